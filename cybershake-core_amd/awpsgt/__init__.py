@@ -1,0 +1,308 @@
+"""Python front end of libawpsgt (include/awpsgt.h): the MI355X AWP-ODC-SGT
+strain-Green-tensor solver.
+
+The library is loaded from ``cybershake-core_amd/lib/libawpsgt.so`` (built by
+``awpsgt.build.build()``).  There is no fallback: if the library is missing or
+a HIP call fails, the call raises.
+
+Typical use mirrors the reference's run of ``pmcl3d IN3D``
+(AWP-ODC-SGT/src/pmcl3d.f):
+
+    p = read_in3d("IN3D")
+    run = Run(p)
+    run.set_media(media)          # float32 [nz][ny][nx][nvar], file layout
+    run.set_source(xyz, stf)      # [nsrc][3], [nsrc][nst][6]
+    run.set_sgt_points(pts)       # [npts][3] cordfile order
+    run.init(); run.step(p.nt)
+    run.write_sgt_file(p.sgtgro)
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Optional, Tuple
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_DIR, "lib", "libawpsgt.so")
+
+AWP_STORAGE_FP32 = 0
+AWP_STORAGE_MIXED = 1
+
+FIELDS = ["u1", "v1", "w1", "xx", "yy", "zz", "xy", "xz", "yz",
+          "r1", "r2", "r3", "r4", "r5", "r6", "d1", "mu", "lam", "qp", "qs"]
+
+_S = ctypes.c_char * 256
+
+
+class In3d(ctypes.Structure):
+    """Mirror of ``awp_in3d`` (include/awpsgt.h)."""
+    _fields_ = [
+        ("igreen", ctypes.c_int), ("tmax", ctypes.c_float), ("dh", ctypes.c_float),
+        ("dt", ctypes.c_float), ("npc", ctypes.c_int), ("nd", ctypes.c_int),
+        ("arbc", ctypes.c_float), ("pht", ctypes.c_float), ("nsrc", ctypes.c_int),
+        ("nst", ctypes.c_int), ("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
+        ("npx", ctypes.c_int), ("npy", ctypes.c_int), ("npz", ctypes.c_int),
+        ("ifault", ctypes.c_int), ("checkpoint", ctypes.c_int), ("isfcvlm", ctypes.c_int),
+        ("imd5", ctypes.c_int), ("ivelocity", ctypes.c_int), ("mediarestart", ctypes.c_int),
+        ("nvar", ctypes.c_int), ("iost", ctypes.c_int), ("partdeg", ctypes.c_int),
+        ("io_opt", ctypes.c_int), ("perf_meas", ctypes.c_int), ("idyna", ctypes.c_int),
+        ("socalq", ctypes.c_int), ("nve", ctypes.c_int), ("mu_s", ctypes.c_float),
+        ("mu_d", ctypes.c_float), ("fl", ctypes.c_float), ("fh", ctypes.c_float),
+        ("fp", ctypes.c_float), ("read_step", ctypes.c_int), ("write_step", ctypes.c_int),
+        ("write_step2", ctypes.c_int),
+        ("nbgx", ctypes.c_int), ("nedx", ctypes.c_int), ("nskpx", ctypes.c_int),
+        ("nbgy", ctypes.c_int), ("nedy", ctypes.c_int), ("nskpy", ctypes.c_int),
+        ("nbgz", ctypes.c_int), ("nedz", ctypes.c_int), ("nskpz", ctypes.c_int),
+        ("ntiskp", ctypes.c_int),
+        ("nbgx2", ctypes.c_int), ("nedx2", ctypes.c_int), ("nskpx2", ctypes.c_int),
+        ("nbgy2", ctypes.c_int), ("nedy2", ctypes.c_int), ("nskpy2", ctypes.c_int),
+        ("nbgz2", ctypes.c_int), ("nedz2", ctypes.c_int), ("nskpz2", ctypes.c_int),
+        ("ntiskp2", ctypes.c_int), ("ntiskp_sgt", ctypes.c_int),
+        ("chkp", _S), ("chkj", _S), ("insrc", _S), ("invel", _S), ("insgt", _S),
+        ("sxgro", _S), ("sygro", _S), ("szgro", _S), ("sxgro2", _S), ("sygro2", _S),
+        ("szgro2", _S), ("sgtgro", _S), ("sgsn", _S),
+        ("nt", ctypes.c_int),
+    ]
+
+    def s(self, name: str) -> str:
+        return getattr(self, name).decode()
+
+
+class AwpError(RuntimeError):
+    pass
+
+
+_lib = None
+
+# every symbol include/awpsgt.h declares
+EXPORTS = [
+    "awp_in3d_read", "awp_in3d_validate", "awp_source_read", "awp_cordfile_count",
+    "awp_cordfile_read", "awp_sgt_select", "awp_sgt_file_geometry", "awp_create",
+    "awp_destroy", "awp_last_error", "awp_set_media", "awp_set_media_device",
+    "awp_set_source", "awp_set_sgt_points", "awp_init", "awp_step", "awp_synchronize",
+    "awp_progress", "awp_get_sgt", "awp_write_sgt_file", "awp_get_field",
+    "awp_set_timing", "awp_get_timing", "awp_device_bytes",
+]
+
+
+def lib() -> ctypes.CDLL:
+    """Load libawpsgt.so; raises when it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise AwpError("libawpsgt.so not built (%s); run awpsgt.build.build()" % LIB_PATH)
+    L = ctypes.CDLL(LIB_PATH)
+    vp, ip, cp, sz = ctypes.c_void_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_size_t
+    P = ctypes.POINTER(In3d)
+    sig = {
+        "awp_in3d_read": ([cp, P, cp, sz], ip),
+        "awp_in3d_validate": ([P, cp, sz], ip),
+        "awp_source_read": ([cp, ip, ip, vp, vp, cp, sz], ip),
+        "awp_cordfile_count": ([cp, ctypes.POINTER(ctypes.c_int), cp, sz], ip),
+        "awp_cordfile_read": ([cp, ip, vp, cp, sz], ip),
+        "awp_sgt_select": ([P, ip, vp, vp, vp, cp, sz], ip),
+        "awp_sgt_file_geometry": ([P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)], ip),
+        "awp_create": ([ctypes.POINTER(vp), P, ip, ip, cp, sz], ip),
+        "awp_destroy": ([vp], ip),
+        "awp_last_error": ([vp], ctypes.c_char_p),
+        "awp_set_media": ([vp, vp], ip),
+        "awp_set_media_device": ([vp, vp], ip),
+        "awp_set_source": ([vp, ip, vp, vp], ip),
+        "awp_set_sgt_points": ([vp, ip, vp, ctypes.POINTER(ctypes.c_int)], ip),
+        "awp_init": ([vp], ip),
+        "awp_step": ([vp, ip], ip),
+        "awp_synchronize": ([vp], ip),
+        "awp_progress": ([vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)], ip),
+        "awp_get_sgt": ([vp, vp, sz], ip),
+        "awp_write_sgt_file": ([vp, cp], ip),
+        "awp_get_field": ([vp, ip, ip, vp], ip),
+        "awp_set_timing": ([vp, ip], ip),
+        "awp_get_timing": ([vp, vp], ip),
+        "awp_device_bytes": ([vp, ctypes.POINTER(ctypes.c_size_t)], ip),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = res
+    _lib = L
+    return L
+
+
+def _check(rc: int, msg: str) -> int:
+    if rc < 0:
+        raise AwpError(msg)
+    return rc
+
+
+def read_in3d(path: str) -> In3d:
+    p = In3d()
+    err = ctypes.create_string_buffer(512)
+    _check(lib().awp_in3d_read(path.encode(), ctypes.byref(p), err, 512), err.value.decode())
+    return p
+
+
+def validate_in3d(p: In3d) -> None:
+    err = ctypes.create_string_buffer(512)
+    _check(lib().awp_in3d_validate(ctypes.byref(p), err, 512), err.value.decode())
+
+
+def read_source(path: str, nsrc: int, nst: int) -> Tuple[np.ndarray, np.ndarray]:
+    xyz = np.zeros((nsrc, 3), dtype=np.int32)
+    stf = np.zeros((nsrc, nst, 6), dtype=np.float32)
+    err = ctypes.create_string_buffer(512)
+    _check(lib().awp_source_read(path.encode(), nsrc, nst, xyz.ctypes.data, stf.ctypes.data, err, 512),
+           err.value.decode())
+    return xyz, stf
+
+
+def read_cordfile(path: str) -> np.ndarray:
+    n = ctypes.c_int(0)
+    err = ctypes.create_string_buffer(512)
+    _check(lib().awp_cordfile_count(path.encode(), ctypes.byref(n), err, 512), err.value.decode())
+    pts = np.zeros((n.value, 3), dtype=np.int32)
+    _check(lib().awp_cordfile_read(path.encode(), n.value, pts.ctypes.data, err, 512), err.value.decode())
+    return pts
+
+
+def sgt_select(p: In3d, pts: np.ndarray) -> Tuple[np.ndarray, np.ndarray]:
+    pts = np.ascontiguousarray(pts, dtype=np.int32).reshape(-1, 3)
+    keep = np.zeros(len(pts), dtype=np.int32)
+    ijk = np.zeros((len(pts), 3), dtype=np.int32)
+    err = ctypes.create_string_buffer(512)
+    _check(lib().awp_sgt_select(ctypes.byref(p), len(pts), pts.ctypes.data, keep.ctypes.data,
+                                ijk.ctypes.data, err, 512), err.value.decode())
+    return keep, ijk
+
+
+def sgt_file_geometry(p: In3d) -> Tuple[int, int]:
+    a, b = ctypes.c_int(0), ctypes.c_int(0)
+    _check(lib().awp_sgt_file_geometry(ctypes.byref(p), ctypes.byref(a), ctypes.byref(b)), "bad geometry")
+    return a.value, b.value
+
+
+def read_media(path: str, p: In3d) -> np.ndarray:
+    n = p.nx * p.ny * p.nz * p.nvar
+    return np.fromfile(path, dtype=np.float32, count=n).reshape(p.nz, p.ny, p.nx, p.nvar)
+
+
+class Run:
+    """One SGT component run on one GPU (an ``awp_ctx``)."""
+
+    def __init__(self, p: In3d, device: int = 0, storage: int = AWP_STORAGE_FP32):
+        self.p = p
+        self.h = ctypes.c_void_p(0)
+        err = ctypes.create_string_buffer(512)
+        _check(lib().awp_create(ctypes.byref(self.h), ctypes.byref(p), device, storage, err, 512),
+               "awp_create: " + err.value.decode())
+        self.nsamp_max = p.nt // p.ntiskp_sgt
+        self.npts = 0
+
+    def _err(self) -> str:
+        return lib().awp_last_error(self.h).decode()
+
+    def _do(self, rc: int, what: str) -> int:
+        if rc < 0:
+            raise AwpError("%s: %s" % (what, self._err()))
+        return rc
+
+    def set_media(self, media: np.ndarray) -> None:
+        m = np.ascontiguousarray(media, dtype=np.float32)
+        if m.size != self.p.nx * self.p.ny * self.p.nz * self.p.nvar:
+            raise ValueError("media has %d values, IN3D needs %d" % (m.size, self.p.nx * self.p.ny * self.p.nz * self.p.nvar))
+        self._do(lib().awp_set_media(self.h, m.ctypes.data), "awp_set_media")
+
+    def set_media_device(self, ptr: int) -> None:
+        self._do(lib().awp_set_media_device(self.h, ctypes.c_void_p(ptr)), "awp_set_media_device")
+
+    def set_source(self, xyz: np.ndarray, stf: np.ndarray) -> None:
+        x = np.ascontiguousarray(xyz, dtype=np.int32).reshape(-1, 3)
+        s = np.ascontiguousarray(stf, dtype=np.float32)
+        if s.size != len(x) * self.p.nst * 6:
+            raise ValueError("stf must be [nsrc][nst][6]")
+        self._do(lib().awp_set_source(self.h, len(x), x.ctypes.data, s.ctypes.data), "awp_set_source")
+
+    def set_sgt_points(self, pts: np.ndarray) -> int:
+        x = np.ascontiguousarray(pts, dtype=np.int32).reshape(-1, 3)
+        n = ctypes.c_int(0)
+        self._do(lib().awp_set_sgt_points(self.h, len(x), x.ctypes.data, ctypes.byref(n)), "awp_set_sgt_points")
+        self.npts = len(x)
+        return n.value
+
+    def init(self) -> None:
+        self._do(lib().awp_init(self.h), "awp_init")
+
+    def step(self, n: int) -> None:
+        self._do(lib().awp_step(self.h, int(n)), "awp_step")
+
+    def synchronize(self) -> None:
+        self._do(lib().awp_synchronize(self.h), "awp_synchronize")
+
+    def progress(self) -> Tuple[int, int]:
+        a, b = ctypes.c_int(0), ctypes.c_int(0)
+        self._do(lib().awp_progress(self.h, ctypes.byref(a), ctypes.byref(b)), "awp_progress")
+        return a.value, b.value
+
+    def sgt(self) -> np.ndarray:
+        out = np.empty((self.npts, 6, self.nsamp_max), dtype=np.float32)
+        self._do(lib().awp_get_sgt(self.h, out.ctypes.data, out.size), "awp_get_sgt")
+        return out
+
+    def write_sgt_file(self, path: str) -> None:
+        self._do(lib().awp_write_sgt_file(self.h, path.encode()), "awp_write_sgt_file")
+
+    def field(self, name: str, kext: int = 0) -> np.ndarray:
+        out = np.empty((self.p.nz + kext, self.p.ny, self.p.nx), dtype=np.float32)
+        self._do(lib().awp_get_field(self.h, FIELDS.index(name), kext, out.ctypes.data), "awp_get_field")
+        return out
+
+    def set_timing(self, on: bool) -> None:
+        self._do(lib().awp_set_timing(self.h, 1 if on else 0), "awp_set_timing")
+
+    def timing(self) -> Tuple[float, float, float]:
+        a = np.zeros(3, dtype=np.float64)
+        self._do(lib().awp_get_timing(self.h, a.ctypes.data), "awp_get_timing")
+        return float(a[0]), float(a[1]), float(a[2])
+
+    def device_bytes(self) -> int:
+        b = ctypes.c_size_t(0)
+        self._do(lib().awp_device_bytes(self.h, ctypes.byref(b)), "awp_device_bytes")
+        return b.value
+
+    def close(self) -> None:
+        if self.h and self.h.value:
+            lib().awp_destroy(self.h)
+            self.h = ctypes.c_void_p(0)
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def run_dir(dirpath: str, device: int = 0, storage: int = AWP_STORAGE_FP32,
+            nsteps: Optional[int] = None) -> Tuple[In3d, Run]:
+    """Read a run directory's IN3D and inputs (paths relative to it), run it
+    and return (params, run).  The SGT file is not written."""
+    p = read_in3d(os.path.join(dirpath, "IN3D"))
+    validate_in3d(p)
+    media = read_media(os.path.join(dirpath, p.s("invel")), p)
+    xyz, stf = read_source(os.path.join(dirpath, p.s("insrc")), p.nsrc, p.nst)
+    pts = read_cordfile(os.path.join(dirpath, p.s("insgt")))
+    r = Run(p, device, storage)
+    r.set_media(media)
+    r.set_source(xyz, stf)
+    r.set_sgt_points(pts)
+    r.init()
+    r.step(p.nt if nsteps is None else nsteps)
+    r.synchronize()
+    return p, r

@@ -1,0 +1,637 @@
+// awp_ctx.hip -- the C ABI of include/awpsgt.h: one context per SGT run,
+// owning its device buffers and HIP stream, driving the kernels of
+// awp_kernels.hip in the reference's step order (pmcl3d.f:1183-1557).
+
+#include <hip/hip_runtime.h>
+
+#include <fcntl.h>
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <unistd.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "awp_kernels.hpp"
+#include "awpsgt.h"
+
+using namespace awp;
+
+struct awp_ctx {
+    awp_in3d p;
+    int device;
+    int storage;
+    Geo g;
+    State s;
+    Coef c;
+    Launch l;
+    hipStream_t stream;
+    float *vol;              // one allocation for the 20 volume arrays
+    float *prof;             // fx | fy | fz
+    // sources
+    int npsrc;
+    long long *d_psrc;       // [npsrc] node index
+    long long *d_ptop;       // [npsrc] index of (x, y, nz)
+    float *d_stf;            // [npsrc][nst][6]
+    float vtst, vtst1;
+    // SGT
+    int npts, nkept, nsamp_max;
+    long long *d_kpos;
+    int *d_kidx;
+    float *d_sg;
+    float *d_sgt;            // [npts][6][nsamp_max]
+    std::vector<int> keep;   // per cordfile point
+    // progress
+    bool media_set, source_set, points_set, initialised;
+    int step, nsamp;
+    // timing
+    bool timing;
+    std::vector<hipEvent_t> ev;  // pairs per timed launch: vel, stress, step
+    double t_vel, t_str, t_all;
+    size_t bytes;
+    char err[512];
+};
+
+static int fail(awp_ctx *ctx, int code, const char *fmt, ...)
+{
+    if (ctx) {
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(ctx->err, sizeof(ctx->err), fmt, ap);
+        va_end(ap);
+    }
+    return code;
+}
+
+#define HIPTRY(ctx, call)                                                                       \
+    do {                                                                                        \
+        hipError_t e_ = (call);                                                                 \
+        if (e_ != hipSuccess)                                                                   \
+            return fail((ctx), AWP_EHIP, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), \
+                        __FILE__, __LINE__);                                                    \
+    } while (0)
+
+static void copy_err(char *dst, size_t n, const char *src)
+{
+    if (dst && n) {
+        strncpy(dst, src, n - 1);
+        dst[n - 1] = 0;
+    }
+}
+
+// ---------------------------------------------------------------------------
+
+extern "C" const char *awp_last_error(const awp_ctx *ctx) { return ctx ? ctx->err : "null context"; }
+
+static Launch choose_launch(const Geo &g)
+{
+    Launch l;
+    l.tile_y = 4;
+    const long long bxy = (long long)((g.nx + TX - 1) / TX) * ((g.ny + l.tile_y - 1) / l.tile_y);
+    int nchunk = (int)std::max<long long>(1, (4096 + bxy - 1) / bxy);
+    nchunk = std::min(nchunk, std::max(1, g.nz / 8));
+    for (;;) {
+        int kc = (g.nz + nchunk - 1) / nchunk;
+        int last = g.nz - (nchunk - 1) * kc;
+        // the block reaching k = nz writes the free-surface stress images
+        // from the two levels below it, so the top chunk holds >= 3 levels
+        if (nchunk == 1 || (last >= 3 && (nchunk - 1) * kc < g.nz)) {
+            l.kc = kc;
+            l.nchunk = (g.nz + kc - 1) / kc;
+            break;
+        }
+        nchunk--;
+    }
+    return l;
+}
+
+extern "C" int awp_create(awp_ctx **out, const awp_in3d *p, int device, int storage, char *err, size_t errlen)
+{
+    if (!out || !p) {
+        copy_err(err, errlen, "null argument");
+        return AWP_EINVAL;
+    }
+    *out = nullptr;
+    int rc = awp_in3d_validate(p, err, errlen);
+    if (rc) return rc;
+    if (storage != AWP_STORAGE_FP32) {
+        copy_err(err, errlen, "storage mode not available in this build");
+        return AWP_EINVAL;
+    }
+    hipError_t e = hipSetDevice(device);
+    if (e != hipSuccess) {
+        snprintf(err, errlen, "hipSetDevice(%d): %s", device, hipGetErrorString(e));
+        return AWP_EHIP;
+    }
+    awp_ctx *ctx = new awp_ctx();
+    memset(ctx->err, 0, sizeof(ctx->err));
+    ctx->p = *p;
+    ctx->device = device;
+    ctx->storage = storage;
+    ctx->g = make_geo(p->nx, p->ny, p->nz);
+    ctx->l = choose_launch(ctx->g);
+    ctx->vol = nullptr;
+    ctx->prof = nullptr;
+    ctx->npsrc = 0;
+    ctx->d_psrc = ctx->d_ptop = nullptr;
+    ctx->d_stf = nullptr;
+    ctx->npts = ctx->nkept = 0;
+    ctx->nsamp_max = p->nt / p->ntiskp_sgt;
+    ctx->d_kpos = nullptr;
+    ctx->d_kidx = nullptr;
+    ctx->d_sg = nullptr;
+    ctx->d_sgt = nullptr;
+    ctx->media_set = ctx->source_set = ctx->points_set = ctx->initialised = false;
+    ctx->step = ctx->nsamp = 0;
+    ctx->timing = false;
+    ctx->t_vel = ctx->t_str = ctx->t_all = 0;
+    ctx->bytes = 0;
+
+    e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        snprintf(err, errlen, "hipStreamCreate: %s", hipGetErrorString(e));
+        delete ctx;
+        return AWP_EHIP;
+    }
+    const size_t nv = (size_t)ctx->g.nvol;
+    const size_t nbytes = nv * 20 * sizeof(float);
+    e = hipMalloc((void **)&ctx->vol, nbytes);
+    if (e != hipSuccess) {
+        snprintf(err, errlen, "hipMalloc(%zu bytes) for the wavefield and media: %s", nbytes, hipGetErrorString(e));
+        hipStreamDestroy(ctx->stream);
+        delete ctx;
+        return AWP_ENOMEM;
+    }
+    ctx->bytes += nbytes;
+    float **arr[20] = {&ctx->s.u1, &ctx->s.v1, &ctx->s.w1, &ctx->s.xx, &ctx->s.yy, &ctx->s.zz,
+                       &ctx->s.xy, &ctx->s.xz, &ctx->s.yz, &ctx->s.r1, &ctx->s.r2, &ctx->s.r3,
+                       &ctx->s.r4, &ctx->s.r5, &ctx->s.r6, &ctx->s.d1, &ctx->s.mu, &ctx->s.lam,
+                       &ctx->s.qp, &ctx->s.qs};
+    for (int a = 0; a < 20; ++a) *arr[a] = ctx->vol + (size_t)a * nv;
+    const size_t np = (size_t)(p->nx + 2) + (p->ny + 2) + (p->nz + 2);
+    e = hipMalloc((void **)&ctx->prof, np * sizeof(float));
+    if (e != hipSuccess) {
+        snprintf(err, errlen, "hipMalloc profiles: %s", hipGetErrorString(e));
+        awp_destroy(ctx);
+        return AWP_ENOMEM;
+    }
+    ctx->bytes += np * sizeof(float);
+    ctx->s.fx = ctx->prof;
+    ctx->s.fy = ctx->prof + (p->nx + 2);
+    ctx->s.fz = ctx->prof + (p->nx + 2) + (p->ny + 2);
+    e = hipMemsetAsync(ctx->vol, 0, nbytes, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) {
+        snprintf(err, errlen, "hipMemset: %s", hipGetErrorString(e));
+        awp_destroy(ctx);
+        return AWP_EHIP;
+    }
+
+    // coefficients of the time step (viscoop.f:67-88), single precision
+    const float dt = p->dt, dh = p->dh;
+    ctx->c.dth = dt / dh;
+    ctx->c.dt = dt;
+    ctx->c.dh1 = 1.0f / dh;
+    ctx->c.c1 = 9.0f / 8.0f;
+    ctx->c.c2 = -1.0f / 24.0f;
+    // sponge profiles, cerjan.f:4-247 (both faces in x and y, bottom in z)
+    {
+        std::vector<float> h(np, 1.0f);
+        const int nd = p->nd;
+        const float alpha = sqrtf(-logf(p->arbc)) / (float)nd;
+        float *fx = h.data(), *fy = fx + (p->nx + 2), *fz = fy + (p->ny + 2);
+        for (int i = 1; i <= p->nx; ++i) {
+            float f = 1.0f;
+            if (i <= nd) { float a = alpha * (float)(nd - i + 1); f = expf(-(a * a)); }
+            else if (i >= p->nx - nd + 1) { float a = alpha * (float)(nd - (p->nx - i)); f = expf(-(a * a)); }
+            fx[i] = f;
+        }
+        for (int j = 1; j <= p->ny; ++j) {
+            float f = 1.0f;
+            if (j <= nd) { float a = alpha * (float)(nd - j + 1); f = expf(-(a * a)); }
+            else if (j >= p->ny - nd + 1) { float a = alpha * (float)(nd - (p->ny - j)); f = expf(-(a * a)); }
+            fy[j] = f;
+        }
+        for (int k = 1; k <= p->nz; ++k) {
+            float f = 1.0f;
+            if (k <= nd) { float a = alpha * (float)(nd - k + 1); f = expf(-(a * a)); }
+            fz[k] = f;
+        }
+        e = hipMemcpy(ctx->prof, h.data(), np * sizeof(float), hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            snprintf(err, errlen, "hipMemcpy profiles: %s", hipGetErrorString(e));
+            awp_destroy(ctx);
+            return AWP_EHIP;
+        }
+    }
+    ctx->vtst = dt / (dh * dh * dh);
+    ctx->vtst1 = 1.0f / (dh * dh);
+    *out = ctx;
+    return AWP_OK;
+}
+
+extern "C" int awp_destroy(awp_ctx *ctx)
+{
+    if (!ctx) return AWP_OK;
+    hipSetDevice(ctx->device);
+    if (ctx->stream) hipStreamSynchronize(ctx->stream);
+    for (auto &e : ctx->ev) hipEventDestroy(e);
+    hipFree(ctx->vol);
+    hipFree(ctx->prof);
+    hipFree(ctx->d_psrc);
+    hipFree(ctx->d_ptop);
+    hipFree(ctx->d_stf);
+    hipFree(ctx->d_kpos);
+    hipFree(ctx->d_kidx);
+    hipFree(ctx->d_sg);
+    hipFree(ctx->d_sgt);
+    if (ctx->stream) hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return AWP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// media
+// ---------------------------------------------------------------------------
+static int media_from_device(awp_ctx *ctx, const float *dmedia)
+{
+    const awp_in3d &p = ctx->p;
+    // constants of structure.f:461-468 and relax.f, single precision
+    const float pi = 4.0f * atanf(1.0f);
+    const float w0 = 2.0f * pi * p.fp;
+    const float ww1 = 2.0f * pi * p.fl;
+    const float w2 = 2.0f * pi * p.fh;
+    const float taumax = 1.0f / ww1;
+    const float taumin = 1.0f / w2;
+    MediaConst mc;
+    mc.pi = pi;
+    mc.tmp1 = 2.0f / pi * (logf(taumax / taumin));
+    mc.tmp2 = 2.0f / pi * logf(w0 * taumin);
+    mc.lw = logf(w2 / w0);
+    mc.socalq = p.socalq;
+    mc.nvar = p.nvar;
+    // relaxation times (relax.f:4-39) and tau1/tau2 (viscoop.f:79-88)
+    static const int tautem[2][2][2] = {{{1, 8}, {7, 2}}, {{6, 3}, {4, 5}}};
+    const float dt1 = 1.0f / p.dt;
+    for (int a = 0; a < 2; ++a)
+        for (int b = 0; b < 2; ++b)
+            for (int c = 0; c < 2; ++c) {
+                float t = (float)tautem[a][b][c];
+                t = (t - 0.5f) / 8;
+                t = 2.0f * t - 1.0f;
+                const float tau = expf(0.5f * (logf(taumax * taumin) + logf(taumax / taumin) * t));
+                ctx->c.tau1[(a * 2 + b) * 2 + c] = 1.0f / ((tau * dt1) + (1.0f / 2.0f));
+                ctx->c.tau2[(a * 2 + b) * 2 + c] = (tau * dt1) - (1.0f / 2.0f);
+            }
+    HIPTRY(ctx, launch_media_setup(ctx->g, ctx->s, dmedia, mc, ctx->stream));
+    HIPTRY(ctx, launch_media_ghosts(ctx->g, ctx->s, ctx->stream));
+    HIPTRY(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->media_set = true;
+    ctx->initialised = false;
+    return AWP_OK;
+}
+
+extern "C" int awp_set_media(awp_ctx *ctx, const float *media_host)
+{
+    if (!ctx || !media_host) return fail(ctx, AWP_EINVAL, "null argument");
+    HIPTRY(ctx, hipSetDevice(ctx->device));
+    const size_t n = (size_t)ctx->p.nx * ctx->p.ny * ctx->p.nz * ctx->p.nvar;
+    float *d = nullptr;
+    hipError_t e = hipMalloc((void **)&d, n * sizeof(float));
+    if (e != hipSuccess) return fail(ctx, AWP_ENOMEM, "hipMalloc(%zu) for the media file: %s", n * 4, hipGetErrorString(e));
+    e = hipMemcpy(d, media_host, n * sizeof(float), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        hipFree(d);
+        return fail(ctx, AWP_EHIP, "hipMemcpy media: %s", hipGetErrorString(e));
+    }
+    int rc = media_from_device(ctx, d);
+    hipFree(d);
+    return rc;
+}
+
+extern "C" int awp_set_media_device(awp_ctx *ctx, const float *media_dev)
+{
+    if (!ctx || !media_dev) return fail(ctx, AWP_EINVAL, "null argument");
+    HIPTRY(ctx, hipSetDevice(ctx->device));
+    return media_from_device(ctx, media_dev);
+}
+
+// ---------------------------------------------------------------------------
+// sources (source.f:170-198, 378-402)
+// ---------------------------------------------------------------------------
+extern "C" int awp_set_source(awp_ctx *ctx, int nsrc, const int *xyz, const float *stf)
+{
+    if (!ctx || nsrc < 1 || !xyz || !stf) return fail(ctx, AWP_EINVAL, "bad source arguments");
+    const awp_in3d &p = ctx->p;
+    std::vector<long long> idx(nsrc), top(nsrc);
+    for (int s = 0; s < nsrc; ++s) {
+        const int x = xyz[3 * s], y = xyz[3 * s + 1], z = xyz[3 * s + 2];
+        // rdsrcpos stops on coordinates outside the grid (source.f:178-193)
+        if (x < 1 || x > p.nx) return fail(ctx, AWP_EINVAL, "X source coordinate %d is outside range 1-%d", x, p.nx);
+        if (y < 1 || y > p.ny) return fail(ctx, AWP_EINVAL, "Y source coordinate %d is outside range 1-%d", y, p.ny);
+        if (z < 1 || z > p.nz) return fail(ctx, AWP_EINVAL, "Z source coordinate %d is outside range 1-%d", z, p.nz);
+        // surface sources write the top plane of the rank that owns them
+        // (source.f:870-902 use nzt): only a source at z = 1 makes that the
+        // free surface of the whole model
+        if (p.igreen >= 4 && z != 1)
+            return fail(ctx, AWP_EINVAL, "igreen=%d is a surface source; its depth index must be 1, got %d", p.igreen, z);
+        idx[s] = gidx(ctx->g, x, y, p.nz + 1 - z);
+        top[s] = gidx(ctx->g, x, y, p.nz);
+    }
+    HIPTRY(ctx, hipSetDevice(ctx->device));
+    hipFree(ctx->d_psrc);
+    hipFree(ctx->d_ptop);
+    hipFree(ctx->d_stf);
+    ctx->d_psrc = ctx->d_ptop = nullptr;
+    ctx->d_stf = nullptr;
+    HIPTRY(ctx, hipMalloc((void **)&ctx->d_psrc, nsrc * sizeof(long long)));
+    HIPTRY(ctx, hipMalloc((void **)&ctx->d_ptop, nsrc * sizeof(long long)));
+    const size_t ns = (size_t)nsrc * p.nst * 6;
+    HIPTRY(ctx, hipMalloc((void **)&ctx->d_stf, ns * sizeof(float)));
+    HIPTRY(ctx, hipMemcpy(ctx->d_psrc, idx.data(), nsrc * sizeof(long long), hipMemcpyHostToDevice));
+    HIPTRY(ctx, hipMemcpy(ctx->d_ptop, top.data(), nsrc * sizeof(long long), hipMemcpyHostToDevice));
+    HIPTRY(ctx, hipMemcpy(ctx->d_stf, stf, ns * sizeof(float), hipMemcpyHostToDevice));
+    ctx->npsrc = nsrc;
+    ctx->source_set = true;
+    ctx->initialised = false;
+    return AWP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// SGT points (station.f:31-80)
+// ---------------------------------------------------------------------------
+__global__ void k_fill(float *a, size_t n, float v)
+{
+    size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; t < n; t += stride) a[t] = v;
+}
+
+extern "C" int awp_set_sgt_points(awp_ctx *ctx, int npts, const int *xyz, int *nkept)
+{
+    if (!ctx || npts < 0 || (npts > 0 && !xyz)) return fail(ctx, AWP_EINVAL, "bad point arguments");
+    std::vector<int> keep(npts > 0 ? npts : 1), ijk(3 * (size_t)(npts > 0 ? npts : 1));
+    int kept = awp_sgt_select(&ctx->p, npts, xyz, keep.data(), ijk.data(), ctx->err, sizeof(ctx->err));
+    if (kept < 0) return kept;
+    std::vector<long long> kpos;
+    std::vector<int> kidx;
+    kpos.reserve(kept);
+    kidx.reserve(kept);
+    for (int n = 0; n < npts; ++n)
+        if (keep[n]) {
+            kpos.push_back(gidx(ctx->g, ijk[3 * n], ijk[3 * n + 1], ijk[3 * n + 2]));
+            kidx.push_back(n);
+        }
+    HIPTRY(ctx, hipSetDevice(ctx->device));
+    hipFree(ctx->d_kpos);
+    hipFree(ctx->d_kidx);
+    hipFree(ctx->d_sg);
+    hipFree(ctx->d_sgt);
+    ctx->d_kpos = nullptr;
+    ctx->d_kidx = nullptr;
+    ctx->d_sg = nullptr;
+    ctx->d_sgt = nullptr;
+    const size_t kk = (size_t)std::max(kept, 1);
+    HIPTRY(ctx, hipMalloc((void **)&ctx->d_kpos, kk * sizeof(long long)));
+    HIPTRY(ctx, hipMalloc((void **)&ctx->d_kidx, kk * sizeof(int)));
+    HIPTRY(ctx, hipMalloc((void **)&ctx->d_sg, kk * 3 * sizeof(float)));
+    const size_t nout = (size_t)std::max(npts, 1) * 6 * std::max(ctx->nsamp_max, 1);
+    hipError_t e = hipMalloc((void **)&ctx->d_sgt, nout * sizeof(float));
+    if (e != hipSuccess)
+        return fail(ctx, AWP_ENOMEM, "hipMalloc(%zu bytes) for the SGT buffer: %s", nout * 4, hipGetErrorString(e));
+    if (kept > 0) {
+        HIPTRY(ctx, hipMemcpy(ctx->d_kpos, kpos.data(), kept * sizeof(long long), hipMemcpyHostToDevice));
+        HIPTRY(ctx, hipMemcpy(ctx->d_kidx, kidx.data(), kept * sizeof(int), hipMemcpyHostToDevice));
+    }
+    ctx->keep.assign(keep.begin(), keep.begin() + npts);
+    ctx->npts = npts;
+    ctx->nkept = kept;
+    ctx->points_set = true;
+    ctx->initialised = false;
+    if (nkept) *nkept = kept;
+    return AWP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// run
+// ---------------------------------------------------------------------------
+extern "C" int awp_init(awp_ctx *ctx)
+{
+    if (!ctx) return AWP_EINVAL;
+    if (!ctx->media_set || !ctx->source_set || !ctx->points_set)
+        return fail(ctx, AWP_ESTATE, "awp_init needs media, source and SGT points first");
+    HIPTRY(ctx, hipSetDevice(ctx->device));
+    // wavefield and memory variables: the first 15 arrays
+    HIPTRY(ctx, hipMemsetAsync(ctx->vol, 0, (size_t)ctx->g.nvol * 15 * sizeof(float), ctx->stream));
+    const size_t nout = (size_t)std::max(ctx->npts, 1) * 6 * std::max(ctx->nsamp_max, 1);
+    hipLaunchKernelGGL(k_fill, dim3(1024), dim3(256), 0, ctx->stream, ctx->d_sgt, nout, -1.0f);
+    HIPTRY(ctx, hipGetLastError());
+    HIPTRY(ctx, launch_sgt_coef(ctx->g, ctx->s, ctx->nkept, ctx->d_kpos, ctx->d_sg, ctx->stream));
+    // first source sample before the loop (pmcl3d.f:988)
+    HIPTRY(ctx, launch_source(ctx->g, ctx->s, ctx->p.igreen, ctx->npsrc, ctx->d_psrc, ctx->d_ptop, ctx->d_stf,
+                              ctx->p.nst, 1, ctx->vtst, ctx->vtst1, ctx->stream));
+    HIPTRY(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->step = 0;
+    ctx->nsamp = 0;
+    ctx->initialised = true;
+    return AWP_OK;
+}
+
+static int ensure_events(awp_ctx *ctx, size_t n)
+{
+    while (ctx->ev.size() < n) {
+        hipEvent_t e;
+        HIPTRY(ctx, hipEventCreate(&e));
+        ctx->ev.push_back(e);
+    }
+    return AWP_OK;
+}
+
+extern "C" int awp_step(awp_ctx *ctx, int nsteps)
+{
+    if (!ctx) return AWP_EINVAL;
+    if (!ctx->initialised) return fail(ctx, AWP_ESTATE, "awp_step before awp_init");
+    if (nsteps < 0) return fail(ctx, AWP_EINVAL, "nsteps < 0");
+    HIPTRY(ctx, hipSetDevice(ctx->device));
+    const awp_in3d &p = ctx->p;
+    const int todo = std::min(nsteps, p.nt - ctx->step);
+    if (ctx->timing) {
+        int rc = ensure_events(ctx, (size_t)6 * std::max(todo, 1));
+        if (rc) return rc;
+    }
+    hipStream_t st = ctx->stream;
+    for (int n = 0; n < todo; ++n) {
+        const int i = ++ctx->step;
+        hipEvent_t *e = ctx->timing ? &ctx->ev[(size_t)6 * n] : nullptr;
+        if (e) HIPTRY(ctx, hipEventRecord(e[0], st));
+        HIPTRY(ctx, launch_velocity(ctx->g, ctx->s, ctx->c, ctx->l, st));
+        if (e) HIPTRY(ctx, hipEventRecord(e[1], st));
+        HIPTRY(ctx, launch_fs_velocity(ctx->g, ctx->s, st));
+        if (e) HIPTRY(ctx, hipEventRecord(e[2], st));
+        HIPTRY(ctx, launch_stress(ctx->g, ctx->s, ctx->c, ctx->l, st));
+        if (e) HIPTRY(ctx, hipEventRecord(e[3], st));
+        if (i + 1 <= p.nst)
+            HIPTRY(ctx, launch_source(ctx->g, ctx->s, p.igreen, ctx->npsrc, ctx->d_psrc, ctx->d_ptop, ctx->d_stf,
+                                      p.nst, i + 1, ctx->vtst, ctx->vtst1, st));
+        if (i % p.ntiskp_sgt == 0 && ctx->nsamp < ctx->nsamp_max) {
+            HIPTRY(ctx, launch_sgt_gather(ctx->g, ctx->s, ctx->nkept, ctx->d_kpos, ctx->d_kidx, ctx->d_sg, ctx->d_sgt,
+                                          ctx->nsamp_max, ctx->nsamp, st));
+            ctx->nsamp++;
+        }
+        if (e) HIPTRY(ctx, hipEventRecord(e[4], st));
+    }
+    if (ctx->timing && todo > 0) {
+        HIPTRY(ctx, hipStreamSynchronize(st));
+        double tv = 0, ts = 0, ta = 0;
+        for (int n = 0; n < todo; ++n) {
+            hipEvent_t *e = &ctx->ev[(size_t)6 * n];
+            float a, b, c;
+            HIPTRY(ctx, hipEventElapsedTime(&a, e[0], e[1]));
+            HIPTRY(ctx, hipEventElapsedTime(&b, e[2], e[3]));
+            HIPTRY(ctx, hipEventElapsedTime(&c, e[0], e[4]));
+            tv += a; ts += b; ta += c;
+        }
+        ctx->t_vel = tv;
+        ctx->t_str = ts;
+        ctx->t_all = ta;
+    }
+    return AWP_OK;
+}
+
+extern "C" int awp_synchronize(awp_ctx *ctx)
+{
+    if (!ctx) return AWP_EINVAL;
+    HIPTRY(ctx, hipSetDevice(ctx->device));
+    HIPTRY(ctx, hipStreamSynchronize(ctx->stream));
+    return AWP_OK;
+}
+
+extern "C" int awp_progress(const awp_ctx *ctx, int *step, int *nsamples)
+{
+    if (!ctx) return AWP_EINVAL;
+    if (step) *step = ctx->step;
+    if (nsamples) *nsamples = ctx->nsamp;
+    return AWP_OK;
+}
+
+extern "C" int awp_set_timing(awp_ctx *ctx, int on)
+{
+    if (!ctx) return AWP_EINVAL;
+    ctx->timing = on != 0;
+    return AWP_OK;
+}
+
+extern "C" int awp_get_timing(awp_ctx *ctx, double *out3)
+{
+    if (!ctx || !out3) return AWP_EINVAL;
+    out3[0] = ctx->t_vel;
+    out3[1] = ctx->t_str;
+    out3[2] = ctx->t_all;
+    return AWP_OK;
+}
+
+extern "C" int awp_device_bytes(const awp_ctx *ctx, size_t *bytes)
+{
+    if (!ctx || !bytes) return AWP_EINVAL;
+    size_t b = ctx->bytes;
+    b += (size_t)std::max(ctx->npts, 1) * 6 * std::max(ctx->nsamp_max, 1) * sizeof(float);
+    *bytes = b;
+    return AWP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// outputs
+// ---------------------------------------------------------------------------
+extern "C" int awp_get_sgt(awp_ctx *ctx, float *host, size_t count)
+{
+    if (!ctx || !host) return AWP_EINVAL;
+    if (!ctx->points_set) return fail(ctx, AWP_ESTATE, "no SGT points");
+    const size_t n = (size_t)ctx->npts * 6 * ctx->nsamp_max;
+    if (count < n) return fail(ctx, AWP_EINVAL, "buffer holds %zu floats, need %zu", count, n);
+    HIPTRY(ctx, hipSetDevice(ctx->device));
+    HIPTRY(ctx, hipStreamSynchronize(ctx->stream));
+    if (n) HIPTRY(ctx, hipMemcpy(host, ctx->d_sgt, n * sizeof(float), hipMemcpyDeviceToHost));
+    return AWP_OK;
+}
+
+extern "C" int awp_write_sgt_file(awp_ctx *ctx, const char *path)
+{
+    if (!ctx || !path) return AWP_EINVAL;
+    if (!ctx->points_set) return fail(ctx, AWP_ESTATE, "no SGT points");
+    const awp_in3d &p = ctx->p;
+    int nt_out = 0, nflush = 0;
+    awp_sgt_file_geometry(&p, &nt_out, &nflush);
+    // buffers that filled during the steps run so far
+    const int avail = std::min(nflush, ctx->nsamp / p.write_step);
+    if (avail <= 0) return AWP_OK;   // the reference opens the file at the first flush
+    const int ws = p.write_step, ms = ctx->nsamp_max;
+    const size_t pts_per_chunk = 16384;
+    std::vector<float> host(pts_per_chunk * 6 * (size_t)ms);
+    int fd = open(path, O_WRONLY | O_CREAT | O_TRUNC, 0644);
+    if (fd < 0) return fail(ctx, AWP_EIO, "cannot create '%s'", path);
+    HIPTRY(ctx, hipSetDevice(ctx->device));
+    HIPTRY(ctx, hipStreamSynchronize(ctx->stream));
+    const size_t slot = (size_t)4 * nt_out;          // bytes per (point, component)
+    std::vector<char> img;
+    for (size_t g0 = 0; g0 < (size_t)ctx->npts; g0 += pts_per_chunk) {
+        const size_t g1 = std::min((size_t)ctx->npts, g0 + pts_per_chunk);
+        size_t last = g1;   // one past the last kept point of the chunk
+        while (last > g0 && !ctx->keep[last - 1]) --last;
+        size_t first = g0;
+        while (first < last && !ctx->keep[first]) ++first;
+        if (first >= last) continue;
+        const size_t n = (size_t)(last - g0) * 6 * ms;
+        hipError_t e = hipMemcpy(host.data(), ctx->d_sgt + g0 * 6 * (size_t)ms, n * sizeof(float), hipMemcpyDeviceToHost);
+        if (e != hipSuccess) {
+            close(fd);
+            return fail(ctx, AWP_EHIP, "hipMemcpy SGT: %s", hipGetErrorString(e));
+        }
+        // image of the file bytes [first*24*nt_out, end of last point's last flush)
+        const size_t off0 = first * 6 * slot;
+        const size_t end = (last - 1) * 6 * slot + 5 * slot + (size_t)(avail - 1) * 4 * ws + 4 * (size_t)ws;
+        img.assign(end - off0, 0);
+        for (size_t g = first; g < last; ++g) {
+            if (!ctx->keep[g]) continue;
+            for (int c = 0; c < 6; ++c)
+                for (int f = 1; f <= avail; ++f) {
+                    const size_t off = g * 6 * slot + c * slot + (size_t)(f - 1) * 4 * ws;
+                    memcpy(&img[off - off0], &host[((g - g0) * 6 + c) * (size_t)ms + (size_t)(f - 1) * ws],
+                           4 * (size_t)ws);
+                }
+        }
+        size_t done = 0;
+        while (done < img.size()) {
+            ssize_t w = pwrite(fd, img.data() + done, img.size() - done, (off_t)(off0 + done));
+            if (w <= 0) {
+                close(fd);
+                return fail(ctx, AWP_EIO, "write '%s' failed", path);
+            }
+            done += (size_t)w;
+        }
+    }
+    if (close(fd)) return fail(ctx, AWP_EIO, "close '%s' failed", path);
+    return AWP_OK;
+}
+
+extern "C" int awp_get_field(awp_ctx *ctx, int which, int kext, float *host)
+{
+    if (!ctx || !host || which < 0 || which >= 20 || (kext != 0 && kext != 2)) return fail(ctx, AWP_EINVAL, "bad field request");
+    HIPTRY(ctx, hipSetDevice(ctx->device));
+    const float *srcs[20] = {ctx->s.u1, ctx->s.v1, ctx->s.w1, ctx->s.xx, ctx->s.yy, ctx->s.zz, ctx->s.xy,
+                             ctx->s.xz, ctx->s.yz, ctx->s.r1, ctx->s.r2, ctx->s.r3, ctx->s.r4, ctx->s.r5,
+                             ctx->s.r6, ctx->s.d1, ctx->s.mu, ctx->s.lam, ctx->s.qp, ctx->s.qs};
+    const size_t n = (size_t)ctx->p.nx * ctx->p.ny * (ctx->p.nz + kext);
+    float *d = nullptr;
+    hipError_t e = hipMalloc((void **)&d, n * sizeof(float));
+    if (e != hipSuccess) return fail(ctx, AWP_ENOMEM, "hipMalloc field: %s", hipGetErrorString(e));
+    e = launch_pack_field(ctx->g, srcs[which], kext, d, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e == hipSuccess) e = hipMemcpy(host, d, n * sizeof(float), hipMemcpyDeviceToHost);
+    hipFree(d);
+    if (e != hipSuccess) return fail(ctx, AWP_EHIP, "field copy: %s", hipGetErrorString(e));
+    return AWP_OK;
+}

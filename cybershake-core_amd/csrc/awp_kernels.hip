@@ -1,0 +1,516 @@
+// awp_kernels.hip -- CDNA4 kernels of the AWP-ODC-SGT time step.
+//
+// Every arithmetic statement follows the reference CPU solver's operation
+// order (AWP-ODC-SGT/src/operator.f, viscoop.f, cerjan.f, source.f,
+// pmcl3d.f) and the library is compiled with -ffp-contract=off, so each
+// FP32 result is the IEEE result of the same sequence of +,-,*,/ as the
+// reference.  No MFMA: the stencil is not a contraction.
+//
+// Kernel shape: a block is one wave64 row in x times tile_y rows in y and
+// marches a chunk of z levels bottom-up.  Values a point needs from the
+// levels below/above its own are carried in registers (z queues), so each
+// level of each array is fetched once per column; x and y neighbours come
+// from the lines the neighbouring lanes and rows of the block just loaded.
+
+#include "awp_kernels.hpp"
+
+namespace awp {
+
+// ---------------------------------------------------------------------------
+// velocity: uxx1/vyy1/wzz1 (operator.f:92-244) fused with addcrjvel
+// (cerjan.f:292-328)
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256)
+k_velocity(const Geo g, const State s, const Coef c, const int kc)
+{
+    const int i = blockIdx.x * TX + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int kb = blockIdx.z * kc + 1;
+    const int ke = min(kb + kc - 1, g.nz);
+    if (i > g.nx || j > g.ny) return;
+    const long long sy = g.sy, sz = g.sz;
+    long long o = g.base + (long long)kb * sz + (long long)j * sy + i;
+
+    const float *__restrict__ xx = s.xx;
+    const float *__restrict__ yy = s.yy;
+    const float *__restrict__ zz = s.zz;
+    const float *__restrict__ xy = s.xy;
+    const float *__restrict__ xz = s.xz;
+    const float *__restrict__ yz = s.yz;
+    const float *__restrict__ d1 = s.d1;
+    const float c1 = c.c1, c2 = c.c2, dth = c.dth;
+    const float fxy = s.fx[i] * s.fy[j];
+
+    // z queues at the column (i, j)
+    float xz_m2 = xz[o - 2 * sz], xz_m1 = xz[o - sz], xz_0 = xz[o];
+    float yz_m2 = yz[o - 2 * sz], yz_m1 = yz[o - sz], yz_0 = yz[o];
+    float zz_m1 = zz[o - sz], zz_0 = zz[o], zz_p1 = zz[o + sz];
+    // density at level k-1: (i,j), (i,j-1), (i+1,j)
+    float dA_m = d1[o - sz], dB_m = d1[o - sz - sy], dC_m = d1[o - sz + 1];
+
+    for (int k = kb; k <= ke; ++k, o += sz) {
+        const float xz_p1 = xz[o + sz];
+        const float yz_p1 = yz[o + sz];
+        const float zz_p2 = zz[o + 2 * sz];
+        const float dA = d1[o], dB = d1[o - sy], dC = d1[o + 1], dD = d1[o + 1 - sy];
+
+        const float xx_0 = xx[o], xx_m1 = xx[o - 1], xx_p1 = xx[o + 1], xx_m2 = xx[o - 2];
+        const float xy_0 = xy[o], xy_jm1 = xy[o - sy], xy_jp1 = xy[o + sy], xy_jm2 = xy[o - 2 * sy];
+        const float xy_ip1 = xy[o + 1], xy_ip2 = xy[o + 2], xy_im1 = xy[o - 1];
+        const float yy_0 = yy[o], yy_jp1 = yy[o + sy], yy_jp2 = yy[o + 2 * sy], yy_jm1 = yy[o - sy];
+        const float xz_ip1 = xz[o + 1], xz_ip2 = xz[o + 2], xz_im1 = xz[o - 1];
+        const float yz_jm1 = yz[o - sy], yz_jp1 = yz[o + sy], yz_jm2 = yz[o - 2 * sy];
+
+        float d, u, v, w;
+        d = 0.25f * ((dA + dB) + (dA_m + dB_m));
+        u = s.u1[o] + (dth / d) * (
+            c1 * (xx_0 - xx_m1) + c2 * (xx_p1 - xx_m2) +
+            c1 * (xy_0 - xy_jm1) + c2 * (xy_jp1 - xy_jm2) +
+            c1 * (xz_0 - xz_m1) + c2 * (xz_p1 - xz_m2));
+        d = 0.25f * ((dA + dC) + (dA_m + dC_m));
+        v = s.v1[o] + (dth / d) * (
+            c1 * (xy_ip1 - xy_0) + c2 * (xy_ip2 - xy_im1) +
+            c1 * (yy_jp1 - yy_0) + c2 * (yy_jp2 - yy_jm1) +
+            c1 * (yz_0 - yz_m1) + c2 * (yz_p1 - yz_m2));
+        d = 0.25f * ((dA + dC) + (dB + dD));
+        w = s.w1[o] + (dth / d) * (
+            c1 * (xz_ip1 - xz_0) + c2 * (xz_ip2 - xz_im1) +
+            c1 * (yz_0 - yz_jm1) + c2 * (yz_jp1 - yz_jm2) +
+            c1 * (zz_p1 - zz_0) + c2 * (zz_p2 - zz_m1));
+        const float D = fxy * s.fz[k];
+        s.u1[o] = u * D;
+        s.v1[o] = v * D;
+        s.w1[o] = w * D;
+
+        xz_m2 = xz_m1; xz_m1 = xz_0; xz_0 = xz_p1;
+        yz_m2 = yz_m1; yz_m1 = yz_0; yz_0 = yz_p1;
+        zz_m1 = zz_0; zz_0 = zz_p1; zz_p1 = zz_p2;
+        dA_m = dA; dB_m = dB; dC_m = dC;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// free-surface velocity images: fvelxy then fvelz (operator.f:246-306).
+// u(i+1), v(j-1) at nz+1 are recomputed from the same inputs fvelxy uses,
+// so no thread reads another thread's output.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256)
+k_fs_velocity(const Geo g, const State s)
+{
+    const int i = blockIdx.x * TX + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    if (i > g.nx || j > g.ny) return;
+    const long long sy = g.sy, sz = g.sz;
+    const long long p = gidx(g, i, j, g.nz);
+    const long long q = p + sz;
+    const float *u1 = s.u1, *v1 = s.v1, *w1 = s.w1;
+    const float w0 = w1[p];
+    const float ut = u1[p] - (w0 - w1[p - 1]);
+    const float vt = v1[p] - (w1[p + sy] - w0);
+    const float ut_ip1 = (i < g.nx) ? (u1[p + 1] - (w1[p + 1] - w0)) : 0.0f;
+    const float vt_jm1 = (j > 1) ? (v1[p - sy] - (w0 - w1[p - sy])) : 0.0f;
+    const float xl = 1.0f / s.lam[p];
+    const float xl2m = 2.0f / s.mu[p] + xl;
+    const float wt = w1[p - sz] - (xl / xl2m) * (
+        (ut_ip1 - ut) +
+        (u1[p + 1] - u1[p]) +
+        (vt - vt_jm1) +
+        (v1[p] - v1[p - sy]));
+    s.u1[q] = ut;
+    s.v1[q] = vt;
+    s.w1[q] = wt;
+}
+
+// ---------------------------------------------------------------------------
+// stress + memory variables: allq (viscoop.f:49-191) fused with addcrjstr
+// (cerjan.f:249-290) and, in the block that reaches k = nz, fstr
+// (operator.f:575-612).
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256)
+k_stress(const Geo g, const State s, const Coef c, const int kc)
+{
+    const int i = blockIdx.x * TX + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int kb = blockIdx.z * kc + 1;
+    const int ke = min(kb + kc - 1, g.nz);
+    if (i > g.nx || j > g.ny) return;
+    const long long sy = g.sy, sz = g.sz;
+    long long o = g.base + (long long)kb * sz + (long long)j * sy + i;
+
+    const float *__restrict__ u1 = s.u1;
+    const float *__restrict__ v1 = s.v1;
+    const float *__restrict__ w1 = s.w1;
+    const float *__restrict__ lam = s.lam;
+    const float *__restrict__ mu = s.mu;
+    const float *__restrict__ qp = s.qp;
+    const float *__restrict__ qs = s.qs;
+    const float c1 = c.c1, c2 = c.c2, dth = c.dth, dt = c.dt, dh1 = c.dh1;
+    const float fxy = s.fx[i] * s.fy[j];
+    const int itxy = ((i & 1) * 2 + (j & 1)) * 2;
+
+    // z queues of the velocities at the column
+    float u_m1 = u1[o - sz], u_0 = u1[o], u_p1 = u1[o + sz];
+    float v_m1 = v1[o - sz], v_0 = v1[o], v_p1 = v1[o + sz];
+    float w_m2 = w1[o - 2 * sz], w_m1 = w1[o - sz], w_0 = w1[o];
+    // media at level k-1 on the 2x2 (x,y) cell: A=(i,j) B=(i+1,j) C=(i,j-1) D=(i+1,j-1)
+    long long om = o - sz;
+    float lA_m = lam[om], lB_m = lam[om + 1], lC_m = lam[om - sy], lD_m = lam[om + 1 - sy];
+    float mA_m = mu[om], mB_m = mu[om + 1], mC_m = mu[om - sy], mD_m = mu[om + 1 - sy];
+    float pA_m = qp[om], pB_m = qp[om + 1], pC_m = qp[om - sy], pD_m = qp[om + 1 - sy];
+    float sA_m = qs[om], sB_m = qs[om + 1], sC_m = qs[om - sy], sD_m = qs[om + 1 - sy];
+    // final xz/yz/zz of the two levels below (for the free-surface images)
+    float xz_f1 = 0.f, xz_f2 = 0.f, yz_f1 = 0.f, yz_f2 = 0.f, zz_f1 = 0.f;
+
+    for (int k = kb; k <= ke; ++k, o += sz) {
+        const float u_p2 = u1[o + 2 * sz];
+        const float v_p2 = v1[o + 2 * sz];
+        const float w_p1 = w1[o + sz];
+        const float lA = lam[o], lB = lam[o + 1], lC = lam[o - sy], lD = lam[o + 1 - sy];
+        const float mA = mu[o], mB = mu[o + 1], mC = mu[o - sy], mD = mu[o + 1 - sy];
+        const float pA = qp[o], pB = qp[o + 1], pC = qp[o - sy], pD = qp[o + 1 - sy];
+        const float sA = qs[o], sB = qs[o + 1], sC = qs[o - sy], sD = qs[o + 1 - sy];
+
+        const float u_ip1 = u1[o + 1], u_ip2 = u1[o + 2], u_im1 = u1[o - 1];
+        const float u_jp1 = u1[o + sy], u_jp2 = u1[o + 2 * sy], u_jm1 = u1[o - sy];
+        const float v_jm1 = v1[o - sy], v_jp1 = v1[o + sy], v_jm2 = v1[o - 2 * sy];
+        const float v_im1 = v1[o - 1], v_ip1 = v1[o + 1], v_im2 = v1[o - 2];
+        const float w_im1 = w1[o - 1], w_ip1 = w1[o + 1], w_im2 = w1[o - 2];
+        const float w_jp1 = w1[o + sy], w_jp2 = w1[o + 2 * sy], w_jm1 = w1[o - sy];
+
+        const int it = itxy + (((g.nz + 1) - k) & 1);
+        const float vx1 = c.tau1[it];
+        const float vx2 = c.tau2[it];
+
+        float vxl = 8.0f / (lA + lB + lC + lD + lA_m + lB_m + lC_m + lD_m);
+        const float vxm = 8.0f / (mA + mB + mC + mD + mA_m + mB_m + mC_m + mD_m);
+        vxl = vxl + 2.0f * vxm;
+        const float vqpa = 0.125f * (pA + pB + pC + pD + pA_m + pB_m + pC_m + pD_m);
+        const float vqsa = 0.125f * (sA + sB + sC + sD + sA_m + sB_m + sC_m + sD_m);
+
+        const float vt1 = c1 * (u_ip1 - u_0) + c2 * (u_ip2 - u_im1);
+        const float vt2 = c1 * (v_0 - v_jm1) + c2 * (v_jp1 - v_jm2);
+        const float vt3 = c1 * (w_0 - w_m1) + c2 * (w_p1 - w_m2);
+        const float vt4 = c1 * (u_jp1 - u_0) + c2 * (u_jp2 - u_jm1) +
+                          c1 * (v_0 - v_im1) + c2 * (v_ip1 - v_im2);
+        const float vt5 = c1 * (u_p1 - u_0) + c2 * (u_p2 - u_m1) +
+                          c1 * (w_0 - w_im1) + c2 * (w_ip1 - w_im2);
+        const float vt6 = c1 * (v_p1 - v_0) + c2 * (v_p2 - v_m1);
+        const float vt7 = c1 * (w_jp1 - w_0) + c2 * (w_jp2 - w_jm1);
+
+        const float vh1 = -vxm * vqsa;
+        const float vtmp = vxl * (vt1 + vt2 + vt3);
+        const float va1 = -vqpa * vxl * dh1 * (vt1 + vt2 + vt3) / 2.0f;
+
+        float r1 = s.r1[o], r2 = s.r2[o], r3 = s.r3[o], r4 = s.r4[o], r5 = s.r5[o], r6 = s.r6[o];
+        float xx = s.xx[o], yy = s.yy[o], zz = s.zz[o], xy = s.xy[o], xz = s.xz[o], yz = s.yz[o];
+
+        xx = xx + dth * (vtmp - 2.0f * vxm * (vt2 + vt3)) + dt * r1;
+        yy = yy + dth * (vtmp - 2.0f * vxm * (vt1 + vt3)) + dt * r2;
+        zz = zz + dth * (vtmp - 2.0f * vxm * (vt1 + vt2)) + dt * r3;
+
+        float vxmu = 2.0f / (mA + mA_m);
+        float vaas = 0.5f * (sA + sA_m);
+        const float vh2_r4 = -vaas * vxmu / 2.0f;
+        xy = xy + vxmu * dth * vt4 + dt * r4;
+        vxmu = 2.0f / (mA + mC);
+        vaas = 0.5f * (sA + sC);
+        const float vh2_r5 = -vaas * vxmu / 2.0f;
+        xz = xz + vxmu * dth * vt5 + dt * r5;
+        vxmu = 2.0f / (mA + mB);
+        vaas = 0.5f * (sA + sB);
+        const float vh2_r6 = -vaas * vxmu / 2.0f;
+        yz = yz + vxmu * (dth * vt6 + dth * vt7) + dt * r6;
+
+        r1 = (vx2 * r1 - vh1 * (vt2 + vt3) * dh1 + va1) * vx1;
+        r2 = (vx2 * r2 - vh1 * (vt1 + vt3) * dh1 + va1) * vx1;
+        r3 = (vx2 * r3 - vh1 * (vt1 + vt2) * dh1 + va1) * vx1;
+        r4 = (vx2 * r4 + vh2_r4 * dh1 * vt4) * vx1;
+        r5 = (vx2 * r5 + vh2_r5 * dh1 * vt5) * vx1;
+        r6 = (vx2 * r6 + vh2_r6 * dh1 * (vt6 + vt7)) * vx1;
+
+        xx = xx + dt * r1;
+        yy = yy + dt * r2;
+        zz = zz + dt * r3;
+        xy = xy + dt * r4;
+        xz = xz + dt * r5;
+        yz = yz + dt * r6;
+
+        const float D = fxy * s.fz[k];
+        xx = xx * D; yy = yy * D; zz = zz * D;
+        xz = xz * D; yz = yz * D; xy = xy * D;
+
+        s.r1[o] = r1; s.r2[o] = r2; s.r3[o] = r3;
+        s.r4[o] = r4; s.r5[o] = r5; s.r6[o] = r6;
+        s.xx[o] = xx; s.yy[o] = yy; s.xy[o] = xy;
+        if (k == g.nz) {
+            // fstr: images above the surface, xz = yz = 0 on it
+            s.zz[o] = zz;
+            s.zz[o + sz] = -zz;
+            s.zz[o + 2 * sz] = -zz_f1;
+            s.xz[o] = 0.0f;
+            s.yz[o] = 0.0f;
+            s.xz[o + sz] = -xz_f1;
+            s.xz[o + 2 * sz] = -xz_f2;
+            s.yz[o + sz] = -yz_f1;
+            s.yz[o + 2 * sz] = -yz_f2;
+        } else {
+            s.zz[o] = zz; s.xz[o] = xz; s.yz[o] = yz;
+        }
+        xz_f2 = xz_f1; xz_f1 = xz;
+        yz_f2 = yz_f1; yz_f1 = yz;
+        zz_f1 = zz;
+
+        u_m1 = u_0; u_0 = u_p1; u_p1 = u_p2;
+        v_m1 = v_0; v_0 = v_p1; v_p1 = v_p2;
+        w_m2 = w_m1; w_m1 = w_0; w_0 = w_p1;
+        lA_m = lA; lB_m = lB; lC_m = lC; lD_m = lD;
+        mA_m = mA; mB_m = mB; mC_m = mC; mD_m = mD;
+        pA_m = pA; pB_m = pB; pC_m = pC; pD_m = pD;
+        sA_m = sA; sB_m = sB; sC_m = sC; sD_m = sD;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// point source, addsrc (source.f:812-913).  One thread walks the sources in
+// file order so coincident sources accumulate in the reference's order.
+// ---------------------------------------------------------------------------
+__global__ void k_source(const State s, const int igreen, const int npsrc, const long long *psrc_idx,
+                         const long long *psrc_top, const long long sz, const float *stf, const int nst,
+                         const int istep, const float vtst, const float vtst1)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    for (int n = 0; n < npsrc; ++n) {
+        const float *a = stf + ((size_t)n * nst + (istep - 1)) * 6;
+        const float axx = a[0], ayy = a[1], azz = a[2];
+        const long long p = psrc_idx[n];
+        const long long t = psrc_top[n];   // (x, y, nz)
+        switch (igreen) {
+        case 1: s.u1[p] = s.u1[p] + axx * vtst / s.d1[p]; break;
+        case 2: s.v1[p] = s.v1[p] + ayy * vtst / s.d1[p]; break;
+        case 3: s.w1[p] = s.w1[p] + azz * vtst / s.d1[p]; break;
+        case 4:
+            s.xz[t] = axx * vtst1;
+            s.xz[t + sz] = 2.0f * axx * vtst1 - s.xz[t - sz];
+            s.xz[t + 2 * sz] = 2.0f * axx * vtst1 - s.xz[t - 2 * sz];
+            break;
+        case 5:
+            s.yz[t] = ayy * vtst1;
+            s.yz[t + sz] = 2.0f * ayy * vtst1 - s.yz[t - sz];
+            s.yz[t + 2 * sz] = 2.0f * ayy * vtst1 - s.yz[t - 2 * sz];
+            break;
+        case 6:
+            s.zz[t + sz] = 2.0f * azz * vtst1 - s.zz[t];
+            s.zz[t + 2 * sz] = 2.0f * azz * vtst1 - s.zz[t - sz];
+            break;
+        default: break;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// SGT: moduli coefficients at the output nodes (pmcl3d.f:1158-1170) and the
+// strain gather (pmcl3d.f:1546-1557).  out is [npts][6][nsamp_max].
+// ---------------------------------------------------------------------------
+__global__ void k_sgt_coef(const State s, const int nkept, const long long *kpos, float *sg)
+{
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= nkept) return;
+    const long long p = kpos[n];
+    const float lam = s.lam[p], mu = s.mu[p];
+    sg[3 * n + 0] = (1.0f / lam + 1.0f / mu) / (1.0f / mu * (3.0f / lam + 2.0f / mu));
+    sg[3 * n + 1] = -((1.0f / lam) / (2.0f / mu * (3.0f / lam + 2.0f / mu)));
+    sg[3 * n + 2] = mu;
+}
+
+__global__ void k_sgt_gather(const State s, const int nkept, const long long *kpos, const int *kidx,
+                             const float *sg, float *out, const int nsamp_max, const int sample)
+{
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= nkept) return;
+    const long long p = kpos[n];
+    const float sg1 = sg[3 * n], sg2 = sg[3 * n + 1], sg3 = sg[3 * n + 2];
+    const float xx = s.xx[p], yy = s.yy[p], zz = s.zz[p];
+    float *dst = out + (size_t)kidx[n] * 6 * nsamp_max + sample;
+    dst[0 * (size_t)nsamp_max] = sg1 * xx + sg2 * yy + sg2 * zz;
+    dst[1 * (size_t)nsamp_max] = sg2 * xx + sg1 * yy + sg2 * zz;
+    dst[2 * (size_t)nsamp_max] = sg2 * xx + sg2 * yy + sg1 * zz;
+    dst[3 * (size_t)nsamp_max] = sg3 * s.xy[p];
+    dst[4 * (size_t)nsamp_max] = sg3 * s.xz[p];
+    dst[5 * (size_t)nsamp_max] = sg3 * s.yz[p];
+}
+
+// ---------------------------------------------------------------------------
+// media set-up: structure.f:449-524 (transform of one file record into the
+// compliances, density and Q weights at k = nz+1-z), then the ghost ring.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256)
+k_media_setup(const Geo g, const State s, const float *media, const MediaConst mc)
+{
+    const int i = blockIdx.x * TX + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int k = blockIdx.z + 1;
+    if (i > g.nx || j > g.ny) return;
+    const int zf = g.nz - k;
+    const int nvar = mc.nvar;
+    const int voff = (nvar == 8) ? 3 : 0;
+    const float *rec = media + (((size_t)zf * g.ny + (j - 1)) * g.nx + (i - 1)) * nvar;
+    float vp = rec[voff + 0], vs = rec[voff + 1], dd = rec[voff + 2];
+    float pq, sq;
+    if (nvar == 3) {
+        sq = 0.05f * vs;
+        pq = 2.0f * sq;
+    } else {
+        pq = rec[voff + 3];
+        sq = rec[voff + 4];
+    }
+    const float pi = mc.pi, lw = mc.lw;
+    if (mc.socalq == 1) {
+        vs = vs * (1.0f + (lw) / (pi * sq));
+        vp = vp * (1.0f + (lw) / (pi * pq));
+        const float vpvs = vp / vs;
+        if (vpvs < 1.45f) vs = vp / 1.45f;
+    }
+    const long long p = gidx(g, i, j, k);
+    s.mu[p] = 1.0f / (dd * vs * vs);
+    s.lam[p] = 1.0f / (dd * (vp * vp - 2.0f * vs * vs));
+    s.d1[p] = dd;
+    float qpinv, qsinv;
+    if (pq <= 0.0f) {
+        qpinv = 0.0f; qsinv = 0.0f;
+    } else {
+        qpinv = 1.0f / pq;
+        qsinv = 1.0f / sq;
+    }
+    s.qp[p] = mc.tmp1 * qpinv / (1.0f - mc.tmp2 * qpinv);
+    s.qs[p] = mc.tmp1 * qsinv / (1.0f - mc.tmp2 * qsinv);
+}
+
+// Ghost ring of the media, i in 0..nx+1, j in 0..ny+1, k in 0..nz+1 with at
+// least one index on the ring.  lam/mu/d1: padmedia (structure.f:563-676)
+// and afsmedia (structure.f:679-707) replicate the nearest interior value;
+// qp/qs are never padded (zero) except the afsmedia copy of the surface to
+// k = nz+1 under the interior.
+__global__ void __launch_bounds__(256)
+k_media_ghosts(const Geo g, const State s)
+{
+    const int i = blockIdx.x * TX + threadIdx.x;       // 0..nx+1
+    const int j = blockIdx.y * blockDim.y + threadIdx.y; // 0..ny+1
+    const int k = blockIdx.z;                          // 0..nz+1
+    if (i > g.nx + 1 || j > g.ny + 1 || k > g.nz + 1) return;
+    const bool ring = (i == 0 || i == g.nx + 1 || j == 0 || j == g.ny + 1 || k == 0 || k == g.nz + 1);
+    if (!ring) return;
+    const int ci = min(max(i, 1), g.nx), cj = min(max(j, 1), g.ny), ck = min(max(k, 1), g.nz);
+    const long long p = gidx(g, i, j, k);
+    const long long q = gidx(g, ci, cj, ck);
+    s.lam[p] = s.lam[q];
+    s.mu[p] = s.mu[q];
+    s.d1[p] = s.d1[q];
+    const bool top_interior = (k == g.nz + 1 && i >= 1 && i <= g.nx && j >= 1 && j <= g.ny);
+    s.qp[p] = top_interior ? s.qp[q] : 0.0f;
+    s.qs[p] = top_interior ? s.qs[q] : 0.0f;
+}
+
+// padmedia's corner (nxt+1,0,0) takes lam(nxt,1,1) for lam, mu and d1
+// (structure.f:653-655); applied after the ring.
+__global__ void k_media_corner(const Geo g, const State s)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const long long p = gidx(g, g.nx + 1, 0, 0);
+    const float v = s.lam[gidx(g, g.nx, 1, 1)];
+    s.lam[p] = v;
+    s.mu[p] = v;
+    s.d1[p] = v;
+}
+
+__global__ void __launch_bounds__(256)
+k_pack_field(const Geo g, const float *src, const int kext, float *dst)
+{
+    const int i = blockIdx.x * TX + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int k = blockIdx.z + 1;
+    if (i > g.nx || j > g.ny || k > g.nz + kext) return;
+    dst[((size_t)(k - 1) * g.ny + (j - 1)) * g.nx + (i - 1)] = src[gidx(g, i, j, k)];
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+hipError_t launch_velocity(const Geo &g, const State &s, const Coef &c, const Launch &l, hipStream_t st)
+{
+    dim3 blk(TX, l.tile_y);
+    dim3 grd((g.nx + TX - 1) / TX, (g.ny + l.tile_y - 1) / l.tile_y, l.nchunk);
+    hipLaunchKernelGGL(k_velocity, grd, blk, 0, st, g, s, c, l.kc);
+    return hipGetLastError();
+}
+
+hipError_t launch_fs_velocity(const Geo &g, const State &s, hipStream_t st)
+{
+    dim3 blk(TX, 4);
+    dim3 grd((g.nx + TX - 1) / TX, (g.ny + 3) / 4, 1);
+    hipLaunchKernelGGL(k_fs_velocity, grd, blk, 0, st, g, s);
+    return hipGetLastError();
+}
+
+hipError_t launch_stress(const Geo &g, const State &s, const Coef &c, const Launch &l, hipStream_t st)
+{
+    dim3 blk(TX, l.tile_y);
+    dim3 grd((g.nx + TX - 1) / TX, (g.ny + l.tile_y - 1) / l.tile_y, l.nchunk);
+    hipLaunchKernelGGL(k_stress, grd, blk, 0, st, g, s, c, l.kc);
+    return hipGetLastError();
+}
+
+hipError_t launch_source(const Geo &g, const State &s, int igreen, int npsrc, const long long *psrc_idx,
+                         const long long *psrc_top, const float *stf, int nst, int istep, float vtst,
+                         float vtst1, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_source, dim3(1), dim3(64), 0, st, s, igreen, npsrc, psrc_idx, psrc_top, g.sz,
+                       stf, nst, istep, vtst, vtst1);
+    return hipGetLastError();
+}
+
+hipError_t launch_sgt_coef(const Geo &g, const State &s, int nkept, const long long *kpos, float *sg, hipStream_t st)
+{
+    (void)g;
+    if (nkept <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_sgt_coef, dim3((nkept + 255) / 256), dim3(256), 0, st, s, nkept, kpos, sg);
+    return hipGetLastError();
+}
+
+hipError_t launch_sgt_gather(const Geo &g, const State &s, int nkept, const long long *kpos, const int *kidx,
+                             const float *sg, float *out, int nsamp_max, int sample, hipStream_t st)
+{
+    (void)g;
+    if (nkept <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_sgt_gather, dim3((nkept + 255) / 256), dim3(256), 0, st, s, nkept, kpos, kidx, sg, out,
+                       nsamp_max, sample);
+    return hipGetLastError();
+}
+
+hipError_t launch_media_setup(const Geo &g, const State &s, const float *media, const MediaConst &mc, hipStream_t st)
+{
+    dim3 blk(TX, 4);
+    dim3 grd((g.nx + TX - 1) / TX, (g.ny + 3) / 4, g.nz);
+    hipLaunchKernelGGL(k_media_setup, grd, blk, 0, st, g, s, media, mc);
+    return hipGetLastError();
+}
+
+hipError_t launch_media_ghosts(const Geo &g, const State &s, hipStream_t st)
+{
+    dim3 blk(TX, 4);
+    dim3 grd((g.nx + 2 + TX - 1) / TX, (g.ny + 2 + 3) / 4, g.nz + 2);
+    hipLaunchKernelGGL(k_media_ghosts, grd, blk, 0, st, g, s);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_media_corner, dim3(1), dim3(64), 0, st, g, s);
+    return hipGetLastError();
+}
+
+hipError_t launch_pack_field(const Geo &g, const float *src, int kext, float *dst, hipStream_t st)
+{
+    dim3 blk(TX, 4);
+    dim3 grd((g.nx + TX - 1) / TX, (g.ny + 3) / 4, g.nz + kext);
+    hipLaunchKernelGGL(k_pack_field, grd, blk, 0, st, g, src, kext, dst);
+    return hipGetLastError();
+}
+
+}  // namespace awp

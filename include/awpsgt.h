@@ -1,0 +1,187 @@
+/*
+ * awpsgt.h -- C ABI of the MI355X AWP-ODC-SGT strain-Green-tensor solver.
+ *
+ * One context is one SGT component run (one IN3D): it owns its device
+ * buffers and its HIP stream.  All functions return 0 on success and a
+ * negative AWP_E* code on failure; the text of the last error of a context is
+ * available from awp_last_error().  Plain pointers and sizes only.
+ *
+ * The reference's host<->device seam is the set of extern "C" launch wrappers
+ * the AWP-GPU-SGT C host calls (AWP-GPU-SGT/src/kernel_kp1_fix_havg_ifault5.cu):
+ * SetDeviceConstValue (:24), dvelcx_H (:73), dstrqc_H (:124), addsrc_H (:145),
+ * ComputeSGT (:380), over buffers the host allocates (pmcl3d.c:537-823).  This
+ * ABI replaces that seam with a context (no __constant__ globals, so several
+ * runs can live in one process) and keeps the CPU solver's semantics
+ * (AWP-ODC-SGT/src/pmcl3d.f:1183-1557).  Each entry point names the reference
+ * code it replaces.
+ */
+#ifndef AWPSGT_H
+#define AWPSGT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AWPSGT_ABI_VERSION 1
+
+enum {
+    AWP_OK = 0,
+    AWP_EINVAL = -1,   /* invalid argument or unsupported IN3D setting */
+    AWP_EHIP = -2,     /* HIP runtime error */
+    AWP_ENOMEM = -3,   /* allocation failed */
+    AWP_EIO = -4,      /* file I/O error */
+    AWP_ESTATE = -5    /* call out of order (e.g. step before media) */
+};
+
+/* Storage modes of the state and media arrays (arithmetic is FP32 in all). */
+enum {
+    AWP_STORAGE_FP32 = 0,   /* everything FP32: the reference's storage */
+    AWP_STORAGE_MIXED = 1   /* r1..r6 BF16, d1/lam/mu/qp/qs FP16 */
+};
+
+/* The IN3D parameters the SGT path uses, in the reference's units.  Filled by
+ * awp_in3d_read() (io.f:327-418) or by the caller. */
+typedef struct {
+    int igreen;          /* 1,2,3 body force x/y/z; 4,5,6 surface xz/yz/zz */
+    float tmax, dh, dt;  /* TMAX (s), DH (m), DT (s) */
+    int npc;             /* must be 0 (Cerjan) */
+    int nd;              /* ND sponge width (cells) */
+    float arbc;          /* ARBC */
+    float pht;
+    int nsrc, nst;       /* NSRC, NST */
+    int nx, ny, nz;      /* grid */
+    int npx, npy, npz;   /* reference MPI decomposition (SGT point rule only) */
+    int ifault;          /* must be 0 (text source) */
+    int checkpoint, isfcvlm, imd5, ivelocity;
+    int mediarestart;    /* must be 1 (media file) */
+    int nvar;            /* 3, 5 or 8 */
+    int iost, partdeg, io_opt, perf_meas, idyna;
+    int socalq;          /* SOCALQ */
+    int nve;             /* must be 1 (visco-elastic) */
+    float mu_s, mu_d, fl, fh, fp;
+    int read_step, write_step, write_step2;
+    int nbgx, nedx, nskpx, nbgy, nedy, nskpy, nbgz, nedz, nskpz, ntiskp;
+    int nbgx2, nedx2, nskpx2, nbgy2, nedy2, nskpy2, nbgz2, nedz2, nskpz2, ntiskp2;
+    int ntiskp_sgt;      /* NTISKP_SGT */
+    char chkp[256], chkj[256], insrc[256], invel[256], insgt[256];
+    char sxgro[256], sygro[256], szgro[256], sxgro2[256], sygro2[256], szgro2[256];
+    char sgtgro[256], sgsn[256];
+    int nt;              /* derived: nint(TMAX/DT)+1 (pmcl3d.f:287) */
+} awp_in3d;
+
+typedef struct awp_ctx awp_ctx;
+
+/* ---------------------------------------------------------------------- */
+/* Host-side formats (C, no device work)                                   */
+/* ---------------------------------------------------------------------- */
+
+/* Parse an IN3D file: 72 positional list-directed records (io.f:327-418,
+ * set_names.f:29-47 for the file-name overrides, which the caller applies).
+ * Computes nt.  Replaces rddt3d (io.f:231). */
+int awp_in3d_read(const char *path, awp_in3d *out, char *err, size_t errlen);
+
+/* Check the IN3D settings against what the SGT path supports (igreen 1..6,
+ * NPC=0, NVE=1, IFAULT=0, MEDIARESTART=1, NVAR 3/5/8, sponge fits, ...).
+ * Unsupported values are errors, never silently ignored. */
+int awp_in3d_validate(const awp_in3d *p, char *err, size_t errlen);
+
+/* Read an IFAULT=0 text source (source.f:170-198): per source an "x y z"
+ * line then NST lines of axx ayy azz axz ayz axy.  xyz [nsrc][3] (file
+ * coordinates, z=1 at the surface), stf [nsrc][nst][6]. */
+int awp_source_read(const char *path, int nsrc, int nst, int *xyz, float *stf,
+                    char *err, size_t errlen);
+
+/* Count of points in a cordfile (io.f:419-442: leading '#' lines skipped). */
+int awp_cordfile_count(const char *path, int *npts, char *err, size_t errlen);
+/* Read the points, [npts][3] (x, y, z; 1-based, z=1 at the surface). */
+int awp_cordfile_read(const char *path, int npts, int *xyz, char *err, size_t errlen);
+
+/* SGT point selection (station.f:31-80), as the reference evaluates it on its
+ * NPX x NPY x NPZ ranks.  For each cordfile point: keep[n] = 1 and
+ * ijk[n] = global node (i, j, k; k = nz is the surface) when some rank owns
+ * it, else keep[n] = 0 (the reference silently drops it).  Points with a
+ * negative x or y, or z > nz+1, make the reference index outside its arrays;
+ * they are rejected with AWP_EINVAL.  Returns the number kept (>= 0). */
+int awp_sgt_select(const awp_in3d *p, int npts, const int *xyz, int *keep, int *ijk,
+                   char *err, size_t errlen);
+
+/* Geometry of the raw SGT file (pmcl3d.f:1582-1601): NT_OUT samples per
+ * (point, component) slot and the number of WRITE_STEP buffers flushed. */
+int awp_sgt_file_geometry(const awp_in3d *p, int *nt_out, int *nflush);
+
+/* ---------------------------------------------------------------------- */
+/* Device run (HIP)                                                        */
+/* ---------------------------------------------------------------------- */
+
+/* Create a context on HIP device `device` (allocates every device buffer;
+ * replaces the host's cudaMalloc block, pmcl3d.c:537-823, and
+ * SetDeviceConstValue, kernel...cu:24). */
+int awp_create(awp_ctx **out, const awp_in3d *p, int device, int storage,
+               char *err, size_t errlen);
+int awp_destroy(awp_ctx *ctx);
+const char *awp_last_error(const awp_ctx *ctx);
+
+/* Media in file layout, float32 [nz][ny][nx][nvar] (structure.f:130-170).
+ * The transform to compliances, density and Q weights (structure.f:449-524),
+ * the ghost replicas (padmedia/afsmedia, structure.f:563-707), the
+ * relaxation times (relax.f:4-39) and the sponge (cerjan.f:4-247) are set up
+ * on the device.  _device takes a device pointer. */
+int awp_set_media(awp_ctx *ctx, const float *media_host);
+int awp_set_media_device(awp_ctx *ctx, const float *media_dev);
+
+/* Point sources: xyz [nsrc][3] file coordinates, stf [nsrc][nst][6]
+ * (inisource/getsrc, source.f:3-402). */
+int awp_set_source(awp_ctx *ctx, int nsrc, const int *xyz, const float *stf);
+
+/* SGT output points in cordfile order (getrec SGT branch, station.f:31-80);
+ * *nkept receives the number the reference rule keeps. */
+int awp_set_sgt_points(awp_ctx *ctx, int npts, const int *xyz, int *nkept);
+
+/* Zero the wavefield and memory variables, inject source sample 1
+ * (pmcl3d.f:988) and reset the step counter. */
+int awp_init(awp_ctx *ctx);
+
+/* Advance nsteps full time steps (pmcl3d.f:1183-1557: velocity, sponge,
+ * free-surface velocity, stress + memory variables, sponge, source sample
+ * i+1, free-surface stress, SGT sample when i % NTISKP_SGT == 0).
+ * Asynchronous on the context's stream.  Replaces the dvelcx_H / dstrqc_H /
+ * addsrc_H / ComputeSGT call sequence of AWP-GPU-SGT/src/pmcl3d.c:841-917. */
+int awp_step(awp_ctx *ctx, int nsteps);
+int awp_synchronize(awp_ctx *ctx);
+/* Completed steps and SGT samples so far. */
+int awp_progress(const awp_ctx *ctx, int *step, int *nsamples);
+
+/* SGT samples so far, [npts][6][nsamp_max] float32 with nsamp_max =
+ * nt / NTISKP_SGT; slots of points the rule drops and samples not yet taken
+ * are -1.  (Strains exx eyy ezz exy exz eyz, pmcl3d.f:1546-1557.) */
+int awp_get_sgt(awp_ctx *ctx, float *host, size_t count);
+
+/* Write the raw SGT file exactly as the reference does (pmcl3d.f:1582-1601):
+ * only full WRITE_STEP buffers, each at byte offset
+ * g*24*NT_OUT + c*4*NT_OUT + (f-1)*4*WRITE_STEP; dropped points leave holes.
+ * No file is created when no buffer fills.  The file is truncated first. */
+int awp_write_sgt_file(awp_ctx *ctx, const char *path);
+
+/* Copy one interior field to the host as [k][j][i], x fastest, k = 1 at the
+ * bottom; kext = 2 also copies the two free-surface image planes.  which:
+ * 0 u1 1 v1 2 w1 3 xx 4 yy 5 zz 6 xy 7 xz 8 yz 9..14 r1..r6 15 d1 16 mu
+ * 17 lam 18 qp 19 qs.  For tests and checkpoints. */
+int awp_get_field(awp_ctx *ctx, int which, int kext, float *host);
+
+/* Device time of the last awp_step() call per kernel family, from HIP events
+ * on the context's stream (milliseconds): out[0] velocity, out[1] stress,
+ * out[2] whole step sequence.  Enabled by awp_set_timing(ctx, 1). */
+int awp_set_timing(awp_ctx *ctx, int on);
+int awp_get_timing(awp_ctx *ctx, double *out3);
+
+/* Bytes of device memory the context holds. */
+int awp_device_bytes(const awp_ctx *ctx, size_t *bytes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* AWPSGT_H */
