@@ -69,7 +69,19 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain device copy of 4 GiB five times (a known-size kernel "
+                         "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE counters)")
     a = ap.parse_args()
+    if a.copy_calibration:
+        import torch
+        x = torch.ones(1 << 30, device="cuda", dtype=torch.float32)
+        y = torch.empty_like(x)
+        for _ in range(5):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": 4 << 30, "copy_bytes_written": 4 << 30, "launches": 5}))
+        return
     if a.gpus != 1:
         print("bench.py measures one GPU: the solver runs one SGT job per GPU and this build "
               "has no multi-GPU path (--gpus %d requested)" % a.gpus, file=sys.stderr)

@@ -90,10 +90,19 @@ extern "C" const char *awp_last_error(const awp_ctx *ctx) { return ctx ? ctx->er
 static Launch choose_launch(const Geo &g)
 {
     Launch l;
-    l.tile_y = 4;
+    l.tile_y = 8;
+    l.unroll = 1;
+    const char *un = getenv("AWP_UNROLL");
+    if (un && atoi(un) == 2) l.unroll = 2;
+    // tuning overrides: AWP_TILE_Y (rows per block: 1, 2, 4, 8) and AWP_KC
+    // (z levels per block)
+    const char *ty = getenv("AWP_TILE_Y");
+    if (ty && (atoi(ty) == 1 || atoi(ty) == 2 || atoi(ty) == 4 || atoi(ty) == 8)) l.tile_y = atoi(ty);
     const long long bxy = (long long)((g.nx + TX - 1) / TX) * ((g.ny + l.tile_y - 1) / l.tile_y);
     int nchunk = (int)std::max<long long>(1, (4096 + bxy - 1) / bxy);
     nchunk = std::min(nchunk, std::max(1, g.nz / 8));
+    const char *kc_env = getenv("AWP_KC");
+    if (kc_env && atoi(kc_env) >= 3) nchunk = std::max(1, (g.nz + atoi(kc_env) - 1) / atoi(kc_env));
     for (;;) {
         int kc = (g.nz + nchunk - 1) / nchunk;
         int last = g.nz - (nchunk - 1) * kc;

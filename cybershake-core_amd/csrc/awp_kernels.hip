@@ -20,12 +20,37 @@ namespace awp {
 // velocity: uxx1/vyy1/wzz1 (operator.f:92-244) fused with addcrjvel
 // (cerjan.f:292-328)
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256)
-k_velocity(const Geo g, const State s, const Coef c, const int kc)
+// Block -> tile map.  The hardware hands consecutive workgroup ids to the 8
+// XCDs in turn, and each XCD has its own L2.  Tiles are numbered x-fastest,
+// then y, then z chunk, and each XCD gets one contiguous range of them, so a
+// tile's x and y neighbours (whose halo lines it reads) run on the same XCD
+// at about the same time and those lines are L2 hits.
+struct Tiles {
+    int nbx, nby, nbz;   // tiles in x, y, z
+    int per;             // tiles per XCD (grid = 8 * per blocks)
+};
+
+__device__ inline bool tile_of(const Tiles &t, int &bx, int &by, int &bz)
 {
-    const int i = blockIdx.x * TX + threadIdx.x + 1;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
-    const int kb = blockIdx.z * kc + 1;
+    const int n = blockIdx.x;
+    const int tile = (n & 7) * t.per + (n >> 3);
+    if (tile >= t.nbx * t.nby * t.nbz) return false;
+    bx = tile % t.nbx;
+    const int r = tile / t.nbx;
+    by = r % t.nby;
+    bz = r / t.nby;
+    return true;
+}
+
+template <int U>
+__global__ void __launch_bounds__(512)
+k_velocity(const Geo g, const State s, const Coef c, const int kc, const Tiles tl)
+{
+    int bx, by, bz;
+    if (!tile_of(tl, bx, by, bz)) return;
+    const int i = bx * TX + threadIdx.x + 1;
+    const int j = by * blockDim.y + threadIdx.y + 1;
+    const int kb = bz * kc + 1;
     const int ke = min(kb + kc - 1, g.nz);
     if (i > g.nx || j > g.ny) return;
     const long long sy = g.sy, sz = g.sz;
@@ -48,6 +73,7 @@ k_velocity(const Geo g, const State s, const Coef c, const int kc)
     // density at level k-1: (i,j), (i,j-1), (i+1,j)
     float dA_m = d1[o - sz], dB_m = d1[o - sz - sy], dC_m = d1[o - sz + 1];
 
+#pragma unroll U
     for (int k = kb; k <= ke; ++k, o += sz) {
         const float xz_p1 = xz[o + sz];
         const float yz_p1 = yz[o + sz];
@@ -126,12 +152,15 @@ k_fs_velocity(const Geo g, const State s)
 // (cerjan.f:249-290) and, in the block that reaches k = nz, fstr
 // (operator.f:575-612).
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256)
-k_stress(const Geo g, const State s, const Coef c, const int kc)
+template <int U>
+__global__ void __launch_bounds__(512)
+k_stress(const Geo g, const State s, const Coef c, const int kc, const Tiles tl)
 {
-    const int i = blockIdx.x * TX + threadIdx.x + 1;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
-    const int kb = blockIdx.z * kc + 1;
+    int bx, by, bz;
+    if (!tile_of(tl, bx, by, bz)) return;
+    const int i = bx * TX + threadIdx.x + 1;
+    const int j = by * blockDim.y + threadIdx.y + 1;
+    const int kb = bz * kc + 1;
     const int ke = min(kb + kc - 1, g.nz);
     if (i > g.nx || j > g.ny) return;
     const long long sy = g.sy, sz = g.sz;
@@ -161,6 +190,7 @@ k_stress(const Geo g, const State s, const Coef c, const int kc)
     // final xz/yz/zz of the two levels below (for the free-surface images)
     float xz_f1 = 0.f, xz_f2 = 0.f, yz_f1 = 0.f, yz_f2 = 0.f, zz_f1 = 0.f;
 
+#pragma unroll U
     for (int k = kb; k <= ke; ++k, o += sz) {
         const float u_p2 = u1[o + 2 * sz];
         const float v_p2 = v1[o + 2 * sz];
@@ -435,11 +465,24 @@ k_pack_field(const Geo g, const float *src, const int kext, float *dst)
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
+static Tiles make_tiles(const Geo &g, const Launch &l)
+{
+    Tiles t;
+    t.nbx = (g.nx + TX - 1) / TX;
+    t.nby = (g.ny + l.tile_y - 1) / l.tile_y;
+    t.nbz = l.nchunk;
+    const long long n = (long long)t.nbx * t.nby * t.nbz;
+    t.per = (int)((n + 7) / 8);
+    return t;
+}
+
 hipError_t launch_velocity(const Geo &g, const State &s, const Coef &c, const Launch &l, hipStream_t st)
 {
-    dim3 blk(TX, l.tile_y);
-    dim3 grd((g.nx + TX - 1) / TX, (g.ny + l.tile_y - 1) / l.tile_y, l.nchunk);
-    hipLaunchKernelGGL(k_velocity, grd, blk, 0, st, g, s, c, l.kc);
+    const Tiles t = make_tiles(g, l);
+    if (l.unroll == 2)
+        hipLaunchKernelGGL(k_velocity<2>, dim3(8 * t.per), dim3(TX, l.tile_y), 0, st, g, s, c, l.kc, t);
+    else
+        hipLaunchKernelGGL(k_velocity<1>, dim3(8 * t.per), dim3(TX, l.tile_y), 0, st, g, s, c, l.kc, t);
     return hipGetLastError();
 }
 
@@ -453,9 +496,11 @@ hipError_t launch_fs_velocity(const Geo &g, const State &s, hipStream_t st)
 
 hipError_t launch_stress(const Geo &g, const State &s, const Coef &c, const Launch &l, hipStream_t st)
 {
-    dim3 blk(TX, l.tile_y);
-    dim3 grd((g.nx + TX - 1) / TX, (g.ny + l.tile_y - 1) / l.tile_y, l.nchunk);
-    hipLaunchKernelGGL(k_stress, grd, blk, 0, st, g, s, c, l.kc);
+    const Tiles t = make_tiles(g, l);
+    if (l.unroll == 2)
+        hipLaunchKernelGGL(k_stress<2>, dim3(8 * t.per), dim3(TX, l.tile_y), 0, st, g, s, c, l.kc, t);
+    else
+        hipLaunchKernelGGL(k_stress<1>, dim3(8 * t.per), dim3(TX, l.tile_y), 0, st, g, s, c, l.kc, t);
     return hipGetLastError();
 }
 

@@ -72,6 +72,7 @@ struct Launch {
     int tile_y;      // threads per block in y
     int kc;          // z levels per block (chunk)
     int nchunk;      // number of z chunks
+    int unroll;      // z-loop unroll factor (1 or 2)
 };
 
 // host launchers (awp_kernels.hip)
