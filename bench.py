@@ -11,12 +11,13 @@ on the device; the timed region is K full time steps (velocity, free surface,
 stress + memory variables, sponge, source, SGT gather) with every input
 already resident in HBM.
 
-    python bench.py [--steps K] [--warmup W] [--config c2|c1]
+    python bench.py [--steps K] [--warmup W] [--config c2|c1] [--storage fp32|mixed]
 
 Prints one JSON line.  value = point-updates per second of the whole job
 (Gpt-updates/s); roofline = the stress kernel (the dominant one): its
-algorithmic bytes per launch (124 B per point: 31 FP32 arrays touched once)
-over its average launch time from HIP events on the run's stream.
+algorithmic bytes per launch (FP32: 124 B per point, 31 FP32 arrays touched
+once; mixed: 96 B per point, r1..r6 BF16 and qp/qs FP16) over its average
+launch time from HIP events on the run's stream.
 """
 from __future__ import annotations
 
@@ -30,8 +31,11 @@ REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(REPO, "cybershake-core_amd"))
 
 HBM_PEAK_GBS = 8000.0
-BYTES_VEL = 52       # per point-update, velocity stage (13 FP32 arrays)
-BYTES_STR = 124      # per point-update, stress stage (31 FP32 arrays)
+BYTES_VEL = 52       # per point-update, velocity stage (13 FP32 arrays, both storage modes)
+# per point-update, stress stage: read u,v,w (12 B); read+write 6 stresses
+# (48 B); read+write r1..r6 (48 B FP32, 24 B BF16); read lam, mu (8 B) and
+# qp, qs (8 B FP32, 4 B FP16)
+BYTES_STR = {"fp32": 124, "mixed": 96}
 
 CONFIGS = {
     "c2": dict(nx=1500, ny=1000, nz=300, dh=100.0, dt=0.005, nsgt=1_500_000, src=(750, 500, 10),
@@ -68,6 +72,7 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
+    ap.add_argument("--storage", default="fp32", choices=["fp32", "mixed"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain device copy of 4 GiB five times (a known-size kernel "
@@ -101,7 +106,8 @@ def main():
           file=sys.stderr, flush=True)
 
     t0 = time.time()
-    run = awpsgt.Run(p, device=0)
+    storage = awpsgt.AWP_STORAGE_MIXED if a.storage == "mixed" else awpsgt.AWP_STORAGE_FP32
+    run = awpsgt.Run(p, device=0, storage=storage)
     if a.config == "c2":
         media = c2_media(p.nx, p.ny, p.nz, p.dh, seed=2)
         torch.cuda.synchronize()
@@ -142,13 +148,15 @@ def main():
     ms_step = el / a.steps * 1e3
     str_ms = ts / a.steps
     vel_ms = tv / a.steps
-    str_gbs = BYTES_STR * npts_grid / (str_ms * 1e-3) / 1e9
+    bstr = BYTES_STR[a.storage]
+    str_gbs = bstr * npts_grid / (str_ms * 1e-3) / 1e9
     vel_gbs = BYTES_VEL * npts_grid / (vel_ms * 1e-3) / 1e9
     print("[bench] %.3f ms/step; velocity %.3f ms (%.0f GB/s), stress %.3f ms (%.0f GB/s); finite=%s"
           % (ms_step, vel_ms, vel_gbs, str_ms, str_gbs, finite), file=sys.stderr, flush=True)
 
     traffic = None
-    tfile = os.path.join(REPO, "profiles", "stress_%s_traffic.txt" % a.config)
+    tfile = os.path.join(REPO, "profiles", "stress_%s%s_traffic.txt"
+                         % (a.config, "" if a.storage == "fp32" else "_" + a.storage))
     if os.path.exists(tfile):
         for ln in open(tfile):
             if ln.startswith("hbm_bytes_per_launch="):
@@ -175,14 +183,15 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": cfg["workload"], "grid": [p.nx, p.ny, p.nz], "points": npts_grid,
-                   "sgt_points": int(nkept), "storage": "fp32", "parallelism": "single GPU"},
+        "config": {"workload": cfg["workload"] if a.storage == "fp32" else
+                   cfg["workload"].replace(", FP32", ", mixed storage (BF16 r1..r6, FP16 qp/qs)"), "grid": [p.nx, p.ny, p.nz], "points": npts_grid,
+                   "sgt_points": int(nkept), "storage": a.storage, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": "k_stress (stress + memory variables + sponge + free surface)",
                      "achieved": round(str_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(str_gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "algorithmic_bytes_per_launch": BYTES_STR * npts_grid,
+                     "algorithmic_bytes_per_launch": bstr * npts_grid,
                      "avg_launch_ms": round(str_ms, 4),
-                     "step_frac": round(value * (BYTES_VEL + BYTES_STR) / HBM_PEAK_GBS, 4)},
+                     "step_frac": round(value * (BYTES_VEL + bstr) / HBM_PEAK_GBS, 4)},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -82,8 +82,8 @@ EXPORTS = [
     "awp_cordfile_read", "awp_sgt_select", "awp_sgt_file_geometry", "awp_create",
     "awp_destroy", "awp_last_error", "awp_set_media", "awp_set_media_device",
     "awp_set_source", "awp_set_sgt_points", "awp_init", "awp_step", "awp_synchronize",
-    "awp_progress", "awp_get_sgt", "awp_write_sgt_file", "awp_get_field",
-    "awp_set_timing", "awp_get_timing", "awp_device_bytes",
+    "awp_progress", "awp_get_sgt", "awp_write_sgt_file", "awp_write_sgt_file_fp16",
+    "awp_sgt16_read", "awp_get_field", "awp_set_timing", "awp_get_timing", "awp_device_bytes",
 ]
 
 
@@ -118,6 +118,8 @@ def lib() -> ctypes.CDLL:
         "awp_progress": ([vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)], ip),
         "awp_get_sgt": ([vp, vp, sz], ip),
         "awp_write_sgt_file": ([vp, cp], ip),
+        "awp_write_sgt_file_fp16": ([vp, cp], ip),
+        "awp_sgt16_read": ([cp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), vp, sz, cp, sz], ip),
         "awp_get_field": ([vp, ip, ip, vp], ip),
         "awp_set_timing": ([vp, ip], ip),
         "awp_get_timing": ([vp, vp], ip),
@@ -181,6 +183,19 @@ def sgt_file_geometry(p: In3d) -> Tuple[int, int]:
     a, b = ctypes.c_int(0), ctypes.c_int(0)
     _check(lib().awp_sgt_file_geometry(ctypes.byref(p), ctypes.byref(a), ctypes.byref(b)), "bad geometry")
     return a.value, b.value
+
+
+def read_sgt16(path: str) -> np.ndarray:
+    """Decode an FP16 SGT file (awp_write_sgt_file_fp16) to float32
+    [npts][6][nsamp]."""
+    n, t = ctypes.c_int(0), ctypes.c_int(0)
+    err = ctypes.create_string_buffer(512)
+    _check(lib().awp_sgt16_read(path.encode(), ctypes.byref(n), ctypes.byref(t), None, 0, err, 512),
+           err.value.decode())
+    out = np.empty((n.value, 6, t.value), dtype=np.float32)
+    _check(lib().awp_sgt16_read(path.encode(), None, None, out.ctypes.data, out.size, err, 512),
+           err.value.decode())
+    return out
 
 
 def read_media(path: str, p: In3d) -> np.ndarray:
@@ -253,6 +268,9 @@ class Run:
     def write_sgt_file(self, path: str) -> None:
         self._do(lib().awp_write_sgt_file(self.h, path.encode()), "awp_write_sgt_file")
 
+    def write_sgt_file_fp16(self, path: str) -> None:
+        self._do(lib().awp_write_sgt_file_fp16(self.h, path.encode()), "awp_write_sgt_file_fp16")
+
     def field(self, name: str, kext: int = 0) -> np.ndarray:
         out = np.empty((self.p.nz + kext, self.p.ny, self.p.nx), dtype=np.float32)
         self._do(lib().awp_get_field(self.h, FIELDS.index(name), kext, out.ctypes.data), "awp_get_field")
@@ -299,9 +317,9 @@ def run_dir(dirpath: str, device: int = 0, storage: int = AWP_STORAGE_FP32,
     xyz, stf = read_source(os.path.join(dirpath, p.s("insrc")), p.nsrc, p.nst)
     pts = read_cordfile(os.path.join(dirpath, p.s("insgt")))
     r = Run(p, device, storage)
+    r.set_sgt_points(pts)
     r.set_media(media)
     r.set_source(xyz, stf)
-    r.set_sgt_points(pts)
     r.init()
     r.step(p.nt if nsteps is None else nsteps)
     r.synchronize()

@@ -53,7 +53,18 @@ struct awp_ctx {
     std::vector<hipEvent_t> ev;  // pairs per timed launch: vel, stress, step
     double t_vel, t_str, t_all;
     size_t bytes;
+    size_t vol_bytes;        // the volume allocation
+    size_t state_bytes;      // its wavefield + memory-variable part (zeroed by awp_init)
+    bool sg_done;            // SGT coefficients computed from FP32 media
     char err[512];
+
+    bool mixed() const { return storage == AWP_STORAGE_MIXED; }
+    Media32 media32() const
+    {
+        // qp/qs are FP32 only in FP32 mode; the SGT coefficients use mu, lam
+        Media32 m = {s.d1, s.mu, s.lam, mixed() ? nullptr : (float *)s.qp, mixed() ? nullptr : (float *)s.qs};
+        return m;
+    }
 };
 
 static int fail(awp_ctx *ctx, int code, const char *fmt, ...)
@@ -127,8 +138,8 @@ extern "C" int awp_create(awp_ctx **out, const awp_in3d *p, int device, int stor
     *out = nullptr;
     int rc = awp_in3d_validate(p, err, errlen);
     if (rc) return rc;
-    if (storage != AWP_STORAGE_FP32) {
-        copy_err(err, errlen, "storage mode not available in this build");
+    if (storage != AWP_STORAGE_FP32 && storage != AWP_STORAGE_MIXED) {
+        copy_err(err, errlen, "unknown storage mode");
         return AWP_EINVAL;
     }
     hipError_t e = hipSetDevice(device);
@@ -159,6 +170,8 @@ extern "C" int awp_create(awp_ctx **out, const awp_in3d *p, int device, int stor
     ctx->timing = false;
     ctx->t_vel = ctx->t_str = ctx->t_all = 0;
     ctx->bytes = 0;
+    ctx->vol_bytes = ctx->state_bytes = 0;
+    ctx->sg_done = false;
 
     e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
@@ -166,8 +179,12 @@ extern "C" int awp_create(awp_ctx **out, const awp_in3d *p, int device, int stor
         delete ctx;
         return AWP_EHIP;
     }
+    // one allocation: 9 FP32 arrays (velocities, stresses), r1..r6 at 4
+    // (FP32) or 2 (mixed) bytes per element -- the state awp_init zeroes --
+    // then d1 mu lam (FP32) and qp qs (4 or 2 bytes)
     const size_t nv = (size_t)ctx->g.nvol;
-    const size_t nbytes = nv * 20 * sizeof(float);
+    const size_t esz = (storage == AWP_STORAGE_MIXED) ? 2 : 4;
+    const size_t nbytes = nv * (12 * sizeof(float) + 8 * esz);
     e = hipMalloc((void **)&ctx->vol, nbytes);
     if (e != hipSuccess) {
         snprintf(err, errlen, "hipMalloc(%zu bytes) for the wavefield and media: %s", nbytes, hipGetErrorString(e));
@@ -176,11 +193,22 @@ extern "C" int awp_create(awp_ctx **out, const awp_in3d *p, int device, int stor
         return AWP_ENOMEM;
     }
     ctx->bytes += nbytes;
-    float **arr[20] = {&ctx->s.u1, &ctx->s.v1, &ctx->s.w1, &ctx->s.xx, &ctx->s.yy, &ctx->s.zz,
-                       &ctx->s.xy, &ctx->s.xz, &ctx->s.yz, &ctx->s.r1, &ctx->s.r2, &ctx->s.r3,
-                       &ctx->s.r4, &ctx->s.r5, &ctx->s.r6, &ctx->s.d1, &ctx->s.mu, &ctx->s.lam,
-                       &ctx->s.qp, &ctx->s.qs};
-    for (int a = 0; a < 20; ++a) *arr[a] = ctx->vol + (size_t)a * nv;
+    ctx->vol_bytes = nbytes;
+    ctx->state_bytes = nv * (9 * sizeof(float) + 6 * esz);
+    float **f32[9] = {&ctx->s.u1, &ctx->s.v1, &ctx->s.w1, &ctx->s.xx, &ctx->s.yy, &ctx->s.zz,
+                      &ctx->s.xy, &ctx->s.xz, &ctx->s.yz};
+    for (int a = 0; a < 9; ++a) *f32[a] = ctx->vol + (size_t)a * nv;
+    char *rest = (char *)(ctx->vol + 9 * nv);
+    void **rr[6] = {&ctx->s.r1, &ctx->s.r2, &ctx->s.r3, &ctx->s.r4, &ctx->s.r5, &ctx->s.r6};
+    for (int a = 0; a < 6; ++a) *rr[a] = rest + (size_t)a * nv * esz;
+    rest += 6 * nv * esz;
+    float **m32[3] = {&ctx->s.d1, &ctx->s.mu, &ctx->s.lam};
+    for (int a = 0; a < 3; ++a) *m32[a] = (float *)(rest + (size_t)a * nv * sizeof(float));
+    rest += 3 * nv * sizeof(float);
+    ctx->s.qp = rest;
+    ctx->s.qs = rest + nv * esz;
+    MScale one = {1, 1, 1, 1};
+    ctx->s.ms = one;
     const size_t np = (size_t)(p->nx + 2) + (p->ny + 2) + (p->nz + 2);
     e = hipMalloc((void **)&ctx->prof, np * sizeof(float));
     if (e != hipSuccess) {
@@ -296,9 +324,52 @@ static int media_from_device(awp_ctx *ctx, const float *dmedia)
                 ctx->c.tau1[(a * 2 + b) * 2 + c] = 1.0f / ((tau * dt1) + (1.0f / 2.0f));
                 ctx->c.tau2[(a * 2 + b) * 2 + c] = (tau * dt1) - (1.0f / 2.0f);
             }
-    HIPTRY(ctx, launch_media_setup(ctx->g, ctx->s, dmedia, mc, ctx->stream));
-    HIPTRY(ctx, launch_media_ghosts(ctx->g, ctx->s, ctx->stream));
-    HIPTRY(ctx, hipStreamSynchronize(ctx->stream));
+    if (!ctx->mixed()) {
+        const Media32 m = ctx->media32();
+        HIPTRY(ctx, launch_media_setup(ctx->g, m, dmedia, mc, ctx->stream));
+        HIPTRY(ctx, launch_media_ghosts(ctx->g, m, ctx->stream));
+        HIPTRY(ctx, hipStreamSynchronize(ctx->stream));
+    } else {
+        // d1, mu, lam straight into their FP32 storage, qp and qs into FP32
+        // temporaries that are then stored as FP16 of value * 2^e
+        const size_t nv = (size_t)ctx->g.nvol;
+        float *tmp = nullptr;
+        hipError_t e = hipMalloc((void **)&tmp, nv * 2 * sizeof(float));
+        if (e != hipSuccess)
+            return fail(ctx, AWP_ENOMEM, "hipMalloc(%zu) for FP32 media set-up: %s", nv * 8, hipGetErrorString(e));
+        Media32 m = {ctx->s.d1, ctx->s.mu, ctx->s.lam, tmp, tmp + nv};
+        e = hipMemsetAsync(tmp, 0, nv * 2 * sizeof(float), ctx->stream);
+        if (e == hipSuccess) e = launch_media_setup(ctx->g, m, dmedia, mc, ctx->stream);
+        if (e == hipSuccess) e = launch_media_ghosts(ctx->g, m, ctx->stream);
+        // per-array power of two putting the largest magnitude in [2^13, 2^14)
+        const int nblk = 1024;
+        float *part = nullptr;
+        if (e == hipSuccess) e = hipMalloc((void **)&part, 2 * nblk * sizeof(float));
+        const float *arrs[2] = {m.qp, m.qs};
+        for (int a = 0; a < 2 && e == hipSuccess; ++a) e = launch_absmax(arrs[a], (long long)nv, part + a * nblk, nblk, ctx->stream);
+        std::vector<float> hp(2 * nblk);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        if (e == hipSuccess) e = hipMemcpy(hp.data(), part, hp.size() * sizeof(float), hipMemcpyDeviceToHost);
+        float sc[2], isc[2];
+        for (int a = 0; a < 2; ++a) {
+            float mx = 0.0f;
+            for (int b = 0; b < nblk; ++b) mx = std::max(mx, hp[a * nblk + b]);
+            int ex = 0;
+            if (mx > 0.0f) frexpf(mx, &ex);   // mx = f * 2^ex, f in [0.5, 1)
+            sc[a] = ldexpf(1.0f, 14 - ex);    // mx * sc in [2^13, 2^14)
+            isc[a] = ldexpf(1.0f, ex - 14);
+        }
+        void *dst[2] = {ctx->s.qp, ctx->s.qs};
+        for (int a = 0; a < 2 && e == hipSuccess; ++a)
+            e = launch_to_half(arrs[a], (_Float16 *)dst[a], sc[a], (long long)nv, ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        hipFree(part);
+        hipFree(tmp);
+        if (e != hipSuccess) return fail(ctx, AWP_EHIP, "mixed media set-up: %s", hipGetErrorString(e));
+        MScale ms = {sc[0], sc[1], isc[0], isc[1]};
+        ctx->s.ms = ms;
+    }
+    ctx->sg_done = false;  // computed at awp_init from the FP32 mu, lam
     ctx->media_set = true;
     ctx->initialised = false;
     return AWP_OK;
@@ -421,6 +492,7 @@ extern "C" int awp_set_sgt_points(awp_ctx *ctx, int npts, const int *xyz, int *n
     ctx->nkept = kept;
     ctx->points_set = true;
     ctx->initialised = false;
+    ctx->sg_done = false;   // d_sg is new: awp_init fills it
     if (nkept) *nkept = kept;
     return AWP_OK;
 }
@@ -434,15 +506,18 @@ extern "C" int awp_init(awp_ctx *ctx)
     if (!ctx->media_set || !ctx->source_set || !ctx->points_set)
         return fail(ctx, AWP_ESTATE, "awp_init needs media, source and SGT points first");
     HIPTRY(ctx, hipSetDevice(ctx->device));
-    // wavefield and memory variables: the first 15 arrays
-    HIPTRY(ctx, hipMemsetAsync(ctx->vol, 0, (size_t)ctx->g.nvol * 15 * sizeof(float), ctx->stream));
+    // wavefield and memory variables: the front of the volume allocation
+    HIPTRY(ctx, hipMemsetAsync(ctx->vol, 0, ctx->state_bytes, ctx->stream));
     const size_t nout = (size_t)std::max(ctx->npts, 1) * 6 * std::max(ctx->nsamp_max, 1);
     hipLaunchKernelGGL(k_fill, dim3(1024), dim3(256), 0, ctx->stream, ctx->d_sgt, nout, -1.0f);
     HIPTRY(ctx, hipGetLastError());
-    HIPTRY(ctx, launch_sgt_coef(ctx->g, ctx->s, ctx->nkept, ctx->d_kpos, ctx->d_sg, ctx->stream));
+    if (!ctx->sg_done) {
+        HIPTRY(ctx, launch_sgt_coef(ctx->media32(), ctx->nkept, ctx->d_kpos, ctx->d_sg, ctx->stream));
+        ctx->sg_done = true;
+    }
     // first source sample before the loop (pmcl3d.f:988)
-    HIPTRY(ctx, launch_source(ctx->g, ctx->s, ctx->p.igreen, ctx->npsrc, ctx->d_psrc, ctx->d_ptop, ctx->d_stf,
-                              ctx->p.nst, 1, ctx->vtst, ctx->vtst1, ctx->stream));
+    HIPTRY(ctx, launch_source(ctx->g, ctx->s, ctx->p.igreen, ctx->npsrc, ctx->d_psrc, ctx->d_ptop,
+                              ctx->d_stf, ctx->p.nst, 1, ctx->vtst, ctx->vtst1, ctx->stream));
     HIPTRY(ctx, hipStreamSynchronize(ctx->stream));
     ctx->step = 0;
     ctx->nsamp = 0;
@@ -477,17 +552,18 @@ extern "C" int awp_step(awp_ctx *ctx, int nsteps)
         const int i = ++ctx->step;
         hipEvent_t *e = ctx->timing ? &ctx->ev[(size_t)6 * n] : nullptr;
         if (e) HIPTRY(ctx, hipEventRecord(e[0], st));
-        HIPTRY(ctx, launch_velocity(ctx->g, ctx->s, ctx->c, ctx->l, st));
+        const bool mx = ctx->mixed();
+        HIPTRY(ctx, launch_velocity(mx, ctx->g, ctx->s, ctx->c, ctx->l, st));
         if (e) HIPTRY(ctx, hipEventRecord(e[1], st));
         HIPTRY(ctx, launch_fs_velocity(ctx->g, ctx->s, st));
         if (e) HIPTRY(ctx, hipEventRecord(e[2], st));
-        HIPTRY(ctx, launch_stress(ctx->g, ctx->s, ctx->c, ctx->l, st));
+        HIPTRY(ctx, launch_stress(mx, ctx->g, ctx->s, ctx->c, ctx->l, st));
         if (e) HIPTRY(ctx, hipEventRecord(e[3], st));
         if (i + 1 <= p.nst)
             HIPTRY(ctx, launch_source(ctx->g, ctx->s, p.igreen, ctx->npsrc, ctx->d_psrc, ctx->d_ptop, ctx->d_stf,
                                       p.nst, i + 1, ctx->vtst, ctx->vtst1, st));
         if (i % p.ntiskp_sgt == 0 && ctx->nsamp < ctx->nsamp_max) {
-            HIPTRY(ctx, launch_sgt_gather(ctx->g, ctx->s, ctx->nkept, ctx->d_kpos, ctx->d_kidx, ctx->d_sg, ctx->d_sgt,
+            HIPTRY(ctx, launch_sgt_gather(ctx->s, ctx->nkept, ctx->d_kpos, ctx->d_kidx, ctx->d_sg, ctx->d_sgt,
                                           ctx->nsamp_max, ctx->nsamp, st));
             ctx->nsamp++;
         }
@@ -626,18 +702,65 @@ extern "C" int awp_write_sgt_file(awp_ctx *ctx, const char *path)
     return AWP_OK;
 }
 
+extern "C" int awp_write_sgt_file_fp16(awp_ctx *ctx, const char *path)
+{
+    if (!ctx || !path) return AWP_EINVAL;
+    if (!ctx->points_set) return fail(ctx, AWP_ESTATE, "no SGT points");
+    const awp_in3d &p = ctx->p;
+    int nt_out = 0, nflush = 0;
+    awp_sgt_file_geometry(&p, &nt_out, &nflush);
+    const int avail = std::min(nflush, ctx->nsamp / p.write_step);
+    const int nsamp = avail * p.write_step;   // the samples the raw file holds
+    const size_t npts = (size_t)ctx->npts;
+    HIPTRY(ctx, hipSetDevice(ctx->device));
+    float *d_scale = nullptr;
+    _Float16 *d_data = nullptr;
+    const size_t ndata = npts * 6 * (size_t)std::max(nsamp, 1);
+    HIPTRY(ctx, hipMalloc((void **)&d_scale, std::max<size_t>(npts, 1) * 6 * sizeof(float)));
+    hipError_t e = hipMalloc((void **)&d_data, ndata * sizeof(_Float16));
+    if (e != hipSuccess) {
+        hipFree(d_scale);
+        return fail(ctx, AWP_ENOMEM, "hipMalloc FP16 SGT: %s", hipGetErrorString(e));
+    }
+    e = hipMemsetAsync(d_scale, 0, std::max<size_t>(npts, 1) * 6 * sizeof(float), ctx->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(d_data, 0, ndata * sizeof(_Float16), ctx->stream);
+    if (e == hipSuccess && nsamp > 0)
+        e = launch_sgt_fp16(ctx->d_sgt, ctx->d_kidx, ctx->nkept, ctx->nsamp_max, nsamp, d_scale, d_data, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    std::vector<float> scale(npts * 6);
+    std::vector<uint16_t> data(npts * 6 * (size_t)nsamp);
+    if (e == hipSuccess && npts) e = hipMemcpy(scale.data(), d_scale, npts * 6 * sizeof(float), hipMemcpyDeviceToHost);
+    if (e == hipSuccess && !data.empty())
+        e = hipMemcpy(data.data(), d_data, data.size() * sizeof(uint16_t), hipMemcpyDeviceToHost);
+    hipFree(d_scale);
+    hipFree(d_data);
+    if (e != hipSuccess) return fail(ctx, AWP_EHIP, "FP16 SGT: %s", hipGetErrorString(e));
+    unsigned char hdr[64];
+    memset(hdr, 0, sizeof(hdr));
+    memcpy(hdr, "AWPSGT16", 8);
+    const uint32_t h32[4] = {1u, (uint32_t)npts, 6u, (uint32_t)nsamp};
+    memcpy(hdr + 8, h32, sizeof(h32));
+    const uint32_t ntk = (uint32_t)p.ntiskp_sgt;
+    memcpy(hdr + 24, &ntk, 4);
+    memcpy(hdr + 28, &p.dt, 4);
+    FILE *f = fopen(path, "wb");
+    if (!f) return fail(ctx, AWP_EIO, "cannot create '%s'", path);
+    bool ok = fwrite(hdr, 1, 64, f) == 64;
+    ok = ok && fwrite(scale.data(), sizeof(float), scale.size(), f) == scale.size();
+    ok = ok && fwrite(data.data(), sizeof(uint16_t), data.size(), f) == data.size();
+    ok = (fclose(f) == 0) && ok;
+    return ok ? AWP_OK : fail(ctx, AWP_EIO, "write '%s' failed", path);
+}
+
 extern "C" int awp_get_field(awp_ctx *ctx, int which, int kext, float *host)
 {
     if (!ctx || !host || which < 0 || which >= 20 || (kext != 0 && kext != 2)) return fail(ctx, AWP_EINVAL, "bad field request");
     HIPTRY(ctx, hipSetDevice(ctx->device));
-    const float *srcs[20] = {ctx->s.u1, ctx->s.v1, ctx->s.w1, ctx->s.xx, ctx->s.yy, ctx->s.zz, ctx->s.xy,
-                             ctx->s.xz, ctx->s.yz, ctx->s.r1, ctx->s.r2, ctx->s.r3, ctx->s.r4, ctx->s.r5,
-                             ctx->s.r6, ctx->s.d1, ctx->s.mu, ctx->s.lam, ctx->s.qp, ctx->s.qs};
     const size_t n = (size_t)ctx->p.nx * ctx->p.ny * (ctx->p.nz + kext);
     float *d = nullptr;
     hipError_t e = hipMalloc((void **)&d, n * sizeof(float));
     if (e != hipSuccess) return fail(ctx, AWP_ENOMEM, "hipMalloc field: %s", hipGetErrorString(e));
-    e = launch_pack_field(ctx->g, srcs[which], kext, d, ctx->stream);
+    e = launch_pack_field(ctx->mixed(), ctx->g, ctx->s, which, kext, d, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e == hipSuccess) e = hipMemcpy(host, d, n * sizeof(float), hipMemcpyDeviceToHost);
     hipFree(d);

@@ -11,15 +11,15 @@
 // levels below/above its own are carried in registers (z queues), so each
 // level of each array is fetched once per column; x and y neighbours come
 // from the lines the neighbouring lanes and rows of the block just loaded.
+//
+// The stress kernel is a template on the storage policy (StoreF32 /
+// StoreMixed); the mixed variant differs only in how r1..r6 and qp/qs are
+// loaded and stored and in the exact power-of-two rescaling of qp/qs.
 
 #include "awp_kernels.hpp"
 
 namespace awp {
 
-// ---------------------------------------------------------------------------
-// velocity: uxx1/vyy1/wzz1 (operator.f:92-244) fused with addcrjvel
-// (cerjan.f:292-328)
-// ---------------------------------------------------------------------------
 // Block -> tile map.  The hardware hands consecutive workgroup ids to the 8
 // XCDs in turn, and each XCD has its own L2.  Tiles are numbered x-fastest,
 // then y, then z chunk, and each XCD gets one contiguous range of them, so a
@@ -42,10 +42,15 @@ __device__ inline bool tile_of(const Tiles &t, int &bx, int &by, int &bz)
     return true;
 }
 
+// ---------------------------------------------------------------------------
+// velocity: uxx1/vyy1/wzz1 (operator.f:92-244) fused with addcrjvel
+// (cerjan.f:292-328)
+// ---------------------------------------------------------------------------
 template <int U>
 __global__ void __launch_bounds__(512)
 k_velocity(const Geo g, const State s, const Coef c, const int kc, const Tiles tl)
 {
+    // the velocity stage reads only FP32 arrays in both storage modes
     int bx, by, bz;
     if (!tile_of(tl, bx, by, bz)) return;
     const int i = bx * TX + threadIdx.x + 1;
@@ -71,14 +76,14 @@ k_velocity(const Geo g, const State s, const Coef c, const int kc, const Tiles t
     float yz_m2 = yz[o - 2 * sz], yz_m1 = yz[o - sz], yz_0 = yz[o];
     float zz_m1 = zz[o - sz], zz_0 = zz[o], zz_p1 = zz[o + sz];
     // density at level k-1: (i,j), (i,j-1), (i+1,j)
-    float dA_m = d1[o - sz], dB_m = d1[o - sz - sy], dC_m = d1[o - sz + 1];
+    float dA_m = ld(d1[o - sz]), dB_m = ld(d1[o - sz - sy]), dC_m = ld(d1[o - sz + 1]);
 
 #pragma unroll U
     for (int k = kb; k <= ke; ++k, o += sz) {
         const float xz_p1 = xz[o + sz];
         const float yz_p1 = yz[o + sz];
         const float zz_p2 = zz[o + 2 * sz];
-        const float dA = d1[o], dB = d1[o - sy], dC = d1[o + 1], dD = d1[o + 1 - sy];
+        const float dA = ld(d1[o]), dB = ld(d1[o - sy]), dC = ld(d1[o + 1]), dD = ld(d1[o + 1 - sy]);
 
         const float xx_0 = xx[o], xx_m1 = xx[o - 1], xx_p1 = xx[o + 1], xx_m2 = xx[o - 2];
         const float xy_0 = xy[o], xy_jm1 = xy[o - sy], xy_jp1 = xy[o + sy], xy_jm2 = xy[o - 2 * sy];
@@ -87,19 +92,22 @@ k_velocity(const Geo g, const State s, const Coef c, const int kc, const Tiles t
         const float xz_ip1 = xz[o + 1], xz_ip2 = xz[o + 2], xz_im1 = xz[o - 1];
         const float yz_jm1 = yz[o - sy], yz_jp1 = yz[o + sy], yz_jm2 = yz[o - 2 * sy];
 
-        float d, u, v, w;
+        float d, q, u, v, w;
         d = 0.25f * ((dA + dB) + (dA_m + dB_m));
-        u = s.u1[o] + (dth / d) * (
+        q = dth / d;
+        u = s.u1[o] + q * (
             c1 * (xx_0 - xx_m1) + c2 * (xx_p1 - xx_m2) +
             c1 * (xy_0 - xy_jm1) + c2 * (xy_jp1 - xy_jm2) +
             c1 * (xz_0 - xz_m1) + c2 * (xz_p1 - xz_m2));
         d = 0.25f * ((dA + dC) + (dA_m + dC_m));
-        v = s.v1[o] + (dth / d) * (
+        q = dth / d;
+        v = s.v1[o] + q * (
             c1 * (xy_ip1 - xy_0) + c2 * (xy_ip2 - xy_im1) +
             c1 * (yy_jp1 - yy_0) + c2 * (yy_jp2 - yy_jm1) +
             c1 * (yz_0 - yz_m1) + c2 * (yz_p1 - yz_m2));
         d = 0.25f * ((dA + dC) + (dB + dD));
-        w = s.w1[o] + (dth / d) * (
+        q = dth / d;
+        w = s.w1[o] + q * (
             c1 * (xz_ip1 - xz_0) + c2 * (xz_ip2 - xz_im1) +
             c1 * (yz_0 - yz_jm1) + c2 * (yz_jp1 - yz_jm2) +
             c1 * (zz_p1 - zz_0) + c2 * (zz_p2 - zz_m1));
@@ -136,7 +144,8 @@ k_fs_velocity(const Geo g, const State s)
     const float ut_ip1 = (i < g.nx) ? (u1[p + 1] - (w1[p + 1] - w0)) : 0.0f;
     const float vt_jm1 = (j > 1) ? (v1[p - sy] - (w0 - w1[p - sy])) : 0.0f;
     const float xl = 1.0f / s.lam[p];
-    const float xl2m = 2.0f / s.mu[p] + xl;
+    const float xm = 2.0f / s.mu[p];
+    const float xl2m = xm + xl;
     const float wt = w1[p - sz] - (xl / xl2m) * (
         (ut_ip1 - ut) +
         (u1[p + 1] - u1[p]) +
@@ -152,10 +161,12 @@ k_fs_velocity(const Geo g, const State s)
 // (cerjan.f:249-290) and, in the block that reaches k = nz, fstr
 // (operator.f:575-612).
 // ---------------------------------------------------------------------------
-template <int U>
+template <class P, int U>
 __global__ void __launch_bounds__(512)
 k_stress(const Geo g, const State s, const Coef c, const int kc, const Tiles tl)
 {
+    using Q = typename P::Q;
+    using R = typename P::R;
     int bx, by, bz;
     if (!tile_of(tl, bx, by, bz)) return;
     const int i = bx * TX + threadIdx.x + 1;
@@ -171,8 +182,14 @@ k_stress(const Geo g, const State s, const Coef c, const int kc, const Tiles tl)
     const float *__restrict__ w1 = s.w1;
     const float *__restrict__ lam = s.lam;
     const float *__restrict__ mu = s.mu;
-    const float *__restrict__ qp = s.qp;
-    const float *__restrict__ qs = s.qs;
+    const Q *__restrict__ qp = (const Q *)s.qp;
+    const Q *__restrict__ qs = (const Q *)s.qs;
+    R *__restrict__ R1 = (R *)s.r1;
+    R *__restrict__ R2 = (R *)s.r2;
+    R *__restrict__ R3 = (R *)s.r3;
+    R *__restrict__ R4 = (R *)s.r4;
+    R *__restrict__ R5 = (R *)s.r5;
+    R *__restrict__ R6 = (R *)s.r6;
     const float c1 = c.c1, c2 = c.c2, dth = c.dth, dt = c.dt, dh1 = c.dh1;
     const float fxy = s.fx[i] * s.fy[j];
     const int itxy = ((i & 1) * 2 + (j & 1)) * 2;
@@ -183,10 +200,10 @@ k_stress(const Geo g, const State s, const Coef c, const int kc, const Tiles tl)
     float w_m2 = w1[o - 2 * sz], w_m1 = w1[o - sz], w_0 = w1[o];
     // media at level k-1 on the 2x2 (x,y) cell: A=(i,j) B=(i+1,j) C=(i,j-1) D=(i+1,j-1)
     long long om = o - sz;
-    float lA_m = lam[om], lB_m = lam[om + 1], lC_m = lam[om - sy], lD_m = lam[om + 1 - sy];
-    float mA_m = mu[om], mB_m = mu[om + 1], mC_m = mu[om - sy], mD_m = mu[om + 1 - sy];
-    float pA_m = qp[om], pB_m = qp[om + 1], pC_m = qp[om - sy], pD_m = qp[om + 1 - sy];
-    float sA_m = qs[om], sB_m = qs[om + 1], sC_m = qs[om - sy], sD_m = qs[om + 1 - sy];
+    float lA_m = ld(lam[om]), lB_m = ld(lam[om + 1]), lC_m = ld(lam[om - sy]), lD_m = ld(lam[om + 1 - sy]);
+    float mA_m = ld(mu[om]), mB_m = ld(mu[om + 1]), mC_m = ld(mu[om - sy]), mD_m = ld(mu[om + 1 - sy]);
+    float pA_m = ld(qp[om]), pB_m = ld(qp[om + 1]), pC_m = ld(qp[om - sy]), pD_m = ld(qp[om + 1 - sy]);
+    float sA_m = ld(qs[om]), sB_m = ld(qs[om + 1]), sC_m = ld(qs[om - sy]), sD_m = ld(qs[om + 1 - sy]);
     // final xz/yz/zz of the two levels below (for the free-surface images)
     float xz_f1 = 0.f, xz_f2 = 0.f, yz_f1 = 0.f, yz_f2 = 0.f, zz_f1 = 0.f;
 
@@ -195,10 +212,10 @@ k_stress(const Geo g, const State s, const Coef c, const int kc, const Tiles tl)
         const float u_p2 = u1[o + 2 * sz];
         const float v_p2 = v1[o + 2 * sz];
         const float w_p1 = w1[o + sz];
-        const float lA = lam[o], lB = lam[o + 1], lC = lam[o - sy], lD = lam[o + 1 - sy];
-        const float mA = mu[o], mB = mu[o + 1], mC = mu[o - sy], mD = mu[o + 1 - sy];
-        const float pA = qp[o], pB = qp[o + 1], pC = qp[o - sy], pD = qp[o + 1 - sy];
-        const float sA = qs[o], sB = qs[o + 1], sC = qs[o - sy], sD = qs[o + 1 - sy];
+        const float lA = ld(lam[o]), lB = ld(lam[o + 1]), lC = ld(lam[o - sy]), lD = ld(lam[o + 1 - sy]);
+        const float mA = ld(mu[o]), mB = ld(mu[o + 1]), mC = ld(mu[o - sy]), mD = ld(mu[o + 1 - sy]);
+        const float pA = ld(qp[o]), pB = ld(qp[o + 1]), pC = ld(qp[o - sy]), pD = ld(qp[o + 1 - sy]);
+        const float sA = ld(qs[o]), sB = ld(qs[o + 1]), sC = ld(qs[o - sy]), sD = ld(qs[o + 1 - sy]);
 
         const float u_ip1 = u1[o + 1], u_ip2 = u1[o + 2], u_im1 = u1[o - 1];
         const float u_jp1 = u1[o + sy], u_jp2 = u1[o + 2 * sy], u_jm1 = u1[o - sy];
@@ -214,8 +231,12 @@ k_stress(const Geo g, const State s, const Coef c, const int kc, const Tiles tl)
         float vxl = 8.0f / (lA + lB + lC + lD + lA_m + lB_m + lC_m + lD_m);
         const float vxm = 8.0f / (mA + mB + mC + mD + mA_m + mB_m + mC_m + mD_m);
         vxl = vxl + 2.0f * vxm;
-        const float vqpa = 0.125f * (pA + pB + pC + pD + pA_m + pB_m + pC_m + pD_m);
-        const float vqsa = 0.125f * (sA + sB + sC + sD + sA_m + sB_m + sC_m + sD_m);
+        float vqpa = 0.125f * (pA + pB + pC + pD + pA_m + pB_m + pC_m + pD_m);
+        float vqsa = 0.125f * (sA + sB + sC + sD + sA_m + sB_m + sC_m + sD_m);
+        if constexpr (P::mixed) {
+            vqpa = vqpa * s.ms.iqp;
+            vqsa = vqsa * s.ms.iqs;
+        }
 
         const float vt1 = c1 * (u_ip1 - u_0) + c2 * (u_ip2 - u_im1);
         const float vt2 = c1 * (v_0 - v_jm1) + c2 * (v_jp1 - v_jm2);
@@ -231,7 +252,7 @@ k_stress(const Geo g, const State s, const Coef c, const int kc, const Tiles tl)
         const float vtmp = vxl * (vt1 + vt2 + vt3);
         const float va1 = -vqpa * vxl * dh1 * (vt1 + vt2 + vt3) / 2.0f;
 
-        float r1 = s.r1[o], r2 = s.r2[o], r3 = s.r3[o], r4 = s.r4[o], r5 = s.r5[o], r6 = s.r6[o];
+        float r1 = ld(R1[o]), r2 = ld(R2[o]), r3 = ld(R3[o]), r4 = ld(R4[o]), r5 = ld(R5[o]), r6 = ld(R6[o]);
         float xx = s.xx[o], yy = s.yy[o], zz = s.zz[o], xy = s.xy[o], xz = s.xz[o], yz = s.yz[o];
 
         xx = xx + dth * (vtmp - 2.0f * vxm * (vt2 + vt3)) + dt * r1;
@@ -240,14 +261,17 @@ k_stress(const Geo g, const State s, const Coef c, const int kc, const Tiles tl)
 
         float vxmu = 2.0f / (mA + mA_m);
         float vaas = 0.5f * (sA + sA_m);
+        if constexpr (P::mixed) vaas = vaas * s.ms.iqs;
         const float vh2_r4 = -vaas * vxmu / 2.0f;
         xy = xy + vxmu * dth * vt4 + dt * r4;
         vxmu = 2.0f / (mA + mC);
         vaas = 0.5f * (sA + sC);
+        if constexpr (P::mixed) vaas = vaas * s.ms.iqs;
         const float vh2_r5 = -vaas * vxmu / 2.0f;
         xz = xz + vxmu * dth * vt5 + dt * r5;
         vxmu = 2.0f / (mA + mB);
         vaas = 0.5f * (sA + sB);
+        if constexpr (P::mixed) vaas = vaas * s.ms.iqs;
         const float vh2_r6 = -vaas * vxmu / 2.0f;
         yz = yz + vxmu * (dth * vt6 + dth * vt7) + dt * r6;
 
@@ -269,8 +293,10 @@ k_stress(const Geo g, const State s, const Coef c, const int kc, const Tiles tl)
         xx = xx * D; yy = yy * D; zz = zz * D;
         xz = xz * D; yz = yz * D; xy = xy * D;
 
-        s.r1[o] = r1; s.r2[o] = r2; s.r3[o] = r3;
-        s.r4[o] = r4; s.r5[o] = r5; s.r6[o] = r6;
+        // the new memory variables enter this step's stresses in FP32 and
+        // are rounded to the storage type once, on the store
+        R1[o] = st<R>(r1); R2[o] = st<R>(r2); R3[o] = st<R>(r3);
+        R4[o] = st<R>(r4); R5[o] = st<R>(r5); R6[o] = st<R>(r6);
         s.xx[o] = xx; s.yy[o] = yy; s.xy[o] = xy;
         if (k == g.nz) {
             // fstr: images above the surface, xz = yz = 0 on it
@@ -309,15 +335,21 @@ __global__ void k_source(const State s, const int igreen, const int npsrc, const
                          const int istep, const float vtst, const float vtst1)
 {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const float *d1 = s.d1;
     for (int n = 0; n < npsrc; ++n) {
         const float *a = stf + ((size_t)n * nst + (istep - 1)) * 6;
         const float axx = a[0], ayy = a[1], azz = a[2];
         const long long p = psrc_idx[n];
         const long long t = psrc_top[n];   // (x, y, nz)
+        float add = 0.0f;
+        if (igreen >= 1 && igreen <= 3) {
+            const float amp = igreen == 1 ? axx : (igreen == 2 ? ayy : azz);
+            add = amp * vtst / d1[p];
+        }
         switch (igreen) {
-        case 1: s.u1[p] = s.u1[p] + axx * vtst / s.d1[p]; break;
-        case 2: s.v1[p] = s.v1[p] + ayy * vtst / s.d1[p]; break;
-        case 3: s.w1[p] = s.w1[p] + azz * vtst / s.d1[p]; break;
+        case 1: s.u1[p] = s.u1[p] + add; break;
+        case 2: s.v1[p] = s.v1[p] + add; break;
+        case 3: s.w1[p] = s.w1[p] + add; break;
         case 4:
             s.xz[t] = axx * vtst1;
             s.xz[t + sz] = 2.0f * axx * vtst1 - s.xz[t - sz];
@@ -338,15 +370,16 @@ __global__ void k_source(const State s, const int igreen, const int npsrc, const
 }
 
 // ---------------------------------------------------------------------------
-// SGT: moduli coefficients at the output nodes (pmcl3d.f:1158-1170) and the
-// strain gather (pmcl3d.f:1546-1557).  out is [npts][6][nsamp_max].
+// SGT: moduli coefficients at the output nodes (pmcl3d.f:1158-1170), from the
+// FP32 media, and the strain gather (pmcl3d.f:1546-1557).  out is
+// [npts][6][nsamp_max].
 // ---------------------------------------------------------------------------
-__global__ void k_sgt_coef(const State s, const int nkept, const long long *kpos, float *sg)
+__global__ void k_sgt_coef(const Media32 m, const int nkept, const long long *kpos, float *sg)
 {
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= nkept) return;
     const long long p = kpos[n];
-    const float lam = s.lam[p], mu = s.mu[p];
+    const float lam = m.lam[p], mu = m.mu[p];
     sg[3 * n + 0] = (1.0f / lam + 1.0f / mu) / (1.0f / mu * (3.0f / lam + 2.0f / mu));
     sg[3 * n + 1] = -((1.0f / lam) / (2.0f / mu * (3.0f / lam + 2.0f / mu)));
     sg[3 * n + 2] = mu;
@@ -370,11 +403,11 @@ __global__ void k_sgt_gather(const State s, const int nkept, const long long *kp
 }
 
 // ---------------------------------------------------------------------------
-// media set-up: structure.f:449-524 (transform of one file record into the
-// compliances, density and Q weights at k = nz+1-z), then the ghost ring.
+// media set-up (FP32): structure.f:449-524 (transform of one file record into
+// the compliances, density and Q weights at k = nz+1-z), then the ghost ring.
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256)
-k_media_setup(const Geo g, const State s, const float *media, const MediaConst mc)
+k_media_setup(const Geo g, const Media32 m, const float *media, const MediaConst mc)
 {
     const int i = blockIdx.x * TX + threadIdx.x + 1;
     const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
@@ -401,9 +434,9 @@ k_media_setup(const Geo g, const State s, const float *media, const MediaConst m
         if (vpvs < 1.45f) vs = vp / 1.45f;
     }
     const long long p = gidx(g, i, j, k);
-    s.mu[p] = 1.0f / (dd * vs * vs);
-    s.lam[p] = 1.0f / (dd * (vp * vp - 2.0f * vs * vs));
-    s.d1[p] = dd;
+    m.mu[p] = 1.0f / (dd * vs * vs);
+    m.lam[p] = 1.0f / (dd * (vp * vp - 2.0f * vs * vs));
+    m.d1[p] = dd;
     float qpinv, qsinv;
     if (pq <= 0.0f) {
         qpinv = 0.0f; qsinv = 0.0f;
@@ -411,8 +444,8 @@ k_media_setup(const Geo g, const State s, const float *media, const MediaConst m
         qpinv = 1.0f / pq;
         qsinv = 1.0f / sq;
     }
-    s.qp[p] = mc.tmp1 * qpinv / (1.0f - mc.tmp2 * qpinv);
-    s.qs[p] = mc.tmp1 * qsinv / (1.0f - mc.tmp2 * qsinv);
+    m.qp[p] = mc.tmp1 * qpinv / (1.0f - mc.tmp2 * qpinv);
+    m.qs[p] = mc.tmp1 * qsinv / (1.0f - mc.tmp2 * qsinv);
 }
 
 // Ghost ring of the media, i in 0..nx+1, j in 0..ny+1, k in 0..nz+1 with at
@@ -421,45 +454,100 @@ k_media_setup(const Geo g, const State s, const float *media, const MediaConst m
 // qp/qs are never padded (zero) except the afsmedia copy of the surface to
 // k = nz+1 under the interior.
 __global__ void __launch_bounds__(256)
-k_media_ghosts(const Geo g, const State s)
+k_media_ghosts(const Geo g, const Media32 m)
 {
-    const int i = blockIdx.x * TX + threadIdx.x;       // 0..nx+1
+    const int i = blockIdx.x * TX + threadIdx.x;         // 0..nx+1
     const int j = blockIdx.y * blockDim.y + threadIdx.y; // 0..ny+1
-    const int k = blockIdx.z;                          // 0..nz+1
+    const int k = blockIdx.z;                            // 0..nz+1
     if (i > g.nx + 1 || j > g.ny + 1 || k > g.nz + 1) return;
     const bool ring = (i == 0 || i == g.nx + 1 || j == 0 || j == g.ny + 1 || k == 0 || k == g.nz + 1);
     if (!ring) return;
     const int ci = min(max(i, 1), g.nx), cj = min(max(j, 1), g.ny), ck = min(max(k, 1), g.nz);
     const long long p = gidx(g, i, j, k);
     const long long q = gidx(g, ci, cj, ck);
-    s.lam[p] = s.lam[q];
-    s.mu[p] = s.mu[q];
-    s.d1[p] = s.d1[q];
+    m.lam[p] = m.lam[q];
+    m.mu[p] = m.mu[q];
+    m.d1[p] = m.d1[q];
     const bool top_interior = (k == g.nz + 1 && i >= 1 && i <= g.nx && j >= 1 && j <= g.ny);
-    s.qp[p] = top_interior ? s.qp[q] : 0.0f;
-    s.qs[p] = top_interior ? s.qs[q] : 0.0f;
+    m.qp[p] = top_interior ? m.qp[q] : 0.0f;
+    m.qs[p] = top_interior ? m.qs[q] : 0.0f;
 }
 
 // padmedia's corner (nxt+1,0,0) takes lam(nxt,1,1) for lam, mu and d1
 // (structure.f:653-655); applied after the ring.
-__global__ void k_media_corner(const Geo g, const State s)
+__global__ void k_media_corner(const Geo g, const Media32 m)
 {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     const long long p = gidx(g, g.nx + 1, 0, 0);
-    const float v = s.lam[gidx(g, g.nx, 1, 1)];
-    s.lam[p] = v;
-    s.mu[p] = v;
-    s.d1[p] = v;
+    const float v = m.lam[gidx(g, g.nx, 1, 1)];
+    m.lam[p] = v;
+    m.mu[p] = v;
+    m.d1[p] = v;
 }
 
+__global__ void __launch_bounds__(256) k_absmax(const float *a, const long long n, float *part)
+{
+    __shared__ float red[256];
+    float mx = 0.0f;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x)
+        mx = fmaxf(mx, fabsf(a[t]));
+    red[threadIdx.x] = mx;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+// FP16 SGT series: one wave per (kept point, component).  scale = 2^e with
+// the series' largest magnitude below 2^15 * scale; values stored as the
+// FP16 nearest (ties to even) to x / scale.
+__global__ void __launch_bounds__(64)
+k_sgt_fp16(const float *sgt, const int *kidx, const int nkept, const int nsamp_max, const int nsamp,
+           float *scales, _Float16 *out)
+{
+    const int item = blockIdx.x;  // kept point * 6 + component
+    if (item >= nkept * 6) return;
+    const int n = kidx[item / 6], c = item % 6;
+    const float *src = sgt + ((size_t)n * 6 + c) * nsamp_max;
+    float mx = 0.0f;
+    for (int t = threadIdx.x; t < nsamp; t += 64) mx = fmaxf(mx, fabsf(src[t]));
+    for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off, 64));
+    int ex = 0;
+    if (mx > 0.0f) frexpf(mx, &ex);   // mx = f * 2^ex, f in [0.5, 1)
+    ex = max(ex, -134);               // scale 2^(ex-15) >= 2^-149 stays representable
+    const float scale = mx > 0.0f ? ldexpf(1.0f, ex - 15) : 0.0f;
+    if (threadIdx.x == 0) scales[(size_t)n * 6 + c] = scale;
+    _Float16 *dst = out + ((size_t)n * 6 + c) * nsamp;
+    // ldexpf scales exactly, also where 2^(15-ex) itself would overflow FP32
+    for (int t = threadIdx.x; t < nsamp; t += 64)
+        dst[t] = (_Float16)(mx > 0.0f ? ldexpf(src[t], 15 - ex) : 0.0f);
+}
+
+hipError_t launch_sgt_fp16(const float *sgt, const int *kidx, int nkept, int nsamp_max, int nsamp, float *scales,
+                           _Float16 *out, hipStream_t st)
+{
+    if (nkept <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_sgt_fp16, dim3(nkept * 6), dim3(64), 0, st, sgt, kidx, nkept, nsamp_max, nsamp, scales, out);
+    return hipGetLastError();
+}
+
+__global__ void __launch_bounds__(256) k_to_half(const float *src, _Float16 *dst, const float scale, const long long n)
+{
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x)
+        dst[t] = (_Float16)(src[t] * scale);
+}
+
+template <class T>
 __global__ void __launch_bounds__(256)
-k_pack_field(const Geo g, const float *src, const int kext, float *dst)
+k_pack_field(const Geo g, const T *src, const float inv_scale, const int kext, float *dst)
 {
     const int i = blockIdx.x * TX + threadIdx.x + 1;
     const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
     const int k = blockIdx.z + 1;
     if (i > g.nx || j > g.ny || k > g.nz + kext) return;
-    dst[((size_t)(k - 1) * g.ny + (j - 1)) * g.nx + (i - 1)] = src[gidx(g, i, j, k)];
+    dst[((size_t)(k - 1) * g.ny + (j - 1)) * g.nx + (i - 1)] = ld(src[gidx(g, i, j, k)]) * inv_scale;
 }
 
 // ---------------------------------------------------------------------------
@@ -476,13 +564,13 @@ static Tiles make_tiles(const Geo &g, const Launch &l)
     return t;
 }
 
-hipError_t launch_velocity(const Geo &g, const State &s, const Coef &c, const Launch &l, hipStream_t st)
+hipError_t launch_velocity(bool, const Geo &g, const State &s, const Coef &c, const Launch &l, hipStream_t st)
 {
     const Tiles t = make_tiles(g, l);
     if (l.unroll == 2)
-        hipLaunchKernelGGL(k_velocity<2>, dim3(8 * t.per), dim3(TX, l.tile_y), 0, st, g, s, c, l.kc, t);
+        hipLaunchKernelGGL((k_velocity<2>), dim3(8 * t.per), dim3(TX, l.tile_y), 0, st, g, s, c, l.kc, t);
     else
-        hipLaunchKernelGGL(k_velocity<1>, dim3(8 * t.per), dim3(TX, l.tile_y), 0, st, g, s, c, l.kc, t);
+        hipLaunchKernelGGL((k_velocity<1>), dim3(8 * t.per), dim3(TX, l.tile_y), 0, st, g, s, c, l.kc, t);
     return hipGetLastError();
 }
 
@@ -494,13 +582,22 @@ hipError_t launch_fs_velocity(const Geo &g, const State &s, hipStream_t st)
     return hipGetLastError();
 }
 
-hipError_t launch_stress(const Geo &g, const State &s, const Coef &c, const Launch &l, hipStream_t st)
+template <class P>
+static void stress_t(const Geo &g, const State &s, const Coef &c, const Launch &l, hipStream_t st)
 {
     const Tiles t = make_tiles(g, l);
     if (l.unroll == 2)
-        hipLaunchKernelGGL(k_stress<2>, dim3(8 * t.per), dim3(TX, l.tile_y), 0, st, g, s, c, l.kc, t);
+        hipLaunchKernelGGL((k_stress<P, 2>), dim3(8 * t.per), dim3(TX, l.tile_y), 0, st, g, s, c, l.kc, t);
     else
-        hipLaunchKernelGGL(k_stress<1>, dim3(8 * t.per), dim3(TX, l.tile_y), 0, st, g, s, c, l.kc, t);
+        hipLaunchKernelGGL((k_stress<P, 1>), dim3(8 * t.per), dim3(TX, l.tile_y), 0, st, g, s, c, l.kc, t);
+}
+
+hipError_t launch_stress(bool mixed, const Geo &g, const State &s, const Coef &c, const Launch &l, hipStream_t st)
+{
+    if (mixed)
+        stress_t<StoreMixed>(g, s, c, l, st);
+    else
+        stress_t<StoreF32>(g, s, c, l, st);
     return hipGetLastError();
 }
 
@@ -508,53 +605,85 @@ hipError_t launch_source(const Geo &g, const State &s, int igreen, int npsrc, co
                          const long long *psrc_top, const float *stf, int nst, int istep, float vtst,
                          float vtst1, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_source, dim3(1), dim3(64), 0, st, s, igreen, npsrc, psrc_idx, psrc_top, g.sz,
-                       stf, nst, istep, vtst, vtst1);
+    hipLaunchKernelGGL(k_source, dim3(1), dim3(64), 0, st, s, igreen, npsrc, psrc_idx, psrc_top, g.sz, stf, nst,
+                       istep, vtst, vtst1);
     return hipGetLastError();
 }
 
-hipError_t launch_sgt_coef(const Geo &g, const State &s, int nkept, const long long *kpos, float *sg, hipStream_t st)
+hipError_t launch_sgt_coef(const Media32 &m, int nkept, const long long *kpos, float *sg, hipStream_t st)
 {
-    (void)g;
     if (nkept <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_sgt_coef, dim3((nkept + 255) / 256), dim3(256), 0, st, s, nkept, kpos, sg);
+    hipLaunchKernelGGL(k_sgt_coef, dim3((nkept + 255) / 256), dim3(256), 0, st, m, nkept, kpos, sg);
     return hipGetLastError();
 }
 
-hipError_t launch_sgt_gather(const Geo &g, const State &s, int nkept, const long long *kpos, const int *kidx,
+hipError_t launch_sgt_gather(const State &s, int nkept, const long long *kpos, const int *kidx,
                              const float *sg, float *out, int nsamp_max, int sample, hipStream_t st)
 {
-    (void)g;
     if (nkept <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_sgt_gather, dim3((nkept + 255) / 256), dim3(256), 0, st, s, nkept, kpos, kidx, sg, out,
                        nsamp_max, sample);
     return hipGetLastError();
 }
 
-hipError_t launch_media_setup(const Geo &g, const State &s, const float *media, const MediaConst &mc, hipStream_t st)
+hipError_t launch_media_setup(const Geo &g, const Media32 &m, const float *media, const MediaConst &mc, hipStream_t st)
 {
     dim3 blk(TX, 4);
     dim3 grd((g.nx + TX - 1) / TX, (g.ny + 3) / 4, g.nz);
-    hipLaunchKernelGGL(k_media_setup, grd, blk, 0, st, g, s, media, mc);
+    hipLaunchKernelGGL(k_media_setup, grd, blk, 0, st, g, m, media, mc);
     return hipGetLastError();
 }
 
-hipError_t launch_media_ghosts(const Geo &g, const State &s, hipStream_t st)
+hipError_t launch_media_ghosts(const Geo &g, const Media32 &m, hipStream_t st)
 {
     dim3 blk(TX, 4);
     dim3 grd((g.nx + 2 + TX - 1) / TX, (g.ny + 2 + 3) / 4, g.nz + 2);
-    hipLaunchKernelGGL(k_media_ghosts, grd, blk, 0, st, g, s);
+    hipLaunchKernelGGL(k_media_ghosts, grd, blk, 0, st, g, m);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_media_corner, dim3(1), dim3(64), 0, st, g, s);
+    hipLaunchKernelGGL(k_media_corner, dim3(1), dim3(64), 0, st, g, m);
     return hipGetLastError();
 }
 
-hipError_t launch_pack_field(const Geo &g, const float *src, int kext, float *dst, hipStream_t st)
+hipError_t launch_absmax(const float *a, long long n, float *part, int nblk, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_absmax, dim3(nblk), dim3(256), 0, st, a, n, part);
+    return hipGetLastError();
+}
+
+hipError_t launch_to_half(const float *src, _Float16 *dst, float scale, long long n, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_to_half, dim3(4096), dim3(256), 0, st, src, dst, scale, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_pack_field(bool mixed, const Geo &g, const State &s, int which, int kext, float *dst,
+                             hipStream_t st)
 {
     dim3 blk(TX, 4);
     dim3 grd((g.nx + TX - 1) / TX, (g.ny + 3) / 4, g.nz + kext);
-    hipLaunchKernelGGL(k_pack_field, grd, blk, 0, st, g, src, kext, dst);
+    const float *f32[9] = {s.u1, s.v1, s.w1, s.xx, s.yy, s.zz, s.xy, s.xz, s.yz};
+    void *rr[6] = {s.r1, s.r2, s.r3, s.r4, s.r5, s.r6};
+    const float *m32[3] = {s.d1, s.mu, s.lam};
+    void *qq[2] = {s.qp, s.qs};
+    const float iqs[2] = {s.ms.iqp, s.ms.iqs};
+    if (which < 9) {
+        hipLaunchKernelGGL(k_pack_field<float>, grd, blk, 0, st, g, f32[which], 1.0f, kext, dst);
+    } else if (which >= 15 && which < 18) {
+        hipLaunchKernelGGL(k_pack_field<float>, grd, blk, 0, st, g, m32[which - 15], 1.0f, kext, dst);
+    } else if (which < 15) {
+        if (mixed)
+            hipLaunchKernelGGL(k_pack_field<Bf16>, grd, blk, 0, st, g, (const Bf16 *)rr[which - 9], 1.0f, kext, dst);
+        else
+            hipLaunchKernelGGL(k_pack_field<float>, grd, blk, 0, st, g, (const float *)rr[which - 9], 1.0f, kext, dst);
+    } else {
+        if (mixed)
+            hipLaunchKernelGGL(k_pack_field<_Float16>, grd, blk, 0, st, g, (const _Float16 *)qq[which - 18],
+                               iqs[which - 18], kext, dst);
+        else
+            hipLaunchKernelGGL(k_pack_field<float>, grd, blk, 0, st, g, (const float *)qq[which - 18], 1.0f, kext,
+                               dst);
+    }
     return hipGetLastError();
 }
 

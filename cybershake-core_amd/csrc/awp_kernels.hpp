@@ -10,6 +10,17 @@
 // reading 64 consecutive x values touches exactly two lines.  Two ghost planes
 // on every side hold the reference's halo values (zero outside the domain,
 // free-surface images at k = nz+1, nz+2).
+//
+// Storage: velocities, stresses, the density d1 and the compliances mu/lam
+// are FP32 always.  The memory variables r1..r6 and the attenuation
+// coefficients qp/qs are FP32 (StoreF32) or, in the mixed mode (StoreMixed),
+// BF16 and FP16.  Each FP16 array holds value * 2^e, a per-array power of two
+// that puts its largest magnitude in [2^13, 2^14); the kernels undo it with
+// an exact multiply, so FP16's only effect is the 11-bit rounding of the
+// stored value.  (Density and the compliances stay FP32: their FP16 rounding
+// is constant across a layer of a layered medium, shifts the wave speeds
+// coherently and took the SGT error past 1e-3.)  Arithmetic is FP32 in both
+// modes.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -49,14 +60,62 @@ __host__ __device__ inline long long gidx(const Geo &g, int i, int j, int k)
     return g.base + (long long)k * g.sz + (long long)j * g.sy + i;
 }
 
-// Pointers of one run's state.  Media and memory variables are FP32 here;
-// the mixed-storage variants reinterpret them (see awp_kernels.hip).
+// ---- storage element types ------------------------------------------------
+struct Bf16 {
+    unsigned short u;
+};
+
+__host__ __device__ inline float ld(float x) { return x; }
+__device__ inline float ld(Bf16 x) { return __uint_as_float(((unsigned)x.u) << 16); }
+__device__ inline float ld(_Float16 x) { return (float)x; }
+
+template <class T> __device__ inline T st(float f);
+template <> __device__ inline float st<float>(float f) { return f; }
+// round to nearest even (finite values)
+template <> __device__ inline Bf16 st<Bf16>(float f)
+{
+    unsigned b = __float_as_uint(f);
+    b += 0x7FFFu + ((b >> 16) & 1u);
+    Bf16 r;
+    r.u = (unsigned short)(b >> 16);
+    return r;
+}
+template <> __device__ inline _Float16 st<_Float16>(float f) { return (_Float16)f; }
+
+struct StoreF32 {
+    using R = float;   // r1..r6
+    using Q = float;   // qp, qs
+    static constexpr bool mixed = false;
+};
+struct StoreMixed {
+    using R = Bf16;
+    using Q = _Float16;
+    static constexpr bool mixed = true;
+};
+
+// Power-of-two storage scales of the FP16 qp/qs arrays (1 in FP32 mode) and
+// their inverses.
+struct MScale {
+    float qp, qs;
+    float iqp, iqs;
+};
+
+// Pointers of one run's state.  r1..r6 and qp/qs point to R / Q arrays of
+// the storage mode.
 struct State {
     float *u1, *v1, *w1;
     float *xx, *yy, *zz, *xy, *xz, *yz;
-    float *r1, *r2, *r3, *r4, *r5, *r6;
-    float *d1, *mu, *lam, *qp, *qs;
+    float *d1, *mu, *lam;
+    void *r1, *r2, *r3, *r4, *r5, *r6;
+    void *qp, *qs;
     float *fx, *fy, *fz;       // sponge profiles, index 1..n (fx[0] unused)
+    MScale ms;
+};
+
+// FP32 media (the storage in FP32 mode; in mixed mode qp/qs point to set-up
+// temporaries)
+struct Media32 {
+    float *d1, *mu, *lam, *qp, *qs;
 };
 
 struct Coef {
@@ -75,22 +134,30 @@ struct Launch {
     int unroll;      // z-loop unroll factor (1 or 2)
 };
 
-// host launchers (awp_kernels.hip)
-hipError_t launch_velocity(const Geo &g, const State &s, const Coef &c, const Launch &l, hipStream_t st);
-hipError_t launch_fs_velocity(const Geo &g, const State &s, hipStream_t st);
-hipError_t launch_stress(const Geo &g, const State &s, const Coef &c, const Launch &l, hipStream_t st);
-hipError_t launch_source(const Geo &g, const State &s, int igreen, int npsrc, const long long *psrc_idx,
-                         const long long *psrc_top, const float *stf, int nst, int istep, float vtst,
-                         float vtst1, hipStream_t st);
-hipError_t launch_sgt_coef(const Geo &g, const State &s, int nkept, const long long *kpos, float *sg, hipStream_t st);
-hipError_t launch_sgt_gather(const Geo &g, const State &s, int nkept, const long long *kpos, const int *kidx,
-                             const float *sg, float *out, int nsamp_max, int sample, hipStream_t st);
 struct MediaConst {
     float pi, tmp1, tmp2, lw;   // 4*atan(1), Q-fit constants, log(w2/w0)
     int socalq, nvar;
 };
-hipError_t launch_media_setup(const Geo &g, const State &s, const float *media, const MediaConst &mc, hipStream_t st);
-hipError_t launch_media_ghosts(const Geo &g, const State &s, hipStream_t st);
-hipError_t launch_pack_field(const Geo &g, const float *src, int kext, float *dst, hipStream_t st);
+
+// host launchers (awp_kernels.hip); `mixed` selects the storage mode
+hipError_t launch_velocity(bool mixed, const Geo &g, const State &s, const Coef &c, const Launch &l, hipStream_t st);
+hipError_t launch_fs_velocity(const Geo &g, const State &s, hipStream_t st);
+hipError_t launch_stress(bool mixed, const Geo &g, const State &s, const Coef &c, const Launch &l, hipStream_t st);
+hipError_t launch_source(const Geo &g, const State &s, int igreen, int npsrc, const long long *psrc_idx,
+                         const long long *psrc_top, const float *stf, int nst, int istep, float vtst,
+                         float vtst1, hipStream_t st);
+hipError_t launch_sgt_coef(const Media32 &m, int nkept, const long long *kpos, float *sg, hipStream_t st);
+hipError_t launch_sgt_gather(const State &s, int nkept, const long long *kpos, const int *kidx,
+                             const float *sg, float *out, int nsamp_max, int sample, hipStream_t st);
+hipError_t launch_media_setup(const Geo &g, const Media32 &m, const float *media, const MediaConst &mc, hipStream_t st);
+hipError_t launch_media_ghosts(const Geo &g, const Media32 &m, hipStream_t st);
+// per-block maxima of |a| over n elements into part[nblk] (nblk <= 4096)
+hipError_t launch_absmax(const float *a, long long n, float *part, int nblk, hipStream_t st);
+hipError_t launch_to_half(const float *src, _Float16 *dst, float scale, long long n, hipStream_t st);
+hipError_t launch_sgt_fp16(const float *sgt, const int *kidx, int nkept, int nsamp_max, int nsamp, float *scales,
+                           _Float16 *out, hipStream_t st);
+// copy field `which` (0..19) to dst as [k][j][i] in physical units
+hipError_t launch_pack_field(bool mixed, const Geo &g, const State &s, int which, int kext, float *dst,
+                             hipStream_t st);
 
 }  // namespace awp

@@ -8,6 +8,7 @@
 #include <errno.h>
 #include <math.h>
 #include <stdarg.h>
+#include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -334,6 +335,70 @@ int awp_sgt_select(const awp_in3d *p, int npts, const int *xyz, int *keep, int *
         kept++;
     }
     return kept;
+}
+
+/* ------------------------------------------------------------------------ */
+/* compact FP16 SGT file (format in awpsgt.h)                                */
+/* ------------------------------------------------------------------------ */
+
+static float half_to_float(uint16_t h)
+{
+    const uint32_t sign = (uint32_t)(h & 0x8000u) << 16;
+    const uint32_t ex = (h >> 10) & 0x1Fu, man = h & 0x3FFu;
+    float v;
+    if (ex == 0)
+        v = ldexpf((float)man, -24);               /* zero and subnormals */
+    else if (ex == 31)
+        v = man ? NAN : INFINITY;
+    else
+        v = ldexpf((float)(man | 0x400u), (int)ex - 25);
+    return sign ? -v : v;
+}
+
+int awp_sgt16_read(const char *path, int *npts, int *nsamp, float *out, size_t count, char *err, size_t errlen)
+{
+    FILE *f = fopen(path, "rb");
+    if (!f) return seterr(err, errlen, AWP_EIO, "cannot open '%s': %s", path, strerror(errno));
+    unsigned char hdr[64];
+    uint32_t h32[4];
+    if (fread(hdr, 1, 64, f) != 64 || memcmp(hdr, "AWPSGT16", 8) != 0) {
+        fclose(f);
+        return seterr(err, errlen, AWP_EINVAL, "'%s' is not an FP16 SGT file", path);
+    }
+    memcpy(h32, hdr + 8, sizeof(h32));
+    if (h32[0] != 1u || h32[2] != 6u) {
+        fclose(f);
+        return seterr(err, errlen, AWP_EINVAL, "'%s': unsupported version or component count", path);
+    }
+    const size_t np = h32[1], ns = h32[3];
+    if (npts) *npts = (int)np;
+    if (nsamp) *nsamp = (int)ns;
+    if (!out) {
+        fclose(f);
+        return AWP_OK;
+    }
+    if (count < np * 6 * ns) {
+        fclose(f);
+        return seterr(err, errlen, AWP_EINVAL, "buffer holds %zu values, file has %zu", count, np * 6 * ns);
+    }
+    float *scale = (float *)malloc(sizeof(float) * 6 * (np ? np : 1));
+    uint16_t *row = (uint16_t *)malloc(sizeof(uint16_t) * (ns ? ns : 1));
+    int rc = AWP_OK;
+    if (!scale || !row || fread(scale, sizeof(float), 6 * np, f) != 6 * np) {
+        rc = seterr(err, errlen, AWP_EIO, "'%s': truncated", path);
+    } else {
+        for (size_t s = 0; s < 6 * np && rc == AWP_OK; ++s) {
+            if (ns && fread(row, sizeof(uint16_t), ns, f) != ns) {
+                rc = seterr(err, errlen, AWP_EIO, "'%s': truncated", path);
+                break;
+            }
+            for (size_t t = 0; t < ns; ++t) out[s * ns + t] = half_to_float(row[t]) * scale[s];
+        }
+    }
+    free(scale);
+    free(row);
+    fclose(f);
+    return rc;
 }
 
 int awp_sgt_file_geometry(const awp_in3d *p, int *nt_out, int *nflush)

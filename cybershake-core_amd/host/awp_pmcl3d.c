@@ -7,7 +7,10 @@
  * the reference writes (SGTGRO), plus the CHKP parameter summary
  * (io.f:481-537).
  *
- * Environment: AWP_DEVICE selects the HIP device (default 0).
+ * Environment: AWP_DEVICE selects the HIP device (default 0);
+ * AWP_STORAGE=mixed selects mixed storage (BF16 memory variables, FP16
+ * attenuation weights); AWP_SGT_FP16=<path> also writes the SGT in the
+ * compact FP16 format to <path>.
  */
 #define _POSIX_C_SOURCE 200809L
 #include <errno.h>
@@ -117,6 +120,13 @@ int main(int argc, char **argv)
 
     const char *dev_env = getenv("AWP_DEVICE");
     const int device = dev_env ? atoi(dev_env) : 0;
+    const char *st_env = getenv("AWP_STORAGE");
+    int storage = AWP_STORAGE_FP32;
+    if (st_env && st_env[0]) {
+        if (!strcmp(st_env, "mixed")) storage = AWP_STORAGE_MIXED;
+        else if (strcmp(st_env, "fp32")) return die("AWP_STORAGE", "expected 'fp32' or 'mixed'");
+    }
+    const char *fp16_path = getenv("AWP_SGT_FP16");
     double t0 = now_s();
 
     int *sxyz = (int *)malloc(sizeof(int) * 3 * (size_t)p.nsrc);
@@ -137,10 +147,10 @@ int main(int argc, char **argv)
     double t1 = now_s();
 
     awp_ctx *ctx = NULL;
-    if (awp_create(&ctx, &p, device, AWP_STORAGE_FP32, err, sizeof(err))) return die("awp_create", err);
+    if (awp_create(&ctx, &p, device, storage, err, sizeof(err))) return die("awp_create", err);
     int nkept = 0;
-    if (awp_set_media(ctx, media) || awp_set_source(ctx, p.nsrc, sxyz, stf) ||
-        awp_set_sgt_points(ctx, npts, pts, &nkept) || awp_init(ctx))
+    if (awp_set_sgt_points(ctx, npts, pts, &nkept) || awp_set_media(ctx, media) ||
+        awp_set_source(ctx, p.nsrc, sxyz, stf) || awp_init(ctx))
         return die("set-up", awp_last_error(ctx));
     write_chkp(&p, media);
     free(media);
@@ -159,6 +169,8 @@ int main(int argc, char **argv)
     }
     double t3 = now_s();
     if (awp_write_sgt_file(ctx, p.sgtgro)) return die("SGT output", awp_last_error(ctx));
+    if (fp16_path && fp16_path[0] && awp_write_sgt_file_fp16(ctx, fp16_path))
+        return die("FP16 SGT output", awp_last_error(ctx));
     double t4 = now_s();
     const double per_step = (t3 - tl) / p.nt;
     printf(" inialization time = %10.3f sec\n", t2 - t0);

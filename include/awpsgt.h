@@ -39,7 +39,7 @@ enum {
 /* Storage modes of the state and media arrays (arithmetic is FP32 in all). */
 enum {
     AWP_STORAGE_FP32 = 0,   /* everything FP32: the reference's storage */
-    AWP_STORAGE_MIXED = 1   /* r1..r6 BF16, d1/lam/mu/qp/qs FP16 */
+    AWP_STORAGE_MIXED = 1   /* r1..r6 BF16, qp/qs FP16 (scaled), d1/lam/mu FP32 */
 };
 
 /* The IN3D parameters the SGT path uses, in the reference's units.  Filled by
@@ -164,6 +164,24 @@ int awp_get_sgt(awp_ctx *ctx, float *host, size_t count);
  * g*24*NT_OUT + c*4*NT_OUT + (f-1)*4*WRITE_STEP; dropped points leave holes.
  * No file is created when no buffer fills.  The file is truncated first. */
 int awp_write_sgt_file(awp_ctx *ctx, const char *path);
+
+/* Write the SGT in the compact FP16 format (no counterpart in the
+ * reference; a storage option for the raw SGT, decoded by awp_sgt16_read):
+ *   bytes 0-7 "AWPSGT16"; u32 version=1, npts, ncomp=6, nsamp, NTISKP_SGT;
+ *   f32 DT; zero padding to byte 64; f32 scale[npts][6]; then
+ *   FP16 data[npts][6][nsamp], value = data * scale.
+ * nsamp is the number of samples the raw file holds (full WRITE_STEP
+ * buffers); scale = 2^(e-15) per series, where the series peak is
+ * f * 2^e' with f in [0.5, 1) and e = max(e', -134) (so scale >= 2^-149,
+ * the smallest FP32), and data = fp16(value * 2^(15-e)), round to nearest
+ * even; points the reference drops and all-zero series have scale 0.  Error
+ * per value <= 2^-11 of the series peak (+ 2^-149 for peaks below 2^-134). */
+int awp_write_sgt_file_fp16(awp_ctx *ctx, const char *path);
+
+/* Dimensions of an FP16 SGT file, and its values decoded to float32 as
+ * [npts][6][nsamp] (out may be NULL to query the dimensions only). */
+int awp_sgt16_read(const char *path, int *npts, int *nsamp, float *out, size_t count,
+                   char *err, size_t errlen);
 
 /* Copy one interior field to the host as [k][j][i], x fastest, k = 1 at the
  * bottom; kext = 2 also copies the two free-surface image planes.  which:

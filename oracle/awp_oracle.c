@@ -42,6 +42,12 @@
 #define LOG log
 #define SQRT sqrt
 #define ATAN atan
+#elif defined(MIXED)
+#define SFX(name) name##_mix
+#define EXP expf
+#define LOG logf
+#define SQRT sqrtf
+#define ATAN atanf
 #else
 #define SFX(name) name##_f32
 #define EXP expf
@@ -51,6 +57,44 @@
 #endif
 
 typedef REAL real;
+
+/*
+ * MIXED: the mixed-storage variant of the device path, restated for
+ * checking it.  The reference has no such mode; this is the FP32
+ * restatement with (a) the memory variables r1..r6 rounded to BF16 (round
+ * to nearest even) whenever they are stored, (b) the attenuation weights
+ * qp, qs stored as FP16 of value * 2^e (d1, mu, lam stay FP32), e
+ * per array putting its largest magnitude in
+ * [2^13, 2^14), and the kernels' exact multiplications by 2^e / 2^-e where
+ * the stored values are used.  Arithmetic stays FP32 in the same order.
+ */
+#if defined(MIXED)
+static inline float as_float(uint32_t u) { float f; memcpy(&f, &u, 4); return f; }
+static inline uint32_t as_bits(float f) { uint32_t u; memcpy(&u, &f, 4); return u; }
+static inline float round_bf16(float f)
+{
+    uint32_t b = as_bits(f);
+    b += 0x7FFFu + ((b >> 16) & 1u);
+    return as_float(b & 0xFFFF0000u);
+}
+/* value of the FP16 nearest (ties to even) to f; |f| < 65504 */
+static inline float round_fp16(float f)
+{
+    uint32_t x = as_bits(f);
+    const uint32_t sign = x & 0x80000000u, ax = x & 0x7FFFFFFFu;
+    if (ax < 0x38800000u) {          /* below 2^-14: FP16 subnormal grid 2^-24 */
+        float q = rintf(fabsf(f) * 16777216.0f) / 16777216.0f;
+        return sign ? -q : q;
+    }
+    uint32_t r = ax + 0xFFFu + ((ax >> 13) & 1u);
+    return as_float(sign | (r & 0xFFFFE000u));
+}
+#define MISCALE(v, a) ((v) = (v) * o->misc[a])
+#define RSTORE(v) round_bf16(v)
+#else
+#define MISCALE(v, a) ((void)0)
+#define RSTORE(v) (v)
+#endif
 
 /* Parameters the time step needs, as parsed from IN3D (io.f:327-418).  Real
  * fields are carried as double and rounded to `real` on entry. */
@@ -84,6 +128,7 @@ typedef struct {
     real *d1, *mu, *lam, *qp, *qs;
     /* separable sponge profiles, index 1..n */
     real *fx, *fy, *fz;
+    real msc[5], misc[5];     /* media storage scales d1 mu lam qp qs (MIXED) */
     real tau[2][2][2];       /* tau(idx,idy,idz), relax.f:15-34 */
     /* sources (global indices) */
     int npsrc;
@@ -279,19 +324,22 @@ static void dvelc(oracle_t *o)
                 const size_t p = IX(o, i, j, k);
                 const real *d1 = o->d1, *xx = o->xx, *yy = o->yy, *zz = o->zz;
                 const real *xy = o->xy, *xz = o->xz, *yz = o->yz;
-                real d;
+                real d, q;
                 d = (real)0.25 * ((d1[p] + d1[p - sy]) + (d1[p - sz] + d1[p - sy - sz]));
-                o->u1[p] = o->u1[p] + (dth / d) * (
+                q = dth / d;
+                o->u1[p] = o->u1[p] + q * (
                     c1 * (xx[p] - xx[p - sx]) + c2 * (xx[p + sx] - xx[p - 2 * sx]) +
                     c1 * (xy[p] - xy[p - sy]) + c2 * (xy[p + sy] - xy[p - 2 * sy]) +
                     c1 * (xz[p] - xz[p - sz]) + c2 * (xz[p + sz] - xz[p - 2 * sz]));
                 d = (real)0.25 * ((d1[p] + d1[p + sx]) + (d1[p - sz] + d1[p + sx - sz]));
-                o->v1[p] = o->v1[p] + (dth / d) * (
+                q = dth / d;
+                o->v1[p] = o->v1[p] + q * (
                     c1 * (xy[p + sx] - xy[p]) + c2 * (xy[p + 2 * sx] - xy[p - sx]) +
                     c1 * (yy[p + sy] - yy[p]) + c2 * (yy[p + 2 * sy] - yy[p - sy]) +
                     c1 * (yz[p] - yz[p - sz]) + c2 * (yz[p + sz] - yz[p - 2 * sz]));
                 d = (real)0.25 * ((d1[p] + d1[p + sx]) + (d1[p - sy] + d1[p + sx - sy]));
-                o->w1[p] = o->w1[p] + (dth / d) * (
+                q = dth / d;
+                o->w1[p] = o->w1[p] + q * (
                     c1 * (xz[p + sx] - xz[p]) + c2 * (xz[p + 2 * sx] - xz[p - sx]) +
                     c1 * (yz[p] - yz[p - sy]) + c2 * (yz[p + sy] - yz[p - 2 * sy]) +
                     c1 * (zz[p + sz] - zz[p]) + c2 * (zz[p + 2 * sz] - zz[p - sz]));
@@ -323,7 +371,8 @@ static void fvel(oracle_t *o)
             size_t p = IX(o, i, j, nz);
             size_t q = p + sz; /* nz+1 */
             real xl = (real)1. / o->lam[p];
-            real xl2m = (real)2. / o->mu[p] + xl;
+            real xm = (real)2. / o->mu[p];
+            real xl2m = xm + xl;
             o->w1[q] = o->w1[p - sz] - (xl / xl2m) * (
                 (o->u1[q + sx] - o->u1[q]) +
                 (o->u1[p + sx] - o->u1[p]) +
@@ -372,6 +421,8 @@ static void dstrqc(oracle_t *o)
                                            qp[p - sz] + qp[p + sx - sz] + qp[p - sy - sz] + qp[p + sx - sy - sz]);
                 real vqsa = (real)0.125 * (qs[p] + qs[p + sx] + qs[p - sy] + qs[p + sx - sy] +
                                            qs[p - sz] + qs[p + sx - sz] + qs[p - sy - sz] + qs[p + sx - sy - sz]);
+                MISCALE(vqpa, 3);
+                MISCALE(vqsa, 4);
                 real vt1 = c1 * (u1[p + sx] - u1[p]) + c2 * (u1[p + 2 * sx] - u1[p - sx]);
                 real vt2 = c1 * (v1[p] - v1[p - sy]) + c2 * (v1[p + sy] - v1[p - 2 * sy]);
                 real vt3 = c1 * (w1[p] - w1[p - sz]) + c2 * (w1[p + sz] - w1[p - 2 * sz]);
@@ -394,30 +445,35 @@ static void dstrqc(oracle_t *o)
 
                 real vxmu = (real)2. / (mu[p] + mu[p - sz]);
                 real vaas = (real)0.5 * (qs[p] + qs[p - sz]);
+                MISCALE(vaas, 4);
                 real vh2_r4 = -vaas * vxmu / (real)2.;
                 xy[p] = xy[p] + vxmu * dth * vt4 + dt * r4[p];
                 vxmu = (real)2. / (mu[p] + mu[p - sy]);
                 vaas = (real)0.5 * (qs[p] + qs[p - sy]);
+                MISCALE(vaas, 4);
                 real vh2_r5 = -vaas * vxmu / (real)2.;
                 xz[p] = xz[p] + vxmu * dth * vt5 + dt * r5[p];
                 vxmu = (real)2. / (mu[p] + mu[p + sx]);
                 vaas = (real)0.5 * (qs[p] + qs[p + sx]);
+                MISCALE(vaas, 4);
                 real vh2_r6 = -vaas * vxmu / (real)2.;
                 yz[p] = yz[p] + vxmu * (dth * vt6 + dth * vt7) + dt * r6[p];
 
-                r1[p] = (vx2 * r1[p] - vh1 * (vt2 + vt3) * dh1 + va1) * vx1;
-                r2[p] = (vx2 * r2[p] - vh1 * (vt1 + vt3) * dh1 + va1) * vx1;
-                r3[p] = (vx2 * r3[p] - vh1 * (vt1 + vt2) * dh1 + va1) * vx1;
-                r4[p] = (vx2 * r4[p] + vh2_r4 * dh1 * vt4) * vx1;
-                r5[p] = (vx2 * r5[p] + vh2_r5 * dh1 * vt5) * vx1;
-                r6[p] = (vx2 * r6[p] + vh2_r6 * dh1 * (vt6 + vt7)) * vx1;
+                const real n1 = (vx2 * r1[p] - vh1 * (vt2 + vt3) * dh1 + va1) * vx1;
+                const real n2 = (vx2 * r2[p] - vh1 * (vt1 + vt3) * dh1 + va1) * vx1;
+                const real n3 = (vx2 * r3[p] - vh1 * (vt1 + vt2) * dh1 + va1) * vx1;
+                const real n4 = (vx2 * r4[p] + vh2_r4 * dh1 * vt4) * vx1;
+                const real n5 = (vx2 * r5[p] + vh2_r5 * dh1 * vt5) * vx1;
+                const real n6 = (vx2 * r6[p] + vh2_r6 * dh1 * (vt6 + vt7)) * vx1;
+                r1[p] = RSTORE(n1); r2[p] = RSTORE(n2); r3[p] = RSTORE(n3);
+                r4[p] = RSTORE(n4); r5[p] = RSTORE(n5); r6[p] = RSTORE(n6);
 
-                xx[p] = xx[p] + dt * r1[p];
-                yy[p] = yy[p] + dt * r2[p];
-                zz[p] = zz[p] + dt * r3[p];
-                xy[p] = xy[p] + dt * r4[p];
-                xz[p] = xz[p] + dt * r5[p];
-                yz[p] = yz[p] + dt * r6[p];
+                xx[p] = xx[p] + dt * n1;
+                yy[p] = yy[p] + dt * n2;
+                zz[p] = zz[p] + dt * n3;
+                xy[p] = xy[p] + dt * n4;
+                xz[p] = xz[p] + dt * n5;
+                yz[p] = yz[p] + dt * n6;
 
                 /* addcrjstr, cerjan.f:249-290 (r1..r6 are not damped) */
                 const real D = damp(o, i, j, k);
@@ -463,9 +519,9 @@ static void addsrc(oracle_t *o, int istep)
         const real axx = a[0], ayy = a[1], azz = a[2];
         const size_t p = IX(o, ps[0], ps[1], ps[2]);
         switch (o->p.igreen) {
-        case 1: o->u1[p] = o->u1[p] + axx * vtst / o->d1[p]; break;
-        case 2: o->v1[p] = o->v1[p] + ayy * vtst / o->d1[p]; break;
-        case 3: o->w1[p] = o->w1[p] + azz * vtst / o->d1[p]; break;
+        case 1: { real a1 = axx * vtst / o->d1[p]; o->u1[p] = o->u1[p] + a1; } break;
+        case 2: { real a1 = ayy * vtst / o->d1[p]; o->v1[p] = o->v1[p] + a1; } break;
+        case 3: { real a1 = azz * vtst / o->d1[p]; o->w1[p] = o->w1[p] + a1; } break;
         case 4:
             o->xz[IX(o, ps[0], ps[1], nz)] = axx * vtst1;
             o->xz[IX(o, ps[0], ps[1], nz + 1)] = (real)2. * axx * vtst1 - o->xz[IX(o, ps[0], ps[1], nz - 1)];
@@ -602,6 +658,23 @@ void *SFX(awp_oracle_create)(const awp_oracle_params *prm, const float *media,
         o->sg[3 * o->nkept + 2] = mu;
         o->nkept++;
     }
+    for (int a = 0; a < 5; ++a) { o->msc[a] = 1; o->misc[a] = 1; }
+#if defined(MIXED)
+    /* media to FP16 storage of value * 2^e (the SGT coefficients above were
+     * taken from the FP32 values, as the device does) */
+    {
+        real *M[5] = {o->d1, o->mu, o->lam, o->qp, o->qs};
+        for (int a = 3; a < 5; ++a) {     /* d1, mu, lam stay FP32 */
+            float mx = 0.0f;
+            for (size_t q = 0; q < o->nvol; ++q) mx = fmaxf(mx, fabsf(M[a][q]));
+            int ex = 0;
+            if (mx > 0.0f) frexpf(mx, &ex);
+            o->msc[a] = ldexpf(1.0f, 14 - ex);
+            o->misc[a] = ldexpf(1.0f, ex - 14);
+            for (size_t q = 0; q < o->nvol; ++q) M[a][q] = round_fp16(M[a][q] * o->msc[a]);
+        }
+    }
+#endif
     o->nsamp_max = prm->nt / prm->ntiskp_sgt;
     o->sgt = (real *)malloc((size_t)(npts > 0 ? npts : 1) * 6 * (o->nsamp_max > 0 ? o->nsamp_max : 1) * sizeof(real));
     for (size_t q = 0; q < (size_t)(npts > 0 ? npts : 1) * 6 * (o->nsamp_max > 0 ? o->nsamp_max : 1); ++q)
@@ -657,10 +730,11 @@ int SFX(awp_oracle_field)(void *h, int which, int kext, real *out)
                    o->d1, o->mu, o->lam, o->qp, o->qs};
     if (which < 0 || which >= (int)(sizeof(src) / sizeof(src[0]))) return -2;
     const real *a = src[which];
+    const real f = which >= 15 ? o->misc[which - 15] : (real)1;
     size_t q = 0;
     for (int k = 1; k <= o->nz + kext; ++k)
         for (int j = 1; j <= o->ny; ++j)
-            for (int i = 1; i <= o->nx; ++i) out[q++] = a[IX(o, i, j, k)];
+            for (int i = 1; i <= o->nx; ++i) out[q++] = a[IX(o, i, j, k)] * f;
     return 0;
 }
 

@@ -135,7 +135,7 @@ def lib():
         if not os.path.exists(LIB_PATH):
             build()
         L = ctypes.CDLL(LIB_PATH)
-        for sfx in ("f32", "f64"):
+        for sfx in ("f32", "f64", "mix"):
             getattr(L, "awp_oracle_create_" + sfx).restype = ctypes.c_void_p
             getattr(L, "awp_oracle_create_" + sfx).argtypes = [
                 ctypes.POINTER(_Params), ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
@@ -156,12 +156,16 @@ FIELDS = ["u1", "v1", "w1", "xx", "yy", "zz", "xy", "xz", "yz",
 
 class Oracle:
     """One SGT run of the CPU restatement.  ``dtype`` float32 (reference
-    operation order) or float64 (FP64 companion)."""
+    operation order) or float64 (FP64 companion); ``mixed=True`` selects the
+    restatement of the device's mixed-storage mode (FP32 arithmetic, BF16
+    memory variables, FP16 media)."""
 
     def __init__(self, in3d: Dict[str, object], media: np.ndarray, src_xyz: np.ndarray,
-                 stf: np.ndarray, pts: np.ndarray, dtype=np.float32):
+                 stf: np.ndarray, pts: np.ndarray, dtype=np.float32, mixed: bool = False):
         self.dtype = np.dtype(dtype)
-        self.sfx = "f32" if self.dtype == np.float32 else "f64"
+        if mixed and self.dtype != np.float32:
+            raise ValueError("the mixed-storage restatement computes in float32")
+        self.sfx = "mix" if mixed else ("f32" if self.dtype == np.float32 else "f64")
         L = lib()
         p = _Params()
         p.nx, p.ny, p.nz = int(in3d["NX"]), int(in3d["NY"]), int(in3d["NZ"])

@@ -48,6 +48,28 @@ C1 = dict(nx=200, ny=200, nz=100, dh=200.0, dt=0.01, nt=1500, igreen=1, src=(100
           npts=2000, pts_seed=1, pts_range=((21, 180), (21, 180), (1, 80)))
 
 
+# Mixed-storage error case: the 200x200x100 recipe (layers, h, dt, steps) on
+# 800x600x200; the FP64 restatement is its reference.
+C4 = dict(nx=800, ny=600, nz=200, dh=200.0, dt=0.01, nt=1500, igreen=1, src=(400, 300, 10),
+          sigma=0.3, t0=1.2, amp=1e9, nd=20, ntiskp_sgt=10, ws=150,
+          npts=2000, pts_seed=4, pts_range=((21, 780), (21, 580), (1, 160)))
+
+
+def c4_spec() -> RunSpec:
+    c = C4
+    return RunSpec(nx=c["nx"], ny=c["ny"], nz=c["nz"], dh=c["dh"], dt=c["dt"], nt=c["nt"],
+                   igreen=c["igreen"], src=c["src"], sigma=c["sigma"], t0=c["t0"],
+                   amp=c["amp"], nd=c["nd"], ntiskp_sgt=c["ntiskp_sgt"], write_step=c["ws"])
+
+
+def c4_inputs():
+    spec = c4_spec()
+    media = layered_media(spec.nx, spec.ny, spec.nz, LAYERS_C1)
+    (xr, yr, zr) = C4["pts_range"]
+    pts = random_cordfile_points(C4["npts"], C4["pts_seed"], xr, yr, zr)
+    return spec, media, pts
+
+
 def tiny_spec(name: str) -> RunSpec:
     c = CASES[name]
     return RunSpec(nx=48, ny=40, nz=32, dh=200.0, dt=0.01, nt=c.get("nt", 120),
