@@ -20,7 +20,7 @@ from __future__ import annotations
 
 import ctypes
 import os
-from typing import Optional, Tuple
+from typing import Optional, Sequence, Tuple
 
 import numpy as np
 
@@ -84,6 +84,9 @@ EXPORTS = [
     "awp_set_source", "awp_set_sgt_points", "awp_init", "awp_step", "awp_synchronize",
     "awp_progress", "awp_get_sgt", "awp_write_sgt_file", "awp_write_sgt_file_fp16",
     "awp_sgt16_read", "awp_get_field", "awp_set_timing", "awp_get_timing", "awp_device_bytes",
+    "awp_estimate_bytes", "awp_batch_create", "awp_batch_tune", "awp_batch_step",
+    "awp_batch_synchronize", "awp_batch_time", "awp_batch_get_launch", "awp_batch_last_error",
+    "awp_batch_destroy",
 ]
 
 
@@ -124,6 +127,16 @@ def lib() -> ctypes.CDLL:
         "awp_set_timing": ([vp, ip], ip),
         "awp_get_timing": ([vp, vp], ip),
         "awp_device_bytes": ([vp, ctypes.POINTER(ctypes.c_size_t)], ip),
+        "awp_estimate_bytes": ([P, ip, ip, ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t)], ip),
+        "awp_batch_create": ([ctypes.POINTER(vp), vp, ip, cp, sz], ip),
+        "awp_batch_tune": ([vp, ip], ip),
+        "awp_batch_step": ([vp, ip], ip),
+        "awp_batch_synchronize": ([vp], ip),
+        "awp_batch_time": ([vp, ctypes.POINTER(ctypes.c_double)], ip),
+        "awp_batch_get_launch": ([vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                                  ctypes.POINTER(ctypes.c_int)], ip),
+        "awp_batch_last_error": ([vp], ctypes.c_char_p),
+        "awp_batch_destroy": ([vp], ip),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -292,6 +305,71 @@ class Run:
     def close(self) -> None:
         if self.h and self.h.value:
             lib().awp_destroy(self.h)
+            self.h = ctypes.c_void_p(0)
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def estimate_bytes(p: In3d, storage: int = AWP_STORAGE_FP32, npts: int = 0) -> Tuple[int, int]:
+    """(steady, peak) device bytes a run of ``p`` with ``npts`` SGT points
+    needs (``awp_estimate_bytes``)."""
+    a, b = ctypes.c_size_t(0), ctypes.c_size_t(0)
+    _check(lib().awp_estimate_bytes(ctypes.byref(p), storage, int(npts), ctypes.byref(a), ctypes.byref(b)),
+           "awp_estimate_bytes: bad arguments")
+    return a.value, b.value
+
+
+class Batch:
+    """Initialised runs advanced together (``awp_batch``): one fused-step
+    launch per time step over every run's tiles, replayed from a HIP graph.
+    The runs stay usable (``sgt()``, ``write_sgt_file``) and must outlive
+    the batch."""
+
+    def __init__(self, runs: Sequence["Run"]):
+        self.runs = list(runs)
+        arr = (ctypes.c_void_p * len(self.runs))(*[r.h.value for r in self.runs])
+        self.h = ctypes.c_void_p(0)
+        err = ctypes.create_string_buffer(512)
+        _check(lib().awp_batch_create(ctypes.byref(self.h), ctypes.cast(arr, ctypes.c_void_p), len(self.runs),
+                                      err, 512), "awp_batch_create: " + err.value.decode())
+
+    def _do(self, rc: int, what: str) -> int:
+        if rc < 0:
+            raise AwpError("%s: %s" % (what, lib().awp_batch_last_error(self.h).decode()))
+        return rc
+
+    def tune(self, reps: int = 2) -> Tuple[int, int, int]:
+        """Pick the fastest kernel variant; re-initialises the runs."""
+        self._do(lib().awp_batch_tune(self.h, int(reps)), "awp_batch_tune")
+        return self.launch()
+
+    def launch(self) -> Tuple[int, int, int]:
+        a, b, c = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_int(0)
+        self._do(lib().awp_batch_get_launch(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)),
+                 "awp_batch_get_launch")
+        return a.value, b.value, c.value
+
+    def step(self, n: int) -> None:
+        self._do(lib().awp_batch_step(self.h, int(n)), "awp_batch_step")
+
+    def time_ms(self) -> float:
+        t = ctypes.c_double(0)
+        self._do(lib().awp_batch_time(self.h, ctypes.byref(t)), "awp_batch_time")
+        return t.value
+
+    def close(self) -> None:
+        if self.h and self.h.value:
+            lib().awp_batch_destroy(self.h)
             self.h = ctypes.c_void_p(0)
 
     def __enter__(self):

@@ -1,0 +1,378 @@
+// awp_batch.hip -- several SGT runs (components, sites) advanced together on
+// one GPU.  Each time step is one launch of the fused kernel over the tiles
+// of every run (a persistent grid walking the concatenated tile list) and
+// one launch for every run's source and SGT sample; two steps (both buffer
+// parities) are captured in a HIP graph once and replayed.  The runs keep
+// their own contexts (buffers, counters, outputs): the batch only drives
+// them, so awp_get_sgt / awp_write_sgt_file work on each run afterwards.
+//
+// The tuner times the fused kernel of the batch for each candidate tile
+// height, prefetch depth and z chunking on the runs' own buffers and keeps
+// the fastest; it needs runs that have not stepped yet and re-initialises
+// them afterwards.
+//
+// The reference runs one SGT component per MPI job (awp_odc_wrapper.sh:14);
+// there is no counterpart of a batch in it.
+
+#include <hip/hip_runtime.h>
+
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "awp_ctx.hpp"
+#include "awp_kernels.hpp"
+#include "awpsgt.h"
+
+using namespace awp;
+
+struct awp_batch {
+    std::vector<awp_ctx *> runs;
+    int device;
+    bool mixed;
+    Launch l;                    // tile_y and prefetch depth (unroll) of the batch
+    std::vector<int> nchunk;     // z chunks per run
+    hipStream_t stream;
+    FJob *d_jobs;
+    AuxJob *d_aux;
+    int *d_counter;
+    int total, grid, max_kept;
+    hipGraphExec_t graph;
+    bool graph_ok;
+    bool use_graph;
+    hipEvent_t e0, e1;
+    double t_last;
+    char err[512];
+};
+
+static int bfail(awp_batch *b, int code, const char *fmt, ...)
+{
+    if (b) {
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(b->err, sizeof(b->err), fmt, ap);
+        va_end(ap);
+    }
+    return code;
+}
+
+#define BTRY(b, call)                                                                             \
+    do {                                                                                          \
+        hipError_t e_ = (call);                                                                   \
+        if (e_ != hipSuccess)                                                                     \
+            return bfail((b), AWP_EHIP, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_),    \
+                         __FILE__, __LINE__);                                                     \
+    } while (0)
+
+extern "C" const char *awp_batch_last_error(const awp_batch *b) { return b ? b->err : "null batch"; }
+
+// z chunks of a run with `nchunk` requested: the top chunk keeps >= 3 levels
+// (the free-surface stress images need the two levels below the surface)
+static Launch run_launch(const Geo &g, const Launch &bl, int nchunk)
+{
+    Launch l = bl;
+    nchunk = std::max(1, std::min(nchunk, std::max(1, g.nz / 16)));
+    for (;;) {
+        const int kc = (g.nz + nchunk - 1) / nchunk;
+        const int last = g.nz - (nchunk - 1) * kc;
+        if (nchunk == 1 || (last >= 3 && (nchunk - 1) * kc < g.nz)) {
+            l.kc = kc;
+            l.nchunk = (g.nz + kc - 1) / kc;
+            return l;
+        }
+        nchunk--;
+    }
+}
+
+// Builds the device job tables from the runs' current buffer parity: run r
+// reads s[0] = its current state and writes s[1].
+static int upload_tables(awp_batch *b)
+{
+    const int n = (int)b->runs.size();
+    std::vector<FJob> fj(n);
+    std::vector<AuxJob> aj(n);
+    int tile0 = 0;
+    b->max_kept = 0;
+    for (int r = 0; r < n; ++r) {
+        awp_ctx *c = b->runs[r];
+        const Launch l = run_launch(c->g, b->l, b->nchunk[r]);
+        FJob &f = fj[r];
+        f.g = c->g;
+        f.s[0] = c->view(c->cur);
+        f.s[1] = c->view(c->cur ^ 1);
+        f.c = c->c;
+        f.kc = l.kc;
+        f.t = fused_tiles(c->g, l);
+        f.tile0 = tile0;
+        tile0 += f.t.nbx * f.t.nby * f.t.nbz;
+        AuxJob &x = aj[r];
+        x.s[0] = f.s[0];
+        x.s[1] = f.s[1];
+        x.g = c->g;
+        x.igreen = c->p.igreen;
+        x.npsrc = c->npsrc;
+        x.nst = c->p.nst;
+        x.ntiskp = c->p.ntiskp_sgt;
+        x.nsamp_max = c->nsamp_max;
+        x.nkept = c->nkept;
+        x.psrc = c->d_psrc;
+        x.ptop = c->d_ptop;
+        x.stf = c->d_stf;
+        x.vtst = c->vtst;
+        x.vtst1 = c->vtst1;
+        x.kpos = c->d_kpos;
+        x.kidx = c->d_kidx;
+        x.sg = c->d_sg;
+        x.sgt = c->d_sgt;
+        x.counter = b->d_counter + r;
+        b->max_kept = std::max(b->max_kept, c->nkept);
+    }
+    b->total = tile0;
+    BTRY(b, hipMemcpyAsync(b->d_jobs, fj.data(), n * sizeof(FJob), hipMemcpyHostToDevice, b->stream));
+    BTRY(b, hipMemcpyAsync(b->d_aux, aj.data(), n * sizeof(AuxJob), hipMemcpyHostToDevice, b->stream));
+    BTRY(b, hipStreamSynchronize(b->stream));   // fj / aj are freed on return
+    return AWP_OK;
+}
+
+static int set_counters(awp_batch *b)
+{
+    const int n = (int)b->runs.size();
+    std::vector<int> cnt(n);
+    for (int r = 0; r < n; ++r) cnt[r] = b->runs[r]->step;
+    BTRY(b, hipMemcpyAsync(b->d_counter, cnt.data(), n * sizeof(int), hipMemcpyHostToDevice, b->stream));
+    // the host vector must outlive the copy
+    BTRY(b, hipStreamSynchronize(b->stream));
+    return AWP_OK;
+}
+
+// one time step of every run at relative parity `par`
+static hipError_t launch_one(awp_batch *b, int par)
+{
+    const int n = (int)b->runs.size();
+    hipError_t e = launch_step_fused_batch(b->mixed, b->l, b->d_jobs, n, b->total, b->grid, par, b->stream);
+    if (e != hipSuccess) return e;
+    return launch_aux_batch(b->d_aux, n, b->max_kept, par, b->stream);
+}
+
+static void drop_graph(awp_batch *b)
+{
+    if (b->graph_ok) hipGraphExecDestroy(b->graph);
+    b->graph_ok = false;
+}
+
+static int build_graph(awp_batch *b)
+{
+    drop_graph(b);
+    hipGraph_t g = nullptr;
+    BTRY(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    hipError_t e = launch_one(b, 0);
+    if (e == hipSuccess) e = launch_one(b, 1);
+    hipError_t e2 = hipStreamEndCapture(b->stream, &g);
+    if (e != hipSuccess || e2 != hipSuccess) {
+        if (g) hipGraphDestroy(g);
+        return bfail(b, AWP_EHIP, "graph capture: %s", hipGetErrorString(e != hipSuccess ? e : e2));
+    }
+    e = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    if (e != hipSuccess) return bfail(b, AWP_EHIP, "graph instantiate: %s", hipGetErrorString(e));
+    b->graph_ok = true;
+    return AWP_OK;
+}
+
+extern "C" int awp_batch_create(awp_batch **out, awp_ctx *const *runs, int nruns, char *err, size_t errlen)
+{
+    if (!out || !runs || nruns < 1) {
+        copy_err(err, errlen, "awp_batch_create: no runs");
+        return AWP_EINVAL;
+    }
+    *out = nullptr;
+    for (int r = 0; r < nruns; ++r) {
+        const awp_ctx *c = runs[r];
+        char msg[160];
+        if (!c) {
+            snprintf(msg, sizeof msg, "run %d is null", r);
+        } else if (!c->initialised) {
+            snprintf(msg, sizeof msg, "run %d is not initialised (awp_init)", r);
+        } else if (!c->fused) {
+            snprintf(msg, sizeof msg, "run %d uses the two-stage step; batches need the fused step", r);
+        } else if (c->device != runs[0]->device || c->storage != runs[0]->storage) {
+            snprintf(msg, sizeof msg, "run %d differs from run 0 in device or storage mode", r);
+        } else {
+            bool dup = false;
+            for (int q = 0; q < r; ++q) dup = dup || runs[q] == c;
+            if (!dup) continue;
+            snprintf(msg, sizeof msg, "run %d appears twice", r);
+        }
+        copy_err(err, errlen, msg);
+        return AWP_EINVAL;
+    }
+    awp_batch *b = new awp_batch();
+    b->runs.assign(runs, runs + nruns);
+    b->device = runs[0]->device;
+    b->mixed = runs[0]->mixed();
+    b->l = runs[0]->l;
+    b->l.tile_y = (b->l.tile_y == 4 || b->l.tile_y == 7 || b->l.tile_y == 8) ? b->l.tile_y : 7;
+    b->nchunk.assign(nruns, 1);
+    b->d_jobs = nullptr;
+    b->d_aux = nullptr;
+    b->d_counter = nullptr;
+    b->graph_ok = false;
+    const char *g_env = getenv("AWP_GRAPH");
+    b->use_graph = !(g_env && !strcmp(g_env, "0"));
+    b->t_last = 0;
+    b->err[0] = 0;
+    hipError_t e = hipSetDevice(b->device);
+    if (e == hipSuccess) e = hipStreamCreate(&b->stream);
+    if (e == hipSuccess) e = hipMalloc((void **)&b->d_jobs, nruns * sizeof(FJob));
+    if (e == hipSuccess) e = hipMalloc((void **)&b->d_aux, nruns * sizeof(AuxJob));
+    if (e == hipSuccess) e = hipMalloc((void **)&b->d_counter, nruns * sizeof(int));
+    if (e == hipSuccess) e = hipEventCreate(&b->e0);
+    if (e == hipSuccess) e = hipEventCreate(&b->e1);
+    if (e != hipSuccess) {
+        snprintf(b->err, sizeof b->err, "batch set-up: %s", hipGetErrorString(e));
+        copy_err(err, errlen, b->err);
+        awp_batch_destroy(b);
+        return AWP_EHIP;
+    }
+    // the runs' own streams may still hold work (awp_init); the batch
+    // stream starts after it
+    for (awp_ctx *c : b->runs) hipStreamSynchronize(c->stream);
+    b->grid = fused_batch_grid(b->mixed, b->l, b->device);
+    *out = b;
+    return AWP_OK;
+}
+
+extern "C" int awp_batch_destroy(awp_batch *b)
+{
+    if (!b) return AWP_EINVAL;
+    hipSetDevice(b->device);
+    drop_graph(b);
+    if (b->stream) hipStreamSynchronize(b->stream);
+    hipFree(b->d_jobs);
+    hipFree(b->d_aux);
+    hipFree(b->d_counter);
+    hipEventDestroy(b->e0);
+    hipEventDestroy(b->e1);
+    if (b->stream) hipStreamDestroy(b->stream);
+    delete b;
+    return AWP_OK;
+}
+
+extern "C" int awp_batch_step(awp_batch *b, int nsteps)
+{
+    if (!b) return AWP_EINVAL;
+    if (nsteps < 0) return bfail(b, AWP_EINVAL, "nsteps < 0");
+    BTRY(b, hipSetDevice(b->device));
+    int todo = nsteps;
+    for (awp_ctx *c : b->runs) {
+        if (!c->initialised) return bfail(b, AWP_ESTATE, "a run was re-created or not initialised");
+        todo = std::min(todo, c->p.nt - c->step);
+    }
+    b->t_last = 0;
+    if (todo <= 0) return AWP_OK;
+    for (awp_ctx *c : b->runs) BTRY(b, hipStreamSynchronize(c->stream));
+    int rc = upload_tables(b);
+    if (rc) return rc;
+    rc = set_counters(b);
+    if (rc) return rc;
+    if (b->use_graph && todo >= 2 && !b->graph_ok) {
+        rc = build_graph(b);
+        if (rc) return rc;
+    }
+    BTRY(b, hipEventRecord(b->e0, b->stream));
+    int done = 0;
+    if (b->use_graph) {
+        for (; done + 2 <= todo; done += 2) BTRY(b, hipGraphLaunch(b->graph, b->stream));
+    }
+    for (int par = 0; done < todo; ++done, par ^= 1) BTRY(b, launch_one(b, par));
+    BTRY(b, hipEventRecord(b->e1, b->stream));
+    BTRY(b, hipEventSynchronize(b->e1));
+    float ms = 0;
+    BTRY(b, hipEventElapsedTime(&ms, b->e0, b->e1));
+    b->t_last = ms;
+    for (awp_ctx *c : b->runs) {
+        c->step += todo;
+        c->nsamp = std::min(c->nsamp_max, c->step / c->p.ntiskp_sgt);
+        c->cur ^= (todo & 1);
+    }
+    return AWP_OK;
+}
+
+extern "C" int awp_batch_synchronize(awp_batch *b)
+{
+    if (!b) return AWP_EINVAL;
+    BTRY(b, hipSetDevice(b->device));
+    BTRY(b, hipStreamSynchronize(b->stream));
+    return AWP_OK;
+}
+
+extern "C" int awp_batch_time(const awp_batch *b, double *ms)
+{
+    if (!b || !ms) return AWP_EINVAL;
+    *ms = b->t_last;
+    return AWP_OK;
+}
+
+extern "C" int awp_batch_get_launch(const awp_batch *b, int *tile_y, int *prefetch, int *nchunk_run0)
+{
+    if (!b) return AWP_EINVAL;
+    if (tile_y) *tile_y = b->l.tile_y;
+    if (prefetch) *prefetch = b->l.unroll;
+    if (nchunk_run0) *nchunk_run0 = b->nchunk.empty() ? 0 : b->nchunk[0];
+    return AWP_OK;
+}
+
+extern "C" int awp_batch_tune(awp_batch *b, int reps)
+{
+    if (!b) return AWP_EINVAL;
+    for (awp_ctx *c : b->runs)
+        if (c->step != 0) return bfail(b, AWP_ESTATE, "tuning needs runs that have not stepped");
+    if (reps < 1) reps = 2;
+    BTRY(b, hipSetDevice(b->device));
+    struct Cand {
+        int ty, pf, nchunk;
+    };
+    std::vector<Cand> cands;
+    for (int ty : {7, 4})
+        for (int pf : {1, 2})
+            for (int nc : {1, 2}) cands.push_back({ty, pf, nc});
+    double best = 1e30;
+    Cand pick = {b->l.tile_y, b->l.unroll, 1};
+    for (const Cand &cd : cands) {
+        b->l.tile_y = cd.ty;
+        b->l.unroll = cd.pf;
+        std::fill(b->nchunk.begin(), b->nchunk.end(), cd.nchunk);
+        int rc = upload_tables(b);
+        if (rc) return rc;
+        b->grid = fused_batch_grid(b->mixed, b->l, b->device);
+        const int n = (int)b->runs.size();
+        // warm-up, then timed launches of the fused kernel alone
+        BTRY(b, launch_step_fused_batch(b->mixed, b->l, b->d_jobs, n, b->total, b->grid, 0, b->stream));
+        BTRY(b, hipEventRecord(b->e0, b->stream));
+        for (int k = 0; k < reps; ++k)
+            BTRY(b, launch_step_fused_batch(b->mixed, b->l, b->d_jobs, n, b->total, b->grid, k & 1, b->stream));
+        BTRY(b, hipEventRecord(b->e1, b->stream));
+        BTRY(b, hipEventSynchronize(b->e1));
+        float ms = 0;
+        BTRY(b, hipEventElapsedTime(&ms, b->e0, b->e1));
+        if (ms < best) {
+            best = ms;
+            pick = cd;
+        }
+    }
+    b->l.tile_y = pick.ty;
+    b->l.unroll = pick.pf;
+    std::fill(b->nchunk.begin(), b->nchunk.end(), pick.nchunk);
+    b->grid = fused_batch_grid(b->mixed, b->l, b->device);
+    drop_graph(b);
+    // the trial steps changed the runs' state: start them again
+    for (awp_ctx *c : b->runs) {
+        int rc = awp_init(c);
+        if (rc) return bfail(b, rc, "re-initialising a run after tuning: %s", awp_last_error(c));
+    }
+    b->t_last = best / reps;
+    return AWP_OK;
+}

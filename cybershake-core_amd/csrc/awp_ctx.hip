@@ -16,102 +16,42 @@
 #include <string>
 #include <vector>
 
+#include "awp_ctx.hpp"
 #include "awp_kernels.hpp"
 #include "awpsgt.h"
 
 using namespace awp;
 
-struct awp_ctx {
-    awp_in3d p;
-    int device;
-    int storage;
-    Geo g;
-    State s;
-    Coef c;
-    Launch l;
-    hipStream_t stream;
-    float *vol;              // one allocation for the 20 volume arrays
-    float *prof;             // fx | fy | fz
-    // sources
-    int npsrc;
-    long long *d_psrc;       // [npsrc] node index
-    long long *d_ptop;       // [npsrc] index of (x, y, nz)
-    float *d_stf;            // [npsrc][nst][6]
-    float vtst, vtst1;
-    // SGT
-    int npts, nkept, nsamp_max;
-    long long *d_kpos;
-    int *d_kidx;
-    float *d_sg;
-    float *d_sgt;            // [npts][6][nsamp_max]
-    std::vector<int> keep;   // per cordfile point
-    // progress
-    bool media_set, source_set, points_set, initialised;
-    int step, nsamp;
-    // timing
-    bool timing;
-    std::vector<hipEvent_t> ev;  // pairs per timed launch: vel, stress, step
-    double t_vel, t_str, t_all;
-    size_t bytes;
-    size_t vol_bytes;        // the volume allocation
-    size_t state_bytes;      // its wavefield + memory-variable part (zeroed by awp_init)
-    bool sg_done;            // SGT coefficients computed from FP32 media
-    char err[512];
-
-    bool mixed() const { return storage == AWP_STORAGE_MIXED; }
-    Media32 media32() const
-    {
-        // qp/qs are FP32 only in FP32 mode; the SGT coefficients use mu, lam
-        Media32 m = {s.d1, s.mu, s.lam, mixed() ? nullptr : (float *)s.qp, mixed() ? nullptr : (float *)s.qs};
-        return m;
-    }
-};
-
-static int fail(awp_ctx *ctx, int code, const char *fmt, ...)
-{
-    if (ctx) {
-        va_list ap;
-        va_start(ap, fmt);
-        vsnprintf(ctx->err, sizeof(ctx->err), fmt, ap);
-        va_end(ap);
-    }
-    return code;
-}
-
-#define HIPTRY(ctx, call)                                                                       \
-    do {                                                                                        \
-        hipError_t e_ = (call);                                                                 \
-        if (e_ != hipSuccess)                                                                   \
-            return fail((ctx), AWP_EHIP, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), \
-                        __FILE__, __LINE__);                                                    \
-    } while (0)
-
-static void copy_err(char *dst, size_t n, const char *src)
-{
-    if (dst && n) {
-        strncpy(dst, src, n - 1);
-        dst[n - 1] = 0;
-    }
-}
-
 // ---------------------------------------------------------------------------
 
 extern "C" const char *awp_last_error(const awp_ctx *ctx) { return ctx ? ctx->err : "null context"; }
 
-static Launch choose_launch(const Geo &g)
+static Launch choose_launch(const Geo &g, bool fused)
 {
     Launch l;
     l.tile_y = 8;
     l.unroll = 1;
     const char *un = getenv("AWP_UNROLL");
     if (un && atoi(un) == 2) l.unroll = 2;
-    // tuning overrides: AWP_TILE_Y (rows per block: 1, 2, 4, 8) and AWP_KC
-    // (z levels per block)
+    // tuning overrides: AWP_TILE_Y (rows per block: 1, 2, 4, 8 for the
+    // two-stage kernels, 4, 7 or 8 for the fused one) and AWP_KC (z levels
+    // per block)
     const char *ty = getenv("AWP_TILE_Y");
-    if (ty && (atoi(ty) == 1 || atoi(ty) == 2 || atoi(ty) == 4 || atoi(ty) == 8)) l.tile_y = atoi(ty);
+    if (fused) l.tile_y = 7;
+    if (ty && (atoi(ty) == 1 || atoi(ty) == 2 || atoi(ty) == 4 || atoi(ty) == 7 || atoi(ty) == 8)) l.tile_y = atoi(ty);
+    if (fused && l.tile_y != 4 && l.tile_y != 7) l.tile_y = 8;
+    if (!fused && l.tile_y == 7) l.tile_y = 8;
+    if (fused) {
+        // the fused step keeps its load prefetch depth (0, 1, 2) in `unroll`
+        const char *pf = getenv("AWP_PF");
+        l.unroll = (pf && atoi(pf) >= 0 && atoi(pf) <= 2) ? atoi(pf) : 1;
+    }
     const long long bxy = (long long)((g.nx + TX - 1) / TX) * ((g.ny + l.tile_y - 1) / l.tile_y);
-    int nchunk = (int)std::max<long long>(1, (4096 + bxy - 1) / bxy);
-    nchunk = std::min(nchunk, std::max(1, g.nz / 8));
+    // enough blocks to fill the 256 CUs several times over; a fused block
+    // recomputes 4 velocity levels per z chunk, so it takes fewer chunks
+    const long long want = fused ? 1024 : 4096;
+    int nchunk = (int)std::max<long long>(1, (want + bxy - 1) / bxy);
+    nchunk = std::min(nchunk, std::max(1, g.nz / (fused ? 16 : 8)));
     const char *kc_env = getenv("AWP_KC");
     if (kc_env && atoi(kc_env) >= 3) nchunk = std::max(1, (g.nz + atoi(kc_env) - 1) / atoi(kc_env));
     for (;;) {
@@ -153,7 +93,13 @@ extern "C" int awp_create(awp_ctx **out, const awp_in3d *p, int device, int stor
     ctx->device = device;
     ctx->storage = storage;
     ctx->g = make_geo(p->nx, p->ny, p->nz);
-    ctx->l = choose_launch(ctx->g);
+    // AWP_SCHEME=split selects the two-stage step (velocity kernel, then
+    // stress kernel); the default is the fused one-sweep step
+    const char *scheme = getenv("AWP_SCHEME");
+    ctx->fused = !(scheme && !strcmp(scheme, "split"));
+    // the fused kernel addresses every array with 32-bit byte offsets
+    if ((unsigned long long)ctx->g.nvol * 4ull >= (1ull << 32)) ctx->fused = false;
+    ctx->l = choose_launch(ctx->g, ctx->fused);
     ctx->vol = nullptr;
     ctx->prof = nullptr;
     ctx->npsrc = 0;
@@ -179,12 +125,17 @@ extern "C" int awp_create(awp_ctx **out, const awp_in3d *p, int device, int stor
         delete ctx;
         return AWP_EHIP;
     }
-    // one allocation: 9 FP32 arrays (velocities, stresses), r1..r6 at 4
-    // (FP32) or 2 (mixed) bytes per element -- the state awp_init zeroes --
-    // then d1 mu lam (FP32) and qp qs (4 or 2 bytes)
+    // one allocation: 9 FP32 arrays (velocities, stresses) per wavefield
+    // buffer (two for the fused step), r1..r6 at 4 (FP32) or 2 (mixed) bytes
+    // per element -- the state awp_init zeroes -- then d1 mu lam (FP32) and
+    // qp qs (4 or 2 bytes)
     const size_t nv = (size_t)ctx->g.nvol;
     const size_t esz = (storage == AWP_STORAGE_MIXED) ? 2 : 4;
-    const size_t nbytes = nv * (12 * sizeof(float) + 8 * esz);
+    const int nbuf = ctx->fused ? 2 : 1;
+    // a 2-byte array gets 128 bytes of slack and starts 64 bytes in, which
+    // puts its interior rows on 128-byte lines (the pitch is a multiple of 64)
+    const size_t eslot = nv * esz + (esz == 2 ? 128 : 0), eoff = (esz == 2 ? 64 : 0);
+    const size_t nbytes = nv * (9 * nbuf + 3) * sizeof(float) + 8 * eslot;
     e = hipMalloc((void **)&ctx->vol, nbytes);
     if (e != hipSuccess) {
         snprintf(err, errlen, "hipMalloc(%zu bytes) for the wavefield and media: %s", nbytes, hipGetErrorString(e));
@@ -194,19 +145,22 @@ extern "C" int awp_create(awp_ctx **out, const awp_in3d *p, int device, int stor
     }
     ctx->bytes += nbytes;
     ctx->vol_bytes = nbytes;
-    ctx->state_bytes = nv * (9 * sizeof(float) + 6 * esz);
+    ctx->state_bytes = nv * 9 * nbuf * sizeof(float) + 6 * eslot;
+    for (int bsel = 0; bsel < 2; ++bsel)
+        for (int a = 0; a < 9; ++a) ctx->wf[bsel][a] = ctx->vol + (size_t)((bsel % nbuf) * 9 + a) * nv;
+    ctx->cur = 0;
     float **f32[9] = {&ctx->s.u1, &ctx->s.v1, &ctx->s.w1, &ctx->s.xx, &ctx->s.yy, &ctx->s.zz,
                       &ctx->s.xy, &ctx->s.xz, &ctx->s.yz};
-    for (int a = 0; a < 9; ++a) *f32[a] = ctx->vol + (size_t)a * nv;
-    char *rest = (char *)(ctx->vol + 9 * nv);
+    for (int a = 0; a < 9; ++a) *f32[a] = ctx->wf[0][a];
+    char *rest = (char *)(ctx->vol + 9 * nbuf * nv);
     void **rr[6] = {&ctx->s.r1, &ctx->s.r2, &ctx->s.r3, &ctx->s.r4, &ctx->s.r5, &ctx->s.r6};
-    for (int a = 0; a < 6; ++a) *rr[a] = rest + (size_t)a * nv * esz;
-    rest += 6 * nv * esz;
+    for (int a = 0; a < 6; ++a) *rr[a] = rest + (size_t)a * eslot + eoff;
+    rest += 6 * eslot;
     float **m32[3] = {&ctx->s.d1, &ctx->s.mu, &ctx->s.lam};
     for (int a = 0; a < 3; ++a) *m32[a] = (float *)(rest + (size_t)a * nv * sizeof(float));
     rest += 3 * nv * sizeof(float);
-    ctx->s.qp = rest;
-    ctx->s.qs = rest + nv * esz;
+    ctx->s.qp = rest + eoff;
+    ctx->s.qs = rest + eslot + eoff;
     MScale one = {1, 1, 1, 1};
     ctx->s.ms = one;
     const size_t np = (size_t)(p->nx + 2) + (p->ny + 2) + (p->nz + 2);
@@ -516,7 +470,8 @@ extern "C" int awp_init(awp_ctx *ctx)
         ctx->sg_done = true;
     }
     // first source sample before the loop (pmcl3d.f:988)
-    HIPTRY(ctx, launch_source(ctx->g, ctx->s, ctx->p.igreen, ctx->npsrc, ctx->d_psrc, ctx->d_ptop,
+    ctx->cur = 0;
+    HIPTRY(ctx, launch_source(ctx->g, ctx->view(0), ctx->p.igreen, ctx->npsrc, ctx->d_psrc, ctx->d_ptop,
                               ctx->d_stf, ctx->p.nst, 1, ctx->vtst, ctx->vtst1, ctx->stream));
     HIPTRY(ctx, hipStreamSynchronize(ctx->stream));
     ctx->step = 0;
@@ -553,17 +508,28 @@ extern "C" int awp_step(awp_ctx *ctx, int nsteps)
         hipEvent_t *e = ctx->timing ? &ctx->ev[(size_t)6 * n] : nullptr;
         if (e) HIPTRY(ctx, hipEventRecord(e[0], st));
         const bool mx = ctx->mixed();
-        HIPTRY(ctx, launch_velocity(mx, ctx->g, ctx->s, ctx->c, ctx->l, st));
-        if (e) HIPTRY(ctx, hipEventRecord(e[1], st));
-        HIPTRY(ctx, launch_fs_velocity(ctx->g, ctx->s, st));
-        if (e) HIPTRY(ctx, hipEventRecord(e[2], st));
-        HIPTRY(ctx, launch_stress(mx, ctx->g, ctx->s, ctx->c, ctx->l, st));
+        State sn;   // the state after this step
+        if (ctx->fused) {
+            const State sa = ctx->view(ctx->cur), sb = ctx->view(ctx->cur ^ 1);
+            if (e) HIPTRY(ctx, hipEventRecord(e[1], st));
+            if (e) HIPTRY(ctx, hipEventRecord(e[2], st));
+            HIPTRY(ctx, launch_step_fused(mx, ctx->g, sa, sb, ctx->c, ctx->l, st));
+            ctx->cur ^= 1;
+            sn = sb;
+        } else {
+            sn = ctx->view(0);
+            HIPTRY(ctx, launch_velocity(mx, ctx->g, sn, ctx->c, ctx->l, st));
+            if (e) HIPTRY(ctx, hipEventRecord(e[1], st));
+            HIPTRY(ctx, launch_fs_velocity(ctx->g, sn, st));
+            if (e) HIPTRY(ctx, hipEventRecord(e[2], st));
+            HIPTRY(ctx, launch_stress(mx, ctx->g, sn, ctx->c, ctx->l, st));
+        }
         if (e) HIPTRY(ctx, hipEventRecord(e[3], st));
         if (i + 1 <= p.nst)
-            HIPTRY(ctx, launch_source(ctx->g, ctx->s, p.igreen, ctx->npsrc, ctx->d_psrc, ctx->d_ptop, ctx->d_stf,
+            HIPTRY(ctx, launch_source(ctx->g, sn, p.igreen, ctx->npsrc, ctx->d_psrc, ctx->d_ptop, ctx->d_stf,
                                       p.nst, i + 1, ctx->vtst, ctx->vtst1, st));
         if (i % p.ntiskp_sgt == 0 && ctx->nsamp < ctx->nsamp_max) {
-            HIPTRY(ctx, launch_sgt_gather(ctx->s, ctx->nkept, ctx->d_kpos, ctx->d_kidx, ctx->d_sg, ctx->d_sgt,
+            HIPTRY(ctx, launch_sgt_gather(sn, ctx->nkept, ctx->d_kpos, ctx->d_kidx, ctx->d_sg, ctx->d_sgt,
                                           ctx->nsamp_max, ctx->nsamp, st));
             ctx->nsamp++;
         }
@@ -619,11 +585,38 @@ extern "C" int awp_get_timing(awp_ctx *ctx, double *out3)
     return AWP_OK;
 }
 
+extern "C" int awp_estimate_bytes(const awp_in3d *p, int storage, int npts, size_t *steady, size_t *peak)
+{
+    if (!p || (storage != AWP_STORAGE_FP32 && storage != AWP_STORAGE_MIXED) || npts < 0) return AWP_EINVAL;
+    const Geo g = make_geo(p->nx, p->ny, p->nz);
+    const char *scheme = getenv("AWP_SCHEME");
+    bool fused = !(scheme && !strcmp(scheme, "split"));
+    if ((unsigned long long)g.nvol * 4ull >= (1ull << 32)) fused = false;
+    const size_t nv = (size_t)g.nvol;
+    const size_t esz = storage == AWP_STORAGE_MIXED ? 2 : 4;
+    const size_t eslot = nv * esz + (esz == 2 ? 128 : 0);
+    const size_t vol = nv * (9 * (fused ? 2 : 1) + 3) * sizeof(float) + 8 * eslot;
+    const size_t prof = ((size_t)p->nx + p->ny + p->nz + 6) * sizeof(float);
+    const size_t nsamp_max = (size_t)std::max(p->nt / std::max(p->ntiskp_sgt, 1), 1);
+    const size_t sgt = (size_t)std::max(npts, 1) * 6 * nsamp_max * sizeof(float);
+    const size_t pts = (size_t)std::max(npts, 1) * (sizeof(long long) + sizeof(int) + 3 * sizeof(float));
+    const size_t src = (size_t)p->nsrc * (2 * sizeof(long long) + (size_t)p->nst * 6 * sizeof(float));
+    const size_t st = vol + prof + sgt + pts + src;
+    // awp_set_media: the media file, plus FP32 qp/qs temporaries in mixed mode
+    size_t tr = (size_t)p->nx * p->ny * p->nz * p->nvar * sizeof(float);
+    if (storage == AWP_STORAGE_MIXED) tr += 2 * nv * sizeof(float) + 2 * 1024 * sizeof(float);
+    if (steady) *steady = st;
+    if (peak) *peak = st + tr;
+    return AWP_OK;
+}
+
 extern "C" int awp_device_bytes(const awp_ctx *ctx, size_t *bytes)
 {
     if (!ctx || !bytes) return AWP_EINVAL;
     size_t b = ctx->bytes;
     b += (size_t)std::max(ctx->npts, 1) * 6 * std::max(ctx->nsamp_max, 1) * sizeof(float);
+    b += (size_t)std::max(ctx->nkept, 1) * (sizeof(long long) + sizeof(int) + 3 * sizeof(float));
+    b += (size_t)ctx->npsrc * (2 * sizeof(long long) + (size_t)ctx->p.nst * 6 * sizeof(float));
     *bytes = b;
     return AWP_OK;
 }
@@ -760,7 +753,7 @@ extern "C" int awp_get_field(awp_ctx *ctx, int which, int kext, float *host)
     float *d = nullptr;
     hipError_t e = hipMalloc((void **)&d, n * sizeof(float));
     if (e != hipSuccess) return fail(ctx, AWP_ENOMEM, "hipMalloc field: %s", hipGetErrorString(e));
-    e = launch_pack_field(ctx->mixed(), ctx->g, ctx->s, which, kext, d, ctx->stream);
+    e = launch_pack_field(ctx->mixed(), ctx->g, ctx->view(ctx->cur), which, kext, d, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e == hipSuccess) e = hipMemcpy(host, d, n * sizeof(float), hipMemcpyDeviceToHost);
     hipFree(d);

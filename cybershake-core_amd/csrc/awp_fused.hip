@@ -1,0 +1,621 @@
+// awp_fused.hip -- the whole time step in one sweep of the volume.
+//
+// The two-stage step (awp_kernels.hip) streams the stresses and velocities
+// through HBM twice per step: once to update the velocities, once more to
+// update the stresses from them.  Here one block marches a column tile of the
+// volume bottom-up and, at every level m, first computes the new velocities
+// of level m, then the new stresses of level m-2 (the highest velocity level
+// a stress at s needs is s+2).  The old state is read from buffer `a` and the
+// new state written to buffer `b` (the caller swaps them every step), so
+// every array is read once and written once per step:
+//   FP32 storage: 140 B per point-update (176 B for the two-stage step)
+//   mixed storage: 112 B (BF16 r1..r6, FP16 qp/qs).
+//
+// Stresses at the tile's edge need new velocities of the neighbouring tiles
+// (two columns / rows on each side).  The block recomputes them: its waves
+// are TY "main" rows (velocity and stress of the tile), 4 halo rows (the two
+// rows below and above the tile) and one wave for the 4 halo columns, all of
+// which compute velocities of level m into a ring of six LDS planes.  The
+// stress of level s reads the new velocities of levels s-2..s+2 from the ring
+// (x, y and z neighbours alike).  One barrier per level separates writing
+// level m from reading levels m-4..m; the sixth plane keeps the next level's
+// writes apart from them without a second barrier.  The halo velocities are
+// the same IEEE operations on the same inputs as the owner tile's, so they
+// are bit-identical to it.
+//
+// The old stresses and density along z at the thread's own column are
+// carried in registers (z queues); x/y neighbours of the old state and the
+// media come from the lines neighbouring lanes just loaded.  Arithmetic statements and their order are those of the
+// two-stage kernels, which follow the reference CPU solver (operator.f:92-306,
+// viscoop.f:49-191, cerjan.f:249-328), so results are bit-identical to them.
+
+#include "awp_kernels.hpp"
+
+#include <algorithm>
+
+namespace awp {
+
+namespace {
+
+template <int TY>
+struct FShape {
+    static constexpr int RW = TX + 4;          // region columns i0-2 .. i0+65
+    static constexpr int RH = TY + 4;          // region rows j0-2 .. j0+TY+1
+    static constexpr int PLANE = RW * RH;
+    static constexpr int SLOT = 3 * PLANE;     // u, v, w of one level
+    static constexpr int NSLOT = 6;            // ring of levels
+    static constexpr int NW = TY + 5;          // waves per block
+};
+
+// Same XCD-contiguous tile order as the two-stage kernels.
+__device__ inline bool ftile_of(const FTiles &t, int &bx, int &by, int &bz)
+{
+    const int n = blockIdx.x;
+    const int tile = (n & 7) * t.per + (n >> 3);
+    if (tile >= t.nbx * t.nby * t.nbz) return false;
+    bx = tile % t.nbx;
+    const int r = tile / t.nbx;
+    by = r % t.nby;
+    bz = r / t.nby;
+    return true;
+}
+
+// Loads and stores at a 32-bit byte offset from an array base.  The bases
+// are kernel arguments (scalar registers), so each access takes the
+// scalar-base + 32-bit vector-offset form and one offset register serves
+// every array; +-1, +-2 x neighbours become immediate offsets.  The caller
+// keeps every array below 4 GiB.
+template <class T>
+__device__ __forceinline__ T gl(const T *base, unsigned off, int imm = 0)
+{
+    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + off + imm);
+}
+template <class T>
+__device__ __forceinline__ void gs(T *base, unsigned off, T v, int imm = 0)
+{
+    *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + off + imm) = v;
+}
+
+}  // namespace
+
+// Old-state values the velocity of one level needs besides the z queues.
+struct VelIn {
+    float xz_p1, yz_p1, zz_p2;                 // queue heads (levels m+1, m+2)
+    float dA, dB, dC, dD;                      // d1 at m: (i,j) (i,j-1) (i+1,j) (i+1,j-1)
+    float xx_0, xx_m1, xx_p1, xx_m2;
+    float xy_0, xy_jm1, xy_jp1, xy_jm2, xy_ip1, xy_ip2, xy_im1;
+    float yy_0, yy_jp1, yy_jp2, yy_jm1;
+    float xz_ip1, xz_ip2, xz_im1;
+    float yz_jm1, yz_jp1, yz_jm2;
+    float u, v, w;                             // old velocities of the point
+    float fz;                                  // sponge profile at m
+};
+
+// Values the stress of one level needs from HBM (new velocities come from LDS).
+struct StrIn {
+    float lA, lB, lC, lD, mA, mB, mC, mD;      // lam, mu on the 2x2 cell
+    float pA, pB, pC, pD, sA, sB, sC, sD;      // qp, qs
+    float r1, r2, r3, r4, r5, r6;
+    float xx, yy, xy;
+    float fz;                                  // sponge profile at s
+};
+
+// PF selects how far loads run ahead of their use: 0 = each stage loads
+// what it needs; 1 = the stress stage's loads are issued before the velocity
+// stage's arithmetic; 2 = as 1, and the next level's velocity loads are
+// issued before the stress arithmetic.
+// One column tile (bx, by, z chunk bz) of one run: the whole block calls it.
+template <class P, int TY, int PF>
+__device__ __forceinline__ void step_tile(const Geo &g, const State &a, const State &b, const Coef &c,
+                                          const int kc, const int bx, const int by, const int bz, float *lds)
+{
+    using Sh = FShape<TY>;
+    using Q = typename P::Q;
+    using R = typename P::R;
+    const int lane = threadIdx.x, wv = threadIdx.y;
+    const int i0 = bx * TX + 1, j0 = by * TY + 1;
+    const int kb = bz * kc + 1;
+    const int ke = min(kb + kc - 1, g.nz);
+
+    // this thread's velocity point, as region column rc / row rr
+    int rc, rr;
+    bool active = true;
+    if (wv < TY) {                 // tile rows: velocity and stress
+        rc = lane + 2;
+        rr = wv + 2;
+    } else if (wv < TY + 4) {      // halo rows j0-2, j0-1, j0+TY, j0+TY+1
+        const int h = wv - TY;
+        rc = lane + 2;
+        rr = h < 2 ? h : TY + h;
+    } else {                       // halo columns i0-2, i0-1, i0+64, i0+65 of the tile rows
+        const int q = lane & 3;
+        rc = q < 2 ? q : TX + q;
+        rr = 2 + (lane >> 2);
+        active = lane < 4 * TY;
+    }
+    const bool is_main = wv < TY;
+    const int i = i0 - 2 + rc, j = j0 - 2 + rr;
+    const bool dom = active && i >= 1 && i <= g.nx && j >= 1 && j <= g.ny;
+    const bool smain = is_main && dom;
+    const long long sy = g.sy, sz = g.sz;
+    // byte offset of (i, j, level 0) in a 4-byte array; level l adds l*sz4
+    // (levels >= -1: the unsigned arithmetic is exact modulo 2^32)
+    const unsigned ob = (unsigned)((g.base + (long long)j * sy + i) * 4);
+    const unsigned sy4 = (unsigned)(sy * 4), sz4 = (unsigned)(sz * 4);
+    constexpr int QSH = sizeof(Q) == 2 ? 1 : 0;   // byte offset shift of qp/qs
+    constexpr int RSH = sizeof(R) == 2 ? 1 : 0;   // and of r1..r6
+    constexpr int QB = (int)sizeof(Q);
+    const float c1 = c.c1, c2 = c.c2, dth = c.dth, dt = c.dt, dh1 = c.dh1;
+    const float fxy = dom ? a.fx[i] * a.fy[j] : 0.0f;
+    // relaxation weights of this column for even / odd depth parity
+    // (selected per level: no per-lane indexing of the argument block)
+    const int itxy = ((i & 1) * 2 + (j & 1)) * 2;
+    const float t1e = c.tau1[itxy], t1o = c.tau1[itxy + 1];
+    const float t2e = c.tau2[itxy], t2o = c.tau2[itxy + 1];
+    const int lpos = rr * Sh::RW + rc;
+    constexpr int PL = Sh::PLANE;
+    const float *__restrict__ lam = a.lam;
+    const float *__restrict__ mu = a.mu;
+    const Q *__restrict__ qp = (const Q *)a.qp;
+    const Q *__restrict__ qs = (const Q *)a.qs;
+    R *__restrict__ R1 = (R *)a.r1;
+    R *__restrict__ R2 = (R *)a.r2;
+    R *__restrict__ R3 = (R *)a.r3;
+    R *__restrict__ R4 = (R *)a.r4;
+    R *__restrict__ R5 = (R *)a.r5;
+    R *__restrict__ R6 = (R *)a.r6;
+
+    // loads of the velocity stage at level m (dom threads, -1 <= m <= nz;
+    // every address lies in the allocation, the ghost levels hold zeros)
+    auto load_vel = [&](int m, VelIn &x) {
+        const unsigned o = ob + (unsigned)m * sz4;
+        const unsigned ojm1 = o - sy4, ojp1 = o + sy4, ojm2 = ojm1 - sy4, ojp2 = ojp1 + sy4;
+        x.xz_p1 = gl(a.xz, o + sz4);
+        x.yz_p1 = gl(a.yz, o + sz4);
+        x.zz_p2 = gl(a.zz, o + 2 * sz4);
+        x.dA = gl(a.d1, o); x.dB = gl(a.d1, ojm1); x.dC = gl(a.d1, o, 4); x.dD = gl(a.d1, ojm1, 4);
+        x.xx_0 = gl(a.xx, o); x.xx_m1 = gl(a.xx, o, -4); x.xx_p1 = gl(a.xx, o, 4); x.xx_m2 = gl(a.xx, o, -8);
+        x.xy_0 = gl(a.xy, o); x.xy_jm1 = gl(a.xy, ojm1); x.xy_jp1 = gl(a.xy, ojp1); x.xy_jm2 = gl(a.xy, ojm2);
+        x.xy_ip1 = gl(a.xy, o, 4); x.xy_ip2 = gl(a.xy, o, 8); x.xy_im1 = gl(a.xy, o, -4);
+        x.yy_0 = gl(a.yy, o); x.yy_jp1 = gl(a.yy, ojp1); x.yy_jp2 = gl(a.yy, ojp2); x.yy_jm1 = gl(a.yy, ojm1);
+        x.xz_ip1 = gl(a.xz, o, 4); x.xz_ip2 = gl(a.xz, o, 8); x.xz_im1 = gl(a.xz, o, -4);
+        x.yz_jm1 = gl(a.yz, ojm1); x.yz_jp1 = gl(a.yz, ojp1); x.yz_jm2 = gl(a.yz, ojm2);
+        x.u = gl(a.u1, o); x.v = gl(a.v1, o); x.w = gl(a.w1, o);
+        x.fz = a.fz[m < 0 ? 0 : m];
+    };
+    // loads of the stress stage at level s (main dom threads)
+    auto load_str = [&](int s, StrIn &x) {
+        const unsigned o = ob + (unsigned)s * sz4, oj = o - sy4;
+        x.lA = gl(lam, o); x.lB = gl(lam, o, 4); x.lC = gl(lam, oj); x.lD = gl(lam, oj, 4);
+        x.mA = gl(mu, o); x.mB = gl(mu, o, 4); x.mC = gl(mu, oj); x.mD = gl(mu, oj, 4);
+        const unsigned qo = o >> QSH, qj = oj >> QSH;
+        x.pA = ld(gl(qp, qo)); x.pB = ld(gl(qp, qo, QB)); x.pC = ld(gl(qp, qj)); x.pD = ld(gl(qp, qj, QB));
+        x.sA = ld(gl(qs, qo)); x.sB = ld(gl(qs, qo, QB)); x.sC = ld(gl(qs, qj)); x.sD = ld(gl(qs, qj, QB));
+        const unsigned orr = o >> RSH;
+        x.r1 = ld(gl(R1, orr)); x.r2 = ld(gl(R2, orr)); x.r3 = ld(gl(R3, orr));
+        x.r4 = ld(gl(R4, orr)); x.r5 = ld(gl(R5, orr)); x.r6 = ld(gl(R6, orr));
+        x.xx = gl(a.xx, o); x.yy = gl(a.yy, o); x.xy = gl(a.xy, o);
+        x.fz = a.fz[s];
+    };
+
+    // --- z queues of the old stresses (velocity stage) at this column -----
+    // levels below -1 lie outside the allocation and are zero
+    const int m0 = kb - 2;
+    float xz_m2 = 0.f, xz_m1 = 0.f, xz_0 = 0.f;     // xz at m-2, m-1, m
+    float yz_m2 = 0.f, yz_m1 = 0.f, yz_0 = 0.f;
+    float zz_m2 = 0.f, zz_m1 = 0.f, zz_0 = 0.f, zz_p1 = 0.f;   // zz at m-2 .. m+1
+    float dA_m = 0.f, dB_m = 0.f, dC_m = 0.f;       // d1 at m-1: (i,j), (i,j-1), (i+1,j)
+    if (dom) {
+        for (int l = m0 - 2; l <= m0; ++l) {
+            if (l < -1) continue;   // leading levels: the queue already holds their zeros
+            const unsigned o = ob + (unsigned)l * sz4;
+            const float x = gl(a.xz, o), y = gl(a.yz, o);
+            xz_m2 = xz_m1; xz_m1 = xz_0; xz_0 = x;
+            yz_m2 = yz_m1; yz_m1 = yz_0; yz_0 = y;
+        }
+        for (int l = m0 - 1; l <= m0 + 1; ++l) {
+            if (l < -1) continue;
+            const float z = gl(a.zz, ob + (unsigned)l * sz4);
+            zz_m1 = zz_0; zz_0 = zz_p1; zz_p1 = z;
+        }
+        if (m0 - 1 >= -1) {
+            const unsigned o = ob + (unsigned)(m0 - 1) * sz4;
+            dA_m = gl(a.d1, o); dB_m = gl(a.d1, o - sy4); dC_m = gl(a.d1, o, 4);
+        }
+    }
+
+    // media of level s-1 on the 2x2 (x,y) cell: A=(i,j) B=(i+1,j) C=(i,j-1) D=(i+1,j-1)
+    float lA_m = 0.f, lB_m = 0.f, lC_m = 0.f, lD_m = 0.f;
+    float mA_m = 0.f, mB_m = 0.f, mC_m = 0.f, mD_m = 0.f;
+    float pA_m = 0.f, pB_m = 0.f, pC_m = 0.f, pD_m = 0.f;
+    float sA_m = 0.f, sB_m = 0.f, sC_m = 0.f, sD_m = 0.f;
+    if (smain) {
+        const unsigned om = ob + (unsigned)(kb - 1) * sz4, omj = om - sy4;
+        lA_m = gl(lam, om); lB_m = gl(lam, om, 4); lC_m = gl(lam, omj); lD_m = gl(lam, omj, 4);
+        mA_m = gl(mu, om); mB_m = gl(mu, om, 4); mC_m = gl(mu, omj); mD_m = gl(mu, omj, 4);
+        const unsigned qo = om >> QSH, qj = omj >> QSH;
+        pA_m = ld(gl(qp, qo)); pB_m = ld(gl(qp, qo, QB)); pC_m = ld(gl(qp, qj)); pD_m = ld(gl(qp, qj, QB));
+        sA_m = ld(gl(qs, qo)); sB_m = ld(gl(qs, qo, QB)); sC_m = ld(gl(qs, qj)); sD_m = ld(gl(qs, qj, QB));
+    }
+    // final xz/yz/zz of the two levels below (free-surface images)
+    float xz_f1 = 0.f, xz_f2 = 0.f, yz_f1 = 0.f, yz_f2 = 0.f, zz_f1 = 0.f;
+
+    VelIn vin = {};
+    if constexpr (PF >= 2) {
+        if (dom && m0 <= g.nz) load_vel(m0, vin);
+    }
+
+    for (int m = m0; m <= ke + 2; ++m) {
+        // ring slot of level l (l >= -1)
+        float *slot = lds + ((m + 6) % Sh::NSLOT) * Sh::SLOT;
+        const int s = m - 2;
+        const bool do_str = smain && s >= kb;
+        StrIn sin_ = {};
+        if constexpr (PF >= 1) {
+            if (do_str) load_str(s, sin_);
+        }
+        if constexpr (PF == 1) {
+            if (dom && m <= g.nz) load_vel(m, vin);
+            // keep this level's loads together, ahead of all arithmetic, so
+            // they are in flight at once
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        // ================= velocity of level m (operator.f:92-244) ==========
+        float un = 0.f, vn = 0.f, wn = 0.f;
+        float xz_p1 = 0.f, yz_p1 = 0.f, zz_p2 = 0.f;
+        if (dom && m <= g.nz) {
+            if constexpr (PF == 0) load_vel(m, vin);
+            const VelIn &x = vin;
+            xz_p1 = x.xz_p1; yz_p1 = x.yz_p1; zz_p2 = x.zz_p2;
+            if (m >= 1) {
+                float d, q, u, v, w;
+                d = 0.25f * ((x.dA + x.dB) + (dA_m + dB_m));
+                q = dth / d;
+                u = x.u + q * (
+                    c1 * (x.xx_0 - x.xx_m1) + c2 * (x.xx_p1 - x.xx_m2) +
+                    c1 * (x.xy_0 - x.xy_jm1) + c2 * (x.xy_jp1 - x.xy_jm2) +
+                    c1 * (xz_0 - xz_m1) + c2 * (x.xz_p1 - xz_m2));
+                d = 0.25f * ((x.dA + x.dC) + (dA_m + dC_m));
+                q = dth / d;
+                v = x.v + q * (
+                    c1 * (x.xy_ip1 - x.xy_0) + c2 * (x.xy_ip2 - x.xy_im1) +
+                    c1 * (x.yy_jp1 - x.yy_0) + c2 * (x.yy_jp2 - x.yy_jm1) +
+                    c1 * (yz_0 - yz_m1) + c2 * (x.yz_p1 - yz_m2));
+                d = 0.25f * ((x.dA + x.dC) + (x.dB + x.dD));
+                q = dth / d;
+                w = x.w + q * (
+                    c1 * (x.xz_ip1 - xz_0) + c2 * (x.xz_ip2 - x.xz_im1) +
+                    c1 * (yz_0 - x.yz_jm1) + c2 * (x.yz_jp1 - x.yz_jm2) +
+                    c1 * (zz_p1 - zz_0) + c2 * (x.zz_p2 - zz_m1));
+                const float D = fxy * x.fz;       // addcrjvel, cerjan.f:292-328
+                un = u * D;
+                vn = v * D;
+                wn = w * D;
+                if (is_main && m >= kb && m <= ke) {
+                    const unsigned o = ob + (unsigned)m * sz4;
+                    gs(b.u1, o, un);
+                    gs(b.v1, o, vn);
+                    gs(b.w1, o, wn);
+                }
+            }
+            dA_m = x.dA; dB_m = x.dB; dC_m = x.dC;
+        }
+        if (m <= g.nz) {
+            if (active) {
+                slot[lpos] = un;
+                slot[PL + lpos] = vn;
+                slot[2 * PL + lpos] = wn;
+            }
+        } else if (is_main) {
+            if (m == g.nz + 1 && smain) {
+                // free-surface velocity images fvelxy / fvelz (operator.f:246-306)
+                // from the level-nz plane (and w at nz-1)
+                const float *pl = lds + ((g.nz + 6) % Sh::NSLOT) * Sh::SLOT;
+                const float *pl1 = lds + ((g.nz - 1 + 6) % Sh::NSLOT) * Sh::SLOT;
+                const float *pu = pl, *pv = pl + PL, *pw = pl + 2 * PL;
+                const float w0 = pw[lpos], u0 = pu[lpos], v0 = pv[lpos];
+                const float ut = u0 - (w0 - pw[lpos - 1]);
+                const float vt = v0 - (pw[lpos + Sh::RW] - w0);
+                const float ut_ip1 = (i < g.nx) ? (pu[lpos + 1] - (pw[lpos + 1] - w0)) : 0.0f;
+                const float vt_jm1 = (j > 1) ? (pv[lpos - Sh::RW] - (w0 - pw[lpos - Sh::RW])) : 0.0f;
+                const unsigned o = ob + (unsigned)g.nz * sz4;
+                const float xl = 1.0f / gl(lam, o);
+                const float xm = 2.0f / gl(mu, o);
+                const float xl2m = xm + xl;
+                const float wt = pl1[2 * PL + lpos] - (xl / xl2m) * (
+                    (ut_ip1 - ut) +
+                    (pu[lpos + 1] - u0) +
+                    (vt - vt_jm1) +
+                    (v0 - pv[lpos - Sh::RW]));
+                un = ut;
+                vn = vt;
+                wn = wt;
+                if (ke == g.nz) {
+                    gs(b.u1, o + sz4, un);
+                    gs(b.v1, o + sz4, vn);
+                    gs(b.w1, o + sz4, wn);
+                }
+            }
+            // level nz+1: the images; level nz+2: zero (the reference never sets it)
+            slot[lpos] = un;
+            slot[PL + lpos] = vn;
+            slot[2 * PL + lpos] = wn;
+        }
+
+        __syncthreads();
+
+        if constexpr (PF >= 2) {
+            if (dom && m + 1 <= g.nz) load_vel(m + 1, vin);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+
+        // ================= stress of level s = m-2 (viscoop.f:49-191) =======
+        if (do_str) {
+            if constexpr (PF < 1) load_str(s, sin_);
+            const StrIn &x = sin_;
+            const unsigned o = ob + (unsigned)s * sz4;
+            const float *ps = lds + ((s + 6) % Sh::NSLOT) * Sh::SLOT;          // level s
+            const float *pm1 = lds + ((s - 1 + 6) % Sh::NSLOT) * Sh::SLOT;     // s-1
+            const float *pm2 = lds + ((s - 2 + 6) % Sh::NSLOT) * Sh::SLOT;     // s-2
+            const float *pp1 = lds + ((s + 1 + 6) % Sh::NSLOT) * Sh::SLOT;     // s+1
+            const float *pp2 = lds + ((s + 2 + 6) % Sh::NSLOT) * Sh::SLOT;     // s+2
+            const float *pu = ps, *pv = ps + PL, *pw = ps + 2 * PL;
+            const float uc = pu[lpos], vc = pv[lpos], wc = pw[lpos];
+            const float u_ip1 = pu[lpos + 1], u_ip2 = pu[lpos + 2], u_im1 = pu[lpos - 1];
+            const float u_jp1 = pu[lpos + Sh::RW], u_jp2 = pu[lpos + 2 * Sh::RW], u_jm1 = pu[lpos - Sh::RW];
+            const float v_jm1 = pv[lpos - Sh::RW], v_jp1 = pv[lpos + Sh::RW], v_jm2 = pv[lpos - 2 * Sh::RW];
+            const float v_im1 = pv[lpos - 1], v_ip1 = pv[lpos + 1], v_im2 = pv[lpos - 2];
+            const float w_im1 = pw[lpos - 1], w_ip1 = pw[lpos + 1], w_im2 = pw[lpos - 2];
+            const float w_jp1 = pw[lpos + Sh::RW], w_jp2 = pw[lpos + 2 * Sh::RW], w_jm1 = pw[lpos - Sh::RW];
+            // own column along z
+            const float uc_p1 = pp1[lpos], uc_p2 = pp2[lpos], uc_m1 = pm1[lpos];
+            const float vc_p1 = pp1[PL + lpos], vc_p2 = pp2[PL + lpos], vc_m1 = pm1[PL + lpos];
+            const float wc_p1 = pp1[2 * PL + lpos], wc_m1 = pm1[2 * PL + lpos], wc_m2 = pm2[2 * PL + lpos];
+
+            const bool odd = (((g.nz + 1) - s) & 1) != 0;
+            const float vx1 = odd ? t1o : t1e;
+            const float vx2 = odd ? t2o : t2e;
+
+            float vxl = 8.0f / (x.lA + x.lB + x.lC + x.lD + lA_m + lB_m + lC_m + lD_m);
+            const float vxm = 8.0f / (x.mA + x.mB + x.mC + x.mD + mA_m + mB_m + mC_m + mD_m);
+            vxl = vxl + 2.0f * vxm;
+            float vqpa = 0.125f * (x.pA + x.pB + x.pC + x.pD + pA_m + pB_m + pC_m + pD_m);
+            float vqsa = 0.125f * (x.sA + x.sB + x.sC + x.sD + sA_m + sB_m + sC_m + sD_m);
+            if constexpr (P::mixed) {
+                vqpa = vqpa * a.ms.iqp;
+                vqsa = vqsa * a.ms.iqs;
+            }
+
+            const float vt1 = c1 * (u_ip1 - uc) + c2 * (u_ip2 - u_im1);
+            const float vt2 = c1 * (vc - v_jm1) + c2 * (v_jp1 - v_jm2);
+            const float vt3 = c1 * (wc - wc_m1) + c2 * (wc_p1 - wc_m2);
+            const float vt4 = c1 * (u_jp1 - uc) + c2 * (u_jp2 - u_jm1) +
+                              c1 * (vc - v_im1) + c2 * (v_ip1 - v_im2);
+            const float vt5 = c1 * (uc_p1 - uc) + c2 * (uc_p2 - uc_m1) +
+                              c1 * (wc - w_im1) + c2 * (w_ip1 - w_im2);
+            const float vt6 = c1 * (vc_p1 - vc) + c2 * (vc_p2 - vc_m1);
+            const float vt7 = c1 * (w_jp1 - wc) + c2 * (w_jp2 - w_jm1);
+
+            const float vh1 = -vxm * vqsa;
+            const float vtmp = vxl * (vt1 + vt2 + vt3);
+            const float va1 = -vqpa * vxl * dh1 * (vt1 + vt2 + vt3) / 2.0f;
+
+            float r1 = x.r1, r2 = x.r2, r3 = x.r3, r4 = x.r4, r5 = x.r5, r6 = x.r6;
+            float xx = x.xx, yy = x.yy, xy = x.xy;
+            float zz = zz_m2, xz = xz_m2, yz = yz_m2;   // old values of level s
+
+            xx = xx + dth * (vtmp - 2.0f * vxm * (vt2 + vt3)) + dt * r1;
+            yy = yy + dth * (vtmp - 2.0f * vxm * (vt1 + vt3)) + dt * r2;
+            zz = zz + dth * (vtmp - 2.0f * vxm * (vt1 + vt2)) + dt * r3;
+
+            float vxmu = 2.0f / (x.mA + mA_m);
+            float vaas = 0.5f * (x.sA + sA_m);
+            if constexpr (P::mixed) vaas = vaas * a.ms.iqs;
+            const float vh2_r4 = -vaas * vxmu / 2.0f;
+            xy = xy + vxmu * dth * vt4 + dt * r4;
+            vxmu = 2.0f / (x.mA + x.mC);
+            vaas = 0.5f * (x.sA + x.sC);
+            if constexpr (P::mixed) vaas = vaas * a.ms.iqs;
+            const float vh2_r5 = -vaas * vxmu / 2.0f;
+            xz = xz + vxmu * dth * vt5 + dt * r5;
+            vxmu = 2.0f / (x.mA + x.mB);
+            vaas = 0.5f * (x.sA + x.sB);
+            if constexpr (P::mixed) vaas = vaas * a.ms.iqs;
+            const float vh2_r6 = -vaas * vxmu / 2.0f;
+            yz = yz + vxmu * (dth * vt6 + dth * vt7) + dt * r6;
+
+            r1 = (vx2 * r1 - vh1 * (vt2 + vt3) * dh1 + va1) * vx1;
+            r2 = (vx2 * r2 - vh1 * (vt1 + vt3) * dh1 + va1) * vx1;
+            r3 = (vx2 * r3 - vh1 * (vt1 + vt2) * dh1 + va1) * vx1;
+            r4 = (vx2 * r4 + vh2_r4 * dh1 * vt4) * vx1;
+            r5 = (vx2 * r5 + vh2_r5 * dh1 * vt5) * vx1;
+            r6 = (vx2 * r6 + vh2_r6 * dh1 * (vt6 + vt7)) * vx1;
+
+            xx = xx + dt * r1;
+            yy = yy + dt * r2;
+            zz = zz + dt * r3;
+            xy = xy + dt * r4;
+            xz = xz + dt * r5;
+            yz = yz + dt * r6;
+
+            const float D = fxy * x.fz;       // addcrjstr, cerjan.f:249-290
+            xx = xx * D; yy = yy * D; zz = zz * D;
+            xz = xz * D; yz = yz * D; xy = xy * D;
+
+            // memory variables in place (own point only), rounded once on the store
+            const unsigned orr = o >> RSH;
+            gs(R1, orr, st<R>(r1)); gs(R2, orr, st<R>(r2)); gs(R3, orr, st<R>(r3));
+            gs(R4, orr, st<R>(r4)); gs(R5, orr, st<R>(r5)); gs(R6, orr, st<R>(r6));
+            gs(b.xx, o, xx); gs(b.yy, o, yy); gs(b.xy, o, xy);
+            if (s == g.nz) {
+                // fstr (operator.f:575-612): images above the surface, xz = yz = 0 on it
+                const unsigned o1 = o + sz4, o2 = o1 + sz4;
+                gs(b.zz, o, zz);
+                gs(b.zz, o1, -zz);
+                gs(b.zz, o2, -zz_f1);
+                gs(b.xz, o, 0.0f);
+                gs(b.yz, o, 0.0f);
+                gs(b.xz, o1, -xz_f1);
+                gs(b.xz, o2, -xz_f2);
+                gs(b.yz, o1, -yz_f1);
+                gs(b.yz, o2, -yz_f2);
+            } else {
+                gs(b.zz, o, zz); gs(b.xz, o, xz); gs(b.yz, o, yz);
+            }
+            xz_f2 = xz_f1; xz_f1 = xz;
+            yz_f2 = yz_f1; yz_f1 = yz;
+            zz_f1 = zz;
+            lA_m = x.lA; lB_m = x.lB; lC_m = x.lC; lD_m = x.lD;
+            mA_m = x.mA; mB_m = x.mB; mC_m = x.mC; mD_m = x.mD;
+            pA_m = x.pA; pB_m = x.pB; pC_m = x.pC; pD_m = x.pD;
+            sA_m = x.sA; sB_m = x.sB; sC_m = x.sC; sD_m = x.sD;
+        }
+
+        // advance the old-stress queues to level m+1
+        xz_m2 = xz_m1; xz_m1 = xz_0; xz_0 = xz_p1;
+        yz_m2 = yz_m1; yz_m1 = yz_0; yz_0 = yz_p1;
+        zz_m2 = zz_m1; zz_m1 = zz_0; zz_0 = zz_p1; zz_p1 = zz_p2;
+    }
+}
+
+template <class P, int TY, int PF>
+__global__ void __launch_bounds__(64 * (TY + 5))
+k_step(const Geo g, const State a, const State b, const Coef c, const int kc, const FTiles tl)
+{
+    using Sh = FShape<TY>;
+    __shared__ float lds[Sh::NSLOT * Sh::SLOT];
+    int bx, by, bz;
+    if (!ftile_of(tl, bx, by, bz)) return;   // uniform over the block
+    step_tile<P, TY, PF>(g, a, b, c, kc, bx, by, bz, lds);
+}
+
+// Several runs in one launch.  Runs' tiles are numbered consecutively (run r
+// owns [tile0, tile0 + ntiles)); the grid is persistent: block n works on
+// XCD n % 8 and walks that XCD's contiguous range of the global tile list
+// with stride gridDim.x / 8, so each XCD sweeps neighbouring tiles of one run
+// at a time.  `par` selects each run's (read, write) buffer pair.
+template <class P, int TY, int PF>
+__global__ void __launch_bounds__(64 * (TY + 5))
+k_step_batch(const FJob *__restrict__ jobs, const int njobs, const int total, const int per, const int par)
+{
+    using Sh = FShape<TY>;
+    __shared__ float lds[Sh::NSLOT * Sh::SLOT];
+    const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3, nslot = gridDim.x >> 3;
+    for (int k = slot; k < per; k += nslot) {
+        const int tile = xcd * per + k;
+        if (tile >= total) break;
+        int jb = 0;
+        while (jb + 1 < njobs && jobs[jb + 1].tile0 <= tile) ++jb;
+        const FJob &J = jobs[jb];
+        const int lt = tile - J.tile0;
+        const int bx = lt % J.t.nbx;
+        const int r = lt / J.t.nbx;
+        const int by = r % J.t.nby;
+        const int bz = r / J.t.nby;
+        step_tile<P, TY, PF>(J.g, J.s[par], J.s[par ^ 1], J.c, J.kc, bx, by, bz, lds);
+        __syncthreads();   // the next tile reuses the LDS ring
+    }
+}
+
+FTiles fused_tiles(const Geo &g, const Launch &l)
+{
+    FTiles t;
+    t.nbx = (g.nx + TX - 1) / TX;
+    t.nby = (g.ny + l.tile_y - 1) / l.tile_y;
+    t.nbz = l.nchunk;
+    const long long n = (long long)t.nbx * t.nby * t.nbz;
+    t.per = (int)((n + 7) / 8);
+    return t;
+}
+
+template <class P, int TY, int PF>
+static hipError_t step_t(const Geo &g, const State &a, const State &b, const Coef &c, const Launch &l,
+                         hipStream_t st)
+{
+    const FTiles t = fused_tiles(g, l);
+    hipLaunchKernelGGL((k_step<P, TY, PF>), dim3(8 * t.per), dim3(TX, TY + 5), 0, st, g, a, b, c, l.kc, t);
+    return hipGetLastError();
+}
+
+template <class P, int TY, int PF>
+static hipError_t batch_t(const FJob *d_jobs, int njobs, int total, int grid, int par, hipStream_t st)
+{
+    const int per = (total + 7) / 8;
+    grid = std::max(8, std::min(grid, 8 * per)) / 8 * 8;   // whole XCD rounds
+    hipLaunchKernelGGL((k_step_batch<P, TY, PF>), dim3(grid), dim3(TX, TY + 5), 0, st, d_jobs, njobs, total, per,
+                       par);
+    return hipGetLastError();
+}
+
+template <class P, int TY, int PF>
+static int grid_t(int device)
+{
+    int nb = 0, ncu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_step_batch<P, TY, PF>, TX * (TY + 5), 0) != hipSuccess)
+        nb = 1;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) ncu = 256;
+    return std::max(1, nb) * std::max(8, ncu);
+}
+
+// kernel variant of (storage, tile rows, prefetch depth): F is called with
+// the matching function template instance
+#define AWP_FUSED_DISPATCH(mixed, l, CALL)                                           \
+    do {                                                                             \
+        const int pf_ = (l).unroll;                                                  \
+        if (mixed) {                                                                 \
+            using P = StoreMixed;                                                    \
+            if ((l).tile_y == 7) {                                                   \
+                if (pf_ == 0) { constexpr int TY = 7, PF = 0; CALL; }                \
+                else if (pf_ == 2) { constexpr int TY = 7, PF = 2; CALL; }           \
+                else { constexpr int TY = 7, PF = 1; CALL; }                         \
+            } else if ((l).tile_y == 4) {                                            \
+                if (pf_ == 0) { constexpr int TY = 4, PF = 0; CALL; }                \
+                else if (pf_ == 2) { constexpr int TY = 4, PF = 2; CALL; }           \
+                else { constexpr int TY = 4, PF = 1; CALL; }                         \
+            } else {                                                                 \
+                if (pf_ == 0) { constexpr int TY = 8, PF = 0; CALL; }                \
+                else if (pf_ == 2) { constexpr int TY = 8, PF = 2; CALL; }           \
+                else { constexpr int TY = 8, PF = 1; CALL; }                         \
+            }                                                                        \
+        } else {                                                                     \
+            using P = StoreF32;                                                      \
+            if ((l).tile_y == 7) {                                                   \
+                if (pf_ == 0) { constexpr int TY = 7, PF = 0; CALL; }                \
+                else if (pf_ == 2) { constexpr int TY = 7, PF = 2; CALL; }           \
+                else { constexpr int TY = 7, PF = 1; CALL; }                         \
+            } else if ((l).tile_y == 4) {                                            \
+                if (pf_ == 0) { constexpr int TY = 4, PF = 0; CALL; }                \
+                else if (pf_ == 2) { constexpr int TY = 4, PF = 2; CALL; }           \
+                else { constexpr int TY = 4, PF = 1; CALL; }                         \
+            } else {                                                                 \
+                if (pf_ == 0) { constexpr int TY = 8, PF = 0; CALL; }                \
+                else if (pf_ == 2) { constexpr int TY = 8, PF = 2; CALL; }           \
+                else { constexpr int TY = 8, PF = 1; CALL; }                         \
+            }                                                                        \
+        }                                                                            \
+    } while (0)
+
+hipError_t launch_step_fused(bool mixed, const Geo &g, const State &a, const State &b, const Coef &c,
+                             const Launch &l, hipStream_t st)
+{
+    AWP_FUSED_DISPATCH(mixed, l, return (step_t<P, TY, PF>(g, a, b, c, l, st)));
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_step_fused_batch(bool mixed, const Launch &l, const FJob *d_jobs, int njobs, int total,
+                                   int grid, int par, hipStream_t st)
+{
+    if (njobs <= 0 || total <= 0) return hipSuccess;
+    AWP_FUSED_DISPATCH(mixed, l, return (batch_t<P, TY, PF>(d_jobs, njobs, total, grid, par, st)));
+    return hipErrorInvalidValue;
+}
+
+int fused_batch_grid(bool mixed, const Launch &l, int device)
+{
+    AWP_FUSED_DISPATCH(mixed, l, return (grid_t<P, TY, PF>(device)));
+    return 256;
+}
+
+#undef AWP_FUSED_DISPATCH
+
+}  // namespace awp

@@ -18,6 +18,8 @@
 
 #include "awp_kernels.hpp"
 
+#include <algorithm>
+
 namespace awp {
 
 // Block -> tile map.  The hardware hands consecutive workgroup ids to the 8
@@ -330,11 +332,10 @@ k_stress(const Geo g, const State s, const Coef c, const int kc, const Tiles tl)
 // point source, addsrc (source.f:812-913).  One thread walks the sources in
 // file order so coincident sources accumulate in the reference's order.
 // ---------------------------------------------------------------------------
-__global__ void k_source(const State s, const int igreen, const int npsrc, const long long *psrc_idx,
-                         const long long *psrc_top, const long long sz, const float *stf, const int nst,
-                         const int istep, const float vtst, const float vtst1)
+__device__ void apply_sources(const State &s, const int igreen, const int npsrc, const long long *psrc_idx,
+                              const long long *psrc_top, const long long sz, const float *stf, const int nst,
+                              const int istep, const float vtst, const float vtst1)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
     const float *d1 = s.d1;
     for (int n = 0; n < npsrc; ++n) {
         const float *a = stf + ((size_t)n * nst + (istep - 1)) * 6;
@@ -369,6 +370,14 @@ __global__ void k_source(const State s, const int igreen, const int npsrc, const
     }
 }
 
+__global__ void k_source(const State s, const int igreen, const int npsrc, const long long *psrc_idx,
+                         const long long *psrc_top, const long long sz, const float *stf, const int nst,
+                         const int istep, const float vtst, const float vtst1)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    apply_sources(s, igreen, npsrc, psrc_idx, psrc_top, sz, stf, nst, istep, vtst, vtst1);
+}
+
 // ---------------------------------------------------------------------------
 // SGT: moduli coefficients at the output nodes (pmcl3d.f:1158-1170), from the
 // FP32 media, and the strain gather (pmcl3d.f:1546-1557).  out is
@@ -385,11 +394,9 @@ __global__ void k_sgt_coef(const Media32 m, const int nkept, const long long *kp
     sg[3 * n + 2] = mu;
 }
 
-__global__ void k_sgt_gather(const State s, const int nkept, const long long *kpos, const int *kidx,
-                             const float *sg, float *out, const int nsamp_max, const int sample)
+__device__ inline void gather_point(const State &s, const int n, const long long *kpos, const int *kidx,
+                                    const float *sg, float *out, const int nsamp_max, const int sample)
 {
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= nkept) return;
     const long long p = kpos[n];
     const float sg1 = sg[3 * n], sg2 = sg[3 * n + 1], sg3 = sg[3 * n + 2];
     const float xx = s.xx[p], yy = s.yy[p], zz = s.zz[p];
@@ -400,6 +407,39 @@ __global__ void k_sgt_gather(const State s, const int nkept, const long long *kp
     dst[3 * (size_t)nsamp_max] = sg3 * s.xy[p];
     dst[4 * (size_t)nsamp_max] = sg3 * s.xz[p];
     dst[5 * (size_t)nsamp_max] = sg3 * s.yz[p];
+}
+
+__global__ void k_sgt_gather(const State s, const int nkept, const long long *kpos, const int *kidx,
+                             const float *sg, float *out, const int nsamp_max, const int sample)
+{
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= nkept) return;
+    gather_point(s, n, kpos, kidx, sg, out, nsamp_max, sample);
+}
+
+// The source and SGT work of one step for every run of a batch (the step's
+// number comes from the run's device counter, so the launch is the same
+// every step and can be replayed from a graph): block (x, r) handles run r,
+// block x = 0 thread 0 its sources, all blocks its SGT points.
+__global__ void __launch_bounds__(256) k_aux_batch(const AuxJob *__restrict__ jobs, const int par)
+{
+    const AuxJob &J = jobs[blockIdx.y];
+    const int i = *J.counter + 1;    // this step
+    const State &s = J.s[par ^ 1];   // the state after it
+    if (blockIdx.x == 0 && threadIdx.x == 0 && i + 1 <= J.nst)
+        apply_sources(s, J.igreen, J.npsrc, J.psrc, J.ptop, J.g.sz, J.stf, J.nst, i + 1, J.vtst, J.vtst1);
+    if (i % J.ntiskp == 0) {
+        const int sample = i / J.ntiskp - 1;
+        if (sample < J.nsamp_max)
+            for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < J.nkept; n += gridDim.x * blockDim.x)
+                gather_point(s, n, J.kpos, J.kidx, J.sg, J.sgt, J.nsamp_max, sample);
+    }
+}
+
+__global__ void k_count_batch(const AuxJob *__restrict__ jobs, const int njobs)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < njobs) *jobs[r].counter += 1;
 }
 
 // ---------------------------------------------------------------------------
@@ -623,6 +663,17 @@ hipError_t launch_sgt_gather(const State &s, int nkept, const long long *kpos, c
     if (nkept <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_sgt_gather, dim3((nkept + 255) / 256), dim3(256), 0, st, s, nkept, kpos, kidx, sg, out,
                        nsamp_max, sample);
+    return hipGetLastError();
+}
+
+hipError_t launch_aux_batch(const AuxJob *d_jobs, int njobs, int max_kept, int par, hipStream_t st)
+{
+    if (njobs <= 0) return hipSuccess;
+    const int gx = std::max(1, std::min((max_kept + 255) / 256, 1024));
+    hipLaunchKernelGGL(k_aux_batch, dim3(gx, njobs), dim3(256), 0, st, d_jobs, par);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_count_batch, dim3((njobs + 63) / 64), dim3(64), 0, st, d_jobs, njobs);
     return hipGetLastError();
 }
 

@@ -6,8 +6,10 @@
 // reference's internal convention, structure.f:488), lives at element
 //     base + k*sz + j*sy + i
 // with base chosen so that (1,1,1) sits at (XO, YO, ZO) = (32, 2, 2).  XO = 32
-// puts the start of every interior row on a 128-byte line, so a 64-lane wave
-// reading 64 consecutive x values touches exactly two lines.  Two ghost planes
+// puts the start of every interior row of a 4-byte array on a 128-byte line,
+// so a 64-lane wave reading 64 consecutive x values touches exactly two
+// lines; 2-byte arrays start 64 bytes into their allocation so that their
+// rows start on a line too (one line per wave access).  Two ghost planes
 // on every side hold the reference's halo values (zero outside the domain,
 // free-surface images at k = nz+1, nz+2).
 //
@@ -44,8 +46,11 @@ static inline Geo make_geo(int nx, int ny, int nz)
 {
     Geo g;
     g.nx = nx; g.ny = ny; g.nz = nz;
+    // pitch a multiple of 64 elements: every row of a 4-byte array starts on
+    // a 256-byte boundary and (with the 64-byte shifted base the 2-byte
+    // arrays get) every row of a 2-byte array on a 128-byte line
     int need = XO + nx + 2;
-    g.px = (need + 31) / 32 * 32;
+    g.px = (need + 63) / 64 * 64;
     g.py = ny + 2 * YO;
     g.pz = nz + 2 * ZO;
     g.sy = g.px;
@@ -156,6 +161,50 @@ hipError_t launch_absmax(const float *a, long long n, float *part, int nblk, hip
 hipError_t launch_to_half(const float *src, _Float16 *dst, float scale, long long n, hipStream_t st);
 hipError_t launch_sgt_fp16(const float *sgt, const int *kidx, int nkept, int nsamp_max, int nsamp, float *scales,
                            _Float16 *out, hipStream_t st);
+// the fused one-sweep step (awp_fused.hip): reads state a, writes state b
+hipError_t launch_step_fused(bool mixed, const Geo &g, const State &a, const State &b, const Coef &c,
+                             const Launch &l, hipStream_t st);
+
+// tiles of one run for the fused step: nbx * nby * nbz column tiles
+struct FTiles {
+    int nbx, nby, nbz;
+    int per;   // tiles per XCD (single-run launch)
+};
+FTiles fused_tiles(const Geo &g, const Launch &l);
+
+// one run of a batched fused step; s[p] is read and s[p ^ 1] written when
+// the launch's parity is p
+struct FJob {
+    Geo g;
+    State s[2];
+    Coef c;
+    int kc;
+    FTiles t;
+    int tile0;   // first global tile of the run
+};
+// one launch for all runs (d_jobs in device memory, tile0 ascending);
+// tile_y / unroll of `l` select the kernel variant, grid = persistent blocks
+hipError_t launch_step_fused_batch(bool mixed, const Launch &l, const FJob *d_jobs, int njobs, int total,
+                                   int grid, int par, hipStream_t st);
+// blocks of the batched kernel that fit on the device at once
+int fused_batch_grid(bool mixed, const Launch &l, int device);
+
+// per-run data of the batched source / SGT stage
+struct AuxJob {
+    State s[2];                 // as FJob: s[p ^ 1] is the state after the step
+    Geo g;
+    int igreen, npsrc, nst, ntiskp, nsamp_max, nkept;
+    const long long *psrc, *ptop;
+    const float *stf;
+    float vtst, vtst1;
+    const long long *kpos;
+    const int *kidx;
+    const float *sg;
+    float *sgt;
+    int *counter;               // steps completed before the current one
+};
+// the source and SGT work of one step for every run, then the counters
+hipError_t launch_aux_batch(const AuxJob *d_jobs, int njobs, int max_kept, int par, hipStream_t st);
 // copy field `which` (0..19) to dst as [k][j][i] in physical units
 hipError_t launch_pack_field(bool mixed, const Geo &g, const State &s, int which, int kext, float *dst,
                              hipStream_t st);

@@ -198,6 +198,38 @@ int awp_get_timing(awp_ctx *ctx, double *out3);
 /* Bytes of device memory the context holds. */
 int awp_device_bytes(const awp_ctx *ctx, size_t *bytes);
 
+/* Device memory a run of these parameters with npts SGT points needs:
+ * *steady while it steps, *peak during awp_set_media (the media file and
+ * set-up temporaries on the device as well).  For admitting runs to a GPU
+ * under a memory budget.  No counterpart in the reference. */
+int awp_estimate_bytes(const awp_in3d *p, int storage, int npts, size_t *steady, size_t *peak);
+
+/* ---- batches ------------------------------------------------------------
+ * Several initialised runs (same device and storage mode, fused step)
+ * advanced together: one launch of the fused step kernel per time step over
+ * the tiles of every run, one launch for every run's source and SGT sample,
+ * replayed from a HIP graph.  Each run keeps its own outputs
+ * (awp_get_sgt, awp_write_sgt_file).  The reference runs one SGT component
+ * per MPI job (AWP-ODC-SGT/utils/awp_odc_wrapper.sh:14); these entry points
+ * have no counterpart in it. */
+typedef struct awp_batch awp_batch;
+int awp_batch_create(awp_batch **out, awp_ctx *const *runs, int nruns, char *err, size_t errlen);
+/* Times the batch's fused kernel for each tile height / prefetch depth /
+ * z chunking candidate (reps launches each) and keeps the fastest.  Needs
+ * runs that have not stepped; re-initialises them (awp_init) afterwards. */
+int awp_batch_tune(awp_batch *b, int reps);
+/* Advances every run min(nsteps, steps it has left) steps; returns when
+ * they are done. */
+int awp_batch_step(awp_batch *b, int nsteps);
+int awp_batch_synchronize(awp_batch *b);
+/* Device milliseconds of the last awp_batch_step (the last tuning launch
+ * after awp_batch_tune). */
+int awp_batch_time(const awp_batch *b, double *ms);
+/* The kernel variant in use: tile rows, prefetch depth, z chunks of run 0. */
+int awp_batch_get_launch(const awp_batch *b, int *tile_y, int *prefetch, int *nchunk);
+const char *awp_batch_last_error(const awp_batch *b);
+int awp_batch_destroy(awp_batch *b);
+
 #ifdef __cplusplus
 }
 #endif

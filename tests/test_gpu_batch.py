@@ -1,0 +1,125 @@
+"""Batched runs (awp_batch_*) through the C ABI.
+
+A batch advances several runs with one fused-kernel launch per step over all
+their tiles and replays the steps from a HIP graph.  Every run's SGT and
+fields must equal, bit for bit, those of the same run stepped on its own
+(which the single-run tests tie to the oracle and the reference files),
+whatever the step counts per call (graph pairs plus an odd remainder), the
+kernel variant the tuner picks, and the storage mode.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import cases
+import awpsgt
+from test_gpu_parity import _inputs
+
+pytestmark = pytest.mark.gpu
+
+SAME_NT = ["g1_fx_layered", "g2_fy_nvar5_nosocalq", "g3_fz_surface", "g4_surface_xz", "g6_surface_zz"]
+
+
+def _make(d, storage, init=True):
+    p, media, xyz, stf, pts = _inputs(d)
+    r = awpsgt.Run(p, storage=storage)
+    r.set_media(media)
+    r.set_source(xyz, stf)
+    r.set_sgt_points(pts)
+    if init:
+        r.init()
+    return r
+
+
+def _single(d, storage, nsteps=None):
+    r = _make(d, storage)
+    r.step(r.p.nt if nsteps is None else nsteps)
+    r.synchronize()
+    return r
+
+
+@pytest.mark.parametrize("storage", [awpsgt.AWP_STORAGE_FP32, awpsgt.AWP_STORAGE_MIXED])
+def test_batch_equals_single_runs(case_dirs, storage):
+    dirs = [case_dirs[n][0] for n in SAME_NT]
+    runs = [_make(d, storage) for d in dirs]
+    nt = runs[0].p.nt
+    assert all(r.p.nt == nt for r in runs)
+    with awpsgt.Batch(runs) as b:
+        # uneven calls: graph pairs, odd remainders, a call past the end
+        for n in (1, 36, 7, 40, 1000):
+            b.step(n)
+    for r in runs:
+        assert r.progress()[0] == nt
+    for name, d, r in zip(SAME_NT, dirs, runs):
+        ref = _single(d, storage)
+        assert np.array_equal(r.sgt(), ref.sgt()), name
+        for f in ("u1", "w1", "xx", "xz", "r1"):
+            k = 2 if f in ("u1", "w1", "xz") else 0
+            assert np.array_equal(r.field(f, k), ref.field(f, k)), (name, f)
+        ref.close()
+    for r in runs:
+        r.close()
+
+
+def test_batch_uneven_grids_and_tuner(case_dirs, tmp_path):
+    """Runs of different grid shapes in one batch (their tiles interleave in
+    the persistent grid), after the tuner has picked a variant; files equal
+    the single-run files byte for byte."""
+    import tempfile
+    dirs = [case_dirs["g1_fx_layered"][0]]
+    # two more shapes with the same step count
+    for k, (nx, ny, nz) in enumerate([(70, 33, 40), (40, 58, 24)]):
+        spec = cases.tiny_spec("g1_fx_layered")
+        spec.nx, spec.ny, spec.nz = nx, ny, nz
+        spec.src = (nx // 2, ny // 2, 6)
+        media = awpsgt.inputs.layered_media(nx, ny, nz, [700.0, 1500.0, 2500.0])
+        pts = awpsgt.inputs.random_cordfile_points(50, 11 + k, (1, nx), (1, ny), (1, nz))
+        d = tempfile.mkdtemp()
+        cases.write_case(d, spec, media, pts)
+        dirs.append(d)
+    runs = [_make(d, awpsgt.AWP_STORAGE_FP32) for d in dirs]
+    nt = min(r.p.nt for r in runs)
+    with awpsgt.Batch(runs) as b:
+        ty, pf, nc = b.tune(1)
+        assert ty in (4, 7) and pf in (1, 2) and nc in (1, 2)
+        assert all(r.progress() == (0, 0) for r in runs)
+        b.step(nt)
+    for k, (d, r) in enumerate(zip(dirs, runs)):
+        ref = _single(d, awpsgt.AWP_STORAGE_FP32, nsteps=nt)
+        a, c = os.path.join(str(tmp_path), "a%d" % k), os.path.join(str(tmp_path), "c%d" % k)
+        r.write_sgt_file(a)
+        ref.write_sgt_file(c)
+        assert open(a, "rb").read() == open(c, "rb").read(), d
+        ref.close()
+        r.close()
+
+
+def test_batch_rejects_bad_runs(case_dirs):
+    d = case_dirs["g1_fx_layered"][0]
+    r1 = _make(d, awpsgt.AWP_STORAGE_FP32)
+    r2 = _make(d, awpsgt.AWP_STORAGE_MIXED)
+    r3 = _make(d, awpsgt.AWP_STORAGE_FP32, init=False)
+    with pytest.raises(awpsgt.AwpError, match="storage"):
+        awpsgt.Batch([r1, r2])
+    with pytest.raises(awpsgt.AwpError, match="initialised"):
+        awpsgt.Batch([r1, r3])
+    with pytest.raises(awpsgt.AwpError, match="twice"):
+        awpsgt.Batch([r1, r1])
+    with awpsgt.Batch([r1]) as b:
+        r1.step(3)
+        with pytest.raises(awpsgt.AwpError, match="not stepped"):
+            b.tune(1)
+    for r in (r1, r2, r3):
+        r.close()
+
+
+def test_estimate_bytes_matches_allocation(case_dirs):
+    for storage in (awpsgt.AWP_STORAGE_FP32, awpsgt.AWP_STORAGE_MIXED):
+        d = case_dirs["g7_random_edges"][0]
+        r = _make(d, storage)
+        steady, peak = awpsgt.estimate_bytes(r.p, storage, r.npts)
+        held = r.device_bytes()
+        assert held <= steady <= held + 64 * 1024, (storage, held, steady)
+        assert peak > steady
+        r.close()

@@ -142,8 +142,9 @@ def test_sgt_fp16_file(case_dirs, name, tmp_path):
     assert raw[:8] == b"AWPSGT16"
     version, npts, ncomp, nsamp, ntiskp = struct.unpack("<5I", raw[8:28])
     (dt,) = struct.unpack("<f", raw[28:32])
-    nt_out, _ = awpsgt.sgt_file_geometry(p)
-    assert (version, npts, ncomp, nsamp, ntiskp) == (1, len(pts), 6, nt_out, p.ntiskp_sgt)
+    # the samples the raw file holds: full WRITE_STEP buffers only
+    _, nflush = awpsgt.sgt_file_geometry(p)
+    assert (version, npts, ncomp, nsamp, ntiskp) == (1, len(pts), 6, nflush * p.write_step, p.ntiskp_sgt)
     assert dt == np.float32(p.dt)
     assert len(raw) == 64 + npts * 6 * 4 + npts * 6 * nsamp * 2
     scale = np.frombuffer(raw, dtype=np.float32, count=npts * 6, offset=64).reshape(npts, 6)
