@@ -7,17 +7,25 @@ depth gradient times a seeded von Karman perturbation (correlation 5 km,
 Hurst 0.1, 5% std, clipped to +-3 sigma), Vp = 1.73 Vs, Gardner density,
 Qs = 50 Vs[km/s], Qp = 2 Qs; X point force with a Gaussian source time
 function; 1.5M SGT points sampled every 10 steps.  Synthetic inputs are built
-on the device; the timed region is K full time steps (velocity, free surface,
-stress + memory variables, sponge, source, SGT gather) with every input
-already resident in HBM.
+on the device; the timed region is K full time steps (fused velocity +
+stress sweep with sponge and free surface, source, SGT gather) with every
+input already resident in HBM.
 
-    python bench.py [--steps K] [--warmup W] [--config c2|c1] [--storage fp32|mixed]
+Batched workloads: "c3" -- 8 runs (4 sites x X/Y force) on grids from
+400x300x150 to 900x700x220, 200k SGT points each; "c5" -- 16 runs (8 sites x
+X/Y) on grids from 500x400x150 to 1200x900x280, admitted in waves under the
+device-memory budget.  Same media recipe per site (seeded), h = 100 m,
+dt = 0.005 s.  Each wave is one awpsgt.Batch (one fused launch per step for
+all its runs, replayed from a HIP graph), tuned before the timed steps.
+
+    python bench.py [--steps K] [--warmup W] [--config c2|c1|c3|c5] [--storage fp32|mixed]
 
 Prints one JSON line.  value = point-updates per second of the whole job
-(Gpt-updates/s); roofline = the stress kernel (the dominant one): its
-algorithmic bytes per launch (FP32: 124 B per point, 31 FP32 arrays touched
-once; mixed: 96 B per point, r1..r6 BF16 and qp/qs FP16) over its average
-launch time from HIP events on the run's stream.
+(Gpt-updates/s).  roofline = the fused step kernel (the only volume kernel):
+achieved = the path's algorithmic bytes per point-update (176 B FP32, 148 B
+mixed storage: every array each of the reference's two stages touches, once,
+at its storage width) x points / the kernel's average time from HIP events
+on the run's stream; the fused kernel itself needs only 140 B / 116 B.
 """
 from __future__ import annotations
 
@@ -31,11 +39,13 @@ REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(REPO, "cybershake-core_amd"))
 
 HBM_PEAK_GBS = 8000.0
-BYTES_VEL = 52       # per point-update, velocity stage (13 FP32 arrays, both storage modes)
-# per point-update, stress stage: read u,v,w (12 B); read+write 6 stresses
-# (48 B); read+write r1..r6 (48 B FP32, 24 B BF16); read lam, mu (8 B) and
-# qp, qs (8 B FP32, 4 B FP16)
-BYTES_STR = {"fp32": 124, "mixed": 96}
+# algorithmic bytes per point-update of the two-stage path: velocity stage
+# 13 FP32 arrays (52 B); stress stage read u,v,w (12), read+write 6 stresses
+# (48), read+write r1..r6 (48 FP32 / 24 BF16), read lam, mu (8) and qp, qs
+# (8 FP32 / 4 FP16)
+BYTES_UPDATE = {"fp32": 176, "mixed": 148}
+BYTES_STR = {"fp32": 124, "mixed": 96}     # the two-stage stress kernel alone
+BYTES_FUSED = {"fp32": 140, "mixed": 116}  # what the fused sweep must move
 
 CONFIGS = {
     "c2": dict(nx=1500, ny=1000, nz=300, dh=100.0, dt=0.005, nsgt=1_500_000, src=(750, 500, 10),
@@ -44,18 +54,24 @@ CONFIGS = {
     "c1": dict(nx=200, ny=200, nz=100, dh=200.0, dt=0.01, nsgt=2000, src=(100, 100, 10),
                workload="reference CPU case 200x200x100, h=200 m, dt=0.01 s, layered medium, "
                         "X force, 2000 SGT points every 10 steps, FP32"),
+    "c3": dict(nsites=4, lo=(400, 300, 150), hi=(900, 700, 220), seed0=10, dh=100.0, dt=0.005, nsgt=200_000,
+               workload="batched: 8 runs (4 sites x X/Y force), grids 400x300x150 .. 900x700x220, h=100 m, "
+                        "dt=0.005 s, von Karman media per site, 200k SGT points per run, FP32"),
+    "c5": dict(nsites=8, lo=(500, 400, 150), hi=(1200, 900, 280), seed0=20, dh=100.0, dt=0.005, nsgt=200_000,
+               workload="batched queue: 16 runs (8 sites x X/Y force), grids 500x400x150 .. 1200x900x280, "
+                        "h=100 m, dt=0.005 s, admitted by device-memory budget, 200k SGT points per run, FP32"),
 }
 
 
-def build_params(cfg, nt):
+def build_params(cfg, nt, nx=None, ny=None, nz=None, igreen=1):
     import awpsgt
     p = awpsgt.In3d()
-    p.igreen = 1
+    p.igreen = igreen
     p.dh, p.dt = cfg["dh"], cfg["dt"]
     p.tmax = (nt - 1) * cfg["dt"]
     p.npc, p.nd, p.arbc, p.pht = 0, 20, 0.92, 0.1
     p.nsrc, p.nst = 1, nt
-    p.nx, p.ny, p.nz = cfg["nx"], cfg["ny"], cfg["nz"]
+    p.nx, p.ny, p.nz = nx or cfg["nx"], ny or cfg["ny"], nz or cfg["nz"]
     p.npx = p.npy = p.npz = 1
     p.ifault, p.mediarestart, p.nvar, p.socalq, p.nve = 0, 1, 5, 1, 1
     p.fl, p.fh, p.fp = 0.01, 25.0, 0.5
@@ -66,6 +82,200 @@ def build_params(cfg, nt):
     return p
 
 
+def log(msg):
+    print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
+def make_run(p, storage, media_fn, src, nsgt, seed_pts):
+    """Run with device media (media_fn() -> torch tensor), a Gaussian force
+    at src and nsgt random SGT points; initialised."""
+    import numpy as np
+    import torch
+    import awpsgt
+    from awpsgt.inputs import gaussian_stf, random_cordfile_points
+    run = awpsgt.Run(p, device=0, storage=storage)
+    pts = random_cordfile_points(nsgt, seed_pts, (1, p.nx), (1, p.ny), (1, p.nz))
+    nkept = run.set_sgt_points(pts.astype(np.int32))
+    media = media_fn()
+    if isinstance(media, np.ndarray):
+        run.set_media(media)
+    else:
+        torch.cuda.synchronize()
+        run.set_media_device(media.data_ptr())
+        torch.cuda.synchronize()
+    del media
+    torch.cuda.empty_cache()
+    stf = np.zeros((1, p.nt, 6), dtype=np.float32)
+    stf[0, :, p.igreen - 1] = gaussian_stf(p.nt, p.dt, 0.3, 1.2, 1e9)
+    run.set_source(np.array([src], dtype=np.int32), stf)
+    run.init()
+    return run, nkept
+
+
+def cpu_leg(a):
+    if a.no_cpu_baseline:
+        return None
+    sys.path.insert(0, REPO)
+    from oracle import cpu_baseline
+    ncores = min(len(os.sched_getaffinity(0)), 16)
+    log("cpu baseline on %d cores ..." % ncores)
+    return cpu_baseline.measure(ncores)
+
+
+def traffic_of(name):
+    tfile = os.path.join(REPO, "profiles", name)
+    if os.path.exists(tfile):
+        for ln in open(tfile):
+            if ln.startswith("hbm_bytes_per_launch="):
+                return float(ln.split("=", 1)[1])
+    return None
+
+
+def bench_single(a, storage):
+    import numpy as np
+    import torch
+    import awpsgt
+    from awpsgt.inputs import layered_media
+    from awpsgt.media import c2_media
+
+    cfg = CONFIGS[a.config]
+    nt = a.warmup + a.steps
+    p = build_params(cfg, nt)
+    npts_grid = p.nx * p.ny * p.nz
+    log("%s: grid %dx%dx%d, %d warmup + %d timed steps" % (a.config, p.nx, p.ny, p.nz, a.warmup, a.steps))
+    t0 = time.time()
+    if a.config == "c2":
+        media_fn = lambda: c2_media(p.nx, p.ny, p.nz, p.dh, seed=2)  # noqa: E731
+    else:
+        def media_fn():
+            m = layered_media(p.nx, p.ny, p.nz, [600.0, 1400.0, 2200.0, 3000.0, 3800.0])
+            vs = m[..., 1]
+            return np.stack([m[..., 0], vs, m[..., 2], 0.1 * vs, 0.05 * vs], -1).astype(np.float32)
+    run, nkept = make_run(p, storage, media_fn, cfg["src"], cfg["nsgt"], 3)
+    log("set-up %.1f s, %d SGT points, %.1f GB on device" % (time.time() - t0, nkept, run.device_bytes() / 1e9))
+
+    run.step(a.warmup)
+    run.synchronize()
+    torch.cuda.synchronize()
+    run.set_timing(True)
+    t = time.perf_counter()
+    run.step(a.steps)
+    run.synchronize()
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t
+    tv, ts, ta = run.timing()
+    step_now, nsamp = run.progress()
+    assert step_now == nt
+    finite = bool(np.isfinite(run.sgt()).all())
+    run.close()
+
+    fused = os.environ.get("AWP_SCHEME", "fused") != "split"
+    value = npts_grid * a.steps / el / 1e9
+    ms_step = el / a.steps * 1e3
+    kern_ms = ts / a.steps                   # fused kernel, or the stress kernel of the two-stage step
+    if fused:
+        kname = "k_step (fused velocity + stress + sponge + free surface sweep)"
+        bytes_pt = BYTES_UPDATE[storage]
+        tname = "step_%s%s_traffic.txt" % (a.config, "" if storage == "fp32" else "_" + storage)
+    else:
+        kname = "k_stress (stress + memory variables + sponge + free surface)"
+        bytes_pt = BYTES_STR[storage]
+        tname = "stress_%s%s_traffic.txt" % (a.config, "" if storage == "fp32" else "_" + storage)
+    ach = bytes_pt * npts_grid / (kern_ms * 1e-3) / 1e9
+    log("%.3f ms/step; kernel %.3f ms (%.0f GB/s algorithmic at %d B/pt); finite=%s"
+        % (ms_step, kern_ms, ach, bytes_pt, finite))
+    roof = {"bound": "hbm", "kernel": kname, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic_of(tname),
+            "algorithmic_bytes_per_point": bytes_pt, "algorithmic_bytes_per_launch": bytes_pt * npts_grid,
+            "avg_launch_ms": round(kern_ms, 4),
+            "step_frac": round(value * BYTES_UPDATE[storage] / HBM_PEAK_GBS, 4)}
+    if fused:
+        roof["fused_kernel_min_bytes_per_point"] = BYTES_FUSED[storage]
+    wl = cfg["workload"] if storage == "fp32" else cfg["workload"].replace(
+        ", FP32", ", mixed storage (BF16 r1..r6, FP16 qp/qs)")
+    config = {"workload": wl, "grid": [p.nx, p.ny, p.nz], "points": npts_grid, "sgt_points": int(nkept),
+              "storage": storage, "step": "fused" if fused else "two-stage", "parallelism": "single GPU"}
+    return value, ms_step, roof, config
+
+
+def bench_batched(a, storage):
+    import numpy as np
+    import torch
+    import awpsgt
+    from awpsgt.batchq import Need, admit, site_grids
+    from awpsgt.media import c2_media
+
+    cfg = CONFIGS[a.config]
+    nt = a.warmup + a.steps
+    st = awpsgt.AWP_STORAGE_MIXED if storage == "mixed" else awpsgt.AWP_STORAGE_FP32
+    jobs = []
+    for s_, (nx, ny, nz) in enumerate(site_grids(cfg["nsites"], cfg["lo"], cfg["hi"])):
+        for ig in (1, 2):
+            p = build_params(cfg, nt, nx, ny, nz, igreen=ig)
+            steady, peak = awpsgt.estimate_bytes(p, st, cfg["nsgt"])
+            jobs.append(dict(site=s_, seed=cfg["seed0"] + s_, igreen=ig, p=p, need=Need(steady, peak)))
+    free, _ = torch.cuda.mem_get_info()
+    budget = int(a.budget_gb * 1e9) if a.budget_gb else free - (8 << 30)
+    waves = admit([j["need"] for j in jobs], budget)
+    log("%s: %d runs, %.1f GB steady in total, budget %.1f GB -> %d wave(s) of %s runs"
+        % (a.config, len(jobs), sum(j["need"].steady for j in jobs) / 1e9, budget / 1e9, len(waves),
+           "/".join(str(len(w)) for w in waves)))
+    tot_pts = 0
+    tot_time = 0.0
+    kern = []
+    launch = None
+    for w in waves:
+        runs = []
+        t0 = time.time()
+        for k in w:
+            j = jobs[k]
+            p = j["p"]
+            fn = (lambda p=p, seed=j["seed"]: c2_media(p.nx, p.ny, p.nz, p.dh, seed=seed))
+            r, _ = make_run(p, st, fn, (p.nx // 2, p.ny // 2, 10), cfg["nsgt"], 100 + k)
+            runs.append(r)
+        pts = sum(r.p.nx * r.p.ny * r.p.nz for r in runs)
+        log("wave of %d runs, %.0fM points, set-up %.1f s, %.1f GB on device"
+            % (len(runs), pts / 1e6, time.time() - t0, sum(r.device_bytes() for r in runs) / 1e9))
+        with awpsgt.Batch(runs) as b:
+            if not a.no_tune:
+                launch = b.tune(2)
+                log("tuned: tile rows %d, prefetch %d, z chunks %d (%.2f ms per fused launch)"
+                    % (launch + (b.time_ms(),)))
+            else:
+                launch = b.launch()
+            b.step(a.warmup)
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            b.step(a.steps)
+            el = time.perf_counter() - t
+            kern.append(b.time_ms() / a.steps)
+        finite = all(bool(np.isfinite(r.sgt()).all()) for r in runs)
+        for r in runs:
+            assert r.progress()[0] == nt
+            r.close()
+        torch.cuda.empty_cache()
+        log("  %.3f ms/step (device %.3f), %.2f Gpt/s, finite=%s" % (el / a.steps * 1e3, kern[-1],
+                                                                    pts * a.steps / el / 1e9, finite))
+        tot_pts += pts * a.steps
+        tot_time += el
+    value = tot_pts / tot_time / 1e9
+    pts_step = tot_pts / a.steps
+    ms_step = tot_time / a.steps * 1e3
+    ach = BYTES_UPDATE[storage] * value
+    roof = {"bound": "hbm", "kernel": "k_step_batch (fused sweep over every run of a wave)",
+            "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+            "traffic": None, "algorithmic_bytes_per_point": BYTES_UPDATE[storage],
+            "fused_kernel_min_bytes_per_point": BYTES_FUSED[storage],
+            "avg_step_ms_per_wave": [round(x, 4) for x in kern]}
+    wl = cfg["workload"] if storage == "fp32" else cfg["workload"].replace(
+        ", FP32", ", mixed storage (BF16 r1..r6, FP16 qp/qs)")
+    config = {"workload": wl, "runs": len(jobs), "waves": [len(w) for w in waves],
+              "points_per_step": int(pts_step), "storage": storage, "step": "fused, batched, HIP graph",
+              "launch": {"tile_rows": launch[0], "prefetch": launch[1], "z_chunks": launch[2]} if launch else None,
+              "budget_gb": round(budget / 1e9, 1), "parallelism": "single GPU"}
+    return value, ms_step, roof, config
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -73,6 +283,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
     ap.add_argument("--storage", default="fp32", choices=["fp32", "mixed"])
+    ap.add_argument("--budget-gb", type=float, default=0.0,
+                    help="device-memory budget for the batched configs (default: free memory - 8 GiB)")
+    ap.add_argument("--no-tune", action="store_true", help="batched configs: skip the tuner")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain device copy of 4 GiB five times (a known-size kernel "
@@ -88,88 +301,15 @@ def main():
         print(json.dumps({"copy_bytes_read": 4 << 30, "copy_bytes_written": 4 << 30, "launches": 5}))
         return
     if a.gpus != 1:
-        print("bench.py measures one GPU: the solver runs one SGT job per GPU and this build "
-              "has no multi-GPU path (--gpus %d requested)" % a.gpus, file=sys.stderr)
+        print("bench.py measures one GPU: the solver runs one SGT job (or one batch of jobs) per GPU "
+              "and this build has no multi-GPU path (--gpus %d requested)" % a.gpus, file=sys.stderr)
         sys.exit(2)
 
-    import numpy as np
-    import torch
-    import awpsgt
-    from awpsgt.inputs import gaussian_stf, random_cordfile_points, layered_media
-    from awpsgt.media import c2_media
-
-    cfg = CONFIGS[a.config]
-    nt = a.warmup + a.steps
-    p = build_params(cfg, nt)
-    npts_grid = p.nx * p.ny * p.nz
-    print("[bench] %s: grid %dx%dx%d, %d warmup + %d timed steps" % (a.config, p.nx, p.ny, p.nz, a.warmup, a.steps),
-          file=sys.stderr, flush=True)
-
-    t0 = time.time()
-    storage = awpsgt.AWP_STORAGE_MIXED if a.storage == "mixed" else awpsgt.AWP_STORAGE_FP32
-    run = awpsgt.Run(p, device=0, storage=storage)
-    if a.config == "c2":
-        media = c2_media(p.nx, p.ny, p.nz, p.dh, seed=2)
-        torch.cuda.synchronize()
-        run.set_media_device(media.data_ptr())
-        del media
-        torch.cuda.empty_cache()
+    if a.config in ("c3", "c5"):
+        value, ms_step, roof, config = bench_batched(a, a.storage)
     else:
-        m = layered_media(p.nx, p.ny, p.nz, [600.0, 1400.0, 2200.0, 3000.0, 3800.0])
-        vs = m[..., 1]
-        m5 = np.stack([m[..., 0], vs, m[..., 2], 0.1 * vs, 0.05 * vs], -1).astype(np.float32)
-        run.set_media(m5)
-    stf = np.zeros((1, nt, 6), dtype=np.float32)
-    stf[0, :, 0] = gaussian_stf(nt, p.dt, 0.3, 1.2, 1e9)
-    run.set_source(np.array([cfg["src"]], dtype=np.int32), stf)
-    pts = random_cordfile_points(cfg["nsgt"], 3, (1, p.nx), (1, p.ny), (1, p.nz))
-    nkept = run.set_sgt_points(pts.astype(np.int32))
-    run.init()
-    print("[bench] set-up %.1f s, %d SGT points, %.1f GB on device"
-          % (time.time() - t0, nkept, run.device_bytes() / 1e9), file=sys.stderr, flush=True)
-
-    run.step(a.warmup)
-    run.synchronize()
-    torch.cuda.synchronize()
-    run.set_timing(True)
-    t = time.perf_counter()
-    run.step(a.steps)
-    run.synchronize()
-    torch.cuda.synchronize()
-    el = time.perf_counter() - t
-    tv, ts, ta = run.timing()
-    step_now, nsamp = run.progress()
-    assert step_now == nt
-    s = run.sgt()
-    finite = bool(np.isfinite(s).all())
-    run.close()
-
-    value = npts_grid * a.steps / el / 1e9
-    ms_step = el / a.steps * 1e3
-    str_ms = ts / a.steps
-    vel_ms = tv / a.steps
-    bstr = BYTES_STR[a.storage]
-    str_gbs = bstr * npts_grid / (str_ms * 1e-3) / 1e9
-    vel_gbs = BYTES_VEL * npts_grid / (vel_ms * 1e-3) / 1e9
-    print("[bench] %.3f ms/step; velocity %.3f ms (%.0f GB/s), stress %.3f ms (%.0f GB/s); finite=%s"
-          % (ms_step, vel_ms, vel_gbs, str_ms, str_gbs, finite), file=sys.stderr, flush=True)
-
-    traffic = None
-    tfile = os.path.join(REPO, "profiles", "stress_%s%s_traffic.txt"
-                         % (a.config, "" if a.storage == "fp32" else "_" + a.storage))
-    if os.path.exists(tfile):
-        for ln in open(tfile):
-            if ln.startswith("hbm_bytes_per_launch="):
-                traffic = float(ln.split("=", 1)[1])
-
-    cpu = None
-    if not a.no_cpu_baseline:
-        sys.path.insert(0, REPO)
-        from oracle import cpu_baseline
-        ncores = min(len(os.sched_getaffinity(0)), 16)
-        print("[bench] cpu baseline on %d cores ..." % ncores, file=sys.stderr, flush=True)
-        cpu = cpu_baseline.measure(ncores)
-
+        value, ms_step, roof, config = bench_single(a, a.storage)
+    cpu = cpu_leg(a)
     out = {
         "metric": "Gpt-updates/s + % HBM roofline; batched mixed-storage vs FP32 at stated SGT error",
         "value": round(value, 4),
@@ -183,15 +323,8 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": cfg["workload"] if a.storage == "fp32" else
-                   cfg["workload"].replace(", FP32", ", mixed storage (BF16 r1..r6, FP16 qp/qs)"), "grid": [p.nx, p.ny, p.nz], "points": npts_grid,
-                   "sgt_points": int(nkept), "storage": a.storage, "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "kernel": "k_stress (stress + memory variables + sponge + free surface)",
-                     "achieved": round(str_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(str_gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "algorithmic_bytes_per_launch": bstr * npts_grid,
-                     "avg_launch_ms": round(str_ms, 4),
-                     "step_frac": round(value * (BYTES_VEL + bstr) / HBM_PEAK_GBS, 4)},
+        "config": config,
+        "roofline": roof,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

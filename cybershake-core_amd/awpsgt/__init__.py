@@ -84,7 +84,7 @@ EXPORTS = [
     "awp_set_source", "awp_set_sgt_points", "awp_init", "awp_step", "awp_synchronize",
     "awp_progress", "awp_get_sgt", "awp_write_sgt_file", "awp_write_sgt_file_fp16",
     "awp_sgt16_read", "awp_get_field", "awp_set_timing", "awp_get_timing", "awp_device_bytes",
-    "awp_estimate_bytes", "awp_batch_create", "awp_batch_tune", "awp_batch_step",
+    "awp_write_sgt_file_rwg", "awp_estimate_bytes", "awp_batch_create", "awp_batch_tune", "awp_batch_step",
     "awp_batch_synchronize", "awp_batch_time", "awp_batch_get_launch", "awp_batch_last_error",
     "awp_batch_destroy",
 ]
@@ -122,6 +122,7 @@ def lib() -> ctypes.CDLL:
         "awp_get_sgt": ([vp, vp, sz], ip),
         "awp_write_sgt_file": ([vp, cp], ip),
         "awp_write_sgt_file_fp16": ([vp, cp], ip),
+        "awp_write_sgt_file_rwg": ([vp, cp, ip], ip),
         "awp_sgt16_read": ([cp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), vp, sz, cp, sz], ip),
         "awp_get_field": ([vp, ip, ip, vp], ip),
         "awp_set_timing": ([vp, ip], ip),
@@ -280,6 +281,11 @@ class Run:
 
     def write_sgt_file(self, path: str) -> None:
         self._do(lib().awp_write_sgt_file(self.h, path.encode()), "awp_write_sgt_file")
+
+    def write_sgt_file_rwg(self, path: str, zsource: bool = False) -> None:
+        """The raw file as SgtHead's reformat_awp would rewrite it (RWG order)."""
+        self._do(lib().awp_write_sgt_file_rwg(self.h, path.encode(), 1 if zsource else 0),
+                 "awp_write_sgt_file_rwg")
 
     def write_sgt_file_fp16(self, path: str) -> None:
         self._do(lib().awp_write_sgt_file_fp16(self.h, path.encode()), "awp_write_sgt_file_fp16")

@@ -695,6 +695,58 @@ extern "C" int awp_write_sgt_file(awp_ctx *ctx, const char *path)
     return AWP_OK;
 }
 
+extern "C" int awp_write_sgt_file_rwg(awp_ctx *ctx, const char *path, int zsource)
+{
+    if (!ctx || !path) return AWP_EINVAL;
+    if (!ctx->points_set) return fail(ctx, AWP_ESTATE, "no SGT points");
+    const awp_in3d &p = ctx->p;
+    int nt_out = 0, nflush = 0;
+    awp_sgt_file_geometry(&p, &nt_out, &nflush);
+    const int avail = std::min(nflush, ctx->nsamp / p.write_step);
+    if (avail <= 0) return AWP_OK;   // no raw file, nothing to reformat
+    // the raw file ends with the last kept point's last full buffer; the
+    // reformatter takes the complete 6 * NT_OUT records of it
+    long long last = -1;
+    for (long long g = (long long)ctx->npts - 1; g >= 0; --g)
+        if (ctx->keep[g]) {
+            last = g;
+            break;
+        }
+    const size_t slot = (size_t)4 * nt_out;
+    const size_t raw_size = last < 0 ? 0 : (size_t)last * 6 * slot + 5 * slot + (size_t)avail * 4 * p.write_step;
+    const size_t nrec = raw_size / (6 * slot);
+    const int navail = avail * p.write_step;
+    FILE *f = fopen(path, "wb");
+    if (!f) return fail(ctx, AWP_EIO, "cannot create '%s'", path);
+    HIPTRY(ctx, hipSetDevice(ctx->device));
+    HIPTRY(ctx, hipStreamSynchronize(ctx->stream));
+    const size_t chunk = 8192;
+    const size_t per = 6 * (size_t)nt_out;
+    float *d_out = nullptr;
+    int *d_keep = nullptr;
+    std::vector<float> host(chunk * per);
+    std::vector<int> kk(chunk);
+    hipError_t e = hipMalloc((void **)&d_out, std::max<size_t>(chunk * per, 1) * sizeof(float));
+    if (e == hipSuccess) e = hipMalloc((void **)&d_keep, chunk * sizeof(int));
+    bool ok = e == hipSuccess;
+    for (size_t g0 = 0; ok && g0 < nrec; g0 += chunk) {
+        const size_t n = std::min(chunk, nrec - g0);
+        for (size_t k = 0; k < n; ++k) kk[k] = ctx->keep[g0 + k];
+        e = hipMemcpy(d_keep, kk.data(), n * sizeof(int), hipMemcpyHostToDevice);
+        if (e == hipSuccess)
+            e = launch_sgt_rwg(ctx->d_sgt, d_keep, (int)g0, (int)n, ctx->nsamp_max, nt_out, navail, zsource ? 1 : 0,
+                               d_out, ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        if (e == hipSuccess) e = hipMemcpy(host.data(), d_out, n * per * sizeof(float), hipMemcpyDeviceToHost);
+        ok = e == hipSuccess && fwrite(host.data(), sizeof(float), n * per, f) == n * per;
+    }
+    hipFree(d_out);
+    hipFree(d_keep);
+    ok = (fclose(f) == 0) && ok;
+    if (e != hipSuccess) return fail(ctx, AWP_EHIP, "RWG SGT: %s", hipGetErrorString(e));
+    return ok ? AWP_OK : fail(ctx, AWP_EIO, "write '%s' failed", path);
+}
+
 extern "C" int awp_write_sgt_file_fp16(awp_ctx *ctx, const char *path)
 {
     if (!ctx || !path) return AWP_EINVAL;

@@ -194,7 +194,11 @@ k_stress(const Geo g, const State s, const Coef c, const int kc, const Tiles tl)
     R *__restrict__ R6 = (R *)s.r6;
     const float c1 = c.c1, c2 = c.c2, dth = c.dth, dt = c.dt, dh1 = c.dh1;
     const float fxy = s.fx[i] * s.fy[j];
+    // relaxation weights of this column for even / odd depth parity
+    // (selected per level: no per-lane indexing of the argument block)
     const int itxy = ((i & 1) * 2 + (j & 1)) * 2;
+    const float t1e = c.tau1[itxy], t1o = c.tau1[itxy + 1];
+    const float t2e = c.tau2[itxy], t2o = c.tau2[itxy + 1];
 
     // z queues of the velocities at the column
     float u_m1 = u1[o - sz], u_0 = u1[o], u_p1 = u1[o + sz];
@@ -226,9 +230,9 @@ k_stress(const Geo g, const State s, const Coef c, const int kc, const Tiles tl)
         const float w_im1 = w1[o - 1], w_ip1 = w1[o + 1], w_im2 = w1[o - 2];
         const float w_jp1 = w1[o + sy], w_jp2 = w1[o + 2 * sy], w_jm1 = w1[o - sy];
 
-        const int it = itxy + (((g.nz + 1) - k) & 1);
-        const float vx1 = c.tau1[it];
-        const float vx2 = c.tau2[it];
+        const bool odd = (((g.nz + 1) - k) & 1) != 0;
+        const float vx1 = odd ? t1o : t1e;
+        const float vx2 = odd ? t2o : t2e;
 
         float vxl = 8.0f / (lA + lB + lC + lD + lA_m + lB_m + lC_m + lD_m);
         const float vxm = 8.0f / (mA + mB + mC + mD + mA_m + mB_m + mC_m + mD_m);
@@ -570,6 +574,39 @@ hipError_t launch_sgt_fp16(const float *sgt, const int *kidx, int nkept, int nsa
 {
     if (nkept <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_sgt_fp16, dim3(nkept * 6), dim3(64), 0, st, sgt, kidx, nkept, nsamp_max, nsamp, scales, out);
+    return hipGetLastError();
+}
+
+// RWG component order and signs of SgtHead/src/reformat_awp.c:38-62 for
+// records g0 .. g0+nrec-1 of the raw file: output component co takes input
+// component (1 0 2 3 5 4)[co]; the Z source first multiplies every value by
+// -2, XZ/YZ are then negated.  Dropped points and samples beyond the full
+// WRITE_STEP buffers are holes (zeros) in the raw file.
+__global__ void __launch_bounds__(256)
+k_sgt_rwg(const float *sgt, const int *keep, const int g0, const int nrec, const int nsamp_max, const int nt_out,
+          const int navail, const int zsrc, float *out)
+{
+    const long long total = (long long)nrec * 6 * nt_out;
+    for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < total;
+         q += (long long)gridDim.x * blockDim.x) {
+        const int t = (int)(q % nt_out);
+        const long long r = q / nt_out;
+        const int co = (int)(r % 6);
+        const int g = (int)(r / 6);
+        const int ci = co == 0 ? 1 : co == 1 ? 0 : co == 4 ? 5 : co == 5 ? 4 : co;
+        float v = (keep[g] && t < navail) ? sgt[((size_t)(g0 + g) * 6 + ci) * nsamp_max + t] : 0.0f;
+        if (zsrc) v = -2.0f * v;
+        if (co >= 4) v = -1.0f * v;
+        out[q] = v;
+    }
+}
+
+hipError_t launch_sgt_rwg(const float *sgt, const int *keep, int g0, int nrec, int nsamp_max, int nt_out,
+                          int navail, int zsrc, float *out, hipStream_t st)
+{
+    if (nrec <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_sgt_rwg, dim3(2048), dim3(256), 0, st, sgt, keep, g0, nrec, nsamp_max, nt_out, navail,
+                       zsrc, out);
     return hipGetLastError();
 }
 

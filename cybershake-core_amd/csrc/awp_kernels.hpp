@@ -205,6 +205,10 @@ struct AuxJob {
 };
 // the source and SGT work of one step for every run, then the counters
 hipError_t launch_aux_batch(const AuxJob *d_jobs, int njobs, int max_kept, int par, hipStream_t st);
+// raw-file records g0 .. g0+nrec-1 in SgtHead's RWG order and signs
+// (keep[nrec] marks the points the reference writes; navail samples filled)
+hipError_t launch_sgt_rwg(const float *sgt, const int *keep, int g0, int nrec, int nsamp_max, int nt_out,
+                          int navail, int zsrc, float *out, hipStream_t st);
 // copy field `which` (0..19) to dst as [k][j][i] in physical units
 hipError_t launch_pack_field(bool mixed, const Geo &g, const State &s, int which, int kext, float *dst,
                              hipStream_t st);

@@ -10,7 +10,9 @@
  * Environment: AWP_DEVICE selects the HIP device (default 0);
  * AWP_STORAGE=mixed selects mixed storage (BF16 memory variables, FP16
  * attenuation weights); AWP_SGT_FP16=<path> also writes the SGT in the
- * compact FP16 format to <path>.
+ * compact FP16 format to <path>; AWP_SGT_RWG=<path> also writes the file
+ * SgtHead's reformat_awp would make of the raw file (RWG component order,
+ * with its -z scaling for a Z-force run, igreen 3), in one pass.
  */
 #define _POSIX_C_SOURCE 200809L
 #include <errno.h>
@@ -127,6 +129,7 @@ int main(int argc, char **argv)
         else if (strcmp(st_env, "fp32")) return die("AWP_STORAGE", "expected 'fp32' or 'mixed'");
     }
     const char *fp16_path = getenv("AWP_SGT_FP16");
+    const char *rwg_path = getenv("AWP_SGT_RWG");
     double t0 = now_s();
 
     int *sxyz = (int *)malloc(sizeof(int) * 3 * (size_t)p.nsrc);
@@ -171,6 +174,8 @@ int main(int argc, char **argv)
     if (awp_write_sgt_file(ctx, p.sgtgro)) return die("SGT output", awp_last_error(ctx));
     if (fp16_path && fp16_path[0] && awp_write_sgt_file_fp16(ctx, fp16_path))
         return die("FP16 SGT output", awp_last_error(ctx));
+    if (rwg_path && rwg_path[0] && awp_write_sgt_file_rwg(ctx, rwg_path, p.igreen == 3))
+        return die("RWG SGT output", awp_last_error(ctx));
     double t4 = now_s();
     const double per_step = (t3 - tl) / p.nt;
     printf(" inialization time = %10.3f sec\n", t2 - t0);

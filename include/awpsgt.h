@@ -165,6 +165,15 @@ int awp_get_sgt(awp_ctx *ctx, float *host, size_t count);
  * No file is created when no buffer fills.  The file is truncated first. */
 int awp_write_sgt_file(awp_ctx *ctx, const char *path);
 
+/* Write, in one pass from the device buffer, the file SgtHead's reformatter
+ * makes of the raw file (`reformat_awp <raw> NT_OUT <out> [-z]`,
+ * SgtHead/src/reformat_awp.c:7-66): RWG component order XX YY ZZ XY XZ YZ
+ * with XX <- YY, YY <- XX, XZ <- -YZ, YZ <- -XZ, every value first times -2
+ * when zsource != 0 (the Z-force component), complete records of the raw
+ * file only, its holes as zeros.  Replaces awp_write_sgt_file + reformat_awp
+ * (reformat_awp_mpi.c:91-122) in a CyberShake SGT workflow. */
+int awp_write_sgt_file_rwg(awp_ctx *ctx, const char *path, int zsource);
+
 /* Write the SGT in the compact FP16 format (no counterpart in the
  * reference; a storage option for the raw SGT, decoded by awp_sgt16_read):
  *   bytes 0-7 "AWPSGT16"; u32 version=1, npts, ncomp=6, nsamp, NTISKP_SGT;
