@@ -81,7 +81,7 @@ EXPORTS = [
     "awp_in3d_read", "awp_in3d_validate", "awp_source_read", "awp_cordfile_count",
     "awp_cordfile_read", "awp_sgt_select", "awp_sgt_file_geometry", "awp_create",
     "awp_destroy", "awp_last_error", "awp_set_media", "awp_set_media_device",
-    "awp_set_source", "awp_set_sgt_points", "awp_init", "awp_step", "awp_synchronize",
+    "awp_set_media_rwg", "awp_set_source", "awp_set_sgt_points", "awp_init", "awp_step", "awp_synchronize",
     "awp_progress", "awp_get_sgt", "awp_write_sgt_file", "awp_write_sgt_file_fp16",
     "awp_sgt16_read", "awp_get_field", "awp_set_timing", "awp_get_timing", "awp_device_bytes",
     "awp_write_sgt_file_rwg", "awp_estimate_bytes", "awp_batch_create", "awp_batch_tune", "awp_batch_step",
@@ -113,6 +113,7 @@ def lib() -> ctypes.CDLL:
         "awp_last_error": ([vp], ctypes.c_char_p),
         "awp_set_media": ([vp, vp], ip),
         "awp_set_media_device": ([vp, vp], ip),
+        "awp_set_media_rwg": ([vp, vp, vp, vp], ip),
         "awp_set_source": ([vp, ip, vp, vp], ip),
         "awp_set_sgt_points": ([vp, ip, vp, ctypes.POINTER(ctypes.c_int)], ip),
         "awp_init": ([vp], ip),
@@ -242,6 +243,14 @@ class Run:
         if m.size != self.p.nx * self.p.ny * self.p.nz * self.p.nvar:
             raise ValueError("media has %d values, IN3D needs %d" % (m.size, self.p.nx * self.p.ny * self.p.nz * self.p.nvar))
         self._do(lib().awp_set_media(self.h, m.ctypes.data), "awp_set_media")
+
+    def set_media_rwg(self, vp_km: np.ndarray, vs_km: np.ndarray, rho_g: np.ndarray) -> None:
+        """Media from RWG mesh arrays [ny_rwg][nz][nx_rwg] (= AWP [nx][nz][ny])."""
+        n = self.p.nx * self.p.ny * self.p.nz
+        arrs = [np.ascontiguousarray(a, dtype=np.float32) for a in (vp_km, vs_km, rho_g)]
+        if any(a.size != n for a in arrs):
+            raise ValueError("RWG arrays must hold %d values each" % n)
+        self._do(lib().awp_set_media_rwg(self.h, *[a.ctypes.data for a in arrs]), "awp_set_media_rwg")
 
     def set_media_device(self, ptr: int) -> None:
         self._do(lib().awp_set_media_device(self.h, ctypes.c_void_p(ptr)), "awp_set_media_device")

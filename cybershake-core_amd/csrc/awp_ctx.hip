@@ -347,6 +347,31 @@ extern "C" int awp_set_media(awp_ctx *ctx, const float *media_host)
     return rc;
 }
 
+extern "C" int awp_set_media_rwg(awp_ctx *ctx, const float *vp_km, const float *vs_km, const float *rho_g)
+{
+    if (!ctx || !vp_km || !vs_km || !rho_g) return fail(ctx, AWP_EINVAL, "null argument");
+    if (ctx->p.nvar != 3) return fail(ctx, AWP_EINVAL, "RWG meshes carry vp, vs, rho: IN3D needs NVAR=3");
+    HIPTRY(ctx, hipSetDevice(ctx->device));
+    const size_t n = (size_t)ctx->p.nx * ctx->p.ny * ctx->p.nz;
+    float *buf = nullptr;
+    hipError_t e = hipMalloc((void **)&buf, 6 * n * sizeof(float));
+    if (e != hipSuccess)
+        return fail(ctx, AWP_ENOMEM, "hipMalloc(%zu) for the RWG mesh: %s", 24 * n, hipGetErrorString(e));
+    const float *src[3] = {vp_km, vs_km, rho_g};
+    for (int a = 0; a < 3 && e == hipSuccess; ++a)
+        e = hipMemcpy(buf + a * n, src[a], n * sizeof(float), hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = launch_rwg_media(buf, buf + n, buf + 2 * n, ctx->p.nx, ctx->p.ny, ctx->p.nz, buf + 3 * n, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) {
+        hipFree(buf);
+        return fail(ctx, AWP_EHIP, "RWG mesh: %s", hipGetErrorString(e));
+    }
+    int rc = media_from_device(ctx, buf + 3 * n);
+    hipFree(buf);
+    return rc;
+}
+
 extern "C" int awp_set_media_device(awp_ctx *ctx, const float *media_dev)
 {
     if (!ctx || !media_dev) return fail(ctx, AWP_EINVAL, "null argument");

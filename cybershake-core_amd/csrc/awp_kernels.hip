@@ -610,6 +610,46 @@ hipError_t launch_sgt_rwg(const float *sgt, const int *keep, int g0, int nrec, i
     return hipGetLastError();
 }
 
+// SgtHead/src/reformat_velocity.c:62-72 on the device: the RWG arrays
+// (x fastest, then z, then y; km/s, g/cm^3) are, in AWP terms, [X][Z][Y]
+// with AWP y fastest; the AWP media file is [Z][Y][X][3] (vp vs rho, m/s,
+// kg/m^3).  32x32 (x, y) tiles of one z plane go through LDS so both the
+// reads (along y) and the writes (along x) are contiguous.
+__global__ void __launch_bounds__(256)
+k_rwg_media(const float *p, const float *s, const float *d, const int NX, const int NY, const int NZ, float *out)
+{
+    __shared__ float tile[3][32][33];
+    const int x0 = blockIdx.x * 32, y0 = blockIdx.y * 32, k = blockIdx.z;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;   // 32 x 8
+    for (int r = ty; r < 32; r += 8) {
+        const int x = x0 + r, y = y0 + tx;
+        if (x < NX && y < NY) {
+            const size_t q = ((size_t)x * NZ + k) * NY + y;
+            tile[0][r][tx] = p[q];
+            tile[1][r][tx] = s[q];
+            tile[2][r][tx] = d[q];
+        }
+    }
+    __syncthreads();
+    for (int r = ty; r < 32; r += 8) {
+        const int y = y0 + r, x = x0 + tx;
+        if (x < NX && y < NY) {
+            float *o = out + (((size_t)k * NY + y) * NX + x) * 3;
+            o[0] = tile[0][tx][r] * 1000.0f;
+            o[1] = tile[1][tx][r] * 1000.0f;
+            o[2] = tile[2][tx][r] * 1000.0f;
+        }
+    }
+}
+
+hipError_t launch_rwg_media(const float *p, const float *s, const float *d, int NX, int NY, int NZ, float *out,
+                            hipStream_t st)
+{
+    dim3 grd((NX + 31) / 32, (NY + 31) / 32, NZ);
+    hipLaunchKernelGGL(k_rwg_media, grd, dim3(256), 0, st, p, s, d, NX, NY, NZ, out);
+    return hipGetLastError();
+}
+
 __global__ void __launch_bounds__(256) k_to_half(const float *src, _Float16 *dst, const float scale, const long long n)
 {
     for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x)

@@ -205,6 +205,9 @@ struct AuxJob {
 };
 // the source and SGT work of one step for every run, then the counters
 hipError_t launch_aux_batch(const AuxJob *d_jobs, int njobs, int max_kept, int par, hipStream_t st);
+// RWG mesh arrays (AWP [X][Z][Y], y fastest) -> AWP media file [Z][Y][X][3] x 1000
+hipError_t launch_rwg_media(const float *p, const float *s, const float *d, int NX, int NY, int NZ, float *out,
+                            hipStream_t st);
 // raw-file records g0 .. g0+nrec-1 in SgtHead's RWG order and signs
 // (keep[nrec] marks the points the reference writes; navail samples filled)
 hipError_t launch_sgt_rwg(const float *sgt, const int *keep, int g0, int nrec, int nsamp_max, int nt_out,

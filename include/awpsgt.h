@@ -130,6 +130,12 @@ const char *awp_last_error(const awp_ctx *ctx);
  * relaxation times (relax.f:4-39) and the sponge (cerjan.f:4-247) are set up
  * on the device.  _device takes a device pointer. */
 int awp_set_media(awp_ctx *ctx, const float *media_host);
+/* The same from an RWG velocity mesh (the .p/.s/.d files, each float32,
+ * x fastest, then z, then y, km/s and g/cm^3, on the RWG grid whose x is
+ * the AWP y): the conversion SgtHead's reformat_velocity makes into the AWP
+ * media file (SgtHead/src/reformat_velocity.c:62-72, x1000 to m/s, kg/m^3)
+ * runs on the device, no media file is written.  Needs NVAR = 3. */
+int awp_set_media_rwg(awp_ctx *ctx, const float *vp_km, const float *vs_km, const float *rho_g);
 int awp_set_media_device(awp_ctx *ctx, const float *media_dev);
 
 /* Point sources: xyz [nsrc][3] file coordinates, stf [nsrc][nst][6]

@@ -5,6 +5,8 @@ the reference tool itself (oracle/_ref/reformat_awp)."""
 import os
 import subprocess
 
+import numpy as np
+
 import pytest
 
 from conftest import REPO
@@ -35,3 +37,27 @@ def test_rwg_file_equals_reformatted_raw_file(case_dirs, name, z, tmp_path):
         subprocess.run([TOOL, raw, str(nt_out), out] + (["-z"] if z else []), check=True,
                        stdout=subprocess.DEVNULL)
         assert mine == open(out, "rb").read()
+
+
+def test_media_from_rwg_mesh_equals_reformatted_file(case_dirs):
+    """awp_set_media_rwg (device reformat of the RWG mesh) gives the media
+    awp_set_media gives from the file SgtHead's reformat_velocity writes
+    (its restatement is pinned to the reference tool in test_builders.py)."""
+    from awpsgt.builders import reformat_velocity_arrays
+    d, _ = case_dirs["g1_fx_layered"]
+    p, media, xyz, stf, pts = _inputs(d)
+    assert p.nvar == 3
+    rng = np.random.default_rng(8)
+    shape = (p.nx, p.nz, p.ny)             # RWG [y][z][x] = AWP [x][z][y]
+    vs = rng.uniform(0.4, 3.8, size=shape).astype(np.float32)
+    vp = (vs * rng.uniform(1.6, 2.0, size=shape)).astype(np.float32)
+    rho = rng.uniform(1.8, 2.9, size=shape).astype(np.float32)
+    awp = reformat_velocity_arrays(vp, vs, rho)      # [nz][AWP y][AWP x][3]
+    r1 = awpsgt.Run(p)
+    r1.set_media_rwg(vp, vs, rho)
+    r2 = awpsgt.Run(p)
+    r2.set_media(awp)
+    for f in ("d1", "mu", "lam", "qp", "qs"):
+        assert np.array_equal(r1.field(f), r2.field(f)), f
+    r1.close()
+    r2.close()
