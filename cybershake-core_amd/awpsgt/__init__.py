@@ -83,7 +83,7 @@ EXPORTS = [
     "awp_destroy", "awp_last_error", "awp_set_media", "awp_set_media_device",
     "awp_set_media_rwg", "awp_set_source", "awp_set_sgt_points", "awp_init", "awp_step", "awp_synchronize",
     "awp_progress", "awp_get_sgt", "awp_write_sgt_file", "awp_write_sgt_file_fp16",
-    "awp_sgt16_read", "awp_get_field", "awp_set_timing", "awp_get_timing", "awp_device_bytes",
+    "awp_sgt16_read", "awp_sgt16_read_points", "awp_get_field", "awp_set_timing", "awp_get_timing", "awp_device_bytes",
     "awp_write_sgt_file_rwg", "awp_estimate_bytes", "awp_batch_create", "awp_batch_tune", "awp_batch_step",
     "awp_batch_synchronize", "awp_batch_time", "awp_batch_get_launch", "awp_batch_last_error",
     "awp_batch_destroy",
@@ -125,6 +125,7 @@ def lib() -> ctypes.CDLL:
         "awp_write_sgt_file_fp16": ([vp, cp], ip),
         "awp_write_sgt_file_rwg": ([vp, cp, ip], ip),
         "awp_sgt16_read": ([cp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), vp, sz, cp, sz], ip),
+        "awp_sgt16_read_points": ([cp, ip, ip, ip, vp, sz, cp, sz], ip),
         "awp_get_field": ([vp, ip, ip, vp], ip),
         "awp_set_timing": ([vp, ip], ip),
         "awp_get_timing": ([vp, vp], ip),
@@ -210,6 +211,17 @@ def read_sgt16(path: str) -> np.ndarray:
     out = np.empty((n.value, 6, t.value), dtype=np.float32)
     _check(lib().awp_sgt16_read(path.encode(), None, None, out.ctypes.data, out.size, err, 512),
            err.value.decode())
+    return out
+
+
+def read_sgt16_points(path: str, first: int, count: int, nt_out: int, layout: int = 0) -> np.ndarray:
+    """Points [first, first+count) of an FP16 SGT file as float32
+    [count][6][nt_out]; layout 0 AWP raw records, 1 RWG (reformat_awp),
+    2 RWG for the Z source."""
+    out = np.empty((count, 6, nt_out), dtype=np.float32)
+    err = ctypes.create_string_buffer(512)
+    _check(lib().awp_sgt16_read_points(path.encode(), int(first), int(count), int(layout), out.ctypes.data,
+                                       out.size, err, 512), err.value.decode())
     return out
 
 

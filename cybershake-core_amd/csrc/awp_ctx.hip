@@ -813,6 +813,8 @@ extern "C" int awp_write_sgt_file_fp16(awp_ctx *ctx, const char *path)
     const uint32_t ntk = (uint32_t)p.ntiskp_sgt;
     memcpy(hdr + 24, &ntk, 4);
     memcpy(hdr + 28, &p.dt, 4);
+    const uint32_t nto = (uint32_t)nt_out;   // the raw file's series length
+    memcpy(hdr + 32, &nto, 4);
     FILE *f = fopen(path, "wb");
     if (!f) return fail(ctx, AWP_EIO, "cannot create '%s'", path);
     bool ok = fwrite(hdr, 1, 64, f) == 64;

@@ -401,6 +401,69 @@ int awp_sgt16_read(const char *path, int *npts, int *nsamp, float *out, size_t c
     return rc;
 }
 
+int awp_sgt16_read_points(const char *path, int first, int count, int layout, float *out, size_t n, char *err,
+                          size_t errlen)
+{
+    if (!path || first < 0 || count < 0 || layout < 0 || layout > 2 || (count > 0 && !out))
+        return seterr(err, errlen, AWP_EINVAL, "bad arguments");
+    FILE *f = fopen(path, "rb");
+    if (!f) return seterr(err, errlen, AWP_EIO, "cannot open '%s': %s", path, strerror(errno));
+    unsigned char hdr[64];
+    uint32_t h32[4], nto = 0;
+    if (fread(hdr, 1, 64, f) != 64 || memcmp(hdr, "AWPSGT16", 8) != 0) {
+        fclose(f);
+        return seterr(err, errlen, AWP_EINVAL, "'%s' is not an FP16 SGT file", path);
+    }
+    memcpy(h32, hdr + 8, sizeof(h32));
+    memcpy(&nto, hdr + 32, 4);
+    const size_t np = h32[1], ns = h32[3];
+    const size_t nt = nto ? nto : ns;   /* series length of the raw file */
+    if (h32[0] != 1u || h32[2] != 6u || nt < ns) {
+        fclose(f);
+        return seterr(err, errlen, AWP_EINVAL, "'%s': unsupported version or geometry", path);
+    }
+    if ((size_t)first + (size_t)count > np) {
+        fclose(f);
+        return seterr(err, errlen, AWP_EINVAL, "points %d..%d beyond the file's %zu", first, first + count - 1, np);
+    }
+    if (n < (size_t)count * 6 * nt) {
+        fclose(f);
+        return seterr(err, errlen, AWP_EINVAL, "buffer holds %zu values, need %zu", n, (size_t)count * 6 * nt);
+    }
+    float sc[6];
+    uint16_t *row = (uint16_t *)malloc(sizeof(uint16_t) * 6 * (ns ? ns : 1));
+    float *v = (float *)malloc(sizeof(float) * 6 * nt);
+    int rc = row && v ? AWP_OK : seterr(err, errlen, AWP_ENOMEM, "out of host memory");
+    /* input component of each output component: AWP order as is, or the
+     * RWG order of SgtHead/src/reformat_awp.c:46-61 */
+    static const int from_rwg[6] = {1, 0, 2, 3, 5, 4};
+    for (int k = 0; k < count && rc == AWP_OK; ++k) {
+        const size_t g = (size_t)first + k;
+        if (fseeko(f, (off_t)(64 + g * 6 * 4), SEEK_SET) || fread(sc, sizeof(float), 6, f) != 6 ||
+            fseeko(f, (off_t)(64 + np * 6 * 4 + g * 6 * ns * 2), SEEK_SET) ||
+            (ns && fread(row, sizeof(uint16_t), 6 * ns, f) != 6 * ns)) {
+            rc = seterr(err, errlen, AWP_EIO, "'%s': truncated", path);
+            break;
+        }
+        for (int c = 0; c < 6; ++c)
+            for (size_t t = 0; t < nt; ++t) {
+                float x = t < ns ? half_to_float(row[c * ns + t]) * sc[c] : 0.0f;
+                if (layout == 2) x = (float)(-2.0 * x);   /* reformat_awp.c:40-44 */
+                v[c * nt + t] = x;
+            }
+        float *o = out + (size_t)k * 6 * nt;
+        for (int c = 0; c < 6; ++c) {
+            const int ci = layout ? from_rwg[c] : c;
+            const int neg = layout && c >= 4;
+            for (size_t t = 0; t < nt; ++t) o[c * nt + t] = neg ? (float)(-1.0 * v[ci * nt + t]) : v[ci * nt + t];
+        }
+    }
+    free(row);
+    free(v);
+    fclose(f);
+    return rc;
+}
+
 int awp_sgt_file_geometry(const awp_in3d *p, int *nt_out, int *nflush)
 {
     if (!p || p->write_step < 1 || p->ntiskp_sgt < 1) return AWP_EINVAL;

@@ -183,7 +183,8 @@ int awp_write_sgt_file_rwg(awp_ctx *ctx, const char *path, int zsource);
 /* Write the SGT in the compact FP16 format (no counterpart in the
  * reference; a storage option for the raw SGT, decoded by awp_sgt16_read):
  *   bytes 0-7 "AWPSGT16"; u32 version=1, npts, ncomp=6, nsamp, NTISKP_SGT;
- *   f32 DT; zero padding to byte 64; f32 scale[npts][6]; then
+ *   f32 DT; u32 NT_OUT (the raw file's series length, >= nsamp); zero
+ *   padding to byte 64; f32 scale[npts][6]; then
  *   FP16 data[npts][6][nsamp], value = data * scale.
  * nsamp is the number of samples the raw file holds (full WRITE_STEP
  * buffers); scale = 2^(e-15) per series, where the series peak is
@@ -197,6 +198,16 @@ int awp_write_sgt_file_fp16(awp_ctx *ctx, const char *path);
  * [npts][6][nsamp] (out may be NULL to query the dimensions only). */
 int awp_sgt16_read(const char *path, int *npts, int *nsamp, float *out, size_t count,
                    char *err, size_t errlen);
+/* Points first .. first+count-1 of an FP16 SGT file decoded to float32
+ * [count][6][NT_OUT] (samples the raw file leaves empty are zero) in the
+ * layout a consumer of the FP32 files reads: layout 0 the raw AWP file's
+ * records; 1 the RWG records SgtHead's reformat_awp writes
+ * (SgtHead/src/reformat_awp.c:38-62), 2 the same for the Z source (-z).
+ * Layouts 1/2 give DirectSynth's reader the slice readSGT_MPI takes from the
+ * reformatted file (DirectSynth/src/sgt_handler.c:282-333), to FP16
+ * rounding. */
+int awp_sgt16_read_points(const char *path, int first, int count, int layout, float *out, size_t n,
+                          char *err, size_t errlen);
 
 /* Copy one interior field to the host as [k][j][i], x fastest, k = 1 at the
  * bottom; kext = 2 also copies the two free-surface image planes.  which:

@@ -142,8 +142,10 @@ def test_sgt_fp16_file(case_dirs, name, tmp_path):
     assert raw[:8] == b"AWPSGT16"
     version, npts, ncomp, nsamp, ntiskp = struct.unpack("<5I", raw[8:28])
     (dt,) = struct.unpack("<f", raw[28:32])
+    (nto,) = struct.unpack("<I", raw[32:36])
     # the samples the raw file holds: full WRITE_STEP buffers only
-    _, nflush = awpsgt.sgt_file_geometry(p)
+    nt_out, nflush = awpsgt.sgt_file_geometry(p)
+    assert nto == nt_out
     assert (version, npts, ncomp, nsamp, ntiskp) == (1, len(pts), 6, nflush * p.write_step, p.ntiskp_sgt)
     assert dt == np.float32(p.dt)
     assert len(raw) == 64 + npts * 6 * 4 + npts * 6 * nsamp * 2
@@ -172,3 +174,13 @@ def test_sgt_fp16_file(case_dirs, name, tmp_path):
     assert np.array_equal(dec[keep], ed.astype(np.float32) * es[..., None])
     if name == "g7_random_edges":
         assert not keep.all()    # the case holds points the rule drops
+    # DirectSynth's view: RWG records decoded from the FP16 file match the
+    # reformatted FP32 raw file to FP16 rounding
+    from oracle.sgthead import reformat_awp
+    rawf = os.path.join(str(tmp_path), "raw")
+    with _run(p, media, xyz, stf, pts, awpsgt.AWP_STORAGE_FP32) as r2:
+        r2.write_sgt_file(rawf)
+    rwg = np.frombuffer(reformat_awp(open(rawf, "rb").read(), nt_out), "<f4").reshape(-1, 6, nt_out)
+    dec = awpsgt.read_sgt16_points(path, 0, rwg.shape[0], nt_out, 1)
+    pk = np.max(np.abs(rwg), axis=-1, keepdims=True).astype(np.float64)
+    assert np.all(np.abs(dec.astype(np.float64) - rwg) <= pk * 2.0 ** -11 + 2.0 ** -149)

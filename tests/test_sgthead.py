@@ -50,3 +50,40 @@ def test_reformat_closed_form():
     outz = np.frombuffer(reformat_awp(raw, ts, True), dtype="<f4").reshape(-1, 6, ts)
     assert np.array_equal(outz[0, 0], -2 * rec[1]) and np.array_equal(outz[0, 4], 2 * rec[5])
     assert not np.signbit(outz[0, 5, 0])                          # -(-2 * +0) = +0
+
+
+def _fp16_file(path, scale, data, nt_out, ntiskp=10, dt=0.01):
+    npts, _, nsamp = data.shape
+    hdr = bytearray(64)
+    hdr[0:8] = b"AWPSGT16"
+    hdr[8:28] = np.array([1, npts, 6, nsamp, ntiskp], "<u4").tobytes()
+    hdr[28:32] = np.array([dt], "<f4").tobytes()
+    hdr[32:36] = np.array([nt_out], "<u4").tobytes()
+    with open(path, "wb") as f:
+        f.write(bytes(hdr))
+        f.write(np.asarray(scale, "<f4").tobytes())
+        f.write(np.asarray(data, "<f2").tobytes())
+
+
+@pytest.mark.parametrize("layout", [0, 1, 2])
+def test_fp16_point_reader_layouts(layout, tmp_path):
+    """Decoding a range of points of the FP16 format gives the raw file's
+    records (zero past the stored samples, zero for a dropped point) or, in
+    the RWG layouts, those records as reformat_awp rewrites them."""
+    import awpsgt
+    rng = np.random.default_rng(5)
+    npts, nsamp, nt_out = 6, 7, 9
+    data = rng.uniform(-3e4, 3e4, size=(npts, 6, nsamp)).astype(np.float16)
+    scale = np.ldexp(np.float32(1), rng.integers(-40, -20, size=(npts, 6))).astype(np.float32)
+    scale[3] = 0.0                     # a point the rule dropped
+    data[3] = 0
+    path = os.path.join(str(tmp_path), "s16")
+    _fp16_file(path, scale, data, nt_out)
+    raw = np.zeros((npts, 6, nt_out), np.float32)
+    raw[:, :, :nsamp] = data.astype(np.float32) * scale[:, :, None]
+    want = raw if layout == 0 else np.frombuffer(
+        reformat_awp(raw.tobytes(), nt_out, layout == 2), "<f4").reshape(npts, 6, nt_out)
+    got = awpsgt.read_sgt16_points(path, 1, 4, nt_out, layout)
+    assert got.tobytes() == want[1:5].tobytes()
+    with pytest.raises(awpsgt.AwpError, match="beyond"):
+        awpsgt.read_sgt16_points(path, 4, 3, nt_out, layout)
