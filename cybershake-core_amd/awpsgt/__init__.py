@@ -84,7 +84,7 @@ EXPORTS = [
     "awp_set_media_rwg", "awp_set_source", "awp_set_sgt_points", "awp_init", "awp_step", "awp_synchronize",
     "awp_progress", "awp_get_sgt", "awp_write_sgt_file", "awp_write_sgt_file_fp16",
     "awp_sgt16_read", "awp_sgt16_read_points", "awp_get_field", "awp_set_timing", "awp_get_timing", "awp_device_bytes",
-    "awp_write_sgt_file_rwg", "awp_estimate_bytes", "awp_batch_create", "awp_batch_tune", "awp_batch_step",
+    "awp_write_sgt_file_rwg", "awp_device_memory", "awp_estimate_bytes", "awp_batch_create", "awp_batch_tune", "awp_batch_step",
     "awp_batch_synchronize", "awp_batch_time", "awp_batch_get_launch", "awp_batch_last_error",
     "awp_batch_destroy",
 ]
@@ -130,6 +130,7 @@ def lib() -> ctypes.CDLL:
         "awp_set_timing": ([vp, ip], ip),
         "awp_get_timing": ([vp, vp], ip),
         "awp_device_bytes": ([vp, ctypes.POINTER(ctypes.c_size_t)], ip),
+        "awp_device_memory": ([ip, ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t)], ip),
         "awp_estimate_bytes": ([P, ip, ip, ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t)], ip),
         "awp_batch_create": ([ctypes.POINTER(vp), vp, ip, cp, sz], ip),
         "awp_batch_tune": ([vp, ip], ip),

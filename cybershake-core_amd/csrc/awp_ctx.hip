@@ -635,6 +635,15 @@ extern "C" int awp_estimate_bytes(const awp_in3d *p, int storage, int npts, size
     return AWP_OK;
 }
 
+extern "C" int awp_device_memory(int device, size_t *free_bytes, size_t *total_bytes)
+{
+    size_t f = 0, t = 0;
+    if (hipSetDevice(device) != hipSuccess || hipMemGetInfo(&f, &t) != hipSuccess) return AWP_EHIP;
+    if (free_bytes) *free_bytes = f;
+    if (total_bytes) *total_bytes = t;
+    return AWP_OK;
+}
+
 extern "C" int awp_device_bytes(const awp_ctx *ctx, size_t *bytes)
 {
     if (!ctx || !bytes) return AWP_EINVAL;

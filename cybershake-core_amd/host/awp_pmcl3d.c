@@ -7,12 +7,25 @@
  * the reference writes (SGTGRO), plus the CHKP parameter summary
  * (io.f:481-537).
  *
+ *   awp_pmcl3d [IN3D] [src] [media] [chkp] [chkj]
+ *   awp_pmcl3d --batch IN3D_1 [IN3D_2 ...]
+ *
+ * The batch form runs several SGT components (e.g. a site's X and Y runs,
+ * each with its own IN3D as the reference's workflow writes them) in one
+ * process: runs are admitted in queue order while their device memory
+ * (awp_estimate_bytes) fits the budget, each admitted wave is advanced as
+ * one batch (awp_batch_*), and every run writes its own outputs.  The
+ * reference runs one component per MPI job (utils/awp_odc_wrapper.sh:14).
+ *
  * Environment: AWP_DEVICE selects the HIP device (default 0);
  * AWP_STORAGE=mixed selects mixed storage (BF16 memory variables, FP16
  * attenuation weights); AWP_SGT_FP16=<path> also writes the SGT in the
  * compact FP16 format to <path>; AWP_SGT_RWG=<path> also writes the file
  * SgtHead's reformat_awp would make of the raw file (RWG component order,
- * with its -z scaling for a Z-force run, igreen 3), in one pass.
+ * with its -z scaling for a Z-force run, igreen 3), in one pass.  In batch
+ * form the two paths get ".<n>" appended (n = position in the list);
+ * AWP_HBM_BUDGET_GB caps the device memory a wave may use (default: the free
+ * memory less 2 GiB) and AWP_TUNE=1 runs the batch tuner first.
  */
 #define _POSIX_C_SOURCE 200809L
 #include <errno.h>
@@ -107,19 +120,143 @@ static void write_chkp(const awp_in3d *p, const float *media)
     fclose(f);
 }
 
+typedef struct {
+    awp_in3d p;
+    int npts;
+    size_t steady, peak;
+    awp_ctx *ctx;
+} job_t;
+
+/* Reads one run's inputs and sets up its context (media, source, points,
+ * init).  The IN3D is read and validated by the caller. */
+static int setup_run(job_t *j, int device, int storage, char *err, size_t errlen)
+{
+    awp_in3d *p = &j->p;
+    int rc = 1;
+    int *sxyz = (int *)malloc(sizeof(int) * 3 * (size_t)p->nsrc);
+    float *stf = (float *)malloc(sizeof(float) * 6 * (size_t)p->nsrc * p->nst);
+    int *pts = (int *)malloc(sizeof(int) * 3 * (size_t)(j->npts > 0 ? j->npts : 1));
+    float *media = NULL;
+    if (!sxyz || !stf || !pts) {
+        snprintf(err, errlen, "out of host memory");
+        goto done;
+    }
+    if (awp_source_read(p->insrc, p->nsrc, p->nst, sxyz, stf, err, errlen)) goto done;
+    if (awp_cordfile_read(p->insgt, j->npts, pts, err, errlen)) goto done;
+    media = read_media(p, err, errlen);
+    if (!media) goto done;
+    if (awp_create(&j->ctx, p, device, storage, err, errlen)) goto done;
+    int nkept = 0;
+    if (awp_set_sgt_points(j->ctx, j->npts, pts, &nkept) || awp_set_media(j->ctx, media) ||
+        awp_set_source(j->ctx, p->nsrc, sxyz, stf) || awp_init(j->ctx)) {
+        snprintf(err, errlen, "%s", awp_last_error(j->ctx));
+        goto done;
+    }
+    write_chkp(p, media);
+    printf(" %s: SGT points kept %d of %d; %d time steps\n", p->sgtgro, nkept, j->npts, p->nt);
+    rc = 0;
+done:
+    free(media);
+    free(sxyz);
+    free(stf);
+    free(pts);
+    return rc;
+}
+
+static int write_outputs(job_t *j, const char *fp16_path, const char *rwg_path)
+{
+    if (awp_write_sgt_file(j->ctx, j->p.sgtgro)) return die("SGT output", awp_last_error(j->ctx));
+    if (fp16_path && fp16_path[0] && awp_write_sgt_file_fp16(j->ctx, fp16_path))
+        return die("FP16 SGT output", awp_last_error(j->ctx));
+    if (rwg_path && rwg_path[0] && awp_write_sgt_file_rwg(j->ctx, rwg_path, j->p.igreen == 3))
+        return die("RWG SGT output", awp_last_error(j->ctx));
+    return 0;
+}
+
+static int run_batch(int n, char **in3d, int device, int storage)
+{
+    char err[512];
+    job_t *jobs = (job_t *)calloc((size_t)n, sizeof(job_t));
+    if (!jobs) return die("batch", "out of memory");
+    for (int k = 0; k < n; ++k) {
+        if (awp_in3d_read(in3d[k], &jobs[k].p, err, sizeof(err)) ||
+            awp_in3d_validate(&jobs[k].p, err, sizeof(err)))
+            return die(in3d[k], err);
+        if (awp_cordfile_count(jobs[k].p.insgt, &jobs[k].npts, err, sizeof(err))) return die(in3d[k], err);
+        if (awp_estimate_bytes(&jobs[k].p, storage, jobs[k].npts, &jobs[k].steady, &jobs[k].peak))
+            return die(in3d[k], "bad parameters");
+    }
+    size_t free_b = 0, total_b = 0;
+    if (awp_device_memory(device, &free_b, &total_b)) return die("device", "cannot query memory");
+    size_t budget = free_b > ((size_t)2 << 30) ? free_b - ((size_t)2 << 30) : free_b;
+    const char *bud = getenv("AWP_HBM_BUDGET_GB");
+    if (bud && atof(bud) > 0) budget = (size_t)(atof(bud) * 1e9);
+    const char *fp16 = getenv("AWP_SGT_FP16"), *rwg = getenv("AWP_SGT_RWG");
+    const int tune = getenv("AWP_TUNE") && atoi(getenv("AWP_TUNE")) == 1;
+    enum { MAXW = 256 };
+    awp_ctx *ctxs[MAXW];
+    int k0 = 0, wave = 0;
+    while (k0 < n) {
+        /* admission in queue order: steady bytes plus the largest set-up
+         * transient of the wave within the budget */
+        size_t st = 0, tr = 0;
+        int k1 = k0;
+        while (k1 < n && k1 - k0 < MAXW) {
+            const size_t t = jobs[k1].peak - jobs[k1].steady;
+            if (k1 == k0 && jobs[k1].peak > budget) return die(in3d[k1], "needs more device memory than the budget");
+            if (k1 > k0 && st + jobs[k1].steady + (t > tr ? t : tr) > budget) break;
+            st += jobs[k1].steady;
+            if (t > tr) tr = t;
+            ++k1;
+        }
+        printf(" wave %d: runs %d..%d, %.1f GB of a %.1f GB budget\n", wave, k0, k1 - 1, st / 1e9, budget / 1e9);
+        int minnt = 0x7fffffff;
+        for (int k = k0; k < k1; ++k) {
+            if (setup_run(&jobs[k], device, storage, err, sizeof(err))) return die(in3d[k], err);
+            ctxs[k - k0] = jobs[k].ctx;
+            if (jobs[k].p.nt < minnt) minnt = jobs[k].p.nt;
+        }
+        awp_batch *b = NULL;
+        if (awp_batch_create(&b, ctxs, k1 - k0, err, sizeof(err))) return die("batch", err);
+        if (tune && awp_batch_tune(b, 2)) return die("batch tune", awp_batch_last_error(b));
+        const double t0 = now_s();
+        for (int done = 0; done < minnt;) {
+            const int m = minnt - done < 100 ? minnt - done : 100;
+            if (awp_batch_step(b, m)) return die("batch step", awp_batch_last_error(b));
+            done += m;
+            printf(" TIME STEP %d OF %d\n", done, minnt);
+            fflush(stdout);
+        }
+        const double t1 = now_s();
+        awp_batch_destroy(b);
+        double pts = 0;
+        for (int k = k0; k < k1; ++k) {
+            /* runs longer than the shortest finish on their own */
+            if (jobs[k].p.nt > minnt && awp_step(jobs[k].ctx, jobs[k].p.nt - minnt))
+                return die("awp_step", awp_last_error(jobs[k].ctx));
+            if (awp_synchronize(jobs[k].ctx)) return die("awp_step", awp_last_error(jobs[k].ctx));
+            char f16[300] = "", rw[300] = "";
+            if (fp16 && fp16[0]) snprintf(f16, sizeof f16, "%s.%d", fp16, k);
+            if (rwg && rwg[0]) snprintf(rw, sizeof rw, "%s.%d", rwg, k);
+            if (write_outputs(&jobs[k], f16, rw)) return 1;
+            pts += (double)jobs[k].p.nx * jobs[k].p.ny * jobs[k].p.nz;
+            awp_destroy(jobs[k].ctx);
+            jobs[k].ctx = NULL;
+        }
+        if (minnt > 0) {
+            printf(" computing time per time step = %12.6f sec (wave of %d runs)\n", (t1 - t0) / minnt, k1 - k0);
+            printf(" point-updates per second = %.4g\n", pts * minnt / (t1 - t0));
+        }
+        k0 = k1;
+        ++wave;
+    }
+    free(jobs);
+    return 0;
+}
+
 int main(int argc, char **argv)
 {
     char err[512];
-    const char *in3d_path = argc >= 2 ? argv[1] : "IN3D";
-    awp_in3d p;
-    if (awp_in3d_read(in3d_path, &p, err, sizeof(err))) return die("IN3D", err);
-    /* command-line overrides of the file names (io.f:444-479) */
-    if (argc >= 3) set_name(p.insrc, argv[2]);
-    if (argc >= 4) set_name(p.invel, argv[3]);
-    if (argc >= 5) set_name(p.chkp, argv[4]);
-    if (argc >= 6) set_name(p.chkj, argv[5]);
-    if (awp_in3d_validate(&p, err, sizeof(err))) return die("IN3D", err);
-
     const char *dev_env = getenv("AWP_DEVICE");
     const int device = dev_env ? atoi(dev_env) : 0;
     const char *st_env = getenv("AWP_STORAGE");
@@ -128,64 +265,50 @@ int main(int argc, char **argv)
         if (!strcmp(st_env, "mixed")) storage = AWP_STORAGE_MIXED;
         else if (strcmp(st_env, "fp32")) return die("AWP_STORAGE", "expected 'fp32' or 'mixed'");
     }
+    if (argc >= 2 && !strcmp(argv[1], "--batch")) {
+        if (argc < 3) return die("--batch", "no IN3D files given");
+        return run_batch(argc - 2, argv + 2, device, storage);
+    }
+
+    const char *in3d_path = argc >= 2 ? argv[1] : "IN3D";
+    job_t j;
+    memset(&j, 0, sizeof j);
+    awp_in3d *p = &j.p;
+    if (awp_in3d_read(in3d_path, p, err, sizeof(err))) return die("IN3D", err);
+    /* command-line overrides of the file names (io.f:444-479) */
+    if (argc >= 3) set_name(p->insrc, argv[2]);
+    if (argc >= 4) set_name(p->invel, argv[3]);
+    if (argc >= 5) set_name(p->chkp, argv[4]);
+    if (argc >= 6) set_name(p->chkj, argv[5]);
+    if (awp_in3d_validate(p, err, sizeof(err))) return die("IN3D", err);
     const char *fp16_path = getenv("AWP_SGT_FP16");
     const char *rwg_path = getenv("AWP_SGT_RWG");
+
     double t0 = now_s();
-
-    int *sxyz = (int *)malloc(sizeof(int) * 3 * (size_t)p.nsrc);
-    float *stf = (float *)malloc(sizeof(float) * 6 * (size_t)p.nsrc * p.nst);
-    if (!sxyz || !stf) return die("source", "out of memory");
-    if (awp_source_read(p.insrc, p.nsrc, p.nst, sxyz, stf, err, sizeof(err))) return die("source", err);
-
-    int npts = 0;
-    if (awp_cordfile_count(p.insgt, &npts, err, sizeof(err)))
+    if (awp_cordfile_count(p->insgt, &j.npts, err, sizeof(err)))
         return die("cordfile", err);  /* no station list: SGT mode needs one */
-    int *pts = (int *)malloc(sizeof(int) * 3 * (size_t)(npts > 0 ? npts : 1));
-    if (!pts) return die("cordfile", "out of memory");
-    if (awp_cordfile_read(p.insgt, npts, pts, err, sizeof(err))) return die("cordfile", err);
-    printf(" Use SGT output mode, sgt_numsta= %d\n", npts);
-
-    float *media = read_media(&p, err, sizeof(err));
-    if (!media) return die("media", err);
-    double t1 = now_s();
-
-    awp_ctx *ctx = NULL;
-    if (awp_create(&ctx, &p, device, storage, err, sizeof(err))) return die("awp_create", err);
-    int nkept = 0;
-    if (awp_set_sgt_points(ctx, npts, pts, &nkept) || awp_set_media(ctx, media) ||
-        awp_set_source(ctx, p.nsrc, sxyz, stf) || awp_init(ctx))
-        return die("set-up", awp_last_error(ctx));
-    write_chkp(&p, media);
-    free(media);
+    printf(" Use SGT output mode, sgt_numsta= %d\n", j.npts);
+    if (setup_run(&j, device, storage, err, sizeof(err))) return die("set-up", err);
     double t2 = now_s();
-    printf(" SGT points kept %d of %d; %d time steps\n", nkept, npts, p.nt);
 
     const int chunk = 100;
     double tl = now_s();
-    for (int done = 0; done < p.nt;) {
-        int n = p.nt - done < chunk ? p.nt - done : chunk;
-        if (awp_step(ctx, n)) return die("awp_step", awp_last_error(ctx));
+    for (int done = 0; done < p->nt;) {
+        int n = p->nt - done < chunk ? p->nt - done : chunk;
+        if (awp_step(j.ctx, n)) return die("awp_step", awp_last_error(j.ctx));
         done += n;
-        if (awp_synchronize(ctx)) return die("awp_step", awp_last_error(ctx));
-        printf(" TIME STEP %d OF %d\n", done, p.nt);
+        if (awp_synchronize(j.ctx)) return die("awp_step", awp_last_error(j.ctx));
+        printf(" TIME STEP %d OF %d\n", done, p->nt);
         fflush(stdout);
     }
     double t3 = now_s();
-    if (awp_write_sgt_file(ctx, p.sgtgro)) return die("SGT output", awp_last_error(ctx));
-    if (fp16_path && fp16_path[0] && awp_write_sgt_file_fp16(ctx, fp16_path))
-        return die("FP16 SGT output", awp_last_error(ctx));
-    if (rwg_path && rwg_path[0] && awp_write_sgt_file_rwg(ctx, rwg_path, p.igreen == 3))
-        return die("RWG SGT output", awp_last_error(ctx));
+    if (write_outputs(&j, fp16_path, rwg_path)) return 1;
     double t4 = now_s();
-    const double per_step = (t3 - tl) / p.nt;
+    const double per_step = (t3 - tl) / p->nt;
     printf(" inialization time = %10.3f sec\n", t2 - t0);
-    printf("    read input time = %10.3f sec\n", t1 - t0);
     printf(" computing time per time step = %12.6f sec\n", per_step);
-    printf(" point-updates per second = %.4g\n", (double)p.nx * p.ny * p.nz / per_step);
+    printf(" point-updates per second = %.4g\n", (double)p->nx * p->ny * p->nz / per_step);
     printf(" SGT write time = %10.3f sec\n", t4 - t3);
-    awp_destroy(ctx);
-    free(sxyz);
-    free(stf);
-    free(pts);
+    awp_destroy(j.ctx);
     return 0;
 }

@@ -230,6 +230,9 @@ int awp_device_bytes(const awp_ctx *ctx, size_t *bytes);
  * under a memory budget.  No counterpart in the reference. */
 int awp_estimate_bytes(const awp_in3d *p, int storage, int npts, size_t *steady, size_t *peak);
 
+/* Free and total device memory of a GPU (for admission decisions). */
+int awp_device_memory(int device, size_t *free_bytes, size_t *total_bytes);
+
 /* ---- batches ------------------------------------------------------------
  * Several initialised runs (same device and storage mode, fused step)
  * advanced together: one launch of the fused step kernel per time step over

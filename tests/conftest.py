@@ -53,6 +53,16 @@ def c1_dir(tmp_path_factory):
     return d, cases.write_case(d, spec, media, pts)
 
 
+@pytest.fixture(scope="session")
+def c4_dir(tmp_path_factory):
+    """800x600x200, 1500 steps, 2000 SGT points (the mixed-storage error case)."""
+    spec, media, pts = cases.c4_inputs()
+    d = os.path.join(str(tmp_path_factory.mktemp("c4")), "c4")
+    sha = cases.write_case(d, spec, media, pts)
+    del media
+    return d, sha
+
+
 def rel_l2(a, b):
     a = np.asarray(a, dtype=np.float64)
     b = np.asarray(b, dtype=np.float64)

@@ -123,3 +123,36 @@ def test_estimate_bytes_matches_allocation(case_dirs):
         assert held <= steady <= held + 64 * 1024, (storage, held, steady)
         assert peak > steady
         r.close()
+
+
+def test_host_batch_mode_writes_each_runs_file(case_dirs, tmp_path):
+    """`awp_pmcl3d --batch IN3D_a IN3D_b ...` (one process, one wave, one
+    batch) writes for every run the file the single-run host writes."""
+    import shutil
+    import subprocess
+    from conftest import REPO
+    exe = os.path.join(REPO, "cybershake-core_amd", "bin", "awp_pmcl3d")
+    run = str(tmp_path)
+    names = ["g1_fx_layered", "g2_fy_nvar5_nosocalq", "g3_fz_surface"]
+    in3ds = []
+    for k, name in enumerate(names):
+        sub = os.path.join(run, "r%d" % k)
+        shutil.copytree(case_dirs[name][0], sub)
+        txt = open(os.path.join(sub, "IN3D")).read()
+        for f in ("source", "media", "cordfile", "sgt", "CHKP", "CHKJ"):
+            txt = txt.replace("'%s'" % f, "'r%d/%s'" % (k, f))
+        open(os.path.join(run, "IN3D.%d" % k), "w").write(txt)
+        in3ds.append("IN3D.%d" % k)
+    env = dict(os.environ, AWP_SGT_RWG="rwg")
+    r = subprocess.run([exe, "--batch"] + in3ds, cwd=run, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                       text=True, timeout=300, env=env)
+    assert r.returncode == 0, r.stdout
+    assert "wave 0: runs 0..2" in r.stdout
+    for k, name in enumerate(names):
+        ref = _single(case_dirs[name][0], awpsgt.AWP_STORAGE_FP32)
+        want = os.path.join(run, "want%d" % k)
+        ref.write_sgt_file(want)
+        ref.write_sgt_file_rwg(want + ".rwg", ref.p.igreen == 3)
+        ref.close()
+        assert open(os.path.join(run, "r%d" % k, "sgt"), "rb").read() == open(want, "rb").read(), name
+        assert open(os.path.join(run, "rwg.%d" % k), "rb").read() == open(want + ".rwg", "rb").read(), name
