@@ -41,7 +41,7 @@ sys.path.insert(0, os.path.join(REPO, "cybershake-core_amd"))
 HBM_PEAK_GBS = 8000.0
 # algorithmic bytes per point-update of the two-stage path: velocity stage
 # 13 FP32 arrays (52 B); stress stage read u,v,w (12), read+write 6 stresses
-# (48), read+write r1..r6 (48 FP32 / 24 BF16), read lam, mu (8) and qp, qs
+# (48), read+write r1..r6 (48 FP32 / 24 FP16), read lam, mu (8) and qp, qs
 # (8 FP32 / 4 FP16)
 BYTES_UPDATE = {"fp32": 176, "mixed": 148}
 BYTES_STR = {"fp32": 124, "mixed": 96}     # the two-stage stress kernel alone
@@ -151,7 +151,8 @@ def bench_single(a, storage):
             m = layered_media(p.nx, p.ny, p.nz, [600.0, 1400.0, 2200.0, 3000.0, 3800.0])
             vs = m[..., 1]
             return np.stack([m[..., 0], vs, m[..., 2], 0.1 * vs, 0.05 * vs], -1).astype(np.float32)
-    run, nkept = make_run(p, storage, media_fn, cfg["src"], cfg["nsgt"], 3)
+    st = awpsgt.AWP_STORAGE_MIXED if storage == "mixed" else awpsgt.AWP_STORAGE_FP32
+    run, nkept = make_run(p, st, media_fn, cfg["src"], cfg["nsgt"], 3)
     log("set-up %.1f s, %d SGT points, %.1f GB on device" % (time.time() - t0, nkept, run.device_bytes() / 1e9))
 
     run.step(a.warmup)
@@ -192,7 +193,7 @@ def bench_single(a, storage):
     if fused:
         roof["fused_kernel_min_bytes_per_point"] = BYTES_FUSED[storage]
     wl = cfg["workload"] if storage == "fp32" else cfg["workload"].replace(
-        ", FP32", ", mixed storage (BF16 r1..r6, FP16 qp/qs)")
+        ", FP32", ", mixed storage (FP16 r1..r6 with per-row exponents, FP16 qp/qs)")
     config = {"workload": wl, "grid": [p.nx, p.ny, p.nz], "points": npts_grid, "sgt_points": int(nkept),
               "storage": storage, "step": "fused" if fused else "two-stage", "parallelism": "single GPU"}
     return value, ms_step, roof, config
@@ -268,7 +269,7 @@ def bench_batched(a, storage):
             "fused_kernel_min_bytes_per_point": BYTES_FUSED[storage],
             "avg_step_ms_per_wave": [round(x, 4) for x in kern]}
     wl = cfg["workload"] if storage == "fp32" else cfg["workload"].replace(
-        ", FP32", ", mixed storage (BF16 r1..r6, FP16 qp/qs)")
+        ", FP32", ", mixed storage (FP16 r1..r6 with per-row exponents, FP16 qp/qs)")
     config = {"workload": wl, "runs": len(jobs), "waves": [len(w) for w in waves],
               "points_per_step": int(pts_step), "storage": storage, "step": "fused, batched, HIP graph",
               "launch": {"tile_rows": launch[0], "prefetch": launch[1], "z_chunks": launch[2]} if launch else None,

@@ -214,7 +214,7 @@ extern "C" int awp_batch_create(awp_batch **out, awp_ctx *const *runs, int nruns
     b->device = runs[0]->device;
     b->mixed = runs[0]->mixed();
     b->l = runs[0]->l;
-    b->l.tile_y = (b->l.tile_y == 4 || b->l.tile_y == 7 || b->l.tile_y == 8) ? b->l.tile_y : 7;
+    b->l.tile_y = (b->l.tile_y == 4 || b->l.tile_y == 7 || b->l.tile_y == 8) ? b->l.tile_y : 8;
     b->nchunk.assign(nruns, 1);
     b->d_jobs = nullptr;
     b->d_aux = nullptr;
@@ -335,10 +335,12 @@ extern "C" int awp_batch_tune(awp_batch *b, int reps)
     struct Cand {
         int ty, pf, nchunk;
     };
+    // tile rows x prefetch depth (64x8 tiles only without prefetch: their
+    // 13-wave blocks have 128 registers per lane) x z chunks
     std::vector<Cand> cands;
-    for (int ty : {7, 4})
-        for (int pf : {1, 2})
-            for (int nc : {1, 2}) cands.push_back({ty, pf, nc});
+    const int tp[6][2] = {{8, 0}, {7, 0}, {7, 1}, {7, 2}, {4, 0}, {4, 2}};
+    for (const auto &v : tp)
+        for (int nc : {1, 2}) cands.push_back({v[0], v[1], nc});
     double best = 1e30;
     Cand pick = {b->l.tile_y, b->l.unroll, 1};
     for (const Cand &cd : cands) {

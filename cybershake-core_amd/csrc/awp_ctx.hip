@@ -26,7 +26,7 @@ using namespace awp;
 
 extern "C" const char *awp_last_error(const awp_ctx *ctx) { return ctx ? ctx->err : "null context"; }
 
-static Launch choose_launch(const Geo &g, bool fused)
+static Launch choose_launch(const Geo &g, bool fused, bool mixed)
 {
     Launch l;
     l.tile_y = 8;
@@ -37,14 +37,29 @@ static Launch choose_launch(const Geo &g, bool fused)
     // two-stage kernels, 4, 7 or 8 for the fused one) and AWP_KC (z levels
     // per block)
     const char *ty = getenv("AWP_TILE_Y");
-    if (fused) l.tile_y = 7;
+    // fused defaults measured at 1500x1000x300: FP32 64x8 tiles, mixed 64x7
+    // tiles, both without load prefetch (awp_batch_tune searches the rest)
+    if (fused) l.tile_y = mixed ? 7 : 8;
     if (ty && (atoi(ty) == 1 || atoi(ty) == 2 || atoi(ty) == 4 || atoi(ty) == 7 || atoi(ty) == 8)) l.tile_y = atoi(ty);
     if (fused && l.tile_y != 4 && l.tile_y != 7) l.tile_y = 8;
     if (!fused && l.tile_y == 7) l.tile_y = 8;
+    // tile patches of the fused step (AWP_PATCH=<w>x<h> overrides): 2 x 16
+    // tiles of 64 x 8 points, about square, 32 of them, one per CU of an XCD
+    l.pw = 2;
+    l.ph = 16;
+    const char *pe = getenv("AWP_PATCH");
+    if (pe) {
+        int w = 0, h = 0;
+        if (sscanf(pe, "%dx%d", &w, &h) == 2 && w >= 1 && h >= 1 && w * h <= 4096) {
+            l.pw = w;
+            l.ph = h;
+        }
+    }
     if (fused) {
         // the fused step keeps its load prefetch depth (0, 1, 2) in `unroll`
         const char *pf = getenv("AWP_PF");
-        l.unroll = (pf && atoi(pf) >= 0 && atoi(pf) <= 2) ? atoi(pf) : 1;
+        l.unroll = (pf && atoi(pf) >= 0 && atoi(pf) <= 2) ? atoi(pf) : 0;
+        if (l.tile_y == 8) l.unroll = 0;
     }
     const long long bxy = (long long)((g.nx + TX - 1) / TX) * ((g.ny + l.tile_y - 1) / l.tile_y);
     // enough blocks to fill the 256 CUs several times over; a fused block
@@ -99,7 +114,7 @@ extern "C" int awp_create(awp_ctx **out, const awp_in3d *p, int device, int stor
     ctx->fused = !(scheme && !strcmp(scheme, "split"));
     // the fused kernel addresses every array with 32-bit byte offsets
     if ((unsigned long long)ctx->g.nvol * 4ull >= (1ull << 32)) ctx->fused = false;
-    ctx->l = choose_launch(ctx->g, ctx->fused);
+    ctx->l = choose_launch(ctx->g, ctx->fused, storage == AWP_STORAGE_MIXED);
     ctx->vol = nullptr;
     ctx->prof = nullptr;
     ctx->npsrc = 0;
@@ -135,7 +150,12 @@ extern "C" int awp_create(awp_ctx **out, const awp_in3d *p, int device, int stor
     // a 2-byte array gets 128 bytes of slack and starts 64 bytes in, which
     // puts its interior rows on 128-byte lines (the pitch is a multiple of 64)
     const size_t eslot = nv * esz + (esz == 2 ? 128 : 0), eoff = (esz == 2 ? 64 : 0);
-    const size_t nbytes = nv * (9 * nbuf + 3) * sizeof(float) + 8 * eslot;
+    // mixed storage: one exponent byte per 64-point row segment, level and
+    // memory variable, after r1..r6 (zeroed with them)
+    const int nseg = (p->nx + TX - 1) / TX;
+    const size_t rexp_n = (storage == AWP_STORAGE_MIXED) ? (size_t)p->nz * p->ny * nseg : 0;
+    const size_t rexp_bytes = (6 * rexp_n + 255) / 256 * 256;
+    const size_t nbytes = nv * (9 * nbuf + 3) * sizeof(float) + 8 * eslot + rexp_bytes;
     e = hipMalloc((void **)&ctx->vol, nbytes);
     if (e != hipSuccess) {
         snprintf(err, errlen, "hipMalloc(%zu bytes) for the wavefield and media: %s", nbytes, hipGetErrorString(e));
@@ -145,7 +165,7 @@ extern "C" int awp_create(awp_ctx **out, const awp_in3d *p, int device, int stor
     }
     ctx->bytes += nbytes;
     ctx->vol_bytes = nbytes;
-    ctx->state_bytes = nv * 9 * nbuf * sizeof(float) + 6 * eslot;
+    ctx->state_bytes = nv * 9 * nbuf * sizeof(float) + 6 * eslot + rexp_bytes;
     for (int bsel = 0; bsel < 2; ++bsel)
         for (int a = 0; a < 9; ++a) ctx->wf[bsel][a] = ctx->vol + (size_t)((bsel % nbuf) * 9 + a) * nv;
     ctx->cur = 0;
@@ -156,6 +176,10 @@ extern "C" int awp_create(awp_ctx **out, const awp_in3d *p, int device, int stor
     void **rr[6] = {&ctx->s.r1, &ctx->s.r2, &ctx->s.r3, &ctx->s.r4, &ctx->s.r5, &ctx->s.r6};
     for (int a = 0; a < 6; ++a) *rr[a] = rest + (size_t)a * eslot + eoff;
     rest += 6 * eslot;
+    ctx->s.rexp = rexp_n ? (signed char *)rest : nullptr;
+    ctx->s.rexp_n = (long long)rexp_n;
+    ctx->s.nseg = nseg;
+    rest += rexp_bytes;
     float **m32[3] = {&ctx->s.d1, &ctx->s.mu, &ctx->s.lam};
     for (int a = 0; a < 3; ++a) *m32[a] = (float *)(rest + (size_t)a * nv * sizeof(float));
     rest += 3 * nv * sizeof(float);
@@ -620,7 +644,8 @@ extern "C" int awp_estimate_bytes(const awp_in3d *p, int storage, int npts, size
     const size_t nv = (size_t)g.nvol;
     const size_t esz = storage == AWP_STORAGE_MIXED ? 2 : 4;
     const size_t eslot = nv * esz + (esz == 2 ? 128 : 0);
-    const size_t vol = nv * (9 * (fused ? 2 : 1) + 3) * sizeof(float) + 8 * eslot;
+    const size_t rexp_n = storage == AWP_STORAGE_MIXED ? (size_t)p->nz * p->ny * ((p->nx + TX - 1) / TX) : 0;
+    const size_t vol = nv * (9 * (fused ? 2 : 1) + 3) * sizeof(float) + 8 * eslot + (6 * rexp_n + 255) / 256 * 256;
     const size_t prof = ((size_t)p->nx + p->ny + p->nz + 6) * sizeof(float);
     const size_t nsamp_max = (size_t)std::max(p->nt / std::max(p->ntiskp_sgt, 1), 1);
     const size_t sgt = (size_t)std::max(npts, 1) * 6 * nsamp_max * sizeof(float);

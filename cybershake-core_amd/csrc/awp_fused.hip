@@ -9,7 +9,7 @@
 // new state written to buffer `b` (the caller swaps them every step), so
 // every array is read once and written once per step:
 //   FP32 storage: 140 B per point-update (176 B for the two-stage step)
-//   mixed storage: 112 B (BF16 r1..r6, FP16 qp/qs).
+//   mixed storage: 116 B (FP16 r1..r6 under per-row exponents, FP16 qp/qs).
 //
 // Stresses at the tile's edge need new velocities of the neighbouring tiles
 // (two columns / rows on each side).  The block recomputes them: its waves
@@ -53,10 +53,7 @@ __device__ inline bool ftile_of(const FTiles &t, int &bx, int &by, int &bz)
     const int n = blockIdx.x;
     const int tile = (n & 7) * t.per + (n >> 3);
     if (tile >= t.nbx * t.nby * t.nbz) return false;
-    bx = tile % t.nbx;
-    const int r = tile / t.nbx;
-    by = r % t.nby;
-    bz = r / t.nby;
+    patch_tile(tile, t.nbx, t.nby, t.pw, t.ph, bx, by, bz);
     return true;
 }
 
@@ -192,8 +189,17 @@ __device__ __forceinline__ void step_tile(const Geo &g, const State &a, const St
         x.pA = ld(gl(qp, qo)); x.pB = ld(gl(qp, qo, QB)); x.pC = ld(gl(qp, qj)); x.pD = ld(gl(qp, qj, QB));
         x.sA = ld(gl(qs, qo)); x.sB = ld(gl(qs, qo, QB)); x.sC = ld(gl(qs, qj)); x.sD = ld(gl(qs, qj, QB));
         const unsigned orr = o >> RSH;
-        x.r1 = ld(gl(R1, orr)); x.r2 = ld(gl(R2, orr)); x.r3 = ld(gl(R3, orr));
-        x.r4 = ld(gl(R4, orr)); x.r5 = ld(gl(R5, orr)); x.r6 = ld(gl(R6, orr));
+        if constexpr (P::mixed) {
+            // FP16 values under the row segment's exponent (one per wave)
+            const signed char *ex = a.rexp + rexp_idx(g, a.nseg, bx, j, s);
+            const long long n = a.rexp_n;
+            x.r1 = r_decode(gl(R1, orr), ex[0]); x.r2 = r_decode(gl(R2, orr), ex[n]);
+            x.r3 = r_decode(gl(R3, orr), ex[2 * n]); x.r4 = r_decode(gl(R4, orr), ex[3 * n]);
+            x.r5 = r_decode(gl(R5, orr), ex[4 * n]); x.r6 = r_decode(gl(R6, orr), ex[5 * n]);
+        } else {
+            x.r1 = gl(R1, orr); x.r2 = gl(R2, orr); x.r3 = gl(R3, orr);
+            x.r4 = gl(R4, orr); x.r5 = gl(R5, orr); x.r6 = gl(R6, orr);
+        }
         x.xx = gl(a.xx, o); x.yy = gl(a.yy, o); x.xy = gl(a.xy, o);
         x.fz = a.fz[s];
     };
@@ -350,6 +356,7 @@ __device__ __forceinline__ void step_tile(const Geo &g, const State &a, const St
         }
 
         // ================= stress of level s = m-2 (viscoop.f:49-191) =======
+        float rn[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};   // new memory variables (mixed: stored below)
         if (do_str) {
             if constexpr (PF < 1) load_str(s, sin_);
             const StrIn &x = sin_;
@@ -442,10 +449,14 @@ __device__ __forceinline__ void step_tile(const Geo &g, const State &a, const St
             xx = xx * D; yy = yy * D; zz = zz * D;
             xz = xz * D; yz = yz * D; xy = xy * D;
 
-            // memory variables in place (own point only), rounded once on the store
-            const unsigned orr = o >> RSH;
-            gs(R1, orr, st<R>(r1)); gs(R2, orr, st<R>(r2)); gs(R3, orr, st<R>(r3));
-            gs(R4, orr, st<R>(r4)); gs(R5, orr, st<R>(r5)); gs(R6, orr, st<R>(r6));
+            // memory variables in place (own point only); in mixed storage
+            // they are encoded below, after the row segment's maximum
+            if constexpr (!P::mixed) {
+                gs(R1, o, r1); gs(R2, o, r2); gs(R3, o, r3);
+                gs(R4, o, r4); gs(R5, o, r5); gs(R6, o, r6);
+            } else {
+                rn[0] = r1; rn[1] = r2; rn[2] = r3; rn[3] = r4; rn[4] = r5; rn[5] = r6;
+            }
             gs(b.xx, o, xx); gs(b.yy, o, yy); gs(b.xy, o, xy);
             if (s == g.nz) {
                 // fstr (operator.f:575-612): images above the surface, xz = yz = 0 on it
@@ -469,6 +480,21 @@ __device__ __forceinline__ void step_tile(const Geo &g, const State &a, const St
             mA_m = x.mA; mB_m = x.mB; mC_m = x.mC; mD_m = x.mD;
             pA_m = x.pA; pB_m = x.pB; pC_m = x.pC; pD_m = x.pD;
             sA_m = x.sA; sB_m = x.sB; sC_m = x.sC; sD_m = x.sD;
+        }
+        if constexpr (P::mixed) {
+            // whole waves (one row segment each): lanes outside the domain hold 0
+            if (is_main && s >= kb) {
+                const unsigned orr = (ob + (unsigned)s * sz4) >> RSH;
+                R *RR[6] = {R1, R2, R3, R4, R5, R6};
+#pragma unroll
+                for (int q = 0; q < 6; ++q) {
+                    const int e = row_exponent(wave_max(fabsf(rn[q])));
+                    if (do_str) {
+                        gs(RR[q], orr, r_encode(rn[q], e));
+                        if (lane == 0) a.rexp[q * a.rexp_n + rexp_idx(g, a.nseg, bx, j, s)] = (signed char)e;
+                    }
+                }
+            }
         }
 
         // advance the old-stress queues to level m+1
@@ -507,11 +533,8 @@ k_step_batch(const FJob *__restrict__ jobs, const int njobs, const int total, co
         int jb = 0;
         while (jb + 1 < njobs && jobs[jb + 1].tile0 <= tile) ++jb;
         const FJob &J = jobs[jb];
-        const int lt = tile - J.tile0;
-        const int bx = lt % J.t.nbx;
-        const int r = lt / J.t.nbx;
-        const int by = r % J.t.nby;
-        const int bz = r / J.t.nby;
+        int bx, by, bz;
+        patch_tile(tile - J.tile0, J.t.nbx, J.t.nby, J.t.pw, J.t.ph, bx, by, bz);
         step_tile<P, TY, PF>(J.g, J.s[par], J.s[par ^ 1], J.c, J.kc, bx, by, bz, lds);
         __syncthreads();   // the next tile reuses the LDS ring
     }
@@ -523,6 +546,8 @@ FTiles fused_tiles(const Geo &g, const Launch &l)
     t.nbx = (g.nx + TX - 1) / TX;
     t.nby = (g.ny + l.tile_y - 1) / l.tile_y;
     t.nbz = l.nchunk;
+    t.pw = l.pw > 0 ? l.pw : 1;
+    t.ph = l.ph > 0 ? l.ph : 1;
     const long long n = (long long)t.nbx * t.nby * t.nbz;
     t.per = (int)((n + 7) / 8);
     return t;
@@ -557,8 +582,10 @@ static int grid_t(int device)
     return std::max(1, nb) * std::max(8, ncu);
 }
 
-// kernel variant of (storage, tile rows, prefetch depth): F is called with
-// the matching function template instance
+// kernel variant of (storage, tile rows, prefetch depth): CALL is run with
+// P, TY, PF of the matching template instance.  64 x 8 tiles come without
+// prefetch only: their 13-wave blocks leave 128 registers a lane, and the
+// prefetching variants spill there.
 #define AWP_FUSED_DISPATCH(mixed, l, CALL)                                           \
     do {                                                                             \
         const int pf_ = (l).unroll;                                                  \
@@ -573,9 +600,7 @@ static int grid_t(int device)
                 else if (pf_ == 2) { constexpr int TY = 4, PF = 2; CALL; }           \
                 else { constexpr int TY = 4, PF = 1; CALL; }                         \
             } else {                                                                 \
-                if (pf_ == 0) { constexpr int TY = 8, PF = 0; CALL; }                \
-                else if (pf_ == 2) { constexpr int TY = 8, PF = 2; CALL; }           \
-                else { constexpr int TY = 8, PF = 1; CALL; }                         \
+                constexpr int TY = 8, PF = 0; CALL;                                  \
             }                                                                        \
         } else {                                                                     \
             using P = StoreF32;                                                      \
@@ -588,9 +613,7 @@ static int grid_t(int device)
                 else if (pf_ == 2) { constexpr int TY = 4, PF = 2; CALL; }           \
                 else { constexpr int TY = 4, PF = 1; CALL; }                         \
             } else {                                                                 \
-                if (pf_ == 0) { constexpr int TY = 8, PF = 0; CALL; }                \
-                else if (pf_ == 2) { constexpr int TY = 8, PF = 2; CALL; }           \
-                else { constexpr int TY = 8, PF = 1; CALL; }                         \
+                constexpr int TY = 8, PF = 0; CALL;                                  \
             }                                                                        \
         }                                                                            \
     } while (0)

@@ -171,11 +171,18 @@ k_stress(const Geo g, const State s, const Coef c, const int kc, const Tiles tl)
     using R = typename P::R;
     int bx, by, bz;
     if (!tile_of(tl, bx, by, bz)) return;
-    const int i = bx * TX + threadIdx.x + 1;
-    const int j = by * blockDim.y + threadIdx.y + 1;
+    const int ir = bx * TX + threadIdx.x + 1;
+    const int jr = by * blockDim.y + threadIdx.y + 1;
     const int kb = bz * kc + 1;
     const int ke = min(kb + kc - 1, g.nz);
-    if (i > g.nx || j > g.ny) return;
+    // mixed storage needs every lane of a wave for the row maximum of the
+    // memory variables: lanes outside the domain compute an in-range
+    // duplicate point and store nothing
+    const bool act = ir <= g.nx && jr <= g.ny;
+    if constexpr (!P::mixed) {
+        if (!act) return;
+    }
+    const int i = min(ir, g.nx), j = min(jr, g.ny);
     const long long sy = g.sy, sz = g.sz;
     long long o = g.base + (long long)kb * sz + (long long)j * sy + i;
 
@@ -258,7 +265,15 @@ k_stress(const Geo g, const State s, const Coef c, const int kc, const Tiles tl)
         const float vtmp = vxl * (vt1 + vt2 + vt3);
         const float va1 = -vqpa * vxl * dh1 * (vt1 + vt2 + vt3) / 2.0f;
 
-        float r1 = ld(R1[o]), r2 = ld(R2[o]), r3 = ld(R3[o]), r4 = ld(R4[o]), r5 = ld(R5[o]), r6 = ld(R6[o]);
+        float r1, r2, r3, r4, r5, r6;
+        if constexpr (P::mixed) {
+            const signed char *ex = s.rexp + rexp_idx(g, s.nseg, bx, j, k);
+            const long long n = s.rexp_n;
+            r1 = r_decode(R1[o], ex[0]); r2 = r_decode(R2[o], ex[n]); r3 = r_decode(R3[o], ex[2 * n]);
+            r4 = r_decode(R4[o], ex[3 * n]); r5 = r_decode(R5[o], ex[4 * n]); r6 = r_decode(R6[o], ex[5 * n]);
+        } else {
+            r1 = R1[o]; r2 = R2[o]; r3 = R3[o]; r4 = R4[o]; r5 = R5[o]; r6 = R6[o];
+        }
         float xx = s.xx[o], yy = s.yy[o], zz = s.zz[o], xy = s.xy[o], xz = s.xz[o], yz = s.yz[o];
 
         xx = xx + dth * (vtmp - 2.0f * vxm * (vt2 + vt3)) + dt * r1;
@@ -300,9 +315,24 @@ k_stress(const Geo g, const State s, const Coef c, const int kc, const Tiles tl)
         xz = xz * D; yz = yz * D; xy = xy * D;
 
         // the new memory variables enter this step's stresses in FP32 and
-        // are rounded to the storage type once, on the store
-        R1[o] = st<R>(r1); R2[o] = st<R>(r2); R3[o] = st<R>(r3);
-        R4[o] = st<R>(r4); R5[o] = st<R>(r5); R6[o] = st<R>(r6);
+        // are rounded to the storage format once, on the store
+        if constexpr (P::mixed) {
+            const float rv[6] = {r1, r2, r3, r4, r5, r6};
+            R *RR[6] = {R1, R2, R3, R4, R5, R6};
+            const long long xi = rexp_idx(g, s.nseg, bx, j, k);
+#pragma unroll
+            for (int q = 0; q < 6; ++q) {
+                const int e = row_exponent(wave_max(act ? fabsf(rv[q]) : 0.0f));
+                if (act) {
+                    RR[q][o] = r_encode(rv[q], e);
+                    if (threadIdx.x == 0) s.rexp[q * s.rexp_n + xi] = (signed char)e;
+                }
+            }
+        } else {
+            R1[o] = r1; R2[o] = r2; R3[o] = r3;
+            R4[o] = r4; R5[o] = r5; R6[o] = r6;
+        }
+        if (act) {
         s.xx[o] = xx; s.yy[o] = yy; s.xy[o] = xy;
         if (k == g.nz) {
             // fstr: images above the surface, xz = yz = 0 on it
@@ -317,6 +347,7 @@ k_stress(const Geo g, const State s, const Coef c, const int kc, const Tiles tl)
             s.yz[o + 2 * sz] = -yz_f2;
         } else {
             s.zz[o] = zz; s.xz[o] = xz; s.yz[o] = yz;
+        }
         }
         xz_f2 = xz_f1; xz_f1 = xz;
         yz_f2 = yz_f1; yz_f1 = yz;
@@ -656,6 +687,18 @@ __global__ void __launch_bounds__(256) k_to_half(const float *src, _Float16 *dst
         dst[t] = (_Float16)(src[t] * scale);
 }
 
+__global__ void __launch_bounds__(256)
+k_pack_r(const Geo g, const _Float16 *src, const signed char *ex, const int nseg, const int kext, float *dst)
+{
+    const int i = blockIdx.x * TX + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int k = blockIdx.z + 1;
+    if (i > g.nx || j > g.ny || k > g.nz + kext) return;
+    // levels above the surface are never written: zero
+    const float v = k <= g.nz ? r_decode(src[gidx(g, i, j, k)], ex[rexp_idx(g, nseg, (i - 1) / TX, j, k)]) : 0.0f;
+    dst[((size_t)(k - 1) * g.ny + (j - 1)) * g.nx + (i - 1)] = v;
+}
+
 template <class T>
 __global__ void __launch_bounds__(256)
 k_pack_field(const Geo g, const T *src, const float inv_scale, const int kext, float *dst)
@@ -801,7 +844,8 @@ hipError_t launch_pack_field(bool mixed, const Geo &g, const State &s, int which
         hipLaunchKernelGGL(k_pack_field<float>, grd, blk, 0, st, g, m32[which - 15], 1.0f, kext, dst);
     } else if (which < 15) {
         if (mixed)
-            hipLaunchKernelGGL(k_pack_field<Bf16>, grd, blk, 0, st, g, (const Bf16 *)rr[which - 9], 1.0f, kext, dst);
+            hipLaunchKernelGGL(k_pack_r, grd, blk, 0, st, g, (const _Float16 *)rr[which - 9],
+                               s.rexp + (which - 9) * s.rexp_n, s.nseg, kext, dst);
         else
             hipLaunchKernelGGL(k_pack_field<float>, grd, blk, 0, st, g, (const float *)rr[which - 9], 1.0f, kext, dst);
     } else {

@@ -16,7 +16,7 @@
 // Storage: velocities, stresses, the density d1 and the compliances mu/lam
 // are FP32 always.  The memory variables r1..r6 and the attenuation
 // coefficients qp/qs are FP32 (StoreF32) or, in the mixed mode (StoreMixed),
-// BF16 and FP16.  Each FP16 array holds value * 2^e, a per-array power of two
+// FP16 (see StoreMixed).  Each FP16 qp/qs array holds value * 2^e, a per-array power of two
 // that puts its largest magnitude in [2^13, 2^14); the kernels undo it with
 // an exact multiply, so FP16's only effect is the 11-bit rounding of the
 // stored value.  (Density and the compliances stay FP32: their FP16 rounding
@@ -65,6 +65,12 @@ __host__ __device__ inline long long gidx(const Geo &g, int i, int j, int k)
     return g.base + (long long)k * g.sz + (long long)j * g.sy + i;
 }
 
+// exponent slot of row segment seg at (j, k) (interior j, k)
+__host__ __device__ inline long long rexp_idx(const Geo &g, int nseg, int seg, int j, int k)
+{
+    return ((long long)(k - 1) * g.ny + (j - 1)) * nseg + seg;
+}
+
 // ---- storage element types ------------------------------------------------
 struct Bf16 {
     unsigned short u;
@@ -93,10 +99,33 @@ struct StoreF32 {
     static constexpr bool mixed = false;
 };
 struct StoreMixed {
-    using R = Bf16;
+    using R = _Float16;   // r * 2^(15-e), e per row segment (see below)
     using Q = _Float16;
     static constexpr bool mixed = true;
 };
+
+// Memory variables in mixed storage: each stored value is the FP16 (round to
+// nearest even) of r * 2^(15-e), where e is shared by the 64 points of one x
+// row segment (i = 64s+1 .. 64s+64, the lanes of one wave) at one (j, k):
+// the exponent of the segment's largest |r| (f * 2^e, f in [0.5, 1)),
+// clamped to [REXP_MIN, REXP_MAX], kept in one byte per segment.  Every
+// value keeps FP16's 11-bit relative precision over 30 binades below the
+// segment's largest; decoding is the exact h * 2^(e-15).  (BF16 storage of r,
+// 8-bit precision, let the SGT error grow past 1e-3 over 1500 steps.)
+constexpr int REXP_MIN = -80, REXP_MAX = 120;
+__device__ inline int row_exponent(float amax)
+{
+    int e = 0;
+    if (amax > 0.0f) frexpf(amax, &e);
+    return min(max(e, REXP_MIN), REXP_MAX);
+}
+__device__ inline float wave_max(float v)
+{
+    for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
+    return v;
+}
+__device__ inline _Float16 r_encode(float r, int e) { return (_Float16)ldexpf(r, 15 - e); }
+__device__ inline float r_decode(_Float16 h, int e) { return ldexpf((float)h, e - 15); }
 
 // Power-of-two storage scales of the FP16 qp/qs arrays (1 in FP32 mode) and
 // their inverses.
@@ -112,6 +141,9 @@ struct State {
     float *xx, *yy, *zz, *xy, *xz, *yz;
     float *d1, *mu, *lam;
     void *r1, *r2, *r3, *r4, *r5, *r6;
+    signed char *rexp;         // mixed: 6 arrays of rexp_n row-segment exponents
+    long long rexp_n;          // nz * ny * nseg
+    int nseg;                  // (nx + 63) / 64
     void *qp, *qs;
     float *fx, *fy, *fz;       // sponge profiles, index 1..n (fx[0] unused)
     MScale ms;
@@ -136,8 +168,30 @@ struct Launch {
     int tile_y;      // threads per block in y
     int kc;          // z levels per block (chunk)
     int nchunk;      // number of z chunks
-    int unroll;      // z-loop unroll factor (1 or 2)
+    int unroll;      // z-loop unroll factor (1 or 2); prefetch depth of the fused step
+    int pw, ph;      // fused step: tile patch width / height (tiles)
 };
+
+// (bx, by, bz) of local tile number lt in the patch order of FTiles
+__host__ __device__ inline void patch_tile(int lt, int nbx, int nby, int pw, int ph, int &bx, int &by, int &bz)
+{
+    const int nxy = nbx * nby;
+    bz = lt / nxy;
+    int q = lt - bz * nxy;
+    const int rowt = nbx * ph;                       // tiles of one full patch row
+    const int npy = (nby + ph - 1) / ph;
+    int pr = q / rowt;
+    if (pr > npy - 1) pr = npy - 1;
+    q -= pr * rowt;
+    const int h = nby - pr * ph < ph ? nby - pr * ph : ph;
+    const int npx = (nbx + pw - 1) / pw;
+    int pc = q / (pw * h);
+    if (pc > npx - 1) pc = npx - 1;
+    q -= pc * pw * h;
+    const int w = nbx - pc * pw < pw ? nbx - pc * pw : pw;
+    bx = pc * pw + q % w;
+    by = pr * ph + q / w;
+}
 
 struct MediaConst {
     float pi, tmp1, tmp2, lw;   // 4*atan(1), Q-fit constants, log(w2/w0)
@@ -166,9 +220,15 @@ hipError_t launch_step_fused(bool mixed, const Geo &g, const State &a, const Sta
                              const Launch &l, hipStream_t st);
 
 // tiles of one run for the fused step: nbx * nby * nbz column tiles
+// Tile order: within each z chunk, tiles are grouped in patches of pw x ph
+// tiles (x fastest inside a patch, patches x fastest, then y); each XCD takes
+// a contiguous range of that order, so the blocks an XCD runs at one time
+// form a compact patch whose tiles read each other's halo lines from L2
+// while they march through z together.
 struct FTiles {
     int nbx, nby, nbz;
     int per;   // tiles per XCD (single-run launch)
+    int pw, ph;
 };
 FTiles fused_tiles(const Geo &g, const Launch &l);
 

@@ -18,7 +18,7 @@
  * reference runs one component per MPI job (utils/awp_odc_wrapper.sh:14).
  *
  * Environment: AWP_DEVICE selects the HIP device (default 0);
- * AWP_STORAGE=mixed selects mixed storage (BF16 memory variables, FP16
+ * AWP_STORAGE=mixed selects mixed storage (FP16 memory variables, FP16
  * attenuation weights); AWP_SGT_FP16=<path> also writes the SGT in the
  * compact FP16 format to <path>; AWP_SGT_RWG=<path> also writes the file
  * SgtHead's reformat_awp would make of the raw file (RWG component order,

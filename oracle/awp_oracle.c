@@ -61,8 +61,11 @@ typedef REAL real;
 /*
  * MIXED: the mixed-storage variant of the device path, restated for
  * checking it.  The reference has no such mode; this is the FP32
- * restatement with (a) the memory variables r1..r6 rounded to BF16 (round
- * to nearest even) whenever they are stored, (b) the attenuation weights
+ * restatement with (a) the memory variables r1..r6 stored as FP16 (round to
+ * nearest even) of r * 2^(15-e), e shared by each 64-point x row segment
+ * (i = 64s+1 .. 64s+64) at one (j, k): the exponent of the segment's
+ * largest |r| (f * 2^e, f in [0.5, 1)) clamped to [-80, 120]; the new values
+ * enter the same step's stresses unrounded, as on the device, (b) the attenuation weights
  * qp, qs stored as FP16 of value * 2^e (d1, mu, lam stay FP32), e
  * per array putting its largest magnitude in
  * [2^13, 2^14), and the kernels' exact multiplications by 2^e / 2^-e where
@@ -71,12 +74,6 @@ typedef REAL real;
 #if defined(MIXED)
 static inline float as_float(uint32_t u) { float f; memcpy(&f, &u, 4); return f; }
 static inline uint32_t as_bits(float f) { uint32_t u; memcpy(&u, &f, 4); return u; }
-static inline float round_bf16(float f)
-{
-    uint32_t b = as_bits(f);
-    b += 0x7FFFu + ((b >> 16) & 1u);
-    return as_float(b & 0xFFFF0000u);
-}
 /* value of the FP16 nearest (ties to even) to f; |f| < 65504 */
 static inline float round_fp16(float f)
 {
@@ -90,7 +87,7 @@ static inline float round_fp16(float f)
     return as_float(sign | (r & 0xFFFFE000u));
 }
 #define MISCALE(v, a) ((v) = (v) * o->misc[a])
-#define RSTORE(v) round_bf16(v)
+#define RSTORE(v) (v)   /* rounded per row segment after the row, quantize_row */
 #else
 #define MISCALE(v, a) ((void)0)
 #define RSTORE(v) (v)
@@ -382,6 +379,28 @@ static void fvel(oracle_t *o)
 }
 
 /* --- stress + memory variables: allq (viscoop.f:49-191) + addcrjstr ----- */
+#if defined(MIXED)
+/* Memory variables of row (j, k) to their stored values: per 64-point row
+ * segment, FP16 (round to nearest even) of r * 2^(15-e) times 2^(e-15). */
+static void quantize_row(oracle_t *o, int j, int k)
+{
+    real *rr[6] = {o->r1, o->r2, o->r3, o->r4, o->r5, o->r6};
+    for (int a = 0; a < 6; ++a)
+        for (int s0 = 1; s0 <= o->nx; s0 += 64) {
+            const int s1 = s0 + 63 < o->nx ? s0 + 63 : o->nx;
+            float mx = 0.0f;
+            for (int i = s0; i <= s1; ++i) mx = fmaxf(mx, fabsf(rr[a][IX(o, i, j, k)]));
+            int e = 0;
+            if (mx > 0.0f) frexpf(mx, &e);
+            e = e < -80 ? -80 : (e > 120 ? 120 : e);
+            for (int i = s0; i <= s1; ++i) {
+                const size_t p = IX(o, i, j, k);
+                rr[a][p] = ldexpf(round_fp16(ldexpf(rr[a][p], 15 - e)), e - 15);
+            }
+        }
+}
+#endif
+
 static void dstrqc(oracle_t *o)
 {
     const int nx = o->nx, ny = o->ny, nz = o->nz;
@@ -484,6 +503,9 @@ static void dstrqc(oracle_t *o)
                 yz[p] = yz[p] * D;
                 xy[p] = xy[p] * D;
             }
+#if defined(MIXED)
+            quantize_row(o, j, k);
+#endif
         }
     }
 }

@@ -157,8 +157,8 @@ FIELDS = ["u1", "v1", "w1", "xx", "yy", "zz", "xy", "xz", "yz",
 class Oracle:
     """One SGT run of the CPU restatement.  ``dtype`` float32 (reference
     operation order) or float64 (FP64 companion); ``mixed=True`` selects the
-    restatement of the device's mixed-storage mode (FP32 arithmetic, BF16
-    memory variables, FP16 media)."""
+    restatement of the device's mixed-storage mode (FP32 arithmetic, FP16
+    memory variables under per-row exponents, FP16 qp/qs)."""
 
     def __init__(self, in3d: Dict[str, object], media: np.ndarray, src_xyz: np.ndarray,
                  stf: np.ndarray, pts: np.ndarray, dtype=np.float32, mixed: bool = False):

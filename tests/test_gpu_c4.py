@@ -2,7 +2,7 @@
 the 800x600x200 / 1500-step case (layered medium of the 200x200x100
 reference case, 2000 SGT points; tests/golden/make_fp64.py), per component:
 relative L2 over every 4th point and peak error relative to the component's
-peak.  Bounds: FP32 <= 1e-4 (the FP32 parity bound), mixed storage (BF16
+peak.  Bounds: FP32 <= 1e-4 (the FP32 parity bound), mixed storage (FP16
 r1..r6, FP16 qp/qs) <= 1e-3, and the mixed run's FP16 SGT file (per point and
 component FP32 scale) <= 1e-3 as well."""
 import os
