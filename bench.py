@@ -7,25 +7,29 @@ depth gradient times a seeded von Karman perturbation (correlation 5 km,
 Hurst 0.1, 5% std, clipped to +-3 sigma), Vp = 1.73 Vs, Gardner density,
 Qs = 50 Vs[km/s], Qp = 2 Qs; X point force with a Gaussian source time
 function; 1.5M SGT points sampled every 10 steps.  Synthetic inputs are built
-on the device; the timed region is K full time steps (fused velocity +
-stress sweep with sponge and free surface, source, SGT gather) with every
-input already resident in HBM.
+on the device; the timed region is K full time steps (velocity kernel,
+free-surface images, stress kernel with sponge and memory variables, source,
+SGT gather -- or, with --scheme fused, the one-sweep fused kernel in place of
+the three) with every input already resident in HBM.
 
 Batched workloads: "c3" -- 8 runs (4 sites x X/Y force) on grids from
 400x300x150 to 900x700x220, 200k SGT points each; "c5" -- 16 runs (8 sites x
 X/Y) on grids from 500x400x150 to 1200x900x280, admitted in waves under the
 device-memory budget.  Same media recipe per site (seeded), h = 100 m,
-dt = 0.005 s.  Each wave is one awpsgt.Batch (one fused launch per step for
-all its runs, replayed from a HIP graph), tuned before the timed steps.
+dt = 0.005 s.  Each wave is one awpsgt.Batch (one persistent launch per stage
+for all its runs, replayed from a HIP graph), tuned before the timed steps.
 
     python bench.py [--steps K] [--warmup W] [--config c2|c1|c3|c5] [--storage fp32|mixed]
+                    [--scheme split|fused] [--compare-fp32]
 
 Prints one JSON line.  value = point-updates per second of the whole job
-(Gpt-updates/s).  roofline = the fused step kernel (the only volume kernel):
-achieved = the path's algorithmic bytes per point-update (176 B FP32, 148 B
-mixed storage: every array each of the reference's two stages touches, once,
-at its storage width) x points / the kernel's average time from HIP events
-on the run's stream; the fused kernel itself needs only 140 B / 116 B.
+(Gpt-updates/s).  roofline = the dominant kernel: the stress kernel of the
+two-stage step (124 B FP32 / 96 B mixed per point: every array it touches,
+once, at its storage width), or the fused kernel (the whole step's 176 B /
+148 B -- every array each of the reference's two stages touches -- although
+the sweep itself must move only 140 B / 116 B); achieved = those bytes x
+points / the kernel's average time from HIP events on the run's stream.
+Batched configs report the whole step (all kernels of a wave) at 176 / 148 B.
 """
 from __future__ import annotations
 
@@ -170,7 +174,7 @@ def bench_single(a, storage):
     finite = bool(np.isfinite(run.sgt()).all())
     run.close()
 
-    fused = os.environ.get("AWP_SCHEME", "fused") != "split"
+    fused = os.environ.get("AWP_SCHEME") == "fused"
     value = npts_grid * a.steps / el / 1e9
     ms_step = el / a.steps * 1e3
     kern_ms = ts / a.steps                   # fused kernel, or the stress kernel of the two-stage step
@@ -225,6 +229,7 @@ def bench_batched(a, storage):
     tot_time = 0.0
     kern = []
     launch = None
+    fused = False
     for w in waves:
         runs = []
         t0 = time.time()
@@ -240,10 +245,12 @@ def bench_batched(a, storage):
         with awpsgt.Batch(runs) as b:
             if not a.no_tune:
                 launch = b.tune(2)
-                log("tuned: tile rows %d, prefetch %d, z chunks %d (%.2f ms per fused launch)"
-                    % (launch + (b.time_ms(),)))
+                log("tuned: %s, tile rows %d, %s %d, z chunks %d (%.2f ms per step's kernels)"
+                    % ("fused" if b.fused() else "two-stage", launch[0],
+                       "prefetch" if b.fused() else "z unroll", launch[1], launch[2], b.time_ms()))
             else:
                 launch = b.launch()
+            fused = b.fused()
             b.step(a.warmup)
             torch.cuda.synchronize()
             t = time.perf_counter()
@@ -263,16 +270,22 @@ def bench_batched(a, storage):
     pts_step = tot_pts / a.steps
     ms_step = tot_time / a.steps * 1e3
     ach = BYTES_UPDATE[storage] * value
-    roof = {"bound": "hbm", "kernel": "k_step_batch (fused sweep over every run of a wave)",
+    roof = {"bound": "hbm",
+            "kernel": ("k_step_batch (fused sweep over every run of a wave)" if fused else
+                       "k_velocity_batch + k_fs_velocity_batch + k_stress_batch (every run of a wave), "
+                       "whole step"),
             "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
             "traffic": None, "algorithmic_bytes_per_point": BYTES_UPDATE[storage],
-            "fused_kernel_min_bytes_per_point": BYTES_FUSED[storage],
             "avg_step_ms_per_wave": [round(x, 4) for x in kern]}
+    if fused:
+        roof["fused_kernel_min_bytes_per_point"] = BYTES_FUSED[storage]
     wl = cfg["workload"] if storage == "fp32" else cfg["workload"].replace(
         ", FP32", ", mixed storage (FP16 r1..r6 with per-row exponents, FP16 qp/qs)")
     config = {"workload": wl, "runs": len(jobs), "waves": [len(w) for w in waves],
-              "points_per_step": int(pts_step), "storage": storage, "step": "fused, batched, HIP graph",
-              "launch": {"tile_rows": launch[0], "prefetch": launch[1], "z_chunks": launch[2]} if launch else None,
+              "points_per_step": int(pts_step), "storage": storage,
+              "step": ("fused" if fused else "two-stage") + ", persistent launch per stage, HIP graph",
+              "launch": {"tile_rows": launch[0], ("prefetch" if fused else "z_unroll"): launch[1],
+                         "z_chunks": launch[2]} if launch else None,
               "budget_gb": round(budget / 1e9, 1), "parallelism": "single GPU"}
     return value, ms_step, roof, config
 
@@ -284,6 +297,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
     ap.add_argument("--storage", default="fp32", choices=["fp32", "mixed"])
+    ap.add_argument("--scheme", default="split", choices=["split", "fused"],
+                    help="time-step scheme of the runs (batches created from fused runs tune over both)")
+    ap.add_argument("--compare-fp32", action="store_true",
+                    help="with --storage mixed: also run the FP32 workload and report the ratio")
     ap.add_argument("--budget-gb", type=float, default=0.0,
                     help="device-memory budget for the batched configs (default: free memory - 8 GiB)")
     ap.add_argument("--no-tune", action="store_true", help="batched configs: skip the tuner")
@@ -306,10 +323,17 @@ def main():
               "and this build has no multi-GPU path (--gpus %d requested)" % a.gpus, file=sys.stderr)
         sys.exit(2)
 
-    if a.config in ("c3", "c5"):
-        value, ms_step, roof, config = bench_batched(a, a.storage)
+    if a.scheme == "fused":
+        os.environ["AWP_SCHEME"] = "fused"
     else:
-        value, ms_step, roof, config = bench_single(a, a.storage)
+        os.environ.pop("AWP_SCHEME", None)
+    bench_fn = bench_batched if a.config in ("c3", "c5") else bench_single
+    value, ms_step, roof, config = bench_fn(a, a.storage)
+    if a.compare_fp32 and a.storage == "mixed":
+        v32, ms32, _, _ = bench_fn(a, "fp32")
+        config["fp32_value"] = round(v32, 4)
+        config["mixed_vs_fp32"] = round(value / v32, 4)
+        log("mixed %.3f Gpt/s vs FP32 %.3f Gpt/s: x%.3f" % (value, v32, value / v32))
     cpu = cpu_leg(a)
     out = {
         "metric": "Gpt-updates/s + % HBM roofline; batched mixed-storage vs FP32 at stated SGT error",

@@ -85,8 +85,8 @@ EXPORTS = [
     "awp_progress", "awp_get_sgt", "awp_write_sgt_file", "awp_write_sgt_file_fp16",
     "awp_sgt16_read", "awp_sgt16_read_points", "awp_get_field", "awp_set_timing", "awp_get_timing", "awp_device_bytes",
     "awp_write_sgt_file_rwg", "awp_device_memory", "awp_estimate_bytes", "awp_batch_create", "awp_batch_tune", "awp_batch_step",
-    "awp_batch_synchronize", "awp_batch_time", "awp_batch_get_launch", "awp_batch_last_error",
-    "awp_batch_destroy",
+    "awp_batch_synchronize", "awp_batch_time", "awp_batch_get_launch", "awp_batch_get_scheme",
+    "awp_batch_last_error", "awp_batch_destroy",
 ]
 
 
@@ -139,6 +139,7 @@ def lib() -> ctypes.CDLL:
         "awp_batch_time": ([vp, ctypes.POINTER(ctypes.c_double)], ip),
         "awp_batch_get_launch": ([vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
                                   ctypes.POINTER(ctypes.c_int)], ip),
+        "awp_batch_get_scheme": ([vp, ctypes.POINTER(ctypes.c_int)], ip),
         "awp_batch_last_error": ([vp], ctypes.c_char_p),
         "awp_batch_destroy": ([vp], ip),
     }
@@ -358,10 +359,11 @@ def estimate_bytes(p: In3d, storage: int = AWP_STORAGE_FP32, npts: int = 0) -> T
 
 
 class Batch:
-    """Initialised runs advanced together (``awp_batch``): one fused-step
-    launch per time step over every run's tiles, replayed from a HIP graph.
-    The runs stay usable (``sgt()``, ``write_sgt_file``) and must outlive
-    the batch."""
+    """Initialised runs advanced together (``awp_batch``): one persistent
+    launch per stage of a time step over every run's tiles (two-stage
+    scheme, or the fused one when every run was created with
+    AWP_SCHEME=fused), replayed from a HIP graph.  The runs stay usable
+    (``sgt()``, ``write_sgt_file``) and must outlive the batch."""
 
     def __init__(self, runs: Sequence["Run"]):
         self.runs = list(runs)
@@ -386,6 +388,12 @@ class Batch:
         self._do(lib().awp_batch_get_launch(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)),
                  "awp_batch_get_launch")
         return a.value, b.value, c.value
+
+    def fused(self) -> bool:
+        """True when the batch uses the fused one-sweep scheme."""
+        f = ctypes.c_int(0)
+        self._do(lib().awp_batch_get_scheme(self.h, ctypes.byref(f)), "awp_batch_get_scheme")
+        return bool(f.value)
 
     def step(self, n: int) -> None:
         self._do(lib().awp_batch_step(self.h, int(n)), "awp_batch_step")

@@ -1,15 +1,20 @@
 // awp_batch.hip -- several SGT runs (components, sites) advanced together on
-// one GPU.  Each time step is one launch of the fused kernel over the tiles
-// of every run (a persistent grid walking the concatenated tile list) and
-// one launch for every run's source and SGT sample; two steps (both buffer
-// parities) are captured in a HIP graph once and replayed.  The runs keep
-// their own contexts (buffers, counters, outputs): the batch only drives
-// them, so awp_get_sgt / awp_write_sgt_file work on each run afterwards.
+// one GPU.  Each stage of a time step is one persistent launch over the
+// tiles of every run (a grid walking the concatenated tile list): in the
+// two-stage scheme the velocity kernel, the free-surface images and the
+// stress kernel, in the fused scheme the one-sweep kernel; then one launch
+// for every run's source and SGT sample.  Two steps (both buffer parities of
+// the fused scheme) are captured in a HIP graph once and replayed.  The runs
+// keep their own contexts (buffers, counters, outputs): the batch only
+// drives them, so awp_get_sgt / awp_write_sgt_file work on each run
+// afterwards.
 //
-// The tuner times the fused kernel of the batch for each candidate tile
-// height, prefetch depth and z chunking on the runs' own buffers and keeps
-// the fastest; it needs runs that have not stepped yet and re-initialises
-// them afterwards.
+// A batch uses the fused scheme when every run was created with it (two
+// wavefield buffers), else the two-stage one.  The tuner times the step
+// kernels of the batch for each candidate scheme (the fused one only where
+// the runs hold two buffers), tile height, prefetch depth or z unroll and z
+// chunking on the runs' own buffers and keeps the fastest; it needs runs
+// that have not stepped yet and re-initialises them afterwards.
 //
 // The reference runs one SGT component per MPI job (awp_odc_wrapper.sh:14);
 // there is no counterpart of a batch in it.
@@ -33,13 +38,17 @@ struct awp_batch {
     std::vector<awp_ctx *> runs;
     int device;
     bool mixed;
-    Launch l;                    // tile_y and prefetch depth (unroll) of the batch
+    bool split;                  // two-stage scheme (else fused)
+    bool can_fuse;               // every run holds the fused scheme's two buffers
+    Launch l;                    // tile_y and prefetch depth / z unroll (unroll) of the batch
     std::vector<int> nchunk;     // z chunks per run
     hipStream_t stream;
     FJob *d_jobs;
+    SJob *d_sjobs;
     AuxJob *d_aux;
     int *d_counter;
-    int total, grid, max_kept;
+    int total, fs_total, max_kept;
+    int grid, grid_v, grid_s;    // persistent blocks: fused kernel; velocity, stress kernels
     hipGraphExec_t graph;
     bool graph_ok;
     bool use_graph;
@@ -69,12 +78,13 @@ static int bfail(awp_batch *b, int code, const char *fmt, ...)
 
 extern "C" const char *awp_batch_last_error(const awp_batch *b) { return b ? b->err : "null batch"; }
 
-// z chunks of a run with `nchunk` requested: the top chunk keeps >= 3 levels
-// (the free-surface stress images need the two levels below the surface)
-static Launch run_launch(const Geo &g, const Launch &bl, int nchunk)
+// z chunks of a run with `nchunk` requested (chunks of >= 16 levels fused,
+// >= 8 two-stage): the top chunk keeps >= 3 levels (the free-surface stress
+// images need the two levels below the surface)
+static Launch run_launch(const Geo &g, const Launch &bl, int nchunk, bool split)
 {
     Launch l = bl;
-    nchunk = std::max(1, std::min(nchunk, std::max(1, g.nz / 16)));
+    nchunk = std::max(1, std::min(nchunk, std::max(1, g.nz / (split ? 8 : 16))));
     for (;;) {
         const int kc = (g.nz + nchunk - 1) / nchunk;
         const int last = g.nz - (nchunk - 1) * kc;
@@ -87,30 +97,48 @@ static Launch run_launch(const Geo &g, const Launch &bl, int nchunk)
     }
 }
 
-// Builds the device job tables from the runs' current buffer parity: run r
-// reads s[0] = its current state and writes s[1].
+// Builds the device job tables from the runs' current buffer parity: in the
+// fused scheme run r reads s[0] = its current state and writes s[1]; the
+// two-stage scheme updates the current state in place.
 static int upload_tables(awp_batch *b)
 {
     const int n = (int)b->runs.size();
     std::vector<FJob> fj(n);
+    std::vector<SJob> sj(n);
     std::vector<AuxJob> aj(n);
-    int tile0 = 0;
+    int tile0 = 0, fs0 = 0;
     b->max_kept = 0;
     for (int r = 0; r < n; ++r) {
         awp_ctx *c = b->runs[r];
-        const Launch l = run_launch(c->g, b->l, b->nchunk[r]);
-        FJob &f = fj[r];
-        f.g = c->g;
-        f.s[0] = c->view(c->cur);
-        f.s[1] = c->view(c->cur ^ 1);
-        f.c = c->c;
-        f.kc = l.kc;
-        f.t = fused_tiles(c->g, l);
-        f.tile0 = tile0;
-        tile0 += f.t.nbx * f.t.nby * f.t.nbz;
+        const Launch l = run_launch(c->g, b->l, b->nchunk[r], b->split);
         AuxJob &x = aj[r];
-        x.s[0] = f.s[0];
-        x.s[1] = f.s[1];
+        if (b->split) {
+            SJob &q = sj[r];
+            q.g = c->g;
+            q.s = c->view(c->cur);
+            q.c = c->c;
+            q.kc = l.kc;
+            q.nbx = (c->g.nx + TX - 1) / TX;
+            q.nby = (c->g.ny + l.tile_y - 1) / l.tile_y;
+            q.nbz = l.nchunk;
+            q.tile0 = tile0;
+            q.fs0 = fs0;
+            tile0 += q.nbx * q.nby * q.nbz;
+            fs0 += q.nbx * ((c->g.ny + 3) / 4);
+            x.s[0] = x.s[1] = q.s;
+        } else {
+            FJob &f = fj[r];
+            f.g = c->g;
+            f.s[0] = c->view(c->cur);
+            f.s[1] = c->view(c->cur ^ 1);
+            f.c = c->c;
+            f.kc = l.kc;
+            f.t = fused_tiles(c->g, l);
+            f.tile0 = tile0;
+            tile0 += f.t.nbx * f.t.nby * f.t.nbz;
+            x.s[0] = f.s[0];
+            x.s[1] = f.s[1];
+        }
         x.g = c->g;
         x.igreen = c->p.igreen;
         x.npsrc = c->npsrc;
@@ -131,9 +159,13 @@ static int upload_tables(awp_batch *b)
         b->max_kept = std::max(b->max_kept, c->nkept);
     }
     b->total = tile0;
-    BTRY(b, hipMemcpyAsync(b->d_jobs, fj.data(), n * sizeof(FJob), hipMemcpyHostToDevice, b->stream));
+    b->fs_total = fs0;
+    if (b->split)
+        BTRY(b, hipMemcpyAsync(b->d_sjobs, sj.data(), n * sizeof(SJob), hipMemcpyHostToDevice, b->stream));
+    else
+        BTRY(b, hipMemcpyAsync(b->d_jobs, fj.data(), n * sizeof(FJob), hipMemcpyHostToDevice, b->stream));
     BTRY(b, hipMemcpyAsync(b->d_aux, aj.data(), n * sizeof(AuxJob), hipMemcpyHostToDevice, b->stream));
-    BTRY(b, hipStreamSynchronize(b->stream));   // fj / aj are freed on return
+    BTRY(b, hipStreamSynchronize(b->stream));   // fj / sj / aj are freed on return
     return AWP_OK;
 }
 
@@ -148,13 +180,31 @@ static int set_counters(awp_batch *b)
     return AWP_OK;
 }
 
-// one time step of every run at relative parity `par`
-static hipError_t launch_one(awp_batch *b, int par)
+// the step kernels of every run (no source / SGT work)
+static hipError_t launch_kernels(awp_batch *b, int par)
 {
     const int n = (int)b->runs.size();
-    hipError_t e = launch_step_fused_batch(b->mixed, b->l, b->d_jobs, n, b->total, b->grid, par, b->stream);
+    if (b->split)
+        return launch_step_split_batch(b->mixed, b->l, b->d_sjobs, n, b->total, b->fs_total, b->grid_v,
+                                       b->grid_s, b->stream);
+    return launch_step_fused_batch(b->mixed, b->l, b->d_jobs, n, b->total, b->grid, par, b->stream);
+}
+
+// persistent grid sizes of the current scheme and launch shape
+static void set_grids(awp_batch *b)
+{
+    if (b->split)
+        split_batch_grids(b->mixed, b->l, b->device, &b->grid_v, &b->grid_s);
+    else
+        b->grid = fused_batch_grid(b->mixed, b->l, b->device);
+}
+
+// one time step of every run at relative parity `par` (fused scheme)
+static hipError_t launch_one(awp_batch *b, int par)
+{
+    hipError_t e = launch_kernels(b, par);
     if (e != hipSuccess) return e;
-    return launch_aux_batch(b->d_aux, n, b->max_kept, par, b->stream);
+    return launch_aux_batch(b->d_aux, (int)b->runs.size(), b->max_kept, b->split ? 0 : par, b->stream);
 }
 
 static void drop_graph(awp_batch *b)
@@ -196,8 +246,6 @@ extern "C" int awp_batch_create(awp_batch **out, awp_ctx *const *runs, int nruns
             snprintf(msg, sizeof msg, "run %d is null", r);
         } else if (!c->initialised) {
             snprintf(msg, sizeof msg, "run %d is not initialised (awp_init)", r);
-        } else if (!c->fused) {
-            snprintf(msg, sizeof msg, "run %d uses the two-stage step; batches need the fused step", r);
         } else if (c->device != runs[0]->device || c->storage != runs[0]->storage) {
             snprintf(msg, sizeof msg, "run %d differs from run 0 in device or storage mode", r);
         } else {
@@ -213,10 +261,20 @@ extern "C" int awp_batch_create(awp_batch **out, awp_ctx *const *runs, int nruns
     b->runs.assign(runs, runs + nruns);
     b->device = runs[0]->device;
     b->mixed = runs[0]->mixed();
+    b->can_fuse = true;
+    for (int r = 0; r < nruns; ++r) b->can_fuse = b->can_fuse && runs[r]->fused;
+    b->split = !b->can_fuse;
     b->l = runs[0]->l;
-    b->l.tile_y = (b->l.tile_y == 4 || b->l.tile_y == 7 || b->l.tile_y == 8) ? b->l.tile_y : 8;
+    if (b->split) {
+        // two-stage kernels: 64 x {1, 2, 4, 8} tiles, z unroll 1 or 2
+        if (b->l.tile_y != 1 && b->l.tile_y != 2 && b->l.tile_y != 4) b->l.tile_y = 8;
+        if (b->l.unroll != 2) b->l.unroll = 1;
+    } else {
+        b->l.tile_y = (b->l.tile_y == 4 || b->l.tile_y == 7 || b->l.tile_y == 8) ? b->l.tile_y : 8;
+    }
     b->nchunk.assign(nruns, 1);
     b->d_jobs = nullptr;
+    b->d_sjobs = nullptr;
     b->d_aux = nullptr;
     b->d_counter = nullptr;
     b->graph_ok = false;
@@ -227,6 +285,7 @@ extern "C" int awp_batch_create(awp_batch **out, awp_ctx *const *runs, int nruns
     hipError_t e = hipSetDevice(b->device);
     if (e == hipSuccess) e = hipStreamCreate(&b->stream);
     if (e == hipSuccess) e = hipMalloc((void **)&b->d_jobs, nruns * sizeof(FJob));
+    if (e == hipSuccess) e = hipMalloc((void **)&b->d_sjobs, nruns * sizeof(SJob));
     if (e == hipSuccess) e = hipMalloc((void **)&b->d_aux, nruns * sizeof(AuxJob));
     if (e == hipSuccess) e = hipMalloc((void **)&b->d_counter, nruns * sizeof(int));
     if (e == hipSuccess) e = hipEventCreate(&b->e0);
@@ -240,7 +299,7 @@ extern "C" int awp_batch_create(awp_batch **out, awp_ctx *const *runs, int nruns
     // the runs' own streams may still hold work (awp_init); the batch
     // stream starts after it
     for (awp_ctx *c : b->runs) hipStreamSynchronize(c->stream);
-    b->grid = fused_batch_grid(b->mixed, b->l, b->device);
+    set_grids(b);
     *out = b;
     return AWP_OK;
 }
@@ -252,6 +311,7 @@ extern "C" int awp_batch_destroy(awp_batch *b)
     drop_graph(b);
     if (b->stream) hipStreamSynchronize(b->stream);
     hipFree(b->d_jobs);
+    hipFree(b->d_sjobs);
     hipFree(b->d_aux);
     hipFree(b->d_counter);
     hipEventDestroy(b->e0);
@@ -296,7 +356,7 @@ extern "C" int awp_batch_step(awp_batch *b, int nsteps)
     for (awp_ctx *c : b->runs) {
         c->step += todo;
         c->nsamp = std::min(c->nsamp_max, c->step / c->p.ntiskp_sgt);
-        c->cur ^= (todo & 1);
+        if (!b->split) c->cur ^= (todo & 1);
     }
     return AWP_OK;
 }
@@ -313,6 +373,13 @@ extern "C" int awp_batch_time(const awp_batch *b, double *ms)
 {
     if (!b || !ms) return AWP_EINVAL;
     *ms = b->t_last;
+    return AWP_OK;
+}
+
+extern "C" int awp_batch_get_scheme(const awp_batch *b, int *fused)
+{
+    if (!b || !fused) return AWP_EINVAL;
+    *fused = b->split ? 0 : 1;
     return AWP_OK;
 }
 
@@ -333,29 +400,34 @@ extern "C" int awp_batch_tune(awp_batch *b, int reps)
     if (reps < 1) reps = 2;
     BTRY(b, hipSetDevice(b->device));
     struct Cand {
+        bool split;
         int ty, pf, nchunk;
     };
-    // tile rows x prefetch depth (64x8 tiles only without prefetch: their
-    // 13-wave blocks have 128 registers per lane) x z chunks
     std::vector<Cand> cands;
+    // two-stage: tile rows x z unroll x z chunks
+    for (int ty : {4, 8})
+        for (int un : {1, 2})
+            for (int nc : {1, 2}) cands.push_back({true, ty, un, nc});
+    // fused: tile rows x prefetch depth (64x8 tiles only without prefetch:
+    // their 13-wave blocks have 128 registers per lane) x z chunks
     const int tp[6][2] = {{8, 0}, {7, 0}, {7, 1}, {7, 2}, {4, 0}, {4, 2}};
-    for (const auto &v : tp)
-        for (int nc : {1, 2}) cands.push_back({v[0], v[1], nc});
+    if (b->can_fuse)
+        for (const auto &v : tp)
+            for (int nc : {1, 2}) cands.push_back({false, v[0], v[1], nc});
     double best = 1e30;
-    Cand pick = {b->l.tile_y, b->l.unroll, 1};
+    Cand pick = {b->split, b->l.tile_y, b->l.unroll, 1};
     for (const Cand &cd : cands) {
+        b->split = cd.split;
         b->l.tile_y = cd.ty;
         b->l.unroll = cd.pf;
         std::fill(b->nchunk.begin(), b->nchunk.end(), cd.nchunk);
         int rc = upload_tables(b);
         if (rc) return rc;
-        b->grid = fused_batch_grid(b->mixed, b->l, b->device);
-        const int n = (int)b->runs.size();
-        // warm-up, then timed launches of the fused kernel alone
-        BTRY(b, launch_step_fused_batch(b->mixed, b->l, b->d_jobs, n, b->total, b->grid, 0, b->stream));
+        set_grids(b);
+        // warm-up, then timed launches of the step kernels alone
+        BTRY(b, launch_kernels(b, 0));
         BTRY(b, hipEventRecord(b->e0, b->stream));
-        for (int k = 0; k < reps; ++k)
-            BTRY(b, launch_step_fused_batch(b->mixed, b->l, b->d_jobs, n, b->total, b->grid, k & 1, b->stream));
+        for (int k = 0; k < reps; ++k) BTRY(b, launch_kernels(b, k & 1));
         BTRY(b, hipEventRecord(b->e1, b->stream));
         BTRY(b, hipEventSynchronize(b->e1));
         float ms = 0;
@@ -365,10 +437,11 @@ extern "C" int awp_batch_tune(awp_batch *b, int reps)
             pick = cd;
         }
     }
+    b->split = pick.split;
     b->l.tile_y = pick.ty;
     b->l.unroll = pick.pf;
     std::fill(b->nchunk.begin(), b->nchunk.end(), pick.nchunk);
-    b->grid = fused_batch_grid(b->mixed, b->l, b->device);
+    set_grids(b);
     drop_graph(b);
     // the trial steps changed the runs' state: start them again
     for (awp_ctx *c : b->runs) {

@@ -108,10 +108,10 @@ extern "C" int awp_create(awp_ctx **out, const awp_in3d *p, int device, int stor
     ctx->device = device;
     ctx->storage = storage;
     ctx->g = make_geo(p->nx, p->ny, p->nz);
-    // AWP_SCHEME=split selects the two-stage step (velocity kernel, then
-    // stress kernel); the default is the fused one-sweep step
+    // the default is the two-stage step (velocity kernel, then stress
+    // kernel); AWP_SCHEME=fused selects the fused one-sweep step
     const char *scheme = getenv("AWP_SCHEME");
-    ctx->fused = !(scheme && !strcmp(scheme, "split"));
+    ctx->fused = scheme && !strcmp(scheme, "fused");
     // the fused kernel addresses every array with 32-bit byte offsets
     if ((unsigned long long)ctx->g.nvol * 4ull >= (1ull << 32)) ctx->fused = false;
     ctx->l = choose_launch(ctx->g, ctx->fused, storage == AWP_STORAGE_MIXED);
@@ -639,7 +639,7 @@ extern "C" int awp_estimate_bytes(const awp_in3d *p, int storage, int npts, size
     if (!p || (storage != AWP_STORAGE_FP32 && storage != AWP_STORAGE_MIXED) || npts < 0) return AWP_EINVAL;
     const Geo g = make_geo(p->nx, p->ny, p->nz);
     const char *scheme = getenv("AWP_SCHEME");
-    bool fused = !(scheme && !strcmp(scheme, "split"));
+    bool fused = scheme && !strcmp(scheme, "fused");
     if ((unsigned long long)g.nvol * 4ull >= (1ull << 32)) fused = false;
     const size_t nv = (size_t)g.nvol;
     const size_t esz = storage == AWP_STORAGE_MIXED ? 2 : 4;

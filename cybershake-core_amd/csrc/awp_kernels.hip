@@ -49,12 +49,10 @@ __device__ inline bool tile_of(const Tiles &t, int &bx, int &by, int &bz)
 // (cerjan.f:292-328)
 // ---------------------------------------------------------------------------
 template <int U>
-__global__ void __launch_bounds__(512)
-k_velocity(const Geo g, const State s, const Coef c, const int kc, const Tiles tl)
+__device__ inline void velocity_tile(const Geo &g, const State &s, const Coef &c, const int kc, const int bx,
+                                     const int by, const int bz)
 {
     // the velocity stage reads only FP32 arrays in both storage modes
-    int bx, by, bz;
-    if (!tile_of(tl, bx, by, bz)) return;
     const int i = bx * TX + threadIdx.x + 1;
     const int j = by * blockDim.y + threadIdx.y + 1;
     const int kb = bz * kc + 1;
@@ -130,11 +128,10 @@ k_velocity(const Geo g, const State s, const Coef c, const int kc, const Tiles t
 // u(i+1), v(j-1) at nz+1 are recomputed from the same inputs fvelxy uses,
 // so no thread reads another thread's output.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256)
-k_fs_velocity(const Geo g, const State s)
+__device__ inline void fs_velocity_block(const Geo &g, const State &s, const int bx, const int by)
 {
-    const int i = blockIdx.x * TX + threadIdx.x + 1;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int i = bx * TX + threadIdx.x + 1;
+    const int j = by * blockDim.y + threadIdx.y + 1;
     if (i > g.nx || j > g.ny) return;
     const long long sy = g.sy, sz = g.sz;
     const long long p = gidx(g, i, j, g.nz);
@@ -164,13 +161,11 @@ k_fs_velocity(const Geo g, const State s)
 // (operator.f:575-612).
 // ---------------------------------------------------------------------------
 template <class P, int U>
-__global__ void __launch_bounds__(512)
-k_stress(const Geo g, const State s, const Coef c, const int kc, const Tiles tl)
+__device__ inline void stress_tile(const Geo &g, const State &s, const Coef &c, const int kc, const int bx,
+                                   const int by, const int bz)
 {
     using Q = typename P::Q;
     using R = typename P::R;
-    int bx, by, bz;
-    if (!tile_of(tl, bx, by, bz)) return;
     const int ir = bx * TX + threadIdx.x + 1;
     const int jr = by * blockDim.y + threadIdx.y + 1;
     const int kb = bz * kc + 1;
@@ -361,6 +356,94 @@ k_stress(const Geo g, const State s, const Coef c, const int kc, const Tiles tl)
         pA_m = pA; pB_m = pB; pC_m = pC; pD_m = pD;
         sA_m = sA; sB_m = sB; sC_m = sC; sD_m = sD;
     }
+}
+
+// ---------------------------------------------------------------------------
+// two-stage step kernels: one run (one block per tile), or a batch of runs
+// (a persistent grid per stage over the concatenated tile lists)
+// ---------------------------------------------------------------------------
+template <int U>
+__global__ void __launch_bounds__(512)
+k_velocity(const Geo g, const State s, const Coef c, const int kc, const Tiles tl)
+{
+    int bx, by, bz;
+    if (tile_of(tl, bx, by, bz)) velocity_tile<U>(g, s, c, kc, bx, by, bz);
+}
+
+__global__ void __launch_bounds__(256)
+k_fs_velocity(const Geo g, const State s)
+{
+    fs_velocity_block(g, s, blockIdx.x, blockIdx.y);
+}
+
+template <class P, int U>
+__global__ void __launch_bounds__(512)
+k_stress(const Geo g, const State s, const Coef c, const int kc, const Tiles tl)
+{
+    int bx, by, bz;
+    if (tile_of(tl, bx, by, bz)) stress_tile<P, U>(g, s, c, kc, bx, by, bz);
+}
+
+// Run of global tile `tile` (jobs ascending in tile0) and the tile's
+// (bx, by, bz) in that run's x-fastest, then y, then z-chunk order.
+__device__ inline const SJob &split_tile(const SJob *__restrict__ jobs, const int njobs, const int tile, int &bx,
+                                         int &by, int &bz)
+{
+    int jb = 0;
+    while (jb + 1 < njobs && jobs[jb + 1].tile0 <= tile) ++jb;
+    const SJob &J = jobs[jb];
+    const int lt = tile - J.tile0;
+    bx = lt % J.nbx;
+    const int r = lt / J.nbx;
+    by = r % J.nby;
+    bz = r / J.nby;
+    return J;
+}
+
+// Persistent stage kernels: block n stays on XCD n % 8 and walks that XCD's
+// contiguous range of the global tile list with stride gridDim.x / 8, so the
+// blocks of one XCD work on neighbouring tiles of one run at a time and read
+// each other's halo lines from that XCD's L2.
+template <int U>
+__global__ void __launch_bounds__(512)
+k_velocity_batch(const SJob *__restrict__ jobs, const int njobs, const int total, const int per)
+{
+    const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3, nslot = gridDim.x >> 3;
+    for (int k = slot; k < per; k += nslot) {
+        const int tile = xcd * per + k;
+        if (tile >= total) break;
+        int bx, by, bz;
+        const SJob &J = split_tile(jobs, njobs, tile, bx, by, bz);
+        velocity_tile<U>(J.g, J.s, J.c, J.kc, bx, by, bz);
+    }
+}
+
+template <class P, int U>
+__global__ void __launch_bounds__(512)
+k_stress_batch(const SJob *__restrict__ jobs, const int njobs, const int total, const int per)
+{
+    const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3, nslot = gridDim.x >> 3;
+    for (int k = slot; k < per; k += nslot) {
+        const int tile = xcd * per + k;
+        if (tile >= total) break;
+        int bx, by, bz;
+        const SJob &J = split_tile(jobs, njobs, tile, bx, by, bz);
+        stress_tile<P, U>(J.g, J.s, J.c, J.kc, bx, by, bz);
+    }
+}
+
+// free-surface images of every run: block b of the launch is block
+// b - fs0 of its run's (nx/64) x (ny/4) surface grid
+__global__ void __launch_bounds__(256)
+k_fs_velocity_batch(const SJob *__restrict__ jobs, const int njobs)
+{
+    const int b = blockIdx.x;
+    int jb = 0;
+    while (jb + 1 < njobs && jobs[jb + 1].fs0 <= b) ++jb;
+    const SJob &J = jobs[jb];
+    const int lb = b - J.fs0;
+    const int fnx = (J.g.nx + TX - 1) / TX;
+    fs_velocity_block(J.g, J.s, lb % fnx, lb / fnx);
 }
 
 // ---------------------------------------------------------------------------
@@ -783,6 +866,60 @@ hipError_t launch_sgt_gather(const State &s, int nkept, const long long *kpos, c
     if (nkept <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_sgt_gather, dim3((nkept + 255) / 256), dim3(256), 0, st, s, nkept, kpos, kidx, sg, out,
                        nsamp_max, sample);
+    return hipGetLastError();
+}
+
+template <class P, int U>
+static int split_grid_t(int device, int tile_y, int *gs)
+{
+    int nv = 0, ns = 0, ncu = 0;
+    const int nt = TX * tile_y;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nv, k_velocity_batch<U>, nt, 0) != hipSuccess) nv = 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&ns, k_stress_batch<P, U>, nt, 0) != hipSuccess) ns = 1;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) ncu = 256;
+    ncu = std::max(8, ncu);
+    *gs = std::max(1, ns) * ncu;
+    return std::max(1, nv) * ncu;
+}
+
+void split_batch_grids(bool mixed, const Launch &l, int device, int *grid_v, int *grid_s)
+{
+    const bool u2 = l.unroll == 2;
+    if (mixed)
+        *grid_v = u2 ? split_grid_t<StoreMixed, 2>(device, l.tile_y, grid_s)
+                     : split_grid_t<StoreMixed, 1>(device, l.tile_y, grid_s);
+    else
+        *grid_v = u2 ? split_grid_t<StoreF32, 2>(device, l.tile_y, grid_s)
+                     : split_grid_t<StoreF32, 1>(device, l.tile_y, grid_s);
+}
+
+// whole XCD rounds of persistent blocks, no more than the tiles need
+static int persistent_grid(int grid, int per) { return std::max(8, std::min(grid, 8 * per)) / 8 * 8; }
+
+template <class P, int U>
+static void split_batch_t(const Launch &l, const SJob *d_jobs, int njobs, int total, int fs_total, int grid_v,
+                          int grid_s, hipStream_t st)
+{
+    const int per = (total + 7) / 8;
+    hipLaunchKernelGGL((k_velocity_batch<U>), dim3(persistent_grid(grid_v, per)), dim3(TX, l.tile_y), 0, st,
+                       d_jobs, njobs, total, per);
+    hipLaunchKernelGGL(k_fs_velocity_batch, dim3(fs_total), dim3(TX, 4), 0, st, d_jobs, njobs);
+    hipLaunchKernelGGL((k_stress_batch<P, U>), dim3(persistent_grid(grid_s, per)), dim3(TX, l.tile_y), 0, st,
+                       d_jobs, njobs, total, per);
+}
+
+hipError_t launch_step_split_batch(bool mixed, const Launch &l, const SJob *d_jobs, int njobs, int total,
+                                   int fs_total, int grid_v, int grid_s, hipStream_t st)
+{
+    if (njobs <= 0 || total <= 0) return hipSuccess;
+    const bool u2 = l.unroll == 2;
+    if (mixed) {
+        if (u2) split_batch_t<StoreMixed, 2>(l, d_jobs, njobs, total, fs_total, grid_v, grid_s, st);
+        else split_batch_t<StoreMixed, 1>(l, d_jobs, njobs, total, fs_total, grid_v, grid_s, st);
+    } else {
+        if (u2) split_batch_t<StoreF32, 2>(l, d_jobs, njobs, total, fs_total, grid_v, grid_s, st);
+        else split_batch_t<StoreF32, 1>(l, d_jobs, njobs, total, fs_total, grid_v, grid_s, st);
+    }
     return hipGetLastError();
 }
 

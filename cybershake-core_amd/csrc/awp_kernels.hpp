@@ -119,10 +119,22 @@ __device__ inline int row_exponent(float amax)
     if (amax > 0.0f) frexpf(amax, &e);
     return min(max(e, REXP_MIN), REXP_MAX);
 }
+// Maximum of v >= 0 over the 64 lanes of the wave (all lanes active), the
+// same in every lane.  Non-negative floats order as their bit patterns, so
+// the reduction runs on the integers: four DPP steps (quad permutes [1,0,3,2]
+// and [2,3,0,1], half-row and row mirrors) leave each 16-lane row's maximum
+// in every lane of the row, and the four rows meet through readlane in
+// scalar registers.  (A shuffle per step would cost an LDS-path permute.)
 __device__ inline float wave_max(float v)
 {
-    for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
-    return v;
+    int x = __float_as_int(v);
+    x = max(x, __builtin_amdgcn_update_dpp(0, x, 0xB1, 0xF, 0xF, false));
+    x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x4E, 0xF, 0xF, false));
+    x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x141, 0xF, 0xF, false));
+    x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x140, 0xF, 0xF, false));
+    const int a = max(__builtin_amdgcn_readlane(x, 0), __builtin_amdgcn_readlane(x, 16));
+    const int b = max(__builtin_amdgcn_readlane(x, 32), __builtin_amdgcn_readlane(x, 48));
+    return __int_as_float(max(a, b));
 }
 __device__ inline _Float16 r_encode(float r, int e) { return (_Float16)ldexpf(r, 15 - e); }
 __device__ inline float r_decode(_Float16 h, int e) { return ldexpf((float)h, e - 15); }
@@ -248,6 +260,24 @@ hipError_t launch_step_fused_batch(bool mixed, const Launch &l, const FJob *d_jo
                                    int grid, int par, hipStream_t st);
 // blocks of the batched kernel that fit on the device at once
 int fused_batch_grid(bool mixed, const Launch &l, int device);
+
+// one run of a batched two-stage step (state updated in place); tiles of
+// 64 x tile_y points, x fastest, then y, then z chunk
+struct SJob {
+    Geo g;
+    State s;
+    Coef c;
+    int kc;
+    int nbx, nby, nbz;
+    int tile0;   // first global tile of the run (velocity and stress stages)
+    int fs0;     // first block of the run in the free-surface launch
+};
+// velocity, free-surface and stress stages of every run: three launches,
+// the first and last persistent (grid_v / grid_s blocks, from
+// split_batch_grids); fs_total = sum of the runs' (nx/64) x (ny/4) blocks
+hipError_t launch_step_split_batch(bool mixed, const Launch &l, const SJob *d_jobs, int njobs, int total,
+                                   int fs_total, int grid_v, int grid_s, hipStream_t st);
+void split_batch_grids(bool mixed, const Launch &l, int device, int *grid_v, int *grid_s);
 
 // per-run data of the batched source / SGT stage
 struct AuxJob {

@@ -118,7 +118,11 @@ int awp_sgt_file_geometry(const awp_in3d *p, int *nt_out, int *nflush);
 
 /* Create a context on HIP device `device` (allocates every device buffer;
  * replaces the host's cudaMalloc block, pmcl3d.c:537-823, and
- * SetDeviceConstValue, kernel...cu:24). */
+ * SetDeviceConstValue, kernel...cu:24).  The time step is the two-stage one
+ * (velocity kernel, free-surface images, stress kernel, in place); with
+ * AWP_SCHEME=fused in the environment it is the one-sweep fused kernel,
+ * which holds a second wavefield buffer (36 more bytes per grid point).
+ * Both give the same bits. */
 int awp_create(awp_ctx **out, const awp_in3d *p, int device, int storage,
                char *err, size_t errlen);
 int awp_destroy(awp_ctx *ctx);
@@ -234,18 +238,22 @@ int awp_estimate_bytes(const awp_in3d *p, int storage, int npts, size_t *steady,
 int awp_device_memory(int device, size_t *free_bytes, size_t *total_bytes);
 
 /* ---- batches ------------------------------------------------------------
- * Several initialised runs (same device and storage mode, fused step)
- * advanced together: one launch of the fused step kernel per time step over
- * the tiles of every run, one launch for every run's source and SGT sample,
- * replayed from a HIP graph.  Each run keeps its own outputs
- * (awp_get_sgt, awp_write_sgt_file).  The reference runs one SGT component
- * per MPI job (AWP-ODC-SGT/utils/awp_odc_wrapper.sh:14); these entry points
- * have no counterpart in it. */
+ * Several initialised runs (same device and storage mode) advanced
+ * together: each stage of a time step is one persistent launch over the
+ * tiles of every run (two-stage scheme: velocity, free-surface images,
+ * stress; fused scheme: the one-sweep kernel), then one launch for every
+ * run's source and SGT sample, replayed from a HIP graph.  The fused scheme
+ * is used when every run was created with it (AWP_SCHEME=fused: two
+ * wavefield buffers), else the two-stage one.  Each run keeps its own
+ * outputs (awp_get_sgt, awp_write_sgt_file).  The reference runs one SGT
+ * component per MPI job (AWP-ODC-SGT/utils/awp_odc_wrapper.sh:14); these
+ * entry points have no counterpart in it. */
 typedef struct awp_batch awp_batch;
 int awp_batch_create(awp_batch **out, awp_ctx *const *runs, int nruns, char *err, size_t errlen);
-/* Times the batch's fused kernel for each tile height / prefetch depth /
- * z chunking candidate (reps launches each) and keeps the fastest.  Needs
- * runs that have not stepped; re-initialises them (awp_init) afterwards. */
+/* Times the batch's step kernels for each scheme (fused only when the runs
+ * hold two buffers) / tile height / prefetch depth or z unroll / z chunking
+ * candidate (reps launches each) and keeps the fastest.  Needs runs that
+ * have not stepped; re-initialises them (awp_init) afterwards. */
 int awp_batch_tune(awp_batch *b, int reps);
 /* Advances every run min(nsteps, steps it has left) steps; returns when
  * they are done. */
@@ -254,8 +262,11 @@ int awp_batch_synchronize(awp_batch *b);
 /* Device milliseconds of the last awp_batch_step (the last tuning launch
  * after awp_batch_tune). */
 int awp_batch_time(const awp_batch *b, double *ms);
-/* The kernel variant in use: tile rows, prefetch depth, z chunks of run 0. */
+/* The kernel variant in use: tile rows, prefetch depth (fused) or z unroll
+ * (two-stage), z chunks of run 0. */
 int awp_batch_get_launch(const awp_batch *b, int *tile_y, int *prefetch, int *nchunk);
+/* *fused = 1 for the fused scheme, 0 for the two-stage one. */
+int awp_batch_get_scheme(const awp_batch *b, int *fused);
 const char *awp_batch_last_error(const awp_batch *b);
 int awp_batch_destroy(awp_batch *b);
 

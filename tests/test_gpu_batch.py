@@ -1,11 +1,13 @@
 """Batched runs (awp_batch_*) through the C ABI.
 
-A batch advances several runs with one fused-kernel launch per step over all
-their tiles and replays the steps from a HIP graph.  Every run's SGT and
-fields must equal, bit for bit, those of the same run stepped on its own
-(which the single-run tests tie to the oracle and the reference files),
-whatever the step counts per call (graph pairs plus an odd remainder), the
-kernel variant the tuner picks, and the storage mode.
+A batch advances several runs with one persistent launch per stage of a
+step over all their tiles (two-stage scheme: velocity, free surface, stress;
+fused scheme: one sweep) and replays the steps from a HIP graph.  Every
+run's SGT and fields must equal, bit for bit, those of the same run stepped
+on its own (which the single-run tests tie to the oracle and the reference
+files), whatever the step counts per call (graph pairs plus an odd
+remainder), the scheme and kernel variant the tuner picks, and the storage
+mode.
 """
 import os
 
@@ -39,13 +41,18 @@ def _single(d, storage, nsteps=None):
     return r
 
 
+@pytest.mark.parametrize("scheme", ["split", "fused"])
 @pytest.mark.parametrize("storage", [awpsgt.AWP_STORAGE_FP32, awpsgt.AWP_STORAGE_MIXED])
-def test_batch_equals_single_runs(case_dirs, storage):
+def test_batch_equals_single_runs(case_dirs, storage, scheme, monkeypatch):
     dirs = [case_dirs[n][0] for n in SAME_NT]
+    if scheme == "fused":
+        monkeypatch.setenv("AWP_SCHEME", "fused")
     runs = [_make(d, storage) for d in dirs]
+    monkeypatch.delenv("AWP_SCHEME", raising=False)
     nt = runs[0].p.nt
     assert all(r.p.nt == nt for r in runs)
     with awpsgt.Batch(runs) as b:
+        assert b.fused() == (scheme == "fused")
         # uneven calls: graph pairs, odd remainders, a call past the end
         for n in (1, 36, 7, 40, 1000):
             b.step(n)
@@ -62,10 +69,12 @@ def test_batch_equals_single_runs(case_dirs, storage):
         r.close()
 
 
-def test_batch_uneven_grids_and_tuner(case_dirs, tmp_path):
+@pytest.mark.parametrize("scheme", ["split", "fused"])
+def test_batch_uneven_grids_and_tuner(case_dirs, tmp_path, scheme, monkeypatch):
     """Runs of different grid shapes in one batch (their tiles interleave in
-    the persistent grid), after the tuner has picked a variant; files equal
-    the single-run files byte for byte."""
+    the persistent grids), after the tuner has picked a variant (among both
+    schemes when the runs hold the fused scheme's buffers); files equal the
+    single-run files byte for byte."""
     import tempfile
     dirs = [case_dirs["g1_fx_layered"][0]]
     # two more shapes with the same step count
@@ -78,11 +87,18 @@ def test_batch_uneven_grids_and_tuner(case_dirs, tmp_path):
         d = tempfile.mkdtemp()
         cases.write_case(d, spec, media, pts)
         dirs.append(d)
+    if scheme == "fused":
+        monkeypatch.setenv("AWP_SCHEME", "fused")
     runs = [_make(d, awpsgt.AWP_STORAGE_FP32) for d in dirs]
+    monkeypatch.delenv("AWP_SCHEME", raising=False)
     nt = min(r.p.nt for r in runs)
     with awpsgt.Batch(runs) as b:
         ty, pf, nc = b.tune(1)
-        assert ty in (4, 7, 8) and pf in (0, 1, 2) and nc in (1, 2)
+        if b.fused():
+            assert scheme == "fused"
+            assert ty in (4, 7, 8) and pf in (0, 1, 2) and nc in (1, 2)
+        else:
+            assert ty in (4, 8) and pf in (1, 2) and nc in (1, 2)
         assert all(r.progress() == (0, 0) for r in runs)
         b.step(nt)
     for k, (d, r) in enumerate(zip(dirs, runs)):

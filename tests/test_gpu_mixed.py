@@ -66,11 +66,14 @@ def test_mixed_media_bitwise(case_dirs):
             assert np.max(np.abs(a - b)) <= np.max(np.abs(b)) * 2.0 ** -11
 
 
+@pytest.mark.parametrize("scheme", ["split", "fused"])
 @pytest.mark.parametrize("name", list(cases.CASES))
-def test_mixed_fields_bitwise_after_steps(case_dirs, name):
+def test_mixed_fields_bitwise_after_steps(case_dirs, name, scheme, monkeypatch):
     d, _ = case_dirs[name]
     p, media, xyz, stf, pts = _inputs(d)
     o = _mixed_oracle(d, media, xyz, stf, pts)
+    if scheme == "fused":
+        monkeypatch.setenv("AWP_SCHEME", "fused")
     with _run(p, media, xyz, stf, pts, awpsgt.AWP_STORAGE_MIXED, nsteps=0) as r:
         for target in (1, 9, 40):
             r.step(target - r.progress()[0])
