@@ -230,6 +230,7 @@ def bench_batched(a, storage):
     kern = []
     launch = None
     fused = False
+    persist = True
     for w in waves:
         runs = []
         t0 = time.time()
@@ -245,12 +246,13 @@ def bench_batched(a, storage):
         with awpsgt.Batch(runs) as b:
             if not a.no_tune:
                 launch = b.tune(2)
-                log("tuned: %s, tile rows %d, %s %d, z chunks %d (%.2f ms per step's kernels)"
-                    % ("fused" if b.fused() else "two-stage", launch[0],
+                log("tuned: %s, %s grids, tile rows %d, %s %d, z chunks %d (%.2f ms per step's kernels)"
+                    % ("fused" if b.fused() else "two-stage",
+                       "persistent" if b.scheme()[1] else "one-block-per-tile", launch[0],
                        "prefetch" if b.fused() else "z unroll", launch[1], launch[2], b.time_ms()))
             else:
                 launch = b.launch()
-            fused = b.fused()
+            fused, persist = b.scheme()
             b.step(a.warmup)
             torch.cuda.synchronize()
             t = time.perf_counter()
@@ -283,9 +285,9 @@ def bench_batched(a, storage):
         ", FP32", ", mixed storage (FP16 r1..r6 with per-row exponents, FP16 qp/qs)")
     config = {"workload": wl, "runs": len(jobs), "waves": [len(w) for w in waves],
               "points_per_step": int(pts_step), "storage": storage,
-              "step": ("fused" if fused else "two-stage") + ", persistent launch per stage, HIP graph",
+              "step": ("fused" if fused else "two-stage") + ", one launch per stage over all runs, HIP graph",
               "launch": {"tile_rows": launch[0], ("prefetch" if fused else "z_unroll"): launch[1],
-                         "z_chunks": launch[2]} if launch else None,
+                         "z_chunks": launch[2], "persistent_grid": persist} if launch else None,
               "budget_gb": round(budget / 1e9, 1), "parallelism": "single GPU"}
     return value, ms_step, roof, config
 

@@ -139,7 +139,7 @@ def lib() -> ctypes.CDLL:
         "awp_batch_time": ([vp, ctypes.POINTER(ctypes.c_double)], ip),
         "awp_batch_get_launch": ([vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
                                   ctypes.POINTER(ctypes.c_int)], ip),
-        "awp_batch_get_scheme": ([vp, ctypes.POINTER(ctypes.c_int)], ip),
+        "awp_batch_get_scheme": ([vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)], ip),
         "awp_batch_last_error": ([vp], ctypes.c_char_p),
         "awp_batch_destroy": ([vp], ip),
     }
@@ -389,11 +389,15 @@ class Batch:
                  "awp_batch_get_launch")
         return a.value, b.value, c.value
 
+    def scheme(self) -> Tuple[bool, bool]:
+        """(fused one-sweep scheme?, persistent stage grids?)"""
+        f, p = ctypes.c_int(0), ctypes.c_int(0)
+        self._do(lib().awp_batch_get_scheme(self.h, ctypes.byref(f), ctypes.byref(p)), "awp_batch_get_scheme")
+        return bool(f.value), bool(p.value)
+
     def fused(self) -> bool:
         """True when the batch uses the fused one-sweep scheme."""
-        f = ctypes.c_int(0)
-        self._do(lib().awp_batch_get_scheme(self.h, ctypes.byref(f)), "awp_batch_get_scheme")
-        return bool(f.value)
+        return self.scheme()[0]
 
     def step(self, n: int) -> None:
         self._do(lib().awp_batch_step(self.h, int(n)), "awp_batch_step")

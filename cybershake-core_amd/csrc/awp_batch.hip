@@ -40,6 +40,7 @@ struct awp_batch {
     bool mixed;
     bool split;                  // two-stage scheme (else fused)
     bool can_fuse;               // every run holds the fused scheme's two buffers
+    bool persist;                // persistent grids (else one block per tile)
     Launch l;                    // tile_y and prefetch depth / z unroll (unroll) of the batch
     std::vector<int> nchunk;     // z chunks per run
     hipStream_t stream;
@@ -190,9 +191,15 @@ static hipError_t launch_kernels(awp_batch *b, int par)
     return launch_step_fused_batch(b->mixed, b->l, b->d_jobs, n, b->total, b->grid, par, b->stream);
 }
 
-// persistent grid sizes of the current scheme and launch shape
+// grid sizes of the current scheme and launch shape: the blocks that fit on
+// the device at once (persistent), or one block per tile (the launchers
+// clamp the grid to the tile count)
 static void set_grids(awp_batch *b)
 {
+    if (!b->persist) {
+        b->grid = b->grid_v = b->grid_s = 1 << 30;
+        return;
+    }
     if (b->split)
         split_batch_grids(b->mixed, b->l, b->device, &b->grid_v, &b->grid_s);
     else
@@ -264,6 +271,7 @@ extern "C" int awp_batch_create(awp_batch **out, awp_ctx *const *runs, int nruns
     b->can_fuse = true;
     for (int r = 0; r < nruns; ++r) b->can_fuse = b->can_fuse && runs[r]->fused;
     b->split = !b->can_fuse;
+    b->persist = true;
     b->l = runs[0]->l;
     if (b->split) {
         // two-stage kernels: 64 x {1, 2, 4, 8} tiles, z unroll 1 or 2
@@ -376,10 +384,11 @@ extern "C" int awp_batch_time(const awp_batch *b, double *ms)
     return AWP_OK;
 }
 
-extern "C" int awp_batch_get_scheme(const awp_batch *b, int *fused)
+extern "C" int awp_batch_get_scheme(const awp_batch *b, int *fused, int *persistent)
 {
-    if (!b || !fused) return AWP_EINVAL;
-    *fused = b->split ? 0 : 1;
+    if (!b) return AWP_EINVAL;
+    if (fused) *fused = b->split ? 0 : 1;
+    if (persistent) *persistent = b->persist ? 1 : 0;
     return AWP_OK;
 }
 
@@ -400,24 +409,28 @@ extern "C" int awp_batch_tune(awp_batch *b, int reps)
     if (reps < 1) reps = 2;
     BTRY(b, hipSetDevice(b->device));
     struct Cand {
-        bool split;
+        bool split, persist;
         int ty, pf, nchunk;
     };
     std::vector<Cand> cands;
-    // two-stage: tile rows x z unroll x z chunks
-    for (int ty : {4, 8})
-        for (int un : {1, 2})
-            for (int nc : {1, 2}) cands.push_back({true, ty, un, nc});
-    // fused: tile rows x prefetch depth (64x8 tiles only without prefetch:
-    // their 13-wave blocks have 128 registers per lane) x z chunks
-    const int tp[6][2] = {{8, 0}, {7, 0}, {7, 1}, {7, 2}, {4, 0}, {4, 2}};
-    if (b->can_fuse)
-        for (const auto &v : tp)
-            for (int nc : {1, 2}) cands.push_back({false, v[0], v[1], nc});
+    for (bool ps : {true, false}) {
+        // two-stage: tile rows x z unroll x z chunks
+        for (int ty : {4, 8})
+            for (int un : {1, 2})
+                for (int nc : {1, 2}) cands.push_back({true, ps, ty, un, nc});
+        // fused: tile rows x prefetch depth (64x8 tiles only without
+        // prefetch: their 13-wave blocks have 128 registers per lane) x z
+        // chunks
+        const int tp[6][2] = {{8, 0}, {7, 0}, {7, 1}, {7, 2}, {4, 0}, {4, 2}};
+        if (b->can_fuse)
+            for (const auto &v : tp)
+                for (int nc : {1, 2}) cands.push_back({false, ps, v[0], v[1], nc});
+    }
     double best = 1e30;
-    Cand pick = {b->split, b->l.tile_y, b->l.unroll, 1};
+    Cand pick = {b->split, b->persist, b->l.tile_y, b->l.unroll, 1};
     for (const Cand &cd : cands) {
         b->split = cd.split;
+        b->persist = cd.persist;
         b->l.tile_y = cd.ty;
         b->l.unroll = cd.pf;
         std::fill(b->nchunk.begin(), b->nchunk.end(), cd.nchunk);
@@ -438,6 +451,7 @@ extern "C" int awp_batch_tune(awp_batch *b, int reps)
         }
     }
     b->split = pick.split;
+    b->persist = pick.persist;
     b->l.tile_y = pick.ty;
     b->l.unroll = pick.pf;
     std::fill(b->nchunk.begin(), b->nchunk.end(), pick.nchunk);

@@ -58,19 +58,19 @@ __device__ inline bool ftile_of(const FTiles &t, int &bx, int &by, int &bz)
 }
 
 // Loads and stores at a 32-bit byte offset from an array base.  The bases
-// are kernel arguments (scalar registers), so each access takes the
+// are uniform (scalar registers) global pointers, so each access takes the
 // scalar-base + 32-bit vector-offset form and one offset register serves
 // every array; +-1, +-2 x neighbours become immediate offsets.  The caller
 // keeps every array below 4 GiB.
 template <class T>
-__device__ __forceinline__ T gl(const T *base, unsigned off, int imm = 0)
+__device__ __forceinline__ T gl(const G<T> *base, unsigned off, int imm = 0)
 {
-    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + off + imm);
+    return *reinterpret_cast<const G<T> *>(reinterpret_cast<const G<char> *>(base) + off + imm);
 }
 template <class T>
-__device__ __forceinline__ void gs(T *base, unsigned off, T v, int imm = 0)
+__device__ __forceinline__ void gs(G<T> *base, unsigned off, T v, int imm = 0)
 {
-    *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + off + imm) = v;
+    *reinterpret_cast<G<T> *>(reinterpret_cast<G<char> *>(base) + off + imm) = v;
 }
 
 }  // namespace
@@ -103,9 +103,10 @@ struct StrIn {
 // issued before the stress arithmetic.
 // One column tile (bx, by, z chunk bz) of one run: the whole block calls it.
 template <class P, int TY, int PF>
-__device__ __forceinline__ void step_tile(const Geo &g, const State &a, const State &b, const Coef &c,
+__device__ __forceinline__ void step_tile(const Geo &g, const State &a0, const State &b0, const Coef &c,
                                           const int kc, const int bx, const int by, const int bz, float *lds)
 {
+    const GState a = as_global(a0), b = as_global(b0);
     using Sh = FShape<TY>;
     using Q = typename P::Q;
     using R = typename P::R;
@@ -151,16 +152,16 @@ __device__ __forceinline__ void step_tile(const Geo &g, const State &a, const St
     const float t2e = c.tau2[itxy], t2o = c.tau2[itxy + 1];
     const int lpos = rr * Sh::RW + rc;
     constexpr int PL = Sh::PLANE;
-    const float *__restrict__ lam = a.lam;
-    const float *__restrict__ mu = a.mu;
-    const Q *__restrict__ qp = (const Q *)a.qp;
-    const Q *__restrict__ qs = (const Q *)a.qs;
-    R *__restrict__ R1 = (R *)a.r1;
-    R *__restrict__ R2 = (R *)a.r2;
-    R *__restrict__ R3 = (R *)a.r3;
-    R *__restrict__ R4 = (R *)a.r4;
-    R *__restrict__ R5 = (R *)a.r5;
-    R *__restrict__ R6 = (R *)a.r6;
+    const G<float> *__restrict__ lam = a.lam;
+    const G<float> *__restrict__ mu = a.mu;
+    const G<Q> *__restrict__ qp = (const G<Q> *)a.qp;
+    const G<Q> *__restrict__ qs = (const G<Q> *)a.qs;
+    G<R> *__restrict__ R1 = (G<R> *)a.r1;
+    G<R> *__restrict__ R2 = (G<R> *)a.r2;
+    G<R> *__restrict__ R3 = (G<R> *)a.r3;
+    G<R> *__restrict__ R4 = (G<R> *)a.r4;
+    G<R> *__restrict__ R5 = (G<R> *)a.r5;
+    G<R> *__restrict__ R6 = (G<R> *)a.r6;
 
     // loads of the velocity stage at level m (dom threads, -1 <= m <= nz;
     // every address lies in the allocation, the ghost levels hold zeros)
@@ -191,7 +192,7 @@ __device__ __forceinline__ void step_tile(const Geo &g, const State &a, const St
         const unsigned orr = o >> RSH;
         if constexpr (P::mixed) {
             // FP16 values under the row segment's exponent (one per wave)
-            const signed char *ex = a.rexp + rexp_idx(g, a.nseg, bx, j, s);
+            const G<signed char> *ex = a.rexp + rexp_idx(g, a.nseg, bx, j, s);
             const long long n = a.rexp_n;
             x.r1 = r_decode(gl(R1, orr), ex[0]); x.r2 = r_decode(gl(R2, orr), ex[n]);
             x.r3 = r_decode(gl(R3, orr), ex[2 * n]); x.r4 = r_decode(gl(R4, orr), ex[3 * n]);
@@ -485,7 +486,7 @@ __device__ __forceinline__ void step_tile(const Geo &g, const State &a, const St
             // whole waves (one row segment each): lanes outside the domain hold 0
             if (is_main && s >= kb) {
                 const unsigned orr = (ob + (unsigned)s * sz4) >> RSH;
-                R *RR[6] = {R1, R2, R3, R4, R5, R6};
+                G<R> *RR[6] = {R1, R2, R3, R4, R5, R6};
 #pragma unroll
                 for (int q = 0; q < 6; ++q) {
                     const int e = row_exponent(wave_max(fabsf(rn[q])));

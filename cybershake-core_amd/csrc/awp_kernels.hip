@@ -49,9 +49,10 @@ __device__ inline bool tile_of(const Tiles &t, int &bx, int &by, int &bz)
 // (cerjan.f:292-328)
 // ---------------------------------------------------------------------------
 template <int U>
-__device__ inline void velocity_tile(const Geo &g, const State &s, const Coef &c, const int kc, const int bx,
+__device__ inline void velocity_tile(const Geo &g, const State &s0, const Coef &c, const int kc, const int bx,
                                      const int by, const int bz)
 {
+    const GState s = as_global(s0);
     // the velocity stage reads only FP32 arrays in both storage modes
     const int i = bx * TX + threadIdx.x + 1;
     const int j = by * blockDim.y + threadIdx.y + 1;
@@ -61,13 +62,13 @@ __device__ inline void velocity_tile(const Geo &g, const State &s, const Coef &c
     const long long sy = g.sy, sz = g.sz;
     long long o = g.base + (long long)kb * sz + (long long)j * sy + i;
 
-    const float *__restrict__ xx = s.xx;
-    const float *__restrict__ yy = s.yy;
-    const float *__restrict__ zz = s.zz;
-    const float *__restrict__ xy = s.xy;
-    const float *__restrict__ xz = s.xz;
-    const float *__restrict__ yz = s.yz;
-    const float *__restrict__ d1 = s.d1;
+    const G<float> *__restrict__ xx = s.xx;
+    const G<float> *__restrict__ yy = s.yy;
+    const G<float> *__restrict__ zz = s.zz;
+    const G<float> *__restrict__ xy = s.xy;
+    const G<float> *__restrict__ xz = s.xz;
+    const G<float> *__restrict__ yz = s.yz;
+    const G<float> *__restrict__ d1 = s.d1;
     const float c1 = c.c1, c2 = c.c2, dth = c.dth;
     const float fxy = s.fx[i] * s.fy[j];
 
@@ -128,15 +129,16 @@ __device__ inline void velocity_tile(const Geo &g, const State &s, const Coef &c
 // u(i+1), v(j-1) at nz+1 are recomputed from the same inputs fvelxy uses,
 // so no thread reads another thread's output.
 // ---------------------------------------------------------------------------
-__device__ inline void fs_velocity_block(const Geo &g, const State &s, const int bx, const int by)
+__device__ inline void fs_velocity_block(const Geo &g, const State &s0, const int bx, const int by)
 {
+    const GState s = as_global(s0);
     const int i = bx * TX + threadIdx.x + 1;
     const int j = by * blockDim.y + threadIdx.y + 1;
     if (i > g.nx || j > g.ny) return;
     const long long sy = g.sy, sz = g.sz;
     const long long p = gidx(g, i, j, g.nz);
     const long long q = p + sz;
-    const float *u1 = s.u1, *v1 = s.v1, *w1 = s.w1;
+    const G<float> *u1 = s.u1, *v1 = s.v1, *w1 = s.w1;
     const float w0 = w1[p];
     const float ut = u1[p] - (w0 - w1[p - 1]);
     const float vt = v1[p] - (w1[p + sy] - w0);
@@ -161,9 +163,10 @@ __device__ inline void fs_velocity_block(const Geo &g, const State &s, const int
 // (operator.f:575-612).
 // ---------------------------------------------------------------------------
 template <class P, int U>
-__device__ inline void stress_tile(const Geo &g, const State &s, const Coef &c, const int kc, const int bx,
+__device__ inline void stress_tile(const Geo &g, const State &s0, const Coef &c, const int kc, const int bx,
                                    const int by, const int bz)
 {
+    const GState s = as_global(s0);
     using Q = typename P::Q;
     using R = typename P::R;
     const int ir = bx * TX + threadIdx.x + 1;
@@ -181,19 +184,19 @@ __device__ inline void stress_tile(const Geo &g, const State &s, const Coef &c, 
     const long long sy = g.sy, sz = g.sz;
     long long o = g.base + (long long)kb * sz + (long long)j * sy + i;
 
-    const float *__restrict__ u1 = s.u1;
-    const float *__restrict__ v1 = s.v1;
-    const float *__restrict__ w1 = s.w1;
-    const float *__restrict__ lam = s.lam;
-    const float *__restrict__ mu = s.mu;
-    const Q *__restrict__ qp = (const Q *)s.qp;
-    const Q *__restrict__ qs = (const Q *)s.qs;
-    R *__restrict__ R1 = (R *)s.r1;
-    R *__restrict__ R2 = (R *)s.r2;
-    R *__restrict__ R3 = (R *)s.r3;
-    R *__restrict__ R4 = (R *)s.r4;
-    R *__restrict__ R5 = (R *)s.r5;
-    R *__restrict__ R6 = (R *)s.r6;
+    const G<float> *__restrict__ u1 = s.u1;
+    const G<float> *__restrict__ v1 = s.v1;
+    const G<float> *__restrict__ w1 = s.w1;
+    const G<float> *__restrict__ lam = s.lam;
+    const G<float> *__restrict__ mu = s.mu;
+    const G<Q> *__restrict__ qp = (const G<Q> *)s.qp;
+    const G<Q> *__restrict__ qs = (const G<Q> *)s.qs;
+    G<R> *__restrict__ R1 = (G<R> *)s.r1;
+    G<R> *__restrict__ R2 = (G<R> *)s.r2;
+    G<R> *__restrict__ R3 = (G<R> *)s.r3;
+    G<R> *__restrict__ R4 = (G<R> *)s.r4;
+    G<R> *__restrict__ R5 = (G<R> *)s.r5;
+    G<R> *__restrict__ R6 = (G<R> *)s.r6;
     const float c1 = c.c1, c2 = c.c2, dth = c.dth, dt = c.dt, dh1 = c.dh1;
     const float fxy = s.fx[i] * s.fy[j];
     // relaxation weights of this column for even / odd depth parity
@@ -262,7 +265,7 @@ __device__ inline void stress_tile(const Geo &g, const State &s, const Coef &c, 
 
         float r1, r2, r3, r4, r5, r6;
         if constexpr (P::mixed) {
-            const signed char *ex = s.rexp + rexp_idx(g, s.nseg, bx, j, k);
+            const G<signed char> *ex = s.rexp + rexp_idx(g, s.nseg, bx, j, k);
             const long long n = s.rexp_n;
             r1 = r_decode(R1[o], ex[0]); r2 = r_decode(R2[o], ex[n]); r3 = r_decode(R3[o], ex[2 * n]);
             r4 = r_decode(R4[o], ex[3 * n]); r5 = r_decode(R5[o], ex[4 * n]); r6 = r_decode(R6[o], ex[5 * n]);
@@ -313,7 +316,7 @@ __device__ inline void stress_tile(const Geo &g, const State &s, const Coef &c, 
         // are rounded to the storage format once, on the store
         if constexpr (P::mixed) {
             const float rv[6] = {r1, r2, r3, r4, r5, r6};
-            R *RR[6] = {R1, R2, R3, R4, R5, R6};
+            G<R> *RR[6] = {R1, R2, R3, R4, R5, R6};
             const long long xi = rexp_idx(g, s.nseg, bx, j, k);
 #pragma unroll
             for (int q = 0; q < 6; ++q) {

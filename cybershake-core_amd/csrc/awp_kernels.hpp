@@ -161,6 +161,46 @@ struct State {
     MScale ms;
 };
 
+// Pointers a kernel reads from a job table in device memory are generic
+// (flat) pointers to the compiler, and loads through them become flat loads,
+// which wait on both the vector-memory and the LDS counters (kernel-argument
+// pointers are promoted to global ones by the compiler; table pointers are
+// not).  The volume kernels therefore access the state through GState, the
+// same pointers typed in the global address space.
+template <class T>
+using G = __attribute__((address_space(1))) T;
+
+struct GState {
+    G<float> *u1, *v1, *w1;
+    G<float> *xx, *yy, *zz, *xy, *xz, *yz;
+    G<float> *d1, *mu, *lam;
+    G<void> *r1, *r2, *r3, *r4, *r5, *r6;
+    G<signed char> *rexp;
+    long long rexp_n;
+    int nseg;
+    G<void> *qp, *qs;
+    G<float> *fx, *fy, *fz;
+    MScale ms;
+};
+
+__device__ __forceinline__ GState as_global(const State &a)
+{
+    GState s;
+    s.u1 = (G<float> *)a.u1; s.v1 = (G<float> *)a.v1; s.w1 = (G<float> *)a.w1;
+    s.xx = (G<float> *)a.xx; s.yy = (G<float> *)a.yy; s.zz = (G<float> *)a.zz;
+    s.xy = (G<float> *)a.xy; s.xz = (G<float> *)a.xz; s.yz = (G<float> *)a.yz;
+    s.d1 = (G<float> *)a.d1; s.mu = (G<float> *)a.mu; s.lam = (G<float> *)a.lam;
+    s.r1 = (G<void> *)a.r1; s.r2 = (G<void> *)a.r2; s.r3 = (G<void> *)a.r3;
+    s.r4 = (G<void> *)a.r4; s.r5 = (G<void> *)a.r5; s.r6 = (G<void> *)a.r6;
+    s.rexp = (G<signed char> *)a.rexp;
+    s.rexp_n = a.rexp_n;
+    s.nseg = a.nseg;
+    s.qp = (G<void> *)a.qp; s.qs = (G<void> *)a.qs;
+    s.fx = (G<float> *)a.fx; s.fy = (G<float> *)a.fy; s.fz = (G<float> *)a.fz;
+    s.ms = a.ms;
+    return s;
+}
+
 // FP32 media (the storage in FP32 mode; in mixed mode qp/qs point to set-up
 // temporaries)
 struct Media32 {

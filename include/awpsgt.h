@@ -252,7 +252,8 @@ typedef struct awp_batch awp_batch;
 int awp_batch_create(awp_batch **out, awp_ctx *const *runs, int nruns, char *err, size_t errlen);
 /* Times the batch's step kernels for each scheme (fused only when the runs
  * hold two buffers) / tile height / prefetch depth or z unroll / z chunking
- * candidate (reps launches each) and keeps the fastest.  Needs runs that
+ * / persistent or one-block-per-tile grid candidate (reps launches each)
+ * and keeps the fastest.  Needs runs that
  * have not stepped; re-initialises them (awp_init) afterwards. */
 int awp_batch_tune(awp_batch *b, int reps);
 /* Advances every run min(nsteps, steps it has left) steps; returns when
@@ -265,8 +266,11 @@ int awp_batch_time(const awp_batch *b, double *ms);
 /* The kernel variant in use: tile rows, prefetch depth (fused) or z unroll
  * (two-stage), z chunks of run 0. */
 int awp_batch_get_launch(const awp_batch *b, int *tile_y, int *prefetch, int *nchunk);
-/* *fused = 1 for the fused scheme, 0 for the two-stage one. */
-int awp_batch_get_scheme(const awp_batch *b, int *fused);
+/* *fused = 1 for the fused scheme, 0 for the two-stage one; *persistent = 1
+ * when the stage kernels run persistent grids (the blocks that fit on the
+ * GPU at once walk the tile list), 0 for one block per tile (the tuner
+ * picks).  Either pointer may be NULL. */
+int awp_batch_get_scheme(const awp_batch *b, int *fused, int *persistent);
 const char *awp_batch_last_error(const awp_batch *b);
 int awp_batch_destroy(awp_batch *b);
 
