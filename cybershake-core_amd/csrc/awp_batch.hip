@@ -41,6 +41,7 @@ struct awp_batch {
     bool split;                  // two-stage scheme (else fused)
     bool can_fuse;               // every run holds the fused scheme's two buffers
     bool persist;                // persistent grids (else one block per tile)
+    bool o32;                    // 32-bit byte offsets reach every run's arrays
     Launch l;                    // tile_y and prefetch depth / z unroll (unroll) of the batch
     std::vector<int> nchunk;     // z chunks per run
     hipStream_t stream;
@@ -186,8 +187,8 @@ static hipError_t launch_kernels(awp_batch *b, int par)
 {
     const int n = (int)b->runs.size();
     if (b->split)
-        return launch_step_split_batch(b->mixed, b->l, b->d_sjobs, n, b->total, b->fs_total, b->grid_v,
-                                       b->grid_s, b->stream);
+        return launch_step_split_batch(b->mixed, b->l, b->o32, b->d_sjobs, n, b->total, b->fs_total,
+                                       b->grid_v, b->grid_s, b->stream);
     return launch_step_fused_batch(b->mixed, b->l, b->d_jobs, n, b->total, b->grid, par, b->stream);
 }
 
@@ -272,6 +273,8 @@ extern "C" int awp_batch_create(awp_batch **out, awp_ctx *const *runs, int nruns
     for (int r = 0; r < nruns; ++r) b->can_fuse = b->can_fuse && runs[r]->fused;
     b->split = !b->can_fuse;
     b->persist = true;
+    b->o32 = true;
+    for (int r = 0; r < nruns; ++r) b->o32 = b->o32 && runs[r]->l.o32;
     b->l = runs[0]->l;
     if (b->split) {
         // two-stage kernels: 64 x {1, 2, 4, 8} tiles, z unroll 1 or 2
@@ -411,26 +414,35 @@ extern "C" int awp_batch_tune(awp_batch *b, int reps)
     struct Cand {
         bool split, persist;
         int ty, pf, nchunk;
+        int sform;   // two-stage stress kernel: 0 = 64-bit offsets, blocks as registers allow;
+                     // 1 = 32-bit offsets, 2 blocks per CU
     };
     std::vector<Cand> cands;
     for (bool ps : {true, false}) {
-        // two-stage: tile rows x z unroll x z chunks
+        // two-stage: tile rows x z unroll x z chunks x stress kernel form
         for (int ty : {4, 8})
             for (int un : {1, 2})
-                for (int nc : {1, 2}) cands.push_back({true, ps, ty, un, nc});
+                for (int nc : {1, 2})
+                    for (int sf : {0, 1}) cands.push_back({true, ps, ty, un, nc, sf});
         // fused: tile rows x prefetch depth (64x8 tiles only without
         // prefetch: their 13-wave blocks have 128 registers per lane) x z
         // chunks
         const int tp[6][2] = {{8, 0}, {7, 0}, {7, 1}, {7, 2}, {4, 0}, {4, 2}};
         if (b->can_fuse)
             for (const auto &v : tp)
-                for (int nc : {1, 2}) cands.push_back({false, ps, v[0], v[1], nc});
+                for (int nc : {1, 2}) cands.push_back({false, ps, v[0], v[1], nc, 0});
     }
+    auto apply_form = [b](int sf) {
+        b->l.o32_s = sf == 1;
+        b->l.bpc_s = sf == 1 ? 2 : 0;
+        set_block_caps(b->l, b->device);
+    };
     double best = 1e30;
-    Cand pick = {b->split, b->persist, b->l.tile_y, b->l.unroll, 1};
+    Cand pick = {b->split, b->persist, b->l.tile_y, b->l.unroll, 1, b->l.o32_s ? 1 : 0};
     for (const Cand &cd : cands) {
         b->split = cd.split;
         b->persist = cd.persist;
+        apply_form(cd.sform);
         b->l.tile_y = cd.ty;
         b->l.unroll = cd.pf;
         std::fill(b->nchunk.begin(), b->nchunk.end(), cd.nchunk);
@@ -452,6 +464,7 @@ extern "C" int awp_batch_tune(awp_batch *b, int reps)
     }
     b->split = pick.split;
     b->persist = pick.persist;
+    apply_form(pick.sform);
     b->l.tile_y = pick.ty;
     b->l.unroll = pick.pf;
     std::fill(b->nchunk.begin(), b->nchunk.end(), pick.nchunk);

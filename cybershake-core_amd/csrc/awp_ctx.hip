@@ -26,7 +26,19 @@ using namespace awp;
 
 extern "C" const char *awp_last_error(const awp_ctx *ctx) { return ctx ? ctx->err : "null context"; }
 
-static Launch choose_launch(const Geo &g, bool fused, bool mixed)
+// 32-bit byte offsets where every array is below 4 GiB
+static bool use_o32(const Geo &g) { return offsets32(g); }
+
+// The default is the two-stage step (velocity kernel, then stress kernel);
+// AWP_SCHEME=fused selects the fused one-sweep step, which addresses every
+// array with 32-bit byte offsets.
+static bool want_fused(const Geo &g)
+{
+    const char *scheme = getenv("AWP_SCHEME");
+    return scheme && !strcmp(scheme, "fused") && use_o32(g);
+}
+
+static Launch choose_launch(const Geo &g, bool fused, bool mixed, int device)
 {
     Launch l;
     l.tile_y = 8;
@@ -55,6 +67,27 @@ static Launch choose_launch(const Geo &g, bool fused, bool mixed)
             l.ph = h;
         }
     }
+    l.o32 = use_o32(g);
+    // Form and blocks per CU of the two-stage stress kernel, measured at
+    // 1500x1000x300: FP32 runs fastest in the element-indexed form with as
+    // many blocks as registers allow (one per CU), mixed storage in the
+    // 32-bit byte-offset form (125 registers) with 2 blocks per CU.
+    // AWP_O32=0/1 (stress form) and AWP_BPC=<velocity>x<stress> (0 = as
+    // registers allow) override.
+    l.o32_s = mixed;
+    l.bpc_v = 0;
+    l.bpc_s = mixed ? 2 : 0;
+    const char *o32e = getenv("AWP_O32");
+    if (o32e && (!strcmp(o32e, "0") || !strcmp(o32e, "1"))) l.o32_s = o32e[0] == '1';
+    const char *bpc = getenv("AWP_BPC");
+    if (bpc) {
+        int v = 0, s2 = 0;
+        if (sscanf(bpc, "%dx%d", &v, &s2) == 2 && v >= 0 && v <= 16 && s2 >= 0 && s2 <= 16) {
+            l.bpc_v = v;
+            l.bpc_s = s2;
+        }
+    }
+    set_block_caps(l, device);
     if (fused) {
         // the fused step keeps its load prefetch depth (0, 1, 2) in `unroll`
         const char *pf = getenv("AWP_PF");
@@ -108,13 +141,8 @@ extern "C" int awp_create(awp_ctx **out, const awp_in3d *p, int device, int stor
     ctx->device = device;
     ctx->storage = storage;
     ctx->g = make_geo(p->nx, p->ny, p->nz);
-    // the default is the two-stage step (velocity kernel, then stress
-    // kernel); AWP_SCHEME=fused selects the fused one-sweep step
-    const char *scheme = getenv("AWP_SCHEME");
-    ctx->fused = scheme && !strcmp(scheme, "fused");
-    // the fused kernel addresses every array with 32-bit byte offsets
-    if ((unsigned long long)ctx->g.nvol * 4ull >= (1ull << 32)) ctx->fused = false;
-    ctx->l = choose_launch(ctx->g, ctx->fused, storage == AWP_STORAGE_MIXED);
+    ctx->fused = want_fused(ctx->g);
+    ctx->l = choose_launch(ctx->g, ctx->fused, storage == AWP_STORAGE_MIXED, device);
     ctx->vol = nullptr;
     ctx->prof = nullptr;
     ctx->npsrc = 0;
@@ -638,9 +666,7 @@ extern "C" int awp_estimate_bytes(const awp_in3d *p, int storage, int npts, size
 {
     if (!p || (storage != AWP_STORAGE_FP32 && storage != AWP_STORAGE_MIXED) || npts < 0) return AWP_EINVAL;
     const Geo g = make_geo(p->nx, p->ny, p->nz);
-    const char *scheme = getenv("AWP_SCHEME");
-    bool fused = scheme && !strcmp(scheme, "fused");
-    if ((unsigned long long)g.nvol * 4ull >= (1ull << 32)) fused = false;
+    const bool fused = want_fused(g);
     const size_t nv = (size_t)g.nvol;
     const size_t esz = storage == AWP_STORAGE_MIXED ? 2 : 4;
     const size_t eslot = nv * esz + (esz == 2 ? 128 : 0);

@@ -57,22 +57,6 @@ __device__ inline bool ftile_of(const FTiles &t, int &bx, int &by, int &bz)
     return true;
 }
 
-// Loads and stores at a 32-bit byte offset from an array base.  The bases
-// are uniform (scalar registers) global pointers, so each access takes the
-// scalar-base + 32-bit vector-offset form and one offset register serves
-// every array; +-1, +-2 x neighbours become immediate offsets.  The caller
-// keeps every array below 4 GiB.
-template <class T>
-__device__ __forceinline__ T gl(const G<T> *base, unsigned off, int imm = 0)
-{
-    return *reinterpret_cast<const G<T> *>(reinterpret_cast<const G<char> *>(base) + off + imm);
-}
-template <class T>
-__device__ __forceinline__ void gs(G<T> *base, unsigned off, T v, int imm = 0)
-{
-    *reinterpret_cast<G<T> *>(reinterpret_cast<G<char> *>(base) + off + imm) = v;
-}
-
 }  // namespace
 
 // Old-state values the velocity of one level needs besides the z queues.

@@ -162,8 +162,16 @@ __device__ inline void fs_velocity_block(const Geo &g, const State &s0, const in
 // (cerjan.f:249-290) and, in the block that reaches k = nz, fstr
 // (operator.f:575-612).
 // ---------------------------------------------------------------------------
+// Two forms of the same statements: stress_tile_e addresses the arrays with
+// 64-bit element indices (any size), stress_tile_b through a 32-bit byte
+// offset cursor (arrays below 4 GiB).  The compiler schedules them
+// differently: the element-indexed form keeps more loads in flight (189
+// registers, one 512-thread block per CU) and is the faster FP32 kernel; the
+// byte-offset form needs 125 registers in mixed storage, where two blocks per
+// CU hide the latency of the exponent round trip, and is the faster mixed
+// kernel.
 template <class P, int U>
-__device__ inline void stress_tile(const Geo &g, const State &s0, const Coef &c, const int kc, const int bx,
+__device__ inline void stress_tile_e(const Geo &g, const State &s0, const Coef &c, const int kc, const int bx,
                                    const int by, const int bz)
 {
     const GState s = as_global(s0);
@@ -361,6 +369,224 @@ __device__ inline void stress_tile(const Geo &g, const State &s0, const Coef &c,
     }
 }
 
+template <class P, int U>
+__device__ inline void stress_tile_b(const Geo &g, const State &s0, const Coef &c, const int kc, const int bx,
+                                   const int by, const int bz)
+{
+    const GState s = as_global(s0);
+    using Q = typename P::Q;
+    using R = typename P::R;
+    const int ir = bx * TX + threadIdx.x + 1;
+    const int jr = by * blockDim.y + threadIdx.y + 1;
+    const int kb = bz * kc + 1;
+    const int ke = min(kb + kc - 1, g.nz);
+    // mixed storage needs every lane of a wave for the row maximum of the
+    // memory variables: lanes outside the domain compute an in-range
+    // duplicate point and store nothing
+    const bool act = ir <= g.nx && jr <= g.ny;
+    if constexpr (!P::mixed) {
+        if (!act) return;
+    }
+    const int i = min(ir, g.nx), j = min(jr, g.ny);
+    Cursor32 p(g, g.base + (long long)kb * g.sz + (long long)j * g.sy + i);
+    using O = unsigned;
+
+    const G<float> *__restrict__ u1 = s.u1;
+    const G<float> *__restrict__ v1 = s.v1;
+    const G<float> *__restrict__ w1 = s.w1;
+    const G<float> *__restrict__ lam = s.lam;
+    const G<float> *__restrict__ mu = s.mu;
+    const G<Q> *__restrict__ qp = (const G<Q> *)s.qp;
+    const G<Q> *__restrict__ qs = (const G<Q> *)s.qs;
+    G<R> *__restrict__ R1 = (G<R> *)s.r1;
+    G<R> *__restrict__ R2 = (G<R> *)s.r2;
+    G<R> *__restrict__ R3 = (G<R> *)s.r3;
+    G<R> *__restrict__ R4 = (G<R> *)s.r4;
+    G<R> *__restrict__ R5 = (G<R> *)s.r5;
+    G<R> *__restrict__ R6 = (G<R> *)s.r6;
+    const float c1 = c.c1, c2 = c.c2, dth = c.dth, dt = c.dt, dh1 = c.dh1;
+    const float fxy = s.fx[i] * s.fy[j];
+    // relaxation weights of this column for even / odd depth parity
+    // (selected per level: no per-lane indexing of the argument block)
+    const int itxy = ((i & 1) * 2 + (j & 1)) * 2;
+    const float t1e = c.tau1[itxy], t1o = c.tau1[itxy + 1];
+    const float t2e = c.tau2[itxy], t2o = c.tau2[itxy + 1];
+
+    // z queues of the velocities at the column
+    float u_m1 = p.ld(u1, 0, 0, -1), u_0 = p.ld(u1), u_p1 = p.ld(u1, 0, 0, 1);
+    float v_m1 = p.ld(v1, 0, 0, -1), v_0 = p.ld(v1), v_p1 = p.ld(v1, 0, 0, 1);
+    float w_m2 = p.ld(w1, 0, 0, -2), w_m1 = p.ld(w1, 0, 0, -1), w_0 = p.ld(w1);
+    // media at level k-1 on the 2x2 (x,y) cell: A=(i,j) B=(i+1,j) C=(i,j-1) D=(i+1,j-1)
+    float lA_m = p.ld(lam, 0, 0, -1), lB_m = p.ld(lam, 1, 0, -1), lC_m = p.ld(lam, 0, -1, -1), lD_m = p.ld(lam, 1, -1, -1);
+    float mA_m = p.ld(mu, 0, 0, -1), mB_m = p.ld(mu, 1, 0, -1), mC_m = p.ld(mu, 0, -1, -1), mD_m = p.ld(mu, 1, -1, -1);
+    float pA_m = ld(p.ld(qp, 0, 0, -1)), pB_m = ld(p.ld(qp, 1, 0, -1));
+    float pC_m = ld(p.ld(qp, 0, -1, -1)), pD_m = ld(p.ld(qp, 1, -1, -1));
+    float sA_m = ld(p.ld(qs, 0, 0, -1)), sB_m = ld(p.ld(qs, 1, 0, -1));
+    float sC_m = ld(p.ld(qs, 0, -1, -1)), sD_m = ld(p.ld(qs, 1, -1, -1));
+    // final xz/yz/zz of the two levels below (for the free-surface images)
+    float xz_f1 = 0.f, xz_f2 = 0.f, yz_f1 = 0.f, yz_f2 = 0.f, zz_f1 = 0.f;
+    // mixed storage: offset of this row segment's exponent at level k in
+    // each exponent array, and its step per level
+    O xo = P::mixed ? (O)rexp_idx(g, s.nseg, bx, j, kb) : (O)0;
+    const O xstep = (O)g.ny * (O)s.nseg;
+
+#pragma unroll U
+    for (int k = kb; k <= ke; ++k, p.up(), xo += xstep) {
+        const float u_p2 = p.ld(u1, 0, 0, 2);
+        const float v_p2 = p.ld(v1, 0, 0, 2);
+        const float w_p1 = p.ld(w1, 0, 0, 1);
+        const float lA = p.ld(lam), lB = p.ld(lam, 1), lC = p.ld(lam, 0, -1), lD = p.ld(lam, 1, -1);
+        const float mA = p.ld(mu), mB = p.ld(mu, 1), mC = p.ld(mu, 0, -1), mD = p.ld(mu, 1, -1);
+        const float pA = ld(p.ld(qp)), pB = ld(p.ld(qp, 1)), pC = ld(p.ld(qp, 0, -1)), pD = ld(p.ld(qp, 1, -1));
+        const float sA = ld(p.ld(qs)), sB = ld(p.ld(qs, 1)), sC = ld(p.ld(qs, 0, -1)), sD = ld(p.ld(qs, 1, -1));
+
+        const float u_ip1 = p.ld(u1, 1), u_ip2 = p.ld(u1, 2), u_im1 = p.ld(u1, -1);
+        const float u_jp1 = p.ld(u1, 0, 1), u_jp2 = p.ld(u1, 0, 2), u_jm1 = p.ld(u1, 0, -1);
+        const float v_jm1 = p.ld(v1, 0, -1), v_jp1 = p.ld(v1, 0, 1), v_jm2 = p.ld(v1, 0, -2);
+        const float v_im1 = p.ld(v1, -1), v_ip1 = p.ld(v1, 1), v_im2 = p.ld(v1, -2);
+        const float w_im1 = p.ld(w1, -1), w_ip1 = p.ld(w1, 1), w_im2 = p.ld(w1, -2);
+        const float w_jp1 = p.ld(w1, 0, 1), w_jp2 = p.ld(w1, 0, 2), w_jm1 = p.ld(w1, 0, -1);
+
+        const bool odd = (((g.nz + 1) - k) & 1) != 0;
+        const float vx1 = odd ? t1o : t1e;
+        const float vx2 = odd ? t2o : t2e;
+
+        float vxl = 8.0f / (lA + lB + lC + lD + lA_m + lB_m + lC_m + lD_m);
+        const float vxm = 8.0f / (mA + mB + mC + mD + mA_m + mB_m + mC_m + mD_m);
+        vxl = vxl + 2.0f * vxm;
+        float vqpa = 0.125f * (pA + pB + pC + pD + pA_m + pB_m + pC_m + pD_m);
+        float vqsa = 0.125f * (sA + sB + sC + sD + sA_m + sB_m + sC_m + sD_m);
+        if constexpr (P::mixed) {
+            vqpa = vqpa * s.ms.iqp;
+            vqsa = vqsa * s.ms.iqs;
+        }
+
+        const float vt1 = c1 * (u_ip1 - u_0) + c2 * (u_ip2 - u_im1);
+        const float vt2 = c1 * (v_0 - v_jm1) + c2 * (v_jp1 - v_jm2);
+        const float vt3 = c1 * (w_0 - w_m1) + c2 * (w_p1 - w_m2);
+        const float vt4 = c1 * (u_jp1 - u_0) + c2 * (u_jp2 - u_jm1) +
+                          c1 * (v_0 - v_im1) + c2 * (v_ip1 - v_im2);
+        const float vt5 = c1 * (u_p1 - u_0) + c2 * (u_p2 - u_m1) +
+                          c1 * (w_0 - w_im1) + c2 * (w_ip1 - w_im2);
+        const float vt6 = c1 * (v_p1 - v_0) + c2 * (v_p2 - v_m1);
+        const float vt7 = c1 * (w_jp1 - w_0) + c2 * (w_jp2 - w_jm1);
+
+        const float vh1 = -vxm * vqsa;
+        const float vtmp = vxl * (vt1 + vt2 + vt3);
+        const float va1 = -vqpa * vxl * dh1 * (vt1 + vt2 + vt3) / 2.0f;
+
+        float r1, r2, r3, r4, r5, r6;
+        if constexpr (P::mixed) {
+            // exponents of this row segment: array q at s.rexp + q * rexp_n
+            // (uniform bases), index xo
+            const G<signed char> *e0 = s.rexp;
+            const long long n = s.rexp_n;
+            r1 = r_decode(p.ld(R1), e0[xo]); r2 = r_decode(p.ld(R2), (e0 + n)[xo]);
+            r3 = r_decode(p.ld(R3), (e0 + 2 * n)[xo]); r4 = r_decode(p.ld(R4), (e0 + 3 * n)[xo]);
+            r5 = r_decode(p.ld(R5), (e0 + 4 * n)[xo]); r6 = r_decode(p.ld(R6), (e0 + 5 * n)[xo]);
+        } else {
+            r1 = p.ld(R1); r2 = p.ld(R2); r3 = p.ld(R3);
+            r4 = p.ld(R4); r5 = p.ld(R5); r6 = p.ld(R6);
+        }
+        float xx = p.ld(s.xx), yy = p.ld(s.yy), zz = p.ld(s.zz);
+        float xy = p.ld(s.xy), xz = p.ld(s.xz), yz = p.ld(s.yz);
+
+        xx = xx + dth * (vtmp - 2.0f * vxm * (vt2 + vt3)) + dt * r1;
+        yy = yy + dth * (vtmp - 2.0f * vxm * (vt1 + vt3)) + dt * r2;
+        zz = zz + dth * (vtmp - 2.0f * vxm * (vt1 + vt2)) + dt * r3;
+
+        float vxmu = 2.0f / (mA + mA_m);
+        float vaas = 0.5f * (sA + sA_m);
+        if constexpr (P::mixed) vaas = vaas * s.ms.iqs;
+        const float vh2_r4 = -vaas * vxmu / 2.0f;
+        xy = xy + vxmu * dth * vt4 + dt * r4;
+        vxmu = 2.0f / (mA + mC);
+        vaas = 0.5f * (sA + sC);
+        if constexpr (P::mixed) vaas = vaas * s.ms.iqs;
+        const float vh2_r5 = -vaas * vxmu / 2.0f;
+        xz = xz + vxmu * dth * vt5 + dt * r5;
+        vxmu = 2.0f / (mA + mB);
+        vaas = 0.5f * (sA + sB);
+        if constexpr (P::mixed) vaas = vaas * s.ms.iqs;
+        const float vh2_r6 = -vaas * vxmu / 2.0f;
+        yz = yz + vxmu * (dth * vt6 + dth * vt7) + dt * r6;
+
+        r1 = (vx2 * r1 - vh1 * (vt2 + vt3) * dh1 + va1) * vx1;
+        r2 = (vx2 * r2 - vh1 * (vt1 + vt3) * dh1 + va1) * vx1;
+        r3 = (vx2 * r3 - vh1 * (vt1 + vt2) * dh1 + va1) * vx1;
+        r4 = (vx2 * r4 + vh2_r4 * dh1 * vt4) * vx1;
+        r5 = (vx2 * r5 + vh2_r5 * dh1 * vt5) * vx1;
+        r6 = (vx2 * r6 + vh2_r6 * dh1 * (vt6 + vt7)) * vx1;
+
+        xx = xx + dt * r1;
+        yy = yy + dt * r2;
+        zz = zz + dt * r3;
+        xy = xy + dt * r4;
+        xz = xz + dt * r5;
+        yz = yz + dt * r6;
+
+        const float D = fxy * s.fz[k];
+        xx = xx * D; yy = yy * D; zz = zz * D;
+        xz = xz * D; yz = yz * D; xy = xy * D;
+
+        // the new memory variables enter this step's stresses in FP32 and
+        // are rounded to the storage format once, on the store
+        if constexpr (P::mixed) {
+            const float rv[6] = {r1, r2, r3, r4, r5, r6};
+            G<R> *RR[6] = {R1, R2, R3, R4, R5, R6};
+#pragma unroll
+            for (int q = 0; q < 6; ++q) {
+                const int e = row_exponent(wave_max(act ? fabsf(rv[q]) : 0.0f));
+                if (act) {
+                    p.st(RR[q], r_encode(rv[q], e));
+                    if (threadIdx.x == 0) (s.rexp + q * s.rexp_n)[xo] = (signed char)e;
+                }
+            }
+        } else {
+            p.st(R1, r1); p.st(R2, r2); p.st(R3, r3);
+            p.st(R4, r4); p.st(R5, r5); p.st(R6, r6);
+        }
+        if (act) {
+            p.st(s.xx, xx); p.st(s.yy, yy); p.st(s.xy, xy);
+            if (k == g.nz) {
+                // fstr: images above the surface, xz = yz = 0 on it
+                p.st(s.zz, zz);
+                p.st(s.zz, -zz, 0, 0, 1);
+                p.st(s.zz, -zz_f1, 0, 0, 2);
+                p.st(s.xz, 0.0f);
+                p.st(s.yz, 0.0f);
+                p.st(s.xz, -xz_f1, 0, 0, 1);
+                p.st(s.xz, -xz_f2, 0, 0, 2);
+                p.st(s.yz, -yz_f1, 0, 0, 1);
+                p.st(s.yz, -yz_f2, 0, 0, 2);
+            } else {
+                p.st(s.zz, zz); p.st(s.xz, xz); p.st(s.yz, yz);
+            }
+        }
+        xz_f2 = xz_f1; xz_f1 = xz;
+        yz_f2 = yz_f1; yz_f1 = yz;
+        zz_f1 = zz;
+
+        u_m1 = u_0; u_0 = u_p1; u_p1 = u_p2;
+        v_m1 = v_0; v_0 = v_p1; v_p1 = v_p2;
+        w_m2 = w_m1; w_m1 = w_0; w_0 = w_p1;
+        lA_m = lA; lB_m = lB; lC_m = lC; lD_m = lD;
+        mA_m = mA; mB_m = mB; mC_m = mC; mD_m = mD;
+        pA_m = pA; pB_m = pB; pC_m = pC; pD_m = pD;
+        sA_m = sA; sB_m = sB; sC_m = sC; sD_m = sD;
+    }
+}
+
+template <class P, int U, bool O32>
+__device__ inline void stress_tile(const Geo &g, const State &s0, const Coef &c, const int kc, const int bx,
+                                   const int by, const int bz)
+{
+    if constexpr (O32)
+        stress_tile_b<P, U>(g, s0, c, kc, bx, by, bz);
+    else
+        stress_tile_e<P, U>(g, s0, c, kc, bx, by, bz);
+}
+
 // ---------------------------------------------------------------------------
 // two-stage step kernels: one run (one block per tile), or a batch of runs
 // (a persistent grid per stage over the concatenated tile lists)
@@ -379,12 +605,12 @@ k_fs_velocity(const Geo g, const State s)
     fs_velocity_block(g, s, blockIdx.x, blockIdx.y);
 }
 
-template <class P, int U>
+template <class P, int U, bool O32>
 __global__ void __launch_bounds__(512)
 k_stress(const Geo g, const State s, const Coef c, const int kc, const Tiles tl)
 {
     int bx, by, bz;
-    if (tile_of(tl, bx, by, bz)) stress_tile<P, U>(g, s, c, kc, bx, by, bz);
+    if (tile_of(tl, bx, by, bz)) stress_tile<P, U, O32>(g, s, c, kc, bx, by, bz);
 }
 
 // Run of global tile `tile` (jobs ascending in tile0) and the tile's
@@ -421,7 +647,7 @@ k_velocity_batch(const SJob *__restrict__ jobs, const int njobs, const int total
     }
 }
 
-template <class P, int U>
+template <class P, int U, bool O32>
 __global__ void __launch_bounds__(512)
 k_stress_batch(const SJob *__restrict__ jobs, const int njobs, const int total, const int per)
 {
@@ -431,7 +657,7 @@ k_stress_batch(const SJob *__restrict__ jobs, const int njobs, const int total, 
         if (tile >= total) break;
         int bx, by, bz;
         const SJob &J = split_tile(jobs, njobs, tile, bx, by, bz);
-        stress_tile<P, U>(J.g, J.s, J.c, J.kc, bx, by, bz);
+        stress_tile<P, U, O32>(J.g, J.s, J.c, J.kc, bx, by, bz);
     }
 }
 
@@ -810,13 +1036,19 @@ static Tiles make_tiles(const Geo &g, const Launch &l)
     return t;
 }
 
-hipError_t launch_velocity(bool, const Geo &g, const State &s, const Coef &c, const Launch &l, hipStream_t st)
+template <int U>
+static void velocity_t(const Geo &g, const State &s, const Coef &c, const Launch &l, hipStream_t st)
 {
     const Tiles t = make_tiles(g, l);
+    hipLaunchKernelGGL((k_velocity<U>), dim3(8 * t.per), dim3(TX, l.tile_y), l.lds_v, st, g, s, c, l.kc, t);
+}
+
+hipError_t launch_velocity(bool, const Geo &g, const State &s, const Coef &c, const Launch &l, hipStream_t st)
+{
     if (l.unroll == 2)
-        hipLaunchKernelGGL((k_velocity<2>), dim3(8 * t.per), dim3(TX, l.tile_y), 0, st, g, s, c, l.kc, t);
+        velocity_t<2>(g, s, c, l, st);
     else
-        hipLaunchKernelGGL((k_velocity<1>), dim3(8 * t.per), dim3(TX, l.tile_y), 0, st, g, s, c, l.kc, t);
+        velocity_t<1>(g, s, c, l, st);
     return hipGetLastError();
 }
 
@@ -828,14 +1060,25 @@ hipError_t launch_fs_velocity(const Geo &g, const State &s, hipStream_t st)
     return hipGetLastError();
 }
 
+template <class P, int U, bool O32>
+static void stress_u(const Geo &g, const State &s, const Coef &c, const Launch &l, hipStream_t st)
+{
+    const Tiles t = make_tiles(g, l);
+    hipLaunchKernelGGL((k_stress<P, U, O32>), dim3(8 * t.per), dim3(TX, l.tile_y), l.lds_s, st, g, s, c,
+                       l.kc, t);
+}
+
 template <class P>
 static void stress_t(const Geo &g, const State &s, const Coef &c, const Launch &l, hipStream_t st)
 {
-    const Tiles t = make_tiles(g, l);
-    if (l.unroll == 2)
-        hipLaunchKernelGGL((k_stress<P, 2>), dim3(8 * t.per), dim3(TX, l.tile_y), 0, st, g, s, c, l.kc, t);
-    else
-        hipLaunchKernelGGL((k_stress<P, 1>), dim3(8 * t.per), dim3(TX, l.tile_y), 0, st, g, s, c, l.kc, t);
+    const bool o32 = l.o32 && l.o32_s;
+    if (l.unroll == 2) {
+        if (o32) stress_u<P, 2, true>(g, s, c, l, st);
+        else stress_u<P, 2, false>(g, s, c, l, st);
+    } else {
+        if (o32) stress_u<P, 1, true>(g, s, c, l, st);
+        else stress_u<P, 1, false>(g, s, c, l, st);
+    }
 }
 
 hipError_t launch_stress(bool mixed, const Geo &g, const State &s, const Coef &c, const Launch &l, hipStream_t st)
@@ -872,13 +1115,28 @@ hipError_t launch_sgt_gather(const State &s, int nkept, const long long *kpos, c
     return hipGetLastError();
 }
 
+void set_block_caps(Launch &l, int device)
+{
+    int cu = 0, blk = 0;
+    if (hipDeviceGetAttribute(&cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, device) != hipSuccess) cu = 0;
+    if (hipDeviceGetAttribute(&blk, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess) blk = 0;
+    l.lds_v = lds_cap(l.bpc_v, cu, blk);
+    l.lds_s = lds_cap(l.bpc_s, cu, blk);
+}
+
 template <class P, int U>
-static int split_grid_t(int device, int tile_y, int *gs)
+static int split_grid_t(int device, const Launch &l, int *gs)
 {
     int nv = 0, ns = 0, ncu = 0;
-    const int nt = TX * tile_y;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nv, k_velocity_batch<U>, nt, 0) != hipSuccess) nv = 1;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&ns, k_stress_batch<P, U>, nt, 0) != hipSuccess) ns = 1;
+    const int nt = TX * l.tile_y;
+    // blocks of the variants in use that fit on a CU at once (a run whose
+    // arrays need 64-bit offsets walks more tiles per block)
+    hipError_t ev = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nv, k_velocity_batch<U>, nt, l.lds_v);
+    hipError_t es = l.o32_s
+                        ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&ns, k_stress_batch<P, U, true>, nt, l.lds_s)
+                        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&ns, k_stress_batch<P, U, false>, nt, l.lds_s);
+    if (ev != hipSuccess) nv = 1;
+    if (es != hipSuccess) ns = 1;
     if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) ncu = 256;
     ncu = std::max(8, ncu);
     *gs = std::max(1, ns) * ncu;
@@ -889,40 +1147,60 @@ void split_batch_grids(bool mixed, const Launch &l, int device, int *grid_v, int
 {
     const bool u2 = l.unroll == 2;
     if (mixed)
-        *grid_v = u2 ? split_grid_t<StoreMixed, 2>(device, l.tile_y, grid_s)
-                     : split_grid_t<StoreMixed, 1>(device, l.tile_y, grid_s);
+        *grid_v = u2 ? split_grid_t<StoreMixed, 2>(device, l, grid_s)
+                     : split_grid_t<StoreMixed, 1>(device, l, grid_s);
     else
-        *grid_v = u2 ? split_grid_t<StoreF32, 2>(device, l.tile_y, grid_s)
-                     : split_grid_t<StoreF32, 1>(device, l.tile_y, grid_s);
+        *grid_v = u2 ? split_grid_t<StoreF32, 2>(device, l, grid_s)
+                     : split_grid_t<StoreF32, 1>(device, l, grid_s);
 }
 
 // whole XCD rounds of persistent blocks, no more than the tiles need
 static int persistent_grid(int grid, int per) { return std::max(8, std::min(grid, 8 * per)) / 8 * 8; }
 
-template <class P, int U>
-static void split_batch_t(const Launch &l, const SJob *d_jobs, int njobs, int total, int fs_total, int grid_v,
-                          int grid_s, hipStream_t st)
+template <int U>
+static void vel_batch_t(const Launch &l, const SJob *d_jobs, int njobs, int total, int per, int grid_v,
+                        hipStream_t st)
 {
-    const int per = (total + 7) / 8;
-    hipLaunchKernelGGL((k_velocity_batch<U>), dim3(persistent_grid(grid_v, per)), dim3(TX, l.tile_y), 0, st,
-                       d_jobs, njobs, total, per);
-    hipLaunchKernelGGL(k_fs_velocity_batch, dim3(fs_total), dim3(TX, 4), 0, st, d_jobs, njobs);
-    hipLaunchKernelGGL((k_stress_batch<P, U>), dim3(persistent_grid(grid_s, per)), dim3(TX, l.tile_y), 0, st,
-                       d_jobs, njobs, total, per);
+    hipLaunchKernelGGL((k_velocity_batch<U>), dim3(persistent_grid(grid_v, per)), dim3(TX, l.tile_y),
+                       l.lds_v, st, d_jobs, njobs, total, per);
 }
 
-hipError_t launch_step_split_batch(bool mixed, const Launch &l, const SJob *d_jobs, int njobs, int total,
-                                   int fs_total, int grid_v, int grid_s, hipStream_t st)
+template <class P, int U, bool O32>
+static void str_batch_t(const Launch &l, const SJob *d_jobs, int njobs, int total, int per, int grid_s,
+                        hipStream_t st)
+{
+    hipLaunchKernelGGL((k_stress_batch<P, U, O32>), dim3(persistent_grid(grid_s, per)), dim3(TX, l.tile_y),
+                       l.lds_s, st, d_jobs, njobs, total, per);
+}
+
+template <class P>
+static void split_batch_p(const Launch &l, bool o32, const SJob *d_jobs, int njobs, int total, int fs_total,
+                          int grid_v, int grid_s, hipStream_t st)
+{
+    const int per = (total + 7) / 8;
+    const bool os = o32 && l.o32_s, u2 = l.unroll == 2;
+    if (u2)
+        vel_batch_t<2>(l, d_jobs, njobs, total, per, grid_v, st);
+    else
+        vel_batch_t<1>(l, d_jobs, njobs, total, per, grid_v, st);
+    hipLaunchKernelGGL(k_fs_velocity_batch, dim3(fs_total), dim3(TX, 4), 0, st, d_jobs, njobs);
+    if (u2) {
+        if (os) str_batch_t<P, 2, true>(l, d_jobs, njobs, total, per, grid_s, st);
+        else str_batch_t<P, 2, false>(l, d_jobs, njobs, total, per, grid_s, st);
+    } else {
+        if (os) str_batch_t<P, 1, true>(l, d_jobs, njobs, total, per, grid_s, st);
+        else str_batch_t<P, 1, false>(l, d_jobs, njobs, total, per, grid_s, st);
+    }
+}
+
+hipError_t launch_step_split_batch(bool mixed, const Launch &l, bool o32, const SJob *d_jobs, int njobs,
+                                   int total, int fs_total, int grid_v, int grid_s, hipStream_t st)
 {
     if (njobs <= 0 || total <= 0) return hipSuccess;
-    const bool u2 = l.unroll == 2;
-    if (mixed) {
-        if (u2) split_batch_t<StoreMixed, 2>(l, d_jobs, njobs, total, fs_total, grid_v, grid_s, st);
-        else split_batch_t<StoreMixed, 1>(l, d_jobs, njobs, total, fs_total, grid_v, grid_s, st);
-    } else {
-        if (u2) split_batch_t<StoreF32, 2>(l, d_jobs, njobs, total, fs_total, grid_v, grid_s, st);
-        else split_batch_t<StoreF32, 1>(l, d_jobs, njobs, total, fs_total, grid_v, grid_s, st);
-    }
+    if (mixed)
+        split_batch_p<StoreMixed>(l, o32, d_jobs, njobs, total, fs_total, grid_v, grid_s, st);
+    else
+        split_batch_p<StoreF32>(l, o32, d_jobs, njobs, total, fs_total, grid_v, grid_s, st);
     return hipGetLastError();
 }
 

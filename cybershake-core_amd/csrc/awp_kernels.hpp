@@ -201,6 +201,53 @@ __device__ __forceinline__ GState as_global(const State &a)
     return s;
 }
 
+// Loads and stores at a byte offset from an array base.  With a 32-bit
+// offset (O = unsigned) and a uniform base (scalar registers), an access
+// takes the scalar-base + 32-bit vector-offset form, one offset register
+// serves every array of an element size, and +-1, +-2 x neighbours become
+// immediate offsets; this needs every array below 4 GiB.  A 64-bit offset
+// (O = unsigned long long) addresses any size.
+template <class T, class O>
+__device__ __forceinline__ T gl(const G<T> *base, O off, int imm = 0)
+{
+    return *reinterpret_cast<const G<T> *>(reinterpret_cast<const G<char> *>(base) + off + imm);
+}
+template <class T, class O>
+__device__ __forceinline__ void gs(G<T> *base, O off, T v, int imm = 0)
+{
+    *reinterpret_cast<G<T> *>(reinterpret_cast<G<char> *>(base) + off + imm) = v;
+}
+// Position of one thread's point in the volume arrays while it marches up in
+// z, as a 32-bit byte offset into a 4-byte array (2-byte arrays take it
+// halved), and access to its neighbours (di, dj, dk): every access is a
+// uniform base plus a shared offset register plus an immediate.  Needs every
+// array below 4 GiB.
+struct Cursor32 {
+    unsigned o;        // the point
+    unsigned sy, sz;   // row and level strides
+    __device__ Cursor32(const Geo &g, long long e)
+        : o((unsigned)(e * 4)), sy((unsigned)(g.sy * 4)), sz((unsigned)(g.sz * 4)) {}
+    // (unsigned arithmetic: exact modulo 2^32 for offsets inside the array)
+    __device__ __forceinline__ unsigned at(int dj, int dk) const { return o + (unsigned)dj * sy + (unsigned)dk * sz; }
+    template <class T>
+    __device__ __forceinline__ T ld(const G<T> *p, int di = 0, int dj = 0, int dk = 0) const
+    {
+        return gl(p, at(dj, dk) >> (sizeof(T) == 2 ? 1 : 0), di * (int)sizeof(T));
+    }
+    template <class T>
+    __device__ __forceinline__ void st(G<T> *p, T v, int di = 0, int dj = 0, int dk = 0) const
+    {
+        gs(p, at(dj, dk) >> (sizeof(T) == 2 ? 1 : 0), v, di * (int)sizeof(T));
+    }
+    __device__ __forceinline__ void up() { o += sz; }
+};
+
+// whether 32-bit byte offsets reach every element of a run's arrays (the
+// 32-bit forms need fewer registers and address instructions, but the
+// compiler then schedules the two-stage FP32 kernels with fewer loads in
+// flight; which form is faster is measured per kernel and storage mode)
+static inline bool offsets32(const Geo &g) { return (unsigned long long)g.nvol * 4ull < (1ull << 32); }
+
 // FP32 media (the storage in FP32 mode; in mixed mode qp/qs point to set-up
 // temporaries)
 struct Media32 {
@@ -222,7 +269,24 @@ struct Launch {
     int nchunk;      // number of z chunks
     int unroll;      // z-loop unroll factor (1 or 2); prefetch depth of the fused step
     int pw, ph;      // fused step: tile patch width / height (tiles)
+    bool o32;        // 32-bit byte offsets reach every array (offsets32() of the run's geometry)
+    bool o32_s;      // two-stage stress kernel: its 32-bit byte-offset form (only where o32)
+    int bpc_v, bpc_s;   // two-stage kernels: most blocks per CU (0 = as registers allow)
+    unsigned lds_v, lds_s;   // the dynamic LDS bytes per block that enforce them (set_block_caps)
 };
+
+// Dynamic LDS a block reserves so that at most `bpc` blocks share a CU:
+// capping the blocks in flight keeps the tiles an XCD works on at once, and
+// with them the halo lines they share, within its L2.  0 (no cap) when bpc
+// <= 0 or when the per-block LDS limit cannot enforce it.
+static inline unsigned lds_cap(int bpc, int lds_cu, int lds_block)
+{
+    if (bpc <= 0 || lds_cu <= 0) return 0u;
+    const int want = lds_cu / bpc - 512;
+    return (want > 0 && want <= lds_block) ? (unsigned)want : 0u;
+}
+// fills lds_v / lds_s of `l` from bpc_v / bpc_s for device `device`
+void set_block_caps(Launch &l, int device);
 
 // (bx, by, bz) of local tile number lt in the patch order of FTiles
 __host__ __device__ inline void patch_tile(int lt, int nbx, int nby, int pw, int ph, int &bx, int &by, int &bz)
@@ -313,10 +377,11 @@ struct SJob {
     int fs0;     // first block of the run in the free-surface launch
 };
 // velocity, free-surface and stress stages of every run: three launches,
-// the first and last persistent (grid_v / grid_s blocks, from
-// split_batch_grids); fs_total = sum of the runs' (nx/64) x (ny/4) blocks
-hipError_t launch_step_split_batch(bool mixed, const Launch &l, const SJob *d_jobs, int njobs, int total,
-                                   int fs_total, int grid_v, int grid_s, hipStream_t st);
+// the first and last over the tile list (grid_v / grid_s blocks, from
+// split_batch_grids, or one block per tile when larger); fs_total = sum of
+// the runs' (nx/64) x (ny/4) blocks; o32 = offsets32() holds for every run
+hipError_t launch_step_split_batch(bool mixed, const Launch &l, bool o32, const SJob *d_jobs, int njobs,
+                                   int total, int fs_total, int grid_v, int grid_s, hipStream_t st);
 void split_batch_grids(bool mixed, const Launch &l, int device, int *grid_v, int *grid_s);
 
 // per-run data of the batched source / SGT stage

@@ -54,16 +54,22 @@ def test_media_setup_bitwise(case_dirs):
             assert np.array_equal(r.field(f), o.field(f)), f
 
 
-@pytest.mark.parametrize("scheme", ["split", "fused"])
+@pytest.mark.parametrize("scheme", ["split", "fused", "split_o32", "split_o64"])
 @pytest.mark.parametrize("name", list(cases.CASES))
 def test_fields_bitwise_after_steps(case_dirs, name, scheme, monkeypatch):
     """Both time-step schemes (two-stage default, fused with AWP_SCHEME=fused)
-    against the restatement, bit for bit."""
+    and both byte-offset forms of the two-stage kernels (32-bit, and 64-bit
+    as grids with arrays of 4 GiB or more take) against the restatement, bit
+    for bit."""
     d, _ = case_dirs[name]
     p, media, xyz, stf, pts = _inputs(d)
     o = _oracle(d, p, media, xyz, stf, pts)
     if scheme == "fused":
         monkeypatch.setenv("AWP_SCHEME", "fused")
+    if scheme in ("split_o32", "split_o64"):
+        # both byte-offset widths of the two-stage kernels, whichever the
+        # storage mode uses by default
+        monkeypatch.setenv("AWP_O32", "1" if scheme == "split_o32" else "0")
     with _gpu_run(p, media, xyz, stf, pts, nsteps=0) as r:
         for target in (1, 7, 45):
             r.step(target - r.progress()[0])
