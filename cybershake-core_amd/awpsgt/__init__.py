@@ -70,6 +70,15 @@ class In3d(ctypes.Structure):
         return getattr(self, name).decode()
 
 
+class HeadParams(ctypes.Structure):
+    """awp_head_params (include/awpsgt.h)."""
+    _fields_ = [("modelbox", ctypes.c_char_p), ("coordfile", ctypes.c_char_p), ("fdloc", ctypes.c_char_p),
+                ("gridout", ctypes.c_char_p), ("media", ctypes.c_char_p), ("out", ctypes.c_char_p),
+                ("spacing", ctypes.c_float), ("nt", ctypes.c_int), ("dt", ctypes.c_float),
+                ("sgt_tinc", ctypes.c_int), ("component", ctypes.c_char), ("moment", ctypes.c_float),
+                ("flo", ctypes.c_float), ("mu_corr", ctypes.c_int)]
+
+
 class AwpError(RuntimeError):
     pass
 
@@ -86,7 +95,7 @@ EXPORTS = [
     "awp_sgt16_read", "awp_sgt16_read_points", "awp_get_field", "awp_set_timing", "awp_get_timing", "awp_device_bytes",
     "awp_write_sgt_file_rwg", "awp_device_memory", "awp_estimate_bytes", "awp_batch_create", "awp_batch_tune", "awp_batch_step",
     "awp_batch_synchronize", "awp_batch_time", "awp_batch_get_launch", "awp_batch_get_scheme",
-    "awp_batch_last_error", "awp_batch_destroy",
+    "awp_batch_last_error", "awp_batch_destroy", "awp_write_sgt_head",
 ]
 
 
@@ -133,6 +142,7 @@ def lib() -> ctypes.CDLL:
         "awp_device_memory": ([ip, ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t)], ip),
         "awp_estimate_bytes": ([P, ip, ip, ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t)], ip),
         "awp_batch_create": ([ctypes.POINTER(vp), vp, ip, cp, sz], ip),
+        "awp_write_sgt_head": ([ctypes.POINTER(HeadParams), cp, sz], ip),
         "awp_batch_tune": ([vp, ip], ip),
         "awp_batch_step": ([vp, ip], ip),
         "awp_batch_synchronize": ([vp], ip),
@@ -347,6 +357,18 @@ class Run:
             self.close()
         except Exception:
             pass
+
+
+def write_sgt_head(modelbox: str, coordfile: str, fdloc: str, gridout: str, spacing: float, nt: int, dt: float,
+                   sgt_tinc: int, component: str, moment: float, flo: float, media: str, out: str,
+                   mu_corr: bool = False) -> None:
+    """SgtHead write_head's header file for an AWP SGT run (the same
+    arguments, SgtHead/src/write_head.c:13-41)."""
+    hp = HeadParams(modelbox.encode(), coordfile.encode(), fdloc.encode(), gridout.encode(), media.encode(),
+                    out.encode(), float(spacing), int(nt), float(dt), int(sgt_tinc), component.encode()[:1],
+                    float(moment), float(flo), 1 if mu_corr else 0)
+    err = ctypes.create_string_buffer(512)
+    _check(lib().awp_write_sgt_head(ctypes.byref(hp), err, 512), err.value.decode())
 
 
 def estimate_bytes(p: In3d, storage: int = AWP_STORAGE_FP32, npts: int = 0) -> Tuple[int, int]:

@@ -29,7 +29,7 @@ HIP_FLAGS = [
 C_FLAGS = ["-std=c11", "-O2", "-fPIC", "-Wall", "-Wextra", "-Wno-unused-parameter", "-I" + INCLUDE]
 
 HIP_SOURCES = ["csrc/awp_kernels.hip", "csrc/awp_fused.hip", "csrc/awp_ctx.hip", "csrc/awp_batch.hip"]
-C_SOURCES = ["host/awp_io.c"]
+C_SOURCES = ["host/awp_io.c", "host/awp_head.c"]
 
 
 def _run(cmd, quiet=True):

@@ -7,6 +7,11 @@ the reference's x/y swap (RWG x = AWP y) and 1-based AWP indexing are kept.
     python -m awpsgt.builders velocity <nx> <ny> <nz> <rwg prefix> <awp media out>
     python -m awpsgt.builders src <site> <fdloc file> <x|y|z> <stf file> <out>
     python -m awpsgt.builders in3d <site> <gridout> <x|y|z> <frequency> <px> <py> <pz> <mesh name> [tmax]
+    python -m awpsgt.builders head <modelbox> <coordfile> <fdloc> <gridout> <spacing> <nt> <dt> <tinc>
+                                   <x|y|z> <moment> <max freq> <awp media> <out> [-c]
+
+The last is the output-side companion (SgtHead write_head: the header file
+of an SGT run, through awp_write_sgt_head in the C ABI).
 
 The velocity reformat also has a device path (awp_set_media_rwg in the C
 ABI) that takes the three RWG arrays straight into a run's media set-up.
@@ -266,6 +271,11 @@ def _main(argv: Sequence[str]) -> int:
         P = in3d_params(site, dims, comp, freq, proc, a[7], ref=ref)
         with open("IN3D.%s.%s" % (site, comp), "w") as f:
             f.write(in3d_text(P, comp))
+        return 0
+    if cmd == "head" and len(a) >= 13:
+        import awpsgt
+        awpsgt.write_sgt_head(a[0], a[1], a[2], a[3], float(a[4]), int(a[5]), float(a[6]), int(a[7]), a[8],
+                              float(a[9]), float(a[10]), a[11], a[12], mu_corr=len(a) > 13 and a[13] == "-c")
         return 0
     print(__doc__)
     return 1

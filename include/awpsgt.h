@@ -237,6 +237,33 @@ int awp_estimate_bytes(const awp_in3d *p, int storage, int npts, size_t *steady,
 /* Free and total device memory of a GPU (for admission decisions). */
 int awp_device_memory(int device, size_t *free_bytes, size_t *total_bytes);
 
+/* ---- SGT header (SgtHead write_head) -------------------------------------
+ * The separate header file SgtHead's write_head writes beside an AWP SGT
+ * run (SgtHead/src/write_head.c:51-469): one sgtmaster record, then one
+ * sgtindex and one sgtheader record per point of the RWG coordfile, in its
+ * order, with the site and point coordinates projected on RWG's great-circle
+ * model frame and lambda, mu (dyne/cm^2) and rho (g/cm^3) read from the
+ * AWP-format media file ([z][x][y][3] Vp, Vs m/s, rho kg/m^3, as
+ * reformat_velocity writes it).  Records and byte layout are the reference's
+ * (structure.h: 36-byte master, 24-byte index, 128-byte header). */
+typedef struct {
+    const char *modelbox;   /* model box file (5th line: "mlon= ... mlat= ... mrot= ...") */
+    const char *coordfile;  /* RWG coordfile: count, then "x y z indx lon lat dep" lines */
+    const char *fdloc;      /* site grid location "x y" */
+    const char *gridout;    /* RWG gridout (nx=, ny=, nz= blocks) */
+    const char *media;      /* AWP-format media file */
+    const char *out;        /* header file to write */
+    float spacing;          /* grid spacing, km */
+    int nt;                 /* time steps of the run */
+    float dt;               /* time step, s */
+    int sgt_tinc;           /* time decimation of the SGT (NTISKP_SGT) */
+    char component;         /* 'x', 'y' or 'z' force */
+    float moment;           /* force moment */
+    float flo;              /* frequency the source was filtered at (tst = -3/flo) */
+    int mu_corr;            /* 1: corrected mu with Qs = 0.05 Vs (write_head -c) */
+} awp_head_params;
+int awp_write_sgt_head(const awp_head_params *hp, char *err, size_t errlen);
+
 /* ---- batches ------------------------------------------------------------
  * Several initialised runs (same device and storage mode) advanced
  * together: each stage of a time step is one persistent launch over the
