@@ -95,7 +95,8 @@ EXPORTS = [
     "awp_sgt16_read", "awp_sgt16_read_points", "awp_get_field", "awp_set_timing", "awp_get_timing", "awp_device_bytes",
     "awp_write_sgt_file_rwg", "awp_device_memory", "awp_estimate_bytes", "awp_batch_create", "awp_batch_tune", "awp_batch_step",
     "awp_batch_synchronize", "awp_batch_time", "awp_batch_get_launch", "awp_batch_get_scheme",
-    "awp_batch_last_error", "awp_batch_destroy", "awp_write_sgt_head",
+    "awp_batch_last_error", "awp_batch_destroy", "awp_write_sgt_head", "awp_checkpoint_save",
+    "awp_checkpoint_load",
 ]
 
 
@@ -143,6 +144,8 @@ def lib() -> ctypes.CDLL:
         "awp_estimate_bytes": ([P, ip, ip, ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t)], ip),
         "awp_batch_create": ([ctypes.POINTER(vp), vp, ip, cp, sz], ip),
         "awp_write_sgt_head": ([ctypes.POINTER(HeadParams), cp, sz], ip),
+        "awp_checkpoint_save": ([vp, cp], ip),
+        "awp_checkpoint_load": ([vp, cp], ip),
         "awp_batch_tune": ([vp, ip], ip),
         "awp_batch_step": ([vp, ip], ip),
         "awp_batch_synchronize": ([vp], ip),
@@ -322,6 +325,15 @@ class Run:
 
     def write_sgt_file_fp16(self, path: str) -> None:
         self._do(lib().awp_write_sgt_file_fp16(self.h, path.encode()), "awp_write_sgt_file_fp16")
+
+    def checkpoint_save(self, path: str) -> None:
+        """Write the evolving state (wavefield, memory variables, SGT samples,
+        step count) to `path`."""
+        self._do(lib().awp_checkpoint_save(self.h, path.encode()), "awp_checkpoint_save")
+
+    def checkpoint_load(self, path: str) -> None:
+        """Restore a checkpoint into this initialised run of the same set-up."""
+        self._do(lib().awp_checkpoint_load(self.h, path.encode()), "awp_checkpoint_load")
 
     def field(self, name: str, kext: int = 0) -> np.ndarray:
         out = np.empty((self.p.nz + kext, self.p.ny, self.p.nx), dtype=np.float32)

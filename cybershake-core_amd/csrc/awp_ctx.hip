@@ -899,3 +899,129 @@ extern "C" int awp_get_field(awp_ctx *ctx, int which, int kext, float *host)
     if (e != hipSuccess) return fail(ctx, AWP_EHIP, "field copy: %s", hipGetErrorString(e));
     return AWP_OK;
 }
+
+// ---------------------------------------------------------------------------
+// checkpoint / restart (the reference's restart files: pmcl3d.f:1016-1097,
+// 1964-2045).  File: a 72-byte header, then the state part of the volume
+// allocation (wavefield buffers, memory variables, their exponents) and the
+// SGT sample buffer, byte for byte as they sit in HBM, streamed through a
+// pinned staging buffer.
+// ---------------------------------------------------------------------------
+namespace {
+
+struct CkpHead {
+    char magic[8];                  // "AWPCKP01"
+    int32_t nx, ny, nz, storage;
+    int32_t fused, cur, step, nsamp;
+    int32_t npts, nsamp_max, nt, reserved;
+    uint64_t state_bytes, sgt_bytes;
+};
+static_assert(sizeof(CkpHead) == 72, "checkpoint header is 72 bytes");
+
+size_t sgt_buffer_bytes(const awp_ctx *ctx)
+{
+    return (size_t)std::max(ctx->npts, 1) * 6 * std::max(ctx->nsamp_max, 1) * sizeof(float);
+}
+
+CkpHead ckp_head_of(const awp_ctx *ctx)
+{
+    CkpHead h;
+    memset(&h, 0, sizeof h);
+    memcpy(h.magic, "AWPCKP01", 8);
+    h.nx = ctx->p.nx;
+    h.ny = ctx->p.ny;
+    h.nz = ctx->p.nz;
+    h.storage = ctx->storage;
+    h.fused = ctx->fused ? 1 : 0;
+    h.cur = ctx->cur;
+    h.step = ctx->step;
+    h.nsamp = ctx->nsamp;
+    h.npts = ctx->npts;
+    h.nsamp_max = ctx->nsamp_max;
+    h.nt = ctx->p.nt;
+    h.state_bytes = ctx->state_bytes;
+    h.sgt_bytes = sgt_buffer_bytes(ctx);
+    return h;
+}
+
+// copies `bytes` between device memory and the file in chunks
+int ckp_stream(awp_ctx *ctx, FILE *f, char *dev, size_t bytes, char *stage, size_t chunk, bool save)
+{
+    for (size_t off = 0; off < bytes; off += chunk) {
+        const size_t n = std::min(chunk, bytes - off);
+        if (save) {
+            HIPTRY(ctx, hipMemcpyAsync(stage, dev + off, n, hipMemcpyDeviceToHost, ctx->stream));
+            HIPTRY(ctx, hipStreamSynchronize(ctx->stream));
+            if (fwrite(stage, 1, n, f) != n) return fail(ctx, AWP_EIO, "checkpoint write failed");
+        } else {
+            if (fread(stage, 1, n, f) != n) return fail(ctx, AWP_EIO, "checkpoint file is short");
+            HIPTRY(ctx, hipMemcpyAsync(dev + off, stage, n, hipMemcpyHostToDevice, ctx->stream));
+            HIPTRY(ctx, hipStreamSynchronize(ctx->stream));
+        }
+    }
+    return AWP_OK;
+}
+
+int ckp_run(awp_ctx *ctx, FILE *f, bool save)
+{
+    const size_t chunk = (size_t)64 << 20;
+    char *stage = nullptr;
+    HIPTRY(ctx, hipHostMalloc((void **)&stage, chunk, hipHostMallocDefault));
+    int rc = ckp_stream(ctx, f, (char *)ctx->vol, ctx->state_bytes, stage, chunk, save);
+    if (!rc) rc = ckp_stream(ctx, f, (char *)ctx->d_sgt, sgt_buffer_bytes(ctx), stage, chunk, save);
+    hipHostFree(stage);
+    return rc;
+}
+
+}  // namespace
+
+extern "C" int awp_checkpoint_save(awp_ctx *ctx, const char *path)
+{
+    if (!ctx || !path) return fail(ctx, AWP_EINVAL, "awp_checkpoint_save: null argument");
+    if (!ctx->initialised) return fail(ctx, AWP_ESTATE, "awp_checkpoint_save before awp_init");
+    HIPTRY(ctx, hipSetDevice(ctx->device));
+    HIPTRY(ctx, hipStreamSynchronize(ctx->stream));
+    const std::string tmp = std::string(path) + ".tmp";
+    FILE *f = fopen(tmp.c_str(), "wb");
+    if (!f) return fail(ctx, AWP_EIO, "cannot create '%s'", tmp.c_str());
+    const CkpHead h = ckp_head_of(ctx);
+    int rc = fwrite(&h, sizeof h, 1, f) == 1 ? AWP_OK : fail(ctx, AWP_EIO, "checkpoint write failed");
+    if (!rc) rc = ckp_run(ctx, f, true);
+    if (fclose(f) != 0 && !rc) rc = fail(ctx, AWP_EIO, "checkpoint write failed");
+    if (!rc && rename(tmp.c_str(), path) != 0) rc = fail(ctx, AWP_EIO, "cannot rename '%s' to '%s'", tmp.c_str(), path);
+    if (rc) remove(tmp.c_str());
+    return rc;
+}
+
+extern "C" int awp_checkpoint_load(awp_ctx *ctx, const char *path)
+{
+    if (!ctx || !path) return fail(ctx, AWP_EINVAL, "awp_checkpoint_load: null argument");
+    if (!ctx->initialised) return fail(ctx, AWP_ESTATE, "awp_checkpoint_load needs an initialised run (awp_init)");
+    HIPTRY(ctx, hipSetDevice(ctx->device));
+    FILE *f = fopen(path, "rb");
+    if (!f) return fail(ctx, AWP_EIO, "cannot open '%s'", path);
+    CkpHead h;
+    const CkpHead want = ckp_head_of(ctx);
+    int rc = AWP_OK;
+    if (fread(&h, sizeof h, 1, f) != 1 || memcmp(h.magic, want.magic, 8)) {
+        rc = fail(ctx, AWP_EINVAL, "'%s' is not a checkpoint file", path);
+    } else if (h.nx != want.nx || h.ny != want.ny || h.nz != want.nz || h.storage != want.storage ||
+               h.fused != want.fused || h.npts != want.npts || h.nsamp_max != want.nsamp_max || h.nt != want.nt ||
+               h.state_bytes != want.state_bytes || h.sgt_bytes != want.sgt_bytes) {
+        rc = fail(ctx, AWP_EINVAL, "checkpoint '%s' is of another run (grid, storage, scheme, points or steps differ)",
+                  path);
+    } else if (h.step < 0 || h.step > h.nt || h.nsamp < 0 || h.nsamp > h.nsamp_max || (h.cur != 0 && h.cur != 1)) {
+        rc = fail(ctx, AWP_EINVAL, "checkpoint '%s' has an invalid step count", path);
+    }
+    if (!rc) {
+        rc = ckp_run(ctx, f, false);
+        // a partly restored state is no state: the run needs awp_init again
+        if (rc) ctx->initialised = false;
+    }
+    fclose(f);
+    if (rc) return rc;
+    ctx->step = h.step;
+    ctx->nsamp = h.nsamp;
+    ctx->cur = h.cur;
+    return AWP_OK;
+}

@@ -33,6 +33,7 @@
 #include <stdlib.h>
 #include <string.h>
 #include <time.h>
+#include <unistd.h>
 
 #include "awpsgt.h"
 
@@ -291,20 +292,39 @@ int main(int argc, char **argv)
     if (setup_run(&j, device, storage, err, sizeof(err))) return die("set-up", err);
     double t2 = now_s();
 
+    /* CHECKPOINT > 0: resume from output_ckp/awp_ckp.bin when it exists and
+       write it every CHECKPOINT steps (pmcl3d.f:1016-1097, 1964-2045; the
+       reference keeps one file set per rank in the same directory) */
+    const char *ckp = "output_ckp/awp_ckp.bin";
+    int done = 0;
+    if (p->checkpoint > 0 && access(ckp, R_OK) == 0) {
+        if (awp_checkpoint_load(j.ctx, ckp)) return die("checkpoint", awp_last_error(j.ctx));
+        if (awp_progress(j.ctx, &done, NULL)) return die("checkpoint", awp_last_error(j.ctx));
+        printf(" Checkpoint found, starting from step %d\n", done);
+    }
     const int chunk = 100;
     double tl = now_s();
-    for (int done = 0; done < p->nt;) {
+    const int first = done;
+    while (done < p->nt) {
         int n = p->nt - done < chunk ? p->nt - done : chunk;
+        if (p->checkpoint > 0) {
+            const int next_ckp = (done / p->checkpoint + 1) * p->checkpoint;
+            if (done + n > next_ckp) n = next_ckp - done;
+        }
         if (awp_step(j.ctx, n)) return die("awp_step", awp_last_error(j.ctx));
         done += n;
         if (awp_synchronize(j.ctx)) return die("awp_step", awp_last_error(j.ctx));
+        if (p->checkpoint > 0 && done % p->checkpoint == 0 && done < p->nt) {
+            if (awp_checkpoint_save(j.ctx, ckp)) return die("checkpoint", awp_last_error(j.ctx));
+            printf(" Writing checkpoint file at step %d\n", done);
+        }
         printf(" TIME STEP %d OF %d\n", done, p->nt);
         fflush(stdout);
     }
     double t3 = now_s();
     if (write_outputs(&j, fp16_path, rwg_path)) return 1;
     double t4 = now_s();
-    const double per_step = (t3 - tl) / p->nt;
+    const double per_step = (t3 - tl) / (p->nt > first ? p->nt - first : 1);
     printf(" inialization time = %10.3f sec\n", t2 - t0);
     printf(" computing time per time step = %12.6f sec\n", per_step);
     printf(" point-updates per second = %.4g\n", (double)p->nx * p->ny * p->nz / per_step);

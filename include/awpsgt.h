@@ -237,6 +237,20 @@ int awp_estimate_bytes(const awp_in3d *p, int storage, int npts, size_t *steady,
 /* Free and total device memory of a GPU (for admission decisions). */
 int awp_device_memory(int device, size_t *free_bytes, size_t *total_bytes);
 
+/* ---- checkpoint / restart ----------------------------------------------
+ * awp_checkpoint_save writes the run's evolving state -- the wavefield and
+ * memory variables in their storage format, the SGT samples so far, the
+ * step count -- to `path` (through path.tmp, renamed when complete);
+ * awp_checkpoint_load restores it into an initialised context of the same
+ * grid, storage mode, scheme and SGT points (media and source are the
+ * caller's inputs as at the start), after which awp_step continues bit for
+ * bit as the uninterrupted run (a load that fails after it has started
+ * copying leaves the run uninitialised).  Replaces the reference's per-rank restart
+ * files (pmcl3d.f:1016-1097 read at start-up, :1964-2045 written every
+ * CHECKPOINT steps), which hold the same state for one MPI rank. */
+int awp_checkpoint_save(awp_ctx *ctx, const char *path);
+int awp_checkpoint_load(awp_ctx *ctx, const char *path);
+
 /* ---- SGT header (SgtHead write_head) -------------------------------------
  * The separate header file SgtHead's write_head writes beside an AWP SGT
  * run (SgtHead/src/write_head.c:51-469): one sgtmaster record, then one
