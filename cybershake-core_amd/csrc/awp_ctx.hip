@@ -182,7 +182,7 @@ extern "C" int awp_create(awp_ctx **out, const awp_in3d *p, int device, int stor
     // memory variable, after r1..r6 (zeroed with them)
     const int nseg = (p->nx + TX - 1) / TX;
     const size_t rexp_n = (storage == AWP_STORAGE_MIXED) ? (size_t)p->nz * p->ny * nseg : 0;
-    const size_t rexp_bytes = (6 * rexp_n + 255) / 256 * 256;
+    const size_t rexp_bytes = (8 * rexp_n + 255) / 256 * 256;   // 8-byte slot per row segment
     const size_t nbytes = nv * (9 * nbuf + 3) * sizeof(float) + 8 * eslot + rexp_bytes;
     e = hipMalloc((void **)&ctx->vol, nbytes);
     if (e != hipSuccess) {
@@ -671,7 +671,7 @@ extern "C" int awp_estimate_bytes(const awp_in3d *p, int storage, int npts, size
     const size_t esz = storage == AWP_STORAGE_MIXED ? 2 : 4;
     const size_t eslot = nv * esz + (esz == 2 ? 128 : 0);
     const size_t rexp_n = storage == AWP_STORAGE_MIXED ? (size_t)p->nz * p->ny * ((p->nx + TX - 1) / TX) : 0;
-    const size_t vol = nv * (9 * (fused ? 2 : 1) + 3) * sizeof(float) + 8 * eslot + (6 * rexp_n + 255) / 256 * 256;
+    const size_t vol = nv * (9 * (fused ? 2 : 1) + 3) * sizeof(float) + 8 * eslot + (8 * rexp_n + 255) / 256 * 256;
     const size_t prof = ((size_t)p->nx + p->ny + p->nz + 6) * sizeof(float);
     const size_t nsamp_max = (size_t)std::max(p->nt / std::max(p->ntiskp_sgt, 1), 1);
     const size_t sgt = (size_t)std::max(npts, 1) * 6 * nsamp_max * sizeof(float);

@@ -176,11 +176,10 @@ __device__ __forceinline__ void step_tile(const Geo &g, const State &a0, const S
         const unsigned orr = o >> RSH;
         if constexpr (P::mixed) {
             // FP16 values under the row segment's exponent (one per wave)
-            const G<signed char> *ex = a.rexp + rexp_idx(g, a.nseg, bx, j, s);
-            const long long n = a.rexp_n;
-            x.r1 = r_decode(gl(R1, orr), ex[0]); x.r2 = r_decode(gl(R2, orr), ex[n]);
-            x.r3 = r_decode(gl(R3, orr), ex[2 * n]); x.r4 = r_decode(gl(R4, orr), ex[3 * n]);
-            x.r5 = r_decode(gl(R5, orr), ex[4 * n]); x.r6 = r_decode(gl(R6, orr), ex[5 * n]);
+            const unsigned long long ew = ((const G<unsigned long long> *)a.rexp)[rexp_idx(g, a.nseg, bx, j, s)];
+            x.r1 = r_decode(gl(R1, orr), rexp_of(ew, 0)); x.r2 = r_decode(gl(R2, orr), rexp_of(ew, 1));
+            x.r3 = r_decode(gl(R3, orr), rexp_of(ew, 2)); x.r4 = r_decode(gl(R4, orr), rexp_of(ew, 3));
+            x.r5 = r_decode(gl(R5, orr), rexp_of(ew, 4)); x.r6 = r_decode(gl(R6, orr), rexp_of(ew, 5));
         } else {
             x.r1 = gl(R1, orr); x.r2 = gl(R2, orr); x.r3 = gl(R3, orr);
             x.r4 = gl(R4, orr); x.r5 = gl(R5, orr); x.r6 = gl(R6, orr);
@@ -471,14 +470,14 @@ __device__ __forceinline__ void step_tile(const Geo &g, const State &a0, const S
             if (is_main && s >= kb) {
                 const unsigned orr = (ob + (unsigned)s * sz4) >> RSH;
                 G<R> *RR[6] = {R1, R2, R3, R4, R5, R6};
+                int e[6];
 #pragma unroll
                 for (int q = 0; q < 6; ++q) {
-                    const int e = row_exponent(wave_max(fabsf(rn[q])));
-                    if (do_str) {
-                        gs(RR[q], orr, r_encode(rn[q], e));
-                        if (lane == 0) a.rexp[q * a.rexp_n + rexp_idx(g, a.nseg, bx, j, s)] = (signed char)e;
-                    }
+                    e[q] = row_exponent(wave_max(fabsf(rn[q])));
+                    if (do_str) gs(RR[q], orr, r_encode(rn[q], e[q]));
                 }
+                if (do_str && lane == 0)
+                    ((G<unsigned long long> *)a.rexp)[rexp_idx(g, a.nseg, bx, j, s)] = rexp_pack(e);
             }
         }
 

@@ -273,10 +273,10 @@ __device__ inline void stress_tile_e(const Geo &g, const State &s0, const Coef &
 
         float r1, r2, r3, r4, r5, r6;
         if constexpr (P::mixed) {
-            const G<signed char> *ex = s.rexp + rexp_idx(g, s.nseg, bx, j, k);
-            const long long n = s.rexp_n;
-            r1 = r_decode(R1[o], ex[0]); r2 = r_decode(R2[o], ex[n]); r3 = r_decode(R3[o], ex[2 * n]);
-            r4 = r_decode(R4[o], ex[3 * n]); r5 = r_decode(R5[o], ex[4 * n]); r6 = r_decode(R6[o], ex[5 * n]);
+            const unsigned long long ew = ((const G<unsigned long long> *)s.rexp)[rexp_idx(g, s.nseg, bx, j, k)];
+            r1 = r_decode(R1[o], rexp_of(ew, 0)); r2 = r_decode(R2[o], rexp_of(ew, 1));
+            r3 = r_decode(R3[o], rexp_of(ew, 2)); r4 = r_decode(R4[o], rexp_of(ew, 3));
+            r5 = r_decode(R5[o], rexp_of(ew, 4)); r6 = r_decode(R6[o], rexp_of(ew, 5));
         } else {
             r1 = R1[o]; r2 = R2[o]; r3 = R3[o]; r4 = R4[o]; r5 = R5[o]; r6 = R6[o];
         }
@@ -325,15 +325,14 @@ __device__ inline void stress_tile_e(const Geo &g, const State &s0, const Coef &
         if constexpr (P::mixed) {
             const float rv[6] = {r1, r2, r3, r4, r5, r6};
             G<R> *RR[6] = {R1, R2, R3, R4, R5, R6};
-            const long long xi = rexp_idx(g, s.nseg, bx, j, k);
+            int e[6];
 #pragma unroll
             for (int q = 0; q < 6; ++q) {
-                const int e = row_exponent(wave_max(act ? fabsf(rv[q]) : 0.0f));
-                if (act) {
-                    RR[q][o] = r_encode(rv[q], e);
-                    if (threadIdx.x == 0) s.rexp[q * s.rexp_n + xi] = (signed char)e;
-                }
+                e[q] = row_exponent(wave_max(act ? fabsf(rv[q]) : 0.0f));
+                if (act) RR[q][o] = r_encode(rv[q], e[q]);
             }
+            if (act && threadIdx.x == 0)
+                ((G<unsigned long long> *)s.rexp)[rexp_idx(g, s.nseg, bx, j, k)] = rexp_pack(e);
         } else {
             R1[o] = r1; R2[o] = r2; R3[o] = r3;
             R4[o] = r4; R5[o] = r5; R6[o] = r6;
@@ -427,8 +426,8 @@ __device__ inline void stress_tile_b(const Geo &g, const State &s0, const Coef &
     float xz_f1 = 0.f, xz_f2 = 0.f, yz_f1 = 0.f, yz_f2 = 0.f, zz_f1 = 0.f;
     // mixed storage: offset of this row segment's exponent at level k in
     // each exponent array, and its step per level
-    O xo = P::mixed ? (O)rexp_idx(g, s.nseg, bx, j, kb) : (O)0;
-    const O xstep = (O)g.ny * (O)s.nseg;
+    O xo = P::mixed ? (O)rexp_idx(g, s.nseg, bx, j, kb) * 8 : (O)0;
+    const O xstep = (O)g.ny * (O)s.nseg * 8;
 
 #pragma unroll U
     for (int k = kb; k <= ke; ++k, p.up(), xo += xstep) {
@@ -477,13 +476,11 @@ __device__ inline void stress_tile_b(const Geo &g, const State &s0, const Coef &
 
         float r1, r2, r3, r4, r5, r6;
         if constexpr (P::mixed) {
-            // exponents of this row segment: array q at s.rexp + q * rexp_n
-            // (uniform bases), index xo
-            const G<signed char> *e0 = s.rexp;
-            const long long n = s.rexp_n;
-            r1 = r_decode(p.ld(R1), e0[xo]); r2 = r_decode(p.ld(R2), (e0 + n)[xo]);
-            r3 = r_decode(p.ld(R3), (e0 + 2 * n)[xo]); r4 = r_decode(p.ld(R4), (e0 + 3 * n)[xo]);
-            r5 = r_decode(p.ld(R5), (e0 + 4 * n)[xo]); r6 = r_decode(p.ld(R6), (e0 + 5 * n)[xo]);
+            // the exponents of this row segment: one 8-byte slot at byte offset xo
+            const unsigned long long ew = gl((const G<unsigned long long> *)s.rexp, xo);
+            r1 = r_decode(p.ld(R1), rexp_of(ew, 0)); r2 = r_decode(p.ld(R2), rexp_of(ew, 1));
+            r3 = r_decode(p.ld(R3), rexp_of(ew, 2)); r4 = r_decode(p.ld(R4), rexp_of(ew, 3));
+            r5 = r_decode(p.ld(R5), rexp_of(ew, 4)); r6 = r_decode(p.ld(R6), rexp_of(ew, 5));
         } else {
             r1 = p.ld(R1); r2 = p.ld(R2); r3 = p.ld(R3);
             r4 = p.ld(R4); r5 = p.ld(R5); r6 = p.ld(R6);
@@ -534,14 +531,13 @@ __device__ inline void stress_tile_b(const Geo &g, const State &s0, const Coef &
         if constexpr (P::mixed) {
             const float rv[6] = {r1, r2, r3, r4, r5, r6};
             G<R> *RR[6] = {R1, R2, R3, R4, R5, R6};
+            int e[6];
 #pragma unroll
             for (int q = 0; q < 6; ++q) {
-                const int e = row_exponent(wave_max(act ? fabsf(rv[q]) : 0.0f));
-                if (act) {
-                    p.st(RR[q], r_encode(rv[q], e));
-                    if (threadIdx.x == 0) (s.rexp + q * s.rexp_n)[xo] = (signed char)e;
-                }
+                e[q] = row_exponent(wave_max(act ? fabsf(rv[q]) : 0.0f));
+                if (act) p.st(RR[q], r_encode(rv[q], e[q]));
             }
+            if (act && threadIdx.x == 0) gs((G<unsigned long long> *)s.rexp, xo, rexp_pack(e));
         } else {
             p.st(R1, r1); p.st(R2, r2); p.st(R3, r3);
             p.st(R4, r4); p.st(R5, r5); p.st(R6, r6);
@@ -1000,14 +996,16 @@ __global__ void __launch_bounds__(256) k_to_half(const float *src, _Float16 *dst
 }
 
 __global__ void __launch_bounds__(256)
-k_pack_r(const Geo g, const _Float16 *src, const signed char *ex, const int nseg, const int kext, float *dst)
+k_pack_r(const Geo g, const _Float16 *src, const signed char *ex, const int q, const int nseg, const int kext,
+         float *dst)
 {
     const int i = blockIdx.x * TX + threadIdx.x + 1;
     const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
     const int k = blockIdx.z + 1;
     if (i > g.nx || j > g.ny || k > g.nz + kext) return;
     // levels above the surface are never written: zero
-    const float v = k <= g.nz ? r_decode(src[gidx(g, i, j, k)], ex[rexp_idx(g, nseg, (i - 1) / TX, j, k)]) : 0.0f;
+    const float v = k <= g.nz ? r_decode(src[gidx(g, i, j, k)], ex[rexp_idx(g, nseg, (i - 1) / TX, j, k) * 8 + q])
+                              : 0.0f;
     dst[((size_t)(k - 1) * g.ny + (j - 1)) * g.nx + (i - 1)] = v;
 }
 
@@ -1263,7 +1261,7 @@ hipError_t launch_pack_field(bool mixed, const Geo &g, const State &s, int which
     } else if (which < 15) {
         if (mixed)
             hipLaunchKernelGGL(k_pack_r, grd, blk, 0, st, g, (const _Float16 *)rr[which - 9],
-                               s.rexp + (which - 9) * s.rexp_n, s.nseg, kext, dst);
+                               s.rexp, which - 9, s.nseg, kext, dst);
         else
             hipLaunchKernelGGL(k_pack_field<float>, grd, blk, 0, st, g, (const float *)rr[which - 9], 1.0f, kext, dst);
     } else {

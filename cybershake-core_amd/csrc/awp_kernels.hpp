@@ -65,10 +65,18 @@ __host__ __device__ inline long long gidx(const Geo &g, int i, int j, int k)
     return g.base + (long long)k * g.sz + (long long)j * g.sy + i;
 }
 
-// exponent slot of row segment seg at (j, k) (interior j, k)
+// exponent slot of row segment seg at (j, k) (interior j, k); each slot is
+// 8 bytes: the exponents of r1..r6 in bytes 0..5 (one load, one store)
 __host__ __device__ inline long long rexp_idx(const Geo &g, int nseg, int seg, int j, int k)
 {
     return ((long long)(k - 1) * g.ny + (j - 1)) * nseg + seg;
+}
+__device__ inline int rexp_of(unsigned long long w, int q) { return (int)(signed char)(w >> (8 * q)); }
+__device__ inline unsigned long long rexp_pack(const int e[6])
+{
+    unsigned long long w = 0;
+    for (int q = 0; q < 6; ++q) w |= (unsigned long long)(unsigned char)(signed char)e[q] << (8 * q);
+    return w;
 }
 
 // ---- storage element types ------------------------------------------------
@@ -108,7 +116,8 @@ struct StoreMixed {
 // nearest even) of r * 2^(15-e), where e is shared by the 64 points of one x
 // row segment (i = 64s+1 .. 64s+64, the lanes of one wave) at one (j, k):
 // the exponent of the segment's largest |r| (f * 2^e, f in [0.5, 1)),
-// clamped to [REXP_MIN, REXP_MAX], kept in one byte per segment.  Every
+// clamped to [REXP_MIN, REXP_MAX], kept in one byte per segment and array
+// (the six bytes of a segment side by side, rexp_idx).  Every
 // value keeps FP16's 11-bit relative precision over 30 binades below the
 // segment's largest; decoding is the exact h * 2^(e-15).  (BF16 storage of r,
 // 8-bit precision, let the SGT error grow past 1e-3 over 1500 steps.)
@@ -153,7 +162,7 @@ struct State {
     float *xx, *yy, *zz, *xy, *xz, *yz;
     float *d1, *mu, *lam;
     void *r1, *r2, *r3, *r4, *r5, *r6;
-    signed char *rexp;         // mixed: 6 arrays of rexp_n row-segment exponents
+    signed char *rexp;         // mixed: rexp_n 8-byte slots of row-segment exponents
     long long rexp_n;          // nz * ny * nseg
     int nseg;                  // (nx + 63) / 64
     void *qp, *qs;
