@@ -125,6 +125,8 @@ static int upload_tables(awp_batch *b)
             q.nbz = l.nchunk;
             q.tile0 = tile0;
             q.fs0 = fs0;
+            q.pw = l.spw > 0 ? std::min(l.spw, q.nbx) : q.nbx;
+            q.ph = l.sph > 0 ? l.sph : 1;
             tile0 += q.nbx * q.nby * q.nbz;
             fs0 += q.nbx * ((c->g.ny + 3) / 4);
             x.s[0] = x.s[1] = q.s;

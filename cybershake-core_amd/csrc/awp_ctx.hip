@@ -88,6 +88,21 @@ static Launch choose_launch(const Geo &g, bool fused, bool mixed, int device)
         }
     }
     set_block_caps(l, device);
+    // tile patches of the two-stage kernels (AWP_SPATCH=<w>x<h>, 0x0 for
+    // whole rows of tiles).  Measured at 1500x1000x300 against rows of
+    // tiles: 3 x 8 patches, FP32 stress 13.13 -> 12.80 ms; 3 x 2, mixed
+    // stress 12.12 -> 11.63 ms (an XCD's resident tiles then share more of
+    // their halo rows in its L2).
+    l.spw = 3;
+    l.sph = mixed ? 2 : 8;
+    const char *sp = getenv("AWP_SPATCH");
+    if (sp) {
+        int w = 0, h = 0;
+        if (sscanf(sp, "%dx%d", &w, &h) == 2 && w >= 0 && h >= 0 && w * h <= 4096) {
+            l.spw = w;
+            l.sph = h;
+        }
+    }
     if (fused) {
         // the fused step keeps its load prefetch depth (0, 1, 2) in `unroll`
         const char *pf = getenv("AWP_PF");

@@ -30,6 +30,7 @@ namespace awp {
 struct Tiles {
     int nbx, nby, nbz;   // tiles in x, y, z
     int per;             // tiles per XCD (grid = 8 * per blocks)
+    int pw, ph;          // tile patches (patch_tile); pw = nbx, ph = 1 is row order
 };
 
 __device__ inline bool tile_of(const Tiles &t, int &bx, int &by, int &bz)
@@ -37,10 +38,7 @@ __device__ inline bool tile_of(const Tiles &t, int &bx, int &by, int &bz)
     const int n = blockIdx.x;
     const int tile = (n & 7) * t.per + (n >> 3);
     if (tile >= t.nbx * t.nby * t.nbz) return false;
-    bx = tile % t.nbx;
-    const int r = tile / t.nbx;
-    by = r % t.nby;
-    bz = r / t.nby;
+    patch_tile(tile, t.nbx, t.nby, t.pw, t.ph, bx, by, bz);
     return true;
 }
 
@@ -610,18 +608,14 @@ k_stress(const Geo g, const State s, const Coef c, const int kc, const Tiles tl)
 }
 
 // Run of global tile `tile` (jobs ascending in tile0) and the tile's
-// (bx, by, bz) in that run's x-fastest, then y, then z-chunk order.
+// (bx, by, bz) in that run's patch order.
 __device__ inline const SJob &split_tile(const SJob *__restrict__ jobs, const int njobs, const int tile, int &bx,
                                          int &by, int &bz)
 {
     int jb = 0;
     while (jb + 1 < njobs && jobs[jb + 1].tile0 <= tile) ++jb;
     const SJob &J = jobs[jb];
-    const int lt = tile - J.tile0;
-    bx = lt % J.nbx;
-    const int r = lt / J.nbx;
-    by = r % J.nby;
-    bz = r / J.nby;
+    patch_tile(tile - J.tile0, J.nbx, J.nby, J.pw, J.ph, bx, by, bz);
     return J;
 }
 
@@ -1029,6 +1023,8 @@ static Tiles make_tiles(const Geo &g, const Launch &l)
     t.nbx = (g.nx + TX - 1) / TX;
     t.nby = (g.ny + l.tile_y - 1) / l.tile_y;
     t.nbz = l.nchunk;
+    t.pw = l.spw > 0 ? std::min(l.spw, t.nbx) : t.nbx;
+    t.ph = l.sph > 0 ? l.sph : 1;
     const long long n = (long long)t.nbx * t.nby * t.nbz;
     t.per = (int)((n + 7) / 8);
     return t;

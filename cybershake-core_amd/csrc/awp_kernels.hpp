@@ -278,6 +278,7 @@ struct Launch {
     int nchunk;      // number of z chunks
     int unroll;      // z-loop unroll factor (1 or 2); prefetch depth of the fused step
     int pw, ph;      // fused step: tile patch width / height (tiles)
+    int spw, sph;    // two-stage kernels: tile patch width / height (0 = rows of tiles)
     bool o32;        // 32-bit byte offsets reach every array (offsets32() of the run's geometry)
     bool o32_s;      // two-stage stress kernel: its 32-bit byte-offset form (only where o32)
     int bpc_v, bpc_s;   // two-stage kernels: most blocks per CU (0 = as registers allow)
@@ -375,7 +376,7 @@ hipError_t launch_step_fused_batch(bool mixed, const Launch &l, const FJob *d_jo
 int fused_batch_grid(bool mixed, const Launch &l, int device);
 
 // one run of a batched two-stage step (state updated in place); tiles of
-// 64 x tile_y points, x fastest, then y, then z chunk
+// 64 x tile_y points in patch order (patch_tile) within each z chunk
 struct SJob {
     Geo g;
     State s;
@@ -384,6 +385,7 @@ struct SJob {
     int nbx, nby, nbz;
     int tile0;   // first global tile of the run (velocity and stress stages)
     int fs0;     // first block of the run in the free-surface launch
+    int pw, ph;  // tile patches (patch_tile); pw = nbx, ph = 1 is row order
 };
 // velocity, free-surface and stress stages of every run: three launches,
 // the first and last over the tile list (grid_v / grid_s blocks, from
