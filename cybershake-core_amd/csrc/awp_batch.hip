@@ -418,33 +418,39 @@ extern "C" int awp_batch_tune(awp_batch *b, int reps)
         int ty, pf, nchunk;
         int sform;   // two-stage stress kernel: 0 = 64-bit offsets, blocks as registers allow;
                      // 1 = 32-bit offsets, 2 blocks per CU
+        bool patch;  // two-stage: the runs' tile patches (else rows of tiles)
     };
     std::vector<Cand> cands;
     for (bool ps : {true, false}) {
-        // two-stage: tile rows x z unroll x z chunks x stress kernel form
+        // two-stage: tile rows x z unroll x z chunks x stress kernel form x
+        // tile order
         for (int ty : {4, 8})
             for (int un : {1, 2})
                 for (int nc : {1, 2})
-                    for (int sf : {0, 1}) cands.push_back({true, ps, ty, un, nc, sf});
+                    for (int sf : {0, 1})
+                        for (bool pt : {true, false}) cands.push_back({true, ps, ty, un, nc, sf, pt});
         // fused: tile rows x prefetch depth (64x8 tiles only without
         // prefetch: their 13-wave blocks have 128 registers per lane) x z
         // chunks
         const int tp[6][2] = {{8, 0}, {7, 0}, {7, 1}, {7, 2}, {4, 0}, {4, 2}};
         if (b->can_fuse)
             for (const auto &v : tp)
-                for (int nc : {1, 2}) cands.push_back({false, ps, v[0], v[1], nc, 0});
+                for (int nc : {1, 2}) cands.push_back({false, ps, v[0], v[1], nc, 0, true});
     }
-    auto apply_form = [b](int sf) {
+    const int spw0 = b->l.spw, sph0 = b->l.sph;
+    auto apply_form = [b, spw0, sph0](int sf, bool patch) {
         b->l.o32_s = sf == 1;
         b->l.bpc_s = sf == 1 ? 2 : 0;
+        b->l.spw = patch ? spw0 : 0;
+        b->l.sph = patch ? sph0 : 0;
         set_block_caps(b->l, b->device);
     };
     double best = 1e30;
-    Cand pick = {b->split, b->persist, b->l.tile_y, b->l.unroll, 1, b->l.o32_s ? 1 : 0};
+    Cand pick = {b->split, b->persist, b->l.tile_y, b->l.unroll, 1, b->l.o32_s ? 1 : 0, true};
     for (const Cand &cd : cands) {
         b->split = cd.split;
         b->persist = cd.persist;
-        apply_form(cd.sform);
+        apply_form(cd.sform, cd.patch);
         b->l.tile_y = cd.ty;
         b->l.unroll = cd.pf;
         std::fill(b->nchunk.begin(), b->nchunk.end(), cd.nchunk);
@@ -466,7 +472,7 @@ extern "C" int awp_batch_tune(awp_batch *b, int reps)
     }
     b->split = pick.split;
     b->persist = pick.persist;
-    apply_form(pick.sform);
+    apply_form(pick.sform, pick.patch);
     b->l.tile_y = pick.ty;
     b->l.unroll = pick.pf;
     std::fill(b->nchunk.begin(), b->nchunk.end(), pick.nchunk);
