@@ -31,6 +31,9 @@ CASES: Dict[str, Dict] = {
     "g3_fz_surface": dict(igreen=3, src=(24, 20, 1)),
     # igreen 4 surface traction source (xz), the CPU builder's mode
     "g4_surface_xz": dict(igreen=4, src=(24, 20, 1)),
+    # igreen 5 surface traction source (yz), 2x4x2 ranks, NVAR 5 with
+    # SOCALQ off
+    "g5_surface_yz": dict(igreen=5, src=(17, 26, 1), npx=2, npy=4, npz=2, nvar=5, socalq=0),
     # igreen 6 surface normal-stress source (zz)
     "g6_surface_zz": dict(igreen=6, src=(30, 12, 1)),
     # random medium, 4x2x2 ranks, WRITE_STEP 5 (two flushes, a dropped partial

@@ -20,7 +20,8 @@ from test_gpu_parity import _inputs
 
 pytestmark = pytest.mark.gpu
 
-SAME_NT = ["g1_fx_layered", "g2_fy_nvar5_nosocalq", "g3_fz_surface", "g4_surface_xz", "g6_surface_zz"]
+SAME_NT = ["g1_fx_layered", "g2_fy_nvar5_nosocalq", "g3_fz_surface", "g4_surface_xz", "g5_surface_yz",
+           "g6_surface_zz"]
 
 
 def _make(d, storage, init=True):
