@@ -43,7 +43,8 @@ struct awp_batch {
     bool persist;                // persistent grids (else one block per tile)
     bool o32;                    // 32-bit byte offsets reach every run's arrays
     Launch l;                    // tile_y and prefetch depth / z unroll (unroll) of the batch
-    std::vector<int> nchunk;     // z chunks per run
+    std::vector<int> nchunk;     // fused scheme: z chunks per run
+    int kc_split;                // two-stage scheme: z chunk length (levels)
     hipStream_t stream;
     FJob *d_jobs;
     SJob *d_sjobs;
@@ -112,23 +113,14 @@ static int upload_tables(awp_batch *b)
     b->max_kept = 0;
     for (int r = 0; r < n; ++r) {
         awp_ctx *c = b->runs[r];
-        const Launch l = run_launch(c->g, b->l, b->nchunk[r], b->split);
+        const Launch l = run_launch(c->g, b->l, b->split ? split_nchunk(c->g.nz, b->kc_split) : b->nchunk[r],
+                                    b->split);
         AuxJob &x = aj[r];
         if (b->split) {
             SJob &q = sj[r];
-            q.g = c->g;
-            q.s = c->view(c->cur);
-            q.c = c->c;
-            q.kc = l.kc;
-            q.nbx = (c->g.nx + TX - 1) / TX;
-            q.nby = (c->g.ny + l.tile_y - 1) / l.tile_y;
-            q.nbz = l.nchunk;
-            q.tile0 = tile0;
-            q.fs0 = fs0;
-            q.pw = l.spw > 0 ? std::min(l.spw, q.nbx) : q.nbx;
-            q.ph = l.sph > 0 ? l.sph : 1;
-            tile0 += q.nbx * q.nby * q.nbz;
-            fs0 += q.nbx * ((c->g.ny + 3) / 4);
+            q = split_job(c->g, c->view(c->cur), c->c, l, tile0, fs0);
+            tile0 += split_tiles(q);
+            fs0 += split_fs_blocks(q);
             x.s[0] = x.s[1] = q.s;
         } else {
             FJob &f = fj[r];
@@ -286,6 +278,7 @@ extern "C" int awp_batch_create(awp_batch **out, awp_ctx *const *runs, int nruns
         b->l.tile_y = (b->l.tile_y == 4 || b->l.tile_y == 7 || b->l.tile_y == 8) ? b->l.tile_y : 8;
     }
     b->nchunk.assign(nruns, 1);
+    b->kc_split = SPLIT_KC;
     b->d_jobs = nullptr;
     b->d_sjobs = nullptr;
     b->d_aux = nullptr;
@@ -402,7 +395,9 @@ extern "C" int awp_batch_get_launch(const awp_batch *b, int *tile_y, int *prefet
     if (!b) return AWP_EINVAL;
     if (tile_y) *tile_y = b->l.tile_y;
     if (prefetch) *prefetch = b->l.unroll;
-    if (nchunk_run0) *nchunk_run0 = b->nchunk.empty() ? 0 : b->nchunk[0];
+    if (nchunk_run0)
+        *nchunk_run0 = b->split ? run_launch(b->runs[0]->g, b->l, split_nchunk(b->runs[0]->g.nz, b->kc_split), true).nchunk
+                                : b->nchunk[0];
     return AWP_OK;
 }
 
@@ -415,20 +410,20 @@ extern "C" int awp_batch_tune(awp_batch *b, int reps)
     BTRY(b, hipSetDevice(b->device));
     struct Cand {
         bool split, persist;
-        int ty, pf, nchunk;
+        int ty, pf, nchunk;   // nchunk: z chunks per run (fused), chunk length in levels (two-stage)
         int sform;   // two-stage stress kernel: 0 = 64-bit offsets, blocks as registers allow;
                      // 1 = 32-bit offsets, 2 blocks per CU
         bool patch;  // two-stage: the runs' tile patches (else rows of tiles)
     };
     std::vector<Cand> cands;
     for (bool ps : {true, false}) {
-        // two-stage: tile rows x z unroll x z chunks x stress kernel form x
-        // tile order
+        // two-stage: tile rows x z unroll x z chunk length (levels) x stress
+        // kernel form x tile order
         for (int ty : {4, 8})
             for (int un : {1, 2})
-                for (int nc : {1, 2})
+                for (int kc : {SPLIT_KC, 2 * SPLIT_KC})
                     for (int sf : {0, 1})
-                        for (bool pt : {true, false}) cands.push_back({true, ps, ty, un, nc, sf, pt});
+                        for (bool pt : {true, false}) cands.push_back({true, ps, ty, un, kc, sf, pt});
         // fused: tile rows x prefetch depth (64x8 tiles only without
         // prefetch: their 13-wave blocks have 128 registers per lane) x z
         // chunks
@@ -446,14 +441,18 @@ extern "C" int awp_batch_tune(awp_batch *b, int reps)
         set_block_caps(b->l, b->device);
     };
     double best = 1e30;
-    Cand pick = {b->split, b->persist, b->l.tile_y, b->l.unroll, 1, b->l.o32_s ? 1 : 0, true};
+    Cand pick = {b->split, b->persist, b->l.tile_y, b->l.unroll, b->split ? b->kc_split : 1, b->l.o32_s ? 1 : 0,
+                 true};
     for (const Cand &cd : cands) {
         b->split = cd.split;
         b->persist = cd.persist;
         apply_form(cd.sform, cd.patch);
         b->l.tile_y = cd.ty;
         b->l.unroll = cd.pf;
-        std::fill(b->nchunk.begin(), b->nchunk.end(), cd.nchunk);
+        if (cd.split)
+            b->kc_split = cd.nchunk;
+        else
+            std::fill(b->nchunk.begin(), b->nchunk.end(), cd.nchunk);
         int rc = upload_tables(b);
         if (rc) return rc;
         set_grids(b);
@@ -475,7 +474,10 @@ extern "C" int awp_batch_tune(awp_batch *b, int reps)
     apply_form(pick.sform, pick.patch);
     b->l.tile_y = pick.ty;
     b->l.unroll = pick.pf;
-    std::fill(b->nchunk.begin(), b->nchunk.end(), pick.nchunk);
+    if (pick.split)
+        b->kc_split = pick.nchunk;
+    else
+        std::fill(b->nchunk.begin(), b->nchunk.end(), pick.nchunk);
     set_grids(b);
     drop_graph(b);
     // the trial steps changed the runs' state: start them again

@@ -69,14 +69,15 @@ static Launch choose_launch(const Geo &g, bool fused, bool mixed, int device)
     }
     l.o32 = use_o32(g);
     // Form and blocks per CU of the two-stage stress kernel, measured at
-    // 1500x1000x300: FP32 runs fastest in the element-indexed form with as
-    // many blocks as registers allow (one per CU), mixed storage in the
-    // 32-bit byte-offset form (125 registers) with 2 blocks per CU.
-    // AWP_O32=0/1 (stress form) and AWP_BPC=<velocity>x<stress> (0 = as
-    // registers allow) override.
-    l.o32_s = mixed;
+    // 1500x1000x300 with the tile patches and 25-level z chunks: the 32-bit
+    // byte-offset form, as many blocks as registers allow, in both storage
+    // modes (FP32 stress 12.38 ms against 13.03 in the element-indexed form;
+    // mixed 10.41 against 10.40 with a cap of 2 blocks per CU).  AWP_O32=0/1
+    // (stress form) and AWP_BPC=<velocity>x<stress> (0 = as registers allow)
+    // override.
+    l.o32_s = true;
     l.bpc_v = 0;
-    l.bpc_s = mixed ? 2 : 0;
+    l.bpc_s = 0;
     const char *o32e = getenv("AWP_O32");
     if (o32e && (!strcmp(o32e, "0") || !strcmp(o32e, "1"))) l.o32_s = o32e[0] == '1';
     const char *bpc = getenv("AWP_BPC");
@@ -110,11 +111,20 @@ static Launch choose_launch(const Geo &g, bool fused, bool mixed, int device)
         if (l.tile_y == 8) l.unroll = 0;
     }
     const long long bxy = (long long)((g.nx + TX - 1) / TX) * ((g.ny + l.tile_y - 1) / l.tile_y);
-    // enough blocks to fill the 256 CUs several times over; a fused block
-    // recomputes 4 velocity levels per z chunk, so it takes fewer chunks
-    const long long want = fused ? 1024 : 4096;
-    int nchunk = (int)std::max<long long>(1, (want + bxy - 1) / bxy);
-    nchunk = std::min(nchunk, std::max(1, g.nz / (fused ? 16 : 8)));
+    int nchunk;
+    if (fused) {
+        // enough blocks to fill the 256 CUs several times over; a fused
+        // block recomputes 4 velocity levels per z chunk, so it takes few
+        nchunk = (int)std::max<long long>(1, (1024 + bxy - 1) / bxy);
+        nchunk = std::min(nchunk, std::max(1, g.nz / 16));
+    } else {
+        // z chunks of about SPLIT_KC levels: with the tile patches, the
+        // tiles an XCD holds at once then span few levels and share their
+        // halo lines in L2 (1500x1000x300: chunks of 150 levels 18.64 ms
+        // per step FP32, 17.46 mixed; of 25 levels 18.29 and 16.25; of 12,
+        // 18.57 and 16.37 -- the z queues' start-up loads take over)
+        nchunk = split_nchunk(g.nz, SPLIT_KC);
+    }
     const char *kc_env = getenv("AWP_KC");
     if (kc_env && atoi(kc_env) >= 3) nchunk = std::max(1, (g.nz + atoi(kc_env) - 1) / atoi(kc_env));
     for (;;) {
@@ -160,6 +170,8 @@ extern "C" int awp_create(awp_ctx **out, const awp_in3d *p, int device, int stor
     ctx->l = choose_launch(ctx->g, ctx->fused, storage == AWP_STORAGE_MIXED, device);
     ctx->vol = nullptr;
     ctx->prof = nullptr;
+    ctx->d_sjob = nullptr;
+    ctx->split_total = ctx->split_fs = 0;
     ctx->npsrc = 0;
     ctx->d_psrc = ctx->d_ptop = nullptr;
     ctx->d_stf = nullptr;
@@ -300,6 +312,7 @@ extern "C" int awp_destroy(awp_ctx *ctx)
     for (auto &e : ctx->ev) hipEventDestroy(e);
     hipFree(ctx->vol);
     hipFree(ctx->prof);
+    hipFree(ctx->d_sjob);
     hipFree(ctx->d_psrc);
     hipFree(ctx->d_ptop);
     hipFree(ctx->d_stf);
@@ -561,6 +574,14 @@ extern "C" int awp_init(awp_ctx *ctx)
         HIPTRY(ctx, launch_sgt_coef(ctx->media32(), ctx->nkept, ctx->d_kpos, ctx->d_sg, ctx->stream));
         ctx->sg_done = true;
     }
+    // the two-stage step's one-run job table (state, coefficients, tiles)
+    if (!ctx->fused) {
+        if (!ctx->d_sjob) HIPTRY(ctx, hipMalloc((void **)&ctx->d_sjob, sizeof(SJob)));
+        const SJob q = split_job(ctx->g, ctx->view(0), ctx->c, ctx->l, 0, 0);
+        ctx->split_total = split_tiles(q);
+        ctx->split_fs = split_fs_blocks(q);
+        HIPTRY(ctx, hipMemcpy(ctx->d_sjob, &q, sizeof(SJob), hipMemcpyHostToDevice));
+    }
     // first source sample before the loop (pmcl3d.f:988)
     ctx->cur = 0;
     HIPTRY(ctx, launch_source(ctx->g, ctx->view(0), ctx->p.igreen, ctx->npsrc, ctx->d_psrc, ctx->d_ptop,
@@ -609,12 +630,15 @@ extern "C" int awp_step(awp_ctx *ctx, int nsteps)
             ctx->cur ^= 1;
             sn = sb;
         } else {
+            // the stage kernels over this run's one-entry job table, one
+            // block per tile
             sn = ctx->view(0);
-            HIPTRY(ctx, launch_velocity(mx, ctx->g, sn, ctx->c, ctx->l, st));
+            const int all = 1 << 30;
+            HIPTRY(ctx, launch_split_velocity(ctx->l, ctx->d_sjob, 1, ctx->split_total, all, st));
             if (e) HIPTRY(ctx, hipEventRecord(e[1], st));
-            HIPTRY(ctx, launch_fs_velocity(ctx->g, sn, st));
+            HIPTRY(ctx, launch_split_fs(ctx->d_sjob, 1, ctx->split_fs, st));
             if (e) HIPTRY(ctx, hipEventRecord(e[2], st));
-            HIPTRY(ctx, launch_stress(mx, ctx->g, sn, ctx->c, ctx->l, st));
+            HIPTRY(ctx, launch_split_stress(mx, ctx->l, ctx->l.o32, ctx->d_sjob, 1, ctx->split_total, all, st));
         }
         if (e) HIPTRY(ctx, hipEventRecord(e[3], st));
         if (i + 1 <= p.nst)

@@ -35,6 +35,10 @@ struct awp_ctx {
     float *wf[2][9];
     int cur;
     bool fused;
+    // the two-stage step's job table (one entry: this run), uploaded by
+    // awp_init, and its tile and free-surface block counts
+    awp::SJob *d_sjob;
+    int split_total, split_fs;
     float *prof;             // fx | fy | fz
     // sources
     int npsrc;

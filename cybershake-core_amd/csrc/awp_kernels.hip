@@ -22,26 +22,6 @@
 
 namespace awp {
 
-// Block -> tile map.  The hardware hands consecutive workgroup ids to the 8
-// XCDs in turn, and each XCD has its own L2.  Tiles are numbered x-fastest,
-// then y, then z chunk, and each XCD gets one contiguous range of them, so a
-// tile's x and y neighbours (whose halo lines it reads) run on the same XCD
-// at about the same time and those lines are L2 hits.
-struct Tiles {
-    int nbx, nby, nbz;   // tiles in x, y, z
-    int per;             // tiles per XCD (grid = 8 * per blocks)
-    int pw, ph;          // tile patches (patch_tile); pw = nbx, ph = 1 is row order
-};
-
-__device__ inline bool tile_of(const Tiles &t, int &bx, int &by, int &bz)
-{
-    const int n = blockIdx.x;
-    const int tile = (n & 7) * t.per + (n >> 3);
-    if (tile >= t.nbx * t.nby * t.nbz) return false;
-    patch_tile(tile, t.nbx, t.nby, t.pw, t.ph, bx, by, bz);
-    return true;
-}
-
 // ---------------------------------------------------------------------------
 // velocity: uxx1/vyy1/wzz1 (operator.f:92-244) fused with addcrjvel
 // (cerjan.f:292-328)
@@ -582,31 +562,16 @@ __device__ inline void stress_tile(const Geo &g, const State &s0, const Coef &c,
 }
 
 // ---------------------------------------------------------------------------
-// two-stage step kernels: one run (one block per tile), or a batch of runs
-// (a persistent grid per stage over the concatenated tile lists)
+// Two-stage step kernels over a job table of one or more runs: one launch per
+// stage covers the concatenated tile lists of all runs (one run: the table
+// has one entry).
+//
+// Block -> tile map.  The hardware hands consecutive workgroup ids to the 8
+// XCDs in turn, and each XCD has its own L2.  Each XCD gets one contiguous
+// range of the tile list (patches of tiles, x fastest inside a patch, within
+// each z chunk), so the tiles it holds at once are x and y neighbours whose
+// halo lines it reads from its own L2.
 // ---------------------------------------------------------------------------
-template <int U>
-__global__ void __launch_bounds__(512)
-k_velocity(const Geo g, const State s, const Coef c, const int kc, const Tiles tl)
-{
-    int bx, by, bz;
-    if (tile_of(tl, bx, by, bz)) velocity_tile<U>(g, s, c, kc, bx, by, bz);
-}
-
-__global__ void __launch_bounds__(256)
-k_fs_velocity(const Geo g, const State s)
-{
-    fs_velocity_block(g, s, blockIdx.x, blockIdx.y);
-}
-
-template <class P, int U, bool O32>
-__global__ void __launch_bounds__(512)
-k_stress(const Geo g, const State s, const Coef c, const int kc, const Tiles tl)
-{
-    int bx, by, bz;
-    if (tile_of(tl, bx, by, bz)) stress_tile<P, U, O32>(g, s, c, kc, bx, by, bz);
-}
-
 // Run of global tile `tile` (jobs ascending in tile0) and the tile's
 // (bx, by, bz) in that run's patch order.
 __device__ inline const SJob &split_tile(const SJob *__restrict__ jobs, const int njobs, const int tile, int &bx,
@@ -1017,73 +982,6 @@ k_pack_field(const Geo g, const T *src, const float inv_scale, const int kext, f
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-static Tiles make_tiles(const Geo &g, const Launch &l)
-{
-    Tiles t;
-    t.nbx = (g.nx + TX - 1) / TX;
-    t.nby = (g.ny + l.tile_y - 1) / l.tile_y;
-    t.nbz = l.nchunk;
-    t.pw = l.spw > 0 ? std::min(l.spw, t.nbx) : t.nbx;
-    t.ph = l.sph > 0 ? l.sph : 1;
-    const long long n = (long long)t.nbx * t.nby * t.nbz;
-    t.per = (int)((n + 7) / 8);
-    return t;
-}
-
-template <int U>
-static void velocity_t(const Geo &g, const State &s, const Coef &c, const Launch &l, hipStream_t st)
-{
-    const Tiles t = make_tiles(g, l);
-    hipLaunchKernelGGL((k_velocity<U>), dim3(8 * t.per), dim3(TX, l.tile_y), l.lds_v, st, g, s, c, l.kc, t);
-}
-
-hipError_t launch_velocity(bool, const Geo &g, const State &s, const Coef &c, const Launch &l, hipStream_t st)
-{
-    if (l.unroll == 2)
-        velocity_t<2>(g, s, c, l, st);
-    else
-        velocity_t<1>(g, s, c, l, st);
-    return hipGetLastError();
-}
-
-hipError_t launch_fs_velocity(const Geo &g, const State &s, hipStream_t st)
-{
-    dim3 blk(TX, 4);
-    dim3 grd((g.nx + TX - 1) / TX, (g.ny + 3) / 4, 1);
-    hipLaunchKernelGGL(k_fs_velocity, grd, blk, 0, st, g, s);
-    return hipGetLastError();
-}
-
-template <class P, int U, bool O32>
-static void stress_u(const Geo &g, const State &s, const Coef &c, const Launch &l, hipStream_t st)
-{
-    const Tiles t = make_tiles(g, l);
-    hipLaunchKernelGGL((k_stress<P, U, O32>), dim3(8 * t.per), dim3(TX, l.tile_y), l.lds_s, st, g, s, c,
-                       l.kc, t);
-}
-
-template <class P>
-static void stress_t(const Geo &g, const State &s, const Coef &c, const Launch &l, hipStream_t st)
-{
-    const bool o32 = l.o32 && l.o32_s;
-    if (l.unroll == 2) {
-        if (o32) stress_u<P, 2, true>(g, s, c, l, st);
-        else stress_u<P, 2, false>(g, s, c, l, st);
-    } else {
-        if (o32) stress_u<P, 1, true>(g, s, c, l, st);
-        else stress_u<P, 1, false>(g, s, c, l, st);
-    }
-}
-
-hipError_t launch_stress(bool mixed, const Geo &g, const State &s, const Coef &c, const Launch &l, hipStream_t st)
-{
-    if (mixed)
-        stress_t<StoreMixed>(g, s, c, l, st);
-    else
-        stress_t<StoreF32>(g, s, c, l, st);
-    return hipGetLastError();
-}
-
 hipError_t launch_source(const Geo &g, const State &s, int igreen, int npsrc, const long long *psrc_idx,
                          const long long *psrc_top, const float *stf, int nst, int istep, float vtst,
                          float vtst1, hipStream_t st)
@@ -1168,17 +1066,10 @@ static void str_batch_t(const Launch &l, const SJob *d_jobs, int njobs, int tota
 }
 
 template <class P>
-static void split_batch_p(const Launch &l, bool o32, const SJob *d_jobs, int njobs, int total, int fs_total,
-                          int grid_v, int grid_s, hipStream_t st)
+static void stress_stage(const Launch &l, bool os, const SJob *d_jobs, int njobs, int total, int per, int grid_s,
+                         hipStream_t st)
 {
-    const int per = (total + 7) / 8;
-    const bool os = o32 && l.o32_s, u2 = l.unroll == 2;
-    if (u2)
-        vel_batch_t<2>(l, d_jobs, njobs, total, per, grid_v, st);
-    else
-        vel_batch_t<1>(l, d_jobs, njobs, total, per, grid_v, st);
-    hipLaunchKernelGGL(k_fs_velocity_batch, dim3(fs_total), dim3(TX, 4), 0, st, d_jobs, njobs);
-    if (u2) {
+    if (l.unroll == 2) {
         if (os) str_batch_t<P, 2, true>(l, d_jobs, njobs, total, per, grid_s, st);
         else str_batch_t<P, 2, false>(l, d_jobs, njobs, total, per, grid_s, st);
     } else {
@@ -1187,15 +1078,45 @@ static void split_batch_p(const Launch &l, bool o32, const SJob *d_jobs, int njo
     }
 }
 
+hipError_t launch_split_velocity(const Launch &l, const SJob *d_jobs, int njobs, int total, int grid_v,
+                                 hipStream_t st)
+{
+    if (njobs <= 0 || total <= 0) return hipSuccess;
+    const int per = (total + 7) / 8;
+    if (l.unroll == 2)
+        vel_batch_t<2>(l, d_jobs, njobs, total, per, grid_v, st);
+    else
+        vel_batch_t<1>(l, d_jobs, njobs, total, per, grid_v, st);
+    return hipGetLastError();
+}
+
+hipError_t launch_split_fs(const SJob *d_jobs, int njobs, int fs_total, hipStream_t st)
+{
+    if (njobs <= 0 || fs_total <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fs_velocity_batch, dim3(fs_total), dim3(TX, 4), 0, st, d_jobs, njobs);
+    return hipGetLastError();
+}
+
+hipError_t launch_split_stress(bool mixed, const Launch &l, bool o32, const SJob *d_jobs, int njobs, int total,
+                               int grid_s, hipStream_t st)
+{
+    if (njobs <= 0 || total <= 0) return hipSuccess;
+    const int per = (total + 7) / 8;
+    const bool os = o32 && l.o32_s;
+    if (mixed)
+        stress_stage<StoreMixed>(l, os, d_jobs, njobs, total, per, grid_s, st);
+    else
+        stress_stage<StoreF32>(l, os, d_jobs, njobs, total, per, grid_s, st);
+    return hipGetLastError();
+}
+
 hipError_t launch_step_split_batch(bool mixed, const Launch &l, bool o32, const SJob *d_jobs, int njobs,
                                    int total, int fs_total, int grid_v, int grid_s, hipStream_t st)
 {
-    if (njobs <= 0 || total <= 0) return hipSuccess;
-    if (mixed)
-        split_batch_p<StoreMixed>(l, o32, d_jobs, njobs, total, fs_total, grid_v, grid_s, st);
-    else
-        split_batch_p<StoreF32>(l, o32, d_jobs, njobs, total, fs_total, grid_v, grid_s, st);
-    return hipGetLastError();
+    hipError_t e = launch_split_velocity(l, d_jobs, njobs, total, grid_v, st);
+    if (e == hipSuccess) e = launch_split_fs(d_jobs, njobs, fs_total, st);
+    if (e == hipSuccess) e = launch_split_stress(mixed, l, o32, d_jobs, njobs, total, grid_s, st);
+    return e;
 }
 
 hipError_t launch_aux_batch(const AuxJob *d_jobs, int njobs, int max_kept, int par, hipStream_t st)

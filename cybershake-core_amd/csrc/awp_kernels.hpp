@@ -251,6 +251,14 @@ struct Cursor32 {
     __device__ __forceinline__ void up() { o += sz; }
 };
 
+// z chunks of the two-stage kernels: about `kc` levels each
+constexpr int SPLIT_KC = 25;
+static inline int split_nchunk(int nz, int kc)
+{
+    const int n = (nz + kc / 2) / kc;
+    return n < 1 ? 1 : n;
+}
+
 // whether 32-bit byte offsets reach every element of a run's arrays (the
 // 32-bit forms need fewer registers and address instructions, but the
 // compiler then schedules the two-stage FP32 kernels with fewer loads in
@@ -325,9 +333,6 @@ struct MediaConst {
 };
 
 // host launchers (awp_kernels.hip); `mixed` selects the storage mode
-hipError_t launch_velocity(bool mixed, const Geo &g, const State &s, const Coef &c, const Launch &l, hipStream_t st);
-hipError_t launch_fs_velocity(const Geo &g, const State &s, hipStream_t st);
-hipError_t launch_stress(bool mixed, const Geo &g, const State &s, const Coef &c, const Launch &l, hipStream_t st);
 hipError_t launch_source(const Geo &g, const State &s, int igreen, int npsrc, const long long *psrc_idx,
                          const long long *psrc_top, const float *stf, int nst, int istep, float vtst,
                          float vtst1, hipStream_t st);
@@ -387,12 +392,38 @@ struct SJob {
     int fs0;     // first block of the run in the free-surface launch
     int pw, ph;  // tile patches (patch_tile); pw = nbx, ph = 1 is row order
 };
+// The job of one run, its tiles numbered from tile0 and its free-surface
+// blocks from fs0.
+static inline SJob split_job(const Geo &g, const State &s, const Coef &c, const Launch &l, int tile0, int fs0)
+{
+    SJob q;
+    q.g = g;
+    q.s = s;
+    q.c = c;
+    q.kc = l.kc;
+    q.nbx = (g.nx + TX - 1) / TX;
+    q.nby = (g.ny + l.tile_y - 1) / l.tile_y;
+    q.nbz = l.nchunk;
+    q.tile0 = tile0;
+    q.fs0 = fs0;
+    q.pw = l.spw > 0 ? (l.spw < q.nbx ? l.spw : q.nbx) : q.nbx;
+    q.ph = l.sph > 0 ? l.sph : 1;
+    return q;
+}
+static inline int split_tiles(const SJob &q) { return q.nbx * q.nby * q.nbz; }
+static inline int split_fs_blocks(const SJob &q) { return q.nbx * ((q.g.ny + 3) / 4); }
+
 // velocity, free-surface and stress stages of every run: three launches,
 // the first and last over the tile list (grid_v / grid_s blocks, from
 // split_batch_grids, or one block per tile when larger); fs_total = sum of
-// the runs' (nx/64) x (ny/4) blocks; o32 = offsets32() holds for every run
+// the runs' free-surface blocks; o32 = offsets32() holds for every run
 hipError_t launch_step_split_batch(bool mixed, const Launch &l, bool o32, const SJob *d_jobs, int njobs,
                                    int total, int fs_total, int grid_v, int grid_s, hipStream_t st);
+hipError_t launch_split_velocity(const Launch &l, const SJob *d_jobs, int njobs, int total, int grid_v,
+                                 hipStream_t st);
+hipError_t launch_split_fs(const SJob *d_jobs, int njobs, int fs_total, hipStream_t st);
+hipError_t launch_split_stress(bool mixed, const Launch &l, bool o32, const SJob *d_jobs, int njobs, int total,
+                               int grid_s, hipStream_t st);
 void split_batch_grids(bool mixed, const Launch &l, int device, int *grid_v, int *grid_s);
 
 // per-run data of the batched source / SGT stage
