@@ -183,7 +183,7 @@ def bench_single(a, storage):
         bytes_pt = BYTES_UPDATE[storage]
         tname = "step_%s%s_traffic.txt" % (a.config, "" if storage == "fp32" else "_" + storage)
     else:
-        kname = "k_stress (stress + memory variables + sponge + free surface)"
+        kname = "k_stress_batch (stress + memory variables + sponge + free surface; the run's one-entry job table)"
         bytes_pt = BYTES_STR[storage]
         tname = "stress_%s%s_traffic.txt" % (a.config, "" if storage == "fp32" else "_" + storage)
     ach = bytes_pt * npts_grid / (kern_ms * 1e-3) / 1e9
