@@ -430,7 +430,7 @@ extern "C" int awp_batch_tune(awp_batch *b, int reps)
         const int tp[6][2] = {{8, 0}, {7, 0}, {7, 1}, {7, 2}, {4, 0}, {4, 2}};
         if (b->can_fuse)
             for (const auto &v : tp)
-                for (int nc : {1, 2}) cands.push_back({false, ps, v[0], v[1], nc, 0, true});
+                for (int nc : {2, 4}) cands.push_back({false, ps, v[0], v[1], nc, 0, true});
     }
     const int spw0 = b->l.spw, sph0 = b->l.sph;
     auto apply_form = [b, spw0, sph0](int sf, bool patch) {

@@ -55,10 +55,11 @@ static Launch choose_launch(const Geo &g, bool fused, bool mixed, int device)
     if (ty && (atoi(ty) == 1 || atoi(ty) == 2 || atoi(ty) == 4 || atoi(ty) == 7 || atoi(ty) == 8)) l.tile_y = atoi(ty);
     if (fused && l.tile_y != 4 && l.tile_y != 7) l.tile_y = 8;
     if (!fused && l.tile_y == 7) l.tile_y = 8;
-    // tile patches of the fused step (AWP_PATCH=<w>x<h> overrides): 2 x 16
-    // tiles of 64 x 8 points, about square, 32 of them, one per CU of an XCD
-    l.pw = 2;
-    l.ph = 16;
+    // tile patches of the fused step (AWP_PATCH=<w>x<h> overrides): 4 x 4
+    // tiles of 64 x 8 points (1500x1000x300 FP32, 4 z chunks: 17.21 ms per
+    // step against 17.62 for 2 x 16 and 17.75 for 3 x 8)
+    l.pw = 4;
+    l.ph = 4;
     const char *pe = getenv("AWP_PATCH");
     if (pe) {
         int w = 0, h = 0;
@@ -113,10 +114,12 @@ static Launch choose_launch(const Geo &g, bool fused, bool mixed, int device)
     const long long bxy = (long long)((g.nx + TX - 1) / TX) * ((g.ny + l.tile_y - 1) / l.tile_y);
     int nchunk;
     if (fused) {
-        // enough blocks to fill the 256 CUs several times over; a fused
-        // block recomputes 4 velocity levels per z chunk, so it takes few
-        nchunk = (int)std::max<long long>(1, (1024 + bxy - 1) / bxy);
+        // an even number of chunks of about 75 levels: at 1500x1000x300 the
+        // fused step took 17.2-17.7 ms with 2 or 4 chunks and 19.0-19.6 ms
+        // with 3 or 5 (each chunk recomputes 4 velocity levels)
+        nchunk = 2 * std::max(1, (g.nz + 75) / 150);
         nchunk = std::min(nchunk, std::max(1, g.nz / 16));
+        (void)bxy;
     } else {
         // z chunks of about SPLIT_KC levels: with the tile patches, the
         // tiles an XCD holds at once then span few levels and share their
