@@ -271,9 +271,12 @@ extern "C" int awp_batch_create(awp_batch **out, awp_ctx *const *runs, int nruns
     for (int r = 0; r < nruns; ++r) b->o32 = b->o32 && runs[r]->l.o32;
     b->l = runs[0]->l;
     if (b->split) {
-        // two-stage kernels: 64 x {1, 2, 4, 8} tiles, z unroll 1 or 2
-        if (b->l.tile_y != 1 && b->l.tile_y != 2 && b->l.tile_y != 4) b->l.tile_y = 8;
+        // two-stage kernels: 64 x {1, 2, 4, 8} tiles, z unroll 1 or 2; 64 x 16
+        // tiles in the byte-offset stress form without z unroll
         if (b->l.unroll != 2) b->l.unroll = 1;
+        const bool tall_ok = b->o32 && b->l.o32_s && b->l.unroll == 1;
+        if (b->l.tile_y != 1 && b->l.tile_y != 2 && b->l.tile_y != 4 && !(b->l.tile_y == 16 && tall_ok))
+            b->l.tile_y = 8;
     } else {
         b->l.tile_y = (b->l.tile_y == 4 || b->l.tile_y == 7 || b->l.tile_y == 8) ? b->l.tile_y : 8;
     }
@@ -424,6 +427,11 @@ extern "C" int awp_batch_tune(awp_batch *b, int reps)
                 for (int kc : {SPLIT_KC, 2 * SPLIT_KC})
                     for (int sf : {0, 1})
                         for (bool pt : {true, false}) cands.push_back({true, ps, ty, un, kc, sf, pt});
+        // 64 x 16 tiles (1024-thread blocks): byte-offset stress form, no z
+        // unroll
+        if (b->o32)
+            for (int kc : {SPLIT_KC, 2 * SPLIT_KC})
+                for (bool pt : {true, false}) cands.push_back({true, ps, 16, 1, kc, 1, pt});
         // fused: tile rows x prefetch depth (64x8 tiles only without
         // prefetch: their 13-wave blocks have 128 registers per lane) x z
         // chunks

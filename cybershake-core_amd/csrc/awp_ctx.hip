@@ -45,14 +45,15 @@ static Launch choose_launch(const Geo &g, bool fused, bool mixed, int device)
     l.unroll = 1;
     const char *un = getenv("AWP_UNROLL");
     if (un && atoi(un) == 2) l.unroll = 2;
-    // tuning overrides: AWP_TILE_Y (rows per block: 1, 2, 4, 8 for the
-    // two-stage kernels, 4, 7 or 8 for the fused one) and AWP_KC (z levels
-    // per block)
+    // tuning overrides: AWP_TILE_Y (rows per block: 1, 2, 4, 8 or 16 for
+    // the two-stage kernels, 4, 7 or 8 for the fused one) and AWP_KC (z
+    // levels per block)
     const char *ty = getenv("AWP_TILE_Y");
     // fused defaults measured at 1500x1000x300: FP32 64x8 tiles, mixed 64x7
     // tiles, both without load prefetch (awp_batch_tune searches the rest)
     if (fused) l.tile_y = mixed ? 7 : 8;
-    if (ty && (atoi(ty) == 1 || atoi(ty) == 2 || atoi(ty) == 4 || atoi(ty) == 7 || atoi(ty) == 8)) l.tile_y = atoi(ty);
+    if (ty && (atoi(ty) == 1 || atoi(ty) == 2 || atoi(ty) == 4 || atoi(ty) == 7 || atoi(ty) == 8 || atoi(ty) == 16))
+        l.tile_y = atoi(ty);
     if (fused && l.tile_y != 4 && l.tile_y != 7) l.tile_y = 8;
     if (!fused && l.tile_y == 7) l.tile_y = 8;
     // tile patches of the fused step (AWP_PATCH=<w>x<h> overrides): 4 x 4
@@ -105,6 +106,9 @@ static Launch choose_launch(const Geo &g, bool fused, bool mixed, int device)
             l.sph = h;
         }
     }
+    // 16-row tiles (1024-thread blocks) exist only in the byte-offset
+    // stress form without z unroll
+    if (!fused && l.tile_y == 16 && (!l.o32 || !l.o32_s || l.unroll != 1)) l.tile_y = 8;
     if (fused) {
         // the fused step keeps its load prefetch depth (0, 1, 2) in `unroll`
         const char *pf = getenv("AWP_PF");

@@ -588,8 +588,10 @@ __device__ inline const SJob &split_tile(const SJob *__restrict__ jobs, const in
 // contiguous range of the global tile list with stride gridDim.x / 8, so the
 // blocks of one XCD work on neighbouring tiles of one run at a time and read
 // each other's halo lines from that XCD's L2.
-template <int U>
-__global__ void __launch_bounds__(512)
+// MAXT: most threads per block -- 512 for tiles of up to 8 rows, 1024 for
+// 16-row tiles (whose 16 waves leave 128 registers a lane)
+template <int U, int MAXT>
+__global__ void __launch_bounds__(MAXT)
 k_velocity_batch(const SJob *__restrict__ jobs, const int njobs, const int total, const int per)
 {
     const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3, nslot = gridDim.x >> 3;
@@ -602,8 +604,8 @@ k_velocity_batch(const SJob *__restrict__ jobs, const int njobs, const int total
     }
 }
 
-template <class P, int U, bool O32>
-__global__ void __launch_bounds__(512)
+template <class P, int U, bool O32, int MAXT>
+__global__ void __launch_bounds__(MAXT)
 k_stress_batch(const SJob *__restrict__ jobs, const int njobs, const int total, const int per)
 {
     const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3, nslot = gridDim.x >> 3;
@@ -1016,6 +1018,10 @@ void set_block_caps(Launch &l, int device)
     l.lds_s = lds_cap(l.bpc_s, cu, blk);
 }
 
+// tiles of more than 8 rows: the 1024-thread variants (byte-offset stress
+// form, no z unroll; choose_launch and the tuner keep to that)
+static bool tall(const Launch &l) { return l.tile_y > 8; }
+
 template <class P, int U>
 static int split_grid_t(int device, const Launch &l, int *gs)
 {
@@ -1023,10 +1029,16 @@ static int split_grid_t(int device, const Launch &l, int *gs)
     const int nt = TX * l.tile_y;
     // blocks of the variants in use that fit on a CU at once (a run whose
     // arrays need 64-bit offsets walks more tiles per block)
-    hipError_t ev = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nv, k_velocity_batch<U>, nt, l.lds_v);
-    hipError_t es = l.o32_s
-                        ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&ns, k_stress_batch<P, U, true>, nt, l.lds_s)
-                        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&ns, k_stress_batch<P, U, false>, nt, l.lds_s);
+    hipError_t ev, es;
+    if (tall(l)) {
+        ev = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nv, k_velocity_batch<1, 1024>, nt, l.lds_v);
+        es = hipOccupancyMaxActiveBlocksPerMultiprocessor(&ns, k_stress_batch<P, 1, true, 1024>, nt, l.lds_s);
+    } else {
+        ev = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nv, k_velocity_batch<U, 512>, nt, l.lds_v);
+        es = l.o32_s ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&ns, k_stress_batch<P, U, true, 512>, nt, l.lds_s)
+                     : hipOccupancyMaxActiveBlocksPerMultiprocessor(&ns, k_stress_batch<P, U, false, 512>, nt,
+                                                                    l.lds_s);
+    }
     if (ev != hipSuccess) nv = 1;
     if (es != hipSuccess) ns = 1;
     if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) ncu = 256;
@@ -1049,33 +1061,39 @@ void split_batch_grids(bool mixed, const Launch &l, int device, int *grid_v, int
 // whole XCD rounds of persistent blocks, no more than the tiles need
 static int persistent_grid(int grid, int per) { return std::max(8, std::min(grid, 8 * per)) / 8 * 8; }
 
-template <int U>
+template <int U, int MAXT>
 static void vel_batch_t(const Launch &l, const SJob *d_jobs, int njobs, int total, int per, int grid_v,
                         hipStream_t st)
 {
-    hipLaunchKernelGGL((k_velocity_batch<U>), dim3(persistent_grid(grid_v, per)), dim3(TX, l.tile_y),
+    hipLaunchKernelGGL((k_velocity_batch<U, MAXT>), dim3(persistent_grid(grid_v, per)), dim3(TX, l.tile_y),
                        l.lds_v, st, d_jobs, njobs, total, per);
 }
 
-template <class P, int U, bool O32>
+template <class P, int U, bool O32, int MAXT>
 static void str_batch_t(const Launch &l, const SJob *d_jobs, int njobs, int total, int per, int grid_s,
                         hipStream_t st)
 {
-    hipLaunchKernelGGL((k_stress_batch<P, U, O32>), dim3(persistent_grid(grid_s, per)), dim3(TX, l.tile_y),
+    hipLaunchKernelGGL((k_stress_batch<P, U, O32, MAXT>), dim3(persistent_grid(grid_s, per)), dim3(TX, l.tile_y),
                        l.lds_s, st, d_jobs, njobs, total, per);
 }
 
 template <class P>
-static void stress_stage(const Launch &l, bool os, const SJob *d_jobs, int njobs, int total, int per, int grid_s,
-                         hipStream_t st)
+static hipError_t stress_stage(const Launch &l, bool os, const SJob *d_jobs, int njobs, int total, int per,
+                               int grid_s, hipStream_t st)
 {
-    if (l.unroll == 2) {
-        if (os) str_batch_t<P, 2, true>(l, d_jobs, njobs, total, per, grid_s, st);
-        else str_batch_t<P, 2, false>(l, d_jobs, njobs, total, per, grid_s, st);
-    } else {
-        if (os) str_batch_t<P, 1, true>(l, d_jobs, njobs, total, per, grid_s, st);
-        else str_batch_t<P, 1, false>(l, d_jobs, njobs, total, per, grid_s, st);
+    if (tall(l)) {
+        if (!os || l.unroll == 2) return hipErrorInvalidValue;   // no such variant
+        str_batch_t<P, 1, true, 1024>(l, d_jobs, njobs, total, per, grid_s, st);
+        return hipSuccess;
     }
+    if (l.unroll == 2) {
+        if (os) str_batch_t<P, 2, true, 512>(l, d_jobs, njobs, total, per, grid_s, st);
+        else str_batch_t<P, 2, false, 512>(l, d_jobs, njobs, total, per, grid_s, st);
+    } else {
+        if (os) str_batch_t<P, 1, true, 512>(l, d_jobs, njobs, total, per, grid_s, st);
+        else str_batch_t<P, 1, false, 512>(l, d_jobs, njobs, total, per, grid_s, st);
+    }
+    return hipSuccess;
 }
 
 hipError_t launch_split_velocity(const Launch &l, const SJob *d_jobs, int njobs, int total, int grid_v,
@@ -1083,10 +1101,14 @@ hipError_t launch_split_velocity(const Launch &l, const SJob *d_jobs, int njobs,
 {
     if (njobs <= 0 || total <= 0) return hipSuccess;
     const int per = (total + 7) / 8;
-    if (l.unroll == 2)
-        vel_batch_t<2>(l, d_jobs, njobs, total, per, grid_v, st);
-    else
-        vel_batch_t<1>(l, d_jobs, njobs, total, per, grid_v, st);
+    if (tall(l)) {
+        if (l.unroll == 2) return hipErrorInvalidValue;   // no such variant
+        vel_batch_t<1, 1024>(l, d_jobs, njobs, total, per, grid_v, st);
+    } else if (l.unroll == 2) {
+        vel_batch_t<2, 512>(l, d_jobs, njobs, total, per, grid_v, st);
+    } else {
+        vel_batch_t<1, 512>(l, d_jobs, njobs, total, per, grid_v, st);
+    }
     return hipGetLastError();
 }
 
@@ -1103,10 +1125,9 @@ hipError_t launch_split_stress(bool mixed, const Launch &l, bool o32, const SJob
     if (njobs <= 0 || total <= 0) return hipSuccess;
     const int per = (total + 7) / 8;
     const bool os = o32 && l.o32_s;
-    if (mixed)
-        stress_stage<StoreMixed>(l, os, d_jobs, njobs, total, per, grid_s, st);
-    else
-        stress_stage<StoreF32>(l, os, d_jobs, njobs, total, per, grid_s, st);
+    const hipError_t e = mixed ? stress_stage<StoreMixed>(l, os, d_jobs, njobs, total, per, grid_s, st)
+                               : stress_stage<StoreF32>(l, os, d_jobs, njobs, total, per, grid_s, st);
+    if (e != hipSuccess) return e;
     return hipGetLastError();
 }
 

@@ -42,18 +42,22 @@ def _single(d, storage, nsteps=None):
     return r
 
 
-@pytest.mark.parametrize("scheme", ["split", "fused"])
+@pytest.mark.parametrize("scheme", ["split", "fused", "split_ty16"])
 @pytest.mark.parametrize("storage", [awpsgt.AWP_STORAGE_FP32, awpsgt.AWP_STORAGE_MIXED])
 def test_batch_equals_single_runs(case_dirs, storage, scheme, monkeypatch):
     dirs = [case_dirs[n][0] for n in SAME_NT]
     if scheme == "fused":
         monkeypatch.setenv("AWP_SCHEME", "fused")
+    if scheme == "split_ty16":
+        monkeypatch.setenv("AWP_TILE_Y", "16")
     runs = [_make(d, storage) for d in dirs]
     monkeypatch.delenv("AWP_SCHEME", raising=False)
     nt = runs[0].p.nt
     assert all(r.p.nt == nt for r in runs)
     with awpsgt.Batch(runs) as b:
         assert b.fused() == (scheme == "fused")
+        if scheme == "split_ty16":
+            assert b.launch()[0] == 16
         # uneven calls: graph pairs, odd remainders, a call past the end
         for n in (1, 36, 7, 40, 1000):
             b.step(n)
@@ -99,7 +103,7 @@ def test_batch_uneven_grids_and_tuner(case_dirs, tmp_path, scheme, monkeypatch):
             assert scheme == "fused"
             assert ty in (4, 7, 8) and pf in (0, 1, 2) and nc >= 1
         else:
-            assert ty in (4, 8) and pf in (1, 2) and nc >= 1
+            assert ty in (4, 8, 16) and pf in (1, 2) and nc >= 1
         assert all(r.progress() == (0, 0) for r in runs)
         b.step(nt)
     for k, (d, r) in enumerate(zip(dirs, runs)):

@@ -66,7 +66,7 @@ def test_mixed_media_bitwise(case_dirs):
             assert np.max(np.abs(a - b)) <= np.max(np.abs(b)) * 2.0 ** -11
 
 
-@pytest.mark.parametrize("scheme", ["split", "fused", "split_o32", "split_o64"])
+@pytest.mark.parametrize("scheme", ["split", "fused", "split_o32", "split_o64", "split_ty16"])
 @pytest.mark.parametrize("name", list(cases.CASES))
 def test_mixed_fields_bitwise_after_steps(case_dirs, name, scheme, monkeypatch):
     d, _ = case_dirs[name]
@@ -78,6 +78,9 @@ def test_mixed_fields_bitwise_after_steps(case_dirs, name, scheme, monkeypatch):
         # both byte-offset widths of the two-stage kernels, whichever the
         # storage mode uses by default
         monkeypatch.setenv("AWP_O32", "1" if scheme == "split_o32" else "0")
+    if scheme == "split_ty16":
+        # 64 x 16 tiles: the 1024-thread variants of the two-stage kernels
+        monkeypatch.setenv("AWP_TILE_Y", "16")
     with _run(p, media, xyz, stf, pts, awpsgt.AWP_STORAGE_MIXED, nsteps=0) as r:
         for target in (1, 9, 40):
             r.step(target - r.progress()[0])

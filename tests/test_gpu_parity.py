@@ -54,7 +54,7 @@ def test_media_setup_bitwise(case_dirs):
             assert np.array_equal(r.field(f), o.field(f)), f
 
 
-@pytest.mark.parametrize("scheme", ["split", "fused", "split_o32", "split_o64"])
+@pytest.mark.parametrize("scheme", ["split", "fused", "split_o32", "split_o64", "split_ty16"])
 @pytest.mark.parametrize("name", list(cases.CASES))
 def test_fields_bitwise_after_steps(case_dirs, name, scheme, monkeypatch):
     """Both time-step schemes (two-stage default, fused with AWP_SCHEME=fused)
@@ -70,6 +70,9 @@ def test_fields_bitwise_after_steps(case_dirs, name, scheme, monkeypatch):
         # both byte-offset widths of the two-stage kernels, whichever the
         # storage mode uses by default
         monkeypatch.setenv("AWP_O32", "1" if scheme == "split_o32" else "0")
+    if scheme == "split_ty16":
+        # 64 x 16 tiles: the 1024-thread variants of the two-stage kernels
+        monkeypatch.setenv("AWP_TILE_Y", "16")
     with _gpu_run(p, media, xyz, stf, pts, nsteps=0) as r:
         for target in (1, 7, 45):
             r.step(target - r.progress()[0])
