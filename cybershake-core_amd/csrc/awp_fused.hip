@@ -177,12 +177,12 @@ __device__ __forceinline__ void step_tile(const Geo &g, const State &a0, const S
         if constexpr (P::mixed) {
             // FP16 values under the row segment's exponent (one per wave)
             const unsigned long long ew = ((const G<unsigned long long> *)a.rexp)[rexp_idx(g, a.nseg, bx, j, s)];
-            x.r1 = r_decode(gl(R1, orr), rexp_of(ew, 0)); x.r2 = r_decode(gl(R2, orr), rexp_of(ew, 1));
-            x.r3 = r_decode(gl(R3, orr), rexp_of(ew, 2)); x.r4 = r_decode(gl(R4, orr), rexp_of(ew, 3));
-            x.r5 = r_decode(gl(R5, orr), rexp_of(ew, 4)); x.r6 = r_decode(gl(R6, orr), rexp_of(ew, 5));
+            x.r1 = r_decode(gln(R1, orr), rexp_of(ew, 0)); x.r2 = r_decode(gln(R2, orr), rexp_of(ew, 1));
+            x.r3 = r_decode(gln(R3, orr), rexp_of(ew, 2)); x.r4 = r_decode(gln(R4, orr), rexp_of(ew, 3));
+            x.r5 = r_decode(gln(R5, orr), rexp_of(ew, 4)); x.r6 = r_decode(gln(R6, orr), rexp_of(ew, 5));
         } else {
-            x.r1 = gl(R1, orr); x.r2 = gl(R2, orr); x.r3 = gl(R3, orr);
-            x.r4 = gl(R4, orr); x.r5 = gl(R5, orr); x.r6 = gl(R6, orr);
+            x.r1 = gln(R1, orr); x.r2 = gln(R2, orr); x.r3 = gln(R3, orr);
+            x.r4 = gln(R4, orr); x.r5 = gln(R5, orr); x.r6 = gln(R6, orr);
         }
         x.xx = gl(a.xx, o); x.yy = gl(a.yy, o); x.xy = gl(a.xy, o);
         x.fz = a.fz[s];
@@ -283,9 +283,9 @@ __device__ __forceinline__ void step_tile(const Geo &g, const State &a0, const S
                 wn = w * D;
                 if (is_main && m >= kb && m <= ke) {
                     const unsigned o = ob + (unsigned)m * sz4;
-                    gs(b.u1, o, un);
-                    gs(b.v1, o, vn);
-                    gs(b.w1, o, wn);
+                    gsn(b.u1, o, un);
+                    gsn(b.v1, o, vn);
+                    gsn(b.w1, o, wn);
                 }
             }
             dA_m = x.dA; dB_m = x.dB; dC_m = x.dC;
@@ -321,9 +321,9 @@ __device__ __forceinline__ void step_tile(const Geo &g, const State &a0, const S
                 vn = vt;
                 wn = wt;
                 if (ke == g.nz) {
-                    gs(b.u1, o + sz4, un);
-                    gs(b.v1, o + sz4, vn);
-                    gs(b.w1, o + sz4, wn);
+                    gsn(b.u1, o + sz4, un);
+                    gsn(b.v1, o + sz4, vn);
+                    gsn(b.w1, o + sz4, wn);
                 }
             }
             // level nz+1: the images; level nz+2: zero (the reference never sets it)
@@ -436,26 +436,26 @@ __device__ __forceinline__ void step_tile(const Geo &g, const State &a0, const S
             // memory variables in place (own point only); in mixed storage
             // they are encoded below, after the row segment's maximum
             if constexpr (!P::mixed) {
-                gs(R1, o, r1); gs(R2, o, r2); gs(R3, o, r3);
-                gs(R4, o, r4); gs(R5, o, r5); gs(R6, o, r6);
+                gsn(R1, o, r1); gsn(R2, o, r2); gsn(R3, o, r3);
+                gsn(R4, o, r4); gsn(R5, o, r5); gsn(R6, o, r6);
             } else {
                 rn[0] = r1; rn[1] = r2; rn[2] = r3; rn[3] = r4; rn[4] = r5; rn[5] = r6;
             }
-            gs(b.xx, o, xx); gs(b.yy, o, yy); gs(b.xy, o, xy);
+            gsn(b.xx, o, xx); gsn(b.yy, o, yy); gsn(b.xy, o, xy);
             if (s == g.nz) {
                 // fstr (operator.f:575-612): images above the surface, xz = yz = 0 on it
                 const unsigned o1 = o + sz4, o2 = o1 + sz4;
-                gs(b.zz, o, zz);
-                gs(b.zz, o1, -zz);
-                gs(b.zz, o2, -zz_f1);
-                gs(b.xz, o, 0.0f);
-                gs(b.yz, o, 0.0f);
-                gs(b.xz, o1, -xz_f1);
-                gs(b.xz, o2, -xz_f2);
-                gs(b.yz, o1, -yz_f1);
-                gs(b.yz, o2, -yz_f2);
+                gsn(b.zz, o, zz);
+                gsn(b.zz, o1, -zz);
+                gsn(b.zz, o2, -zz_f1);
+                gsn(b.xz, o, 0.0f);
+                gsn(b.yz, o, 0.0f);
+                gsn(b.xz, o1, -xz_f1);
+                gsn(b.xz, o2, -xz_f2);
+                gsn(b.yz, o1, -yz_f1);
+                gsn(b.yz, o2, -yz_f2);
             } else {
-                gs(b.zz, o, zz); gs(b.xz, o, xz); gs(b.yz, o, yz);
+                gsn(b.zz, o, zz); gsn(b.xz, o, xz); gsn(b.yz, o, yz);
             }
             xz_f2 = xz_f1; xz_f1 = xz;
             yz_f2 = yz_f1; yz_f1 = yz;
@@ -474,7 +474,7 @@ __device__ __forceinline__ void step_tile(const Geo &g, const State &a0, const S
 #pragma unroll
                 for (int q = 0; q < 6; ++q) {
                     e[q] = row_exponent(wave_max(fabsf(rn[q])));
-                    if (do_str) gs(RR[q], orr, r_encode(rn[q], e[q]));
+                    if (do_str) gsn(RR[q], orr, r_encode(rn[q], e[q]));
                 }
                 if (do_str && lane == 0)
                     ((G<unsigned long long> *)a.rexp)[rexp_idx(g, a.nseg, bx, j, s)] = rexp_pack(e);

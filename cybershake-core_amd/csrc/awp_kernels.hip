@@ -74,26 +74,26 @@ __device__ inline void velocity_tile(const Geo &g, const State &s0, const Coef &
         float d, q, u, v, w;
         d = 0.25f * ((dA + dB) + (dA_m + dB_m));
         q = dth / d;
-        u = s.u1[o] + q * (
+        u = nt_ld(&s.u1[o]) + q * (
             c1 * (xx_0 - xx_m1) + c2 * (xx_p1 - xx_m2) +
             c1 * (xy_0 - xy_jm1) + c2 * (xy_jp1 - xy_jm2) +
             c1 * (xz_0 - xz_m1) + c2 * (xz_p1 - xz_m2));
         d = 0.25f * ((dA + dC) + (dA_m + dC_m));
         q = dth / d;
-        v = s.v1[o] + q * (
+        v = nt_ld(&s.v1[o]) + q * (
             c1 * (xy_ip1 - xy_0) + c2 * (xy_ip2 - xy_im1) +
             c1 * (yy_jp1 - yy_0) + c2 * (yy_jp2 - yy_jm1) +
             c1 * (yz_0 - yz_m1) + c2 * (yz_p1 - yz_m2));
         d = 0.25f * ((dA + dC) + (dB + dD));
         q = dth / d;
-        w = s.w1[o] + q * (
+        w = nt_ld(&s.w1[o]) + q * (
             c1 * (xz_ip1 - xz_0) + c2 * (xz_ip2 - xz_im1) +
             c1 * (yz_0 - yz_jm1) + c2 * (yz_jp1 - yz_jm2) +
             c1 * (zz_p1 - zz_0) + c2 * (zz_p2 - zz_m1));
         const float D = fxy * s.fz[k];
-        s.u1[o] = u * D;
-        s.v1[o] = v * D;
-        s.w1[o] = w * D;
+        nt_st(&s.u1[o], u * D);
+        nt_st(&s.v1[o], v * D);
+        nt_st(&s.w1[o], w * D);
 
         xz_m2 = xz_m1; xz_m1 = xz_0; xz_0 = xz_p1;
         yz_m2 = yz_m1; yz_m1 = yz_0; yz_0 = yz_p1;
@@ -455,16 +455,16 @@ __device__ inline void stress_tile_b(const Geo &g, const State &s0, const Coef &
         float r1, r2, r3, r4, r5, r6;
         if constexpr (P::mixed) {
             // the exponents of this row segment: one 8-byte slot at byte offset xo
-            const unsigned long long ew = gl((const G<unsigned long long> *)s.rexp, xo);
-            r1 = r_decode(p.ld(R1), rexp_of(ew, 0)); r2 = r_decode(p.ld(R2), rexp_of(ew, 1));
-            r3 = r_decode(p.ld(R3), rexp_of(ew, 2)); r4 = r_decode(p.ld(R4), rexp_of(ew, 3));
-            r5 = r_decode(p.ld(R5), rexp_of(ew, 4)); r6 = r_decode(p.ld(R6), rexp_of(ew, 5));
+            const unsigned long long ew = gln((const G<unsigned long long> *)s.rexp, xo);
+            r1 = r_decode(p.ldn(R1), rexp_of(ew, 0)); r2 = r_decode(p.ldn(R2), rexp_of(ew, 1));
+            r3 = r_decode(p.ldn(R3), rexp_of(ew, 2)); r4 = r_decode(p.ldn(R4), rexp_of(ew, 3));
+            r5 = r_decode(p.ldn(R5), rexp_of(ew, 4)); r6 = r_decode(p.ldn(R6), rexp_of(ew, 5));
         } else {
-            r1 = p.ld(R1); r2 = p.ld(R2); r3 = p.ld(R3);
-            r4 = p.ld(R4); r5 = p.ld(R5); r6 = p.ld(R6);
+            r1 = p.ldn(R1); r2 = p.ldn(R2); r3 = p.ldn(R3);
+            r4 = p.ldn(R4); r5 = p.ldn(R5); r6 = p.ldn(R6);
         }
-        float xx = p.ld(s.xx), yy = p.ld(s.yy), zz = p.ld(s.zz);
-        float xy = p.ld(s.xy), xz = p.ld(s.xz), yz = p.ld(s.yz);
+        float xx = p.ldn(s.xx), yy = p.ldn(s.yy), zz = p.ldn(s.zz);
+        float xy = p.ldn(s.xy), xz = p.ldn(s.xz), yz = p.ldn(s.yz);
 
         xx = xx + dth * (vtmp - 2.0f * vxm * (vt2 + vt3)) + dt * r1;
         yy = yy + dth * (vtmp - 2.0f * vxm * (vt1 + vt3)) + dt * r2;
@@ -513,15 +513,15 @@ __device__ inline void stress_tile_b(const Geo &g, const State &s0, const Coef &
 #pragma unroll
             for (int q = 0; q < 6; ++q) {
                 e[q] = row_exponent(wave_max(act ? fabsf(rv[q]) : 0.0f));
-                if (act) p.st(RR[q], r_encode(rv[q], e[q]));
+                if (act) p.stn(RR[q], r_encode(rv[q], e[q]));
             }
-            if (act && threadIdx.x == 0) gs((G<unsigned long long> *)s.rexp, xo, rexp_pack(e));
+            if (act && threadIdx.x == 0) gsn((G<unsigned long long> *)s.rexp, xo, rexp_pack(e));
         } else {
-            p.st(R1, r1); p.st(R2, r2); p.st(R3, r3);
-            p.st(R4, r4); p.st(R5, r5); p.st(R6, r6);
+            p.stn(R1, r1); p.stn(R2, r2); p.stn(R3, r3);
+            p.stn(R4, r4); p.stn(R5, r5); p.stn(R6, r6);
         }
         if (act) {
-            p.st(s.xx, xx); p.st(s.yy, yy); p.st(s.xy, xy);
+            p.stn(s.xx, xx); p.stn(s.yy, yy); p.stn(s.xy, xy);
             if (k == g.nz) {
                 // fstr: images above the surface, xz = yz = 0 on it
                 p.st(s.zz, zz);
@@ -534,7 +534,7 @@ __device__ inline void stress_tile_b(const Geo &g, const State &s0, const Coef &
                 p.st(s.yz, -yz_f1, 0, 0, 1);
                 p.st(s.yz, -yz_f2, 0, 0, 2);
             } else {
-                p.st(s.zz, zz); p.st(s.xz, xz); p.st(s.yz, yz);
+                p.stn(s.zz, zz); p.stn(s.xz, xz); p.stn(s.yz, yz);
             }
         }
         xz_f2 = xz_f1; xz_f1 = xz;

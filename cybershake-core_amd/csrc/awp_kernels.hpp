@@ -226,6 +226,40 @@ __device__ __forceinline__ void gs(G<T> *base, O off, T v, int imm = 0)
 {
     *reinterpret_cast<G<T> *>(reinterpret_cast<G<char> *>(base) + off + imm) = v;
 }
+// Streamed access.  A value of a pointwise array (one read and one write per
+// step at the point itself, never a neighbour's halo) goes through the
+// caches with the non-temporal hint, leaving L2 to the halo lines of the
+// stencil arrays that neighbouring tiles read again.  AWP_NT=0 at compile
+// time turns the hint off.
+#ifndef AWP_NT
+#define AWP_NT 1
+#endif
+template <class T>
+__device__ __forceinline__ T nt_ld(const G<T> *a)
+{
+    if constexpr (AWP_NT != 0)
+        return __builtin_nontemporal_load(a);
+    else
+        return *a;
+}
+template <class T>
+__device__ __forceinline__ void nt_st(G<T> *a, T v)
+{
+    if constexpr (AWP_NT != 0)
+        __builtin_nontemporal_store(v, a);
+    else
+        *a = v;
+}
+template <class T, class O>
+__device__ __forceinline__ T gln(const G<T> *base, O off)
+{
+    return nt_ld(reinterpret_cast<const G<T> *>(reinterpret_cast<const G<char> *>(base) + off));
+}
+template <class T, class O>
+__device__ __forceinline__ void gsn(G<T> *base, O off, T v)
+{
+    nt_st(reinterpret_cast<G<T> *>(reinterpret_cast<G<char> *>(base) + off), v);
+}
 // Position of one thread's point in the volume arrays while it marches up in
 // z, as a 32-bit byte offset into a 4-byte array (2-byte arrays take it
 // halved), and access to its neighbours (di, dj, dk): every access is a
@@ -247,6 +281,17 @@ struct Cursor32 {
     __device__ __forceinline__ void st(G<T> *p, T v, int di = 0, int dj = 0, int dk = 0) const
     {
         gs(p, at(dj, dk) >> (sizeof(T) == 2 ? 1 : 0), v, di * (int)sizeof(T));
+    }
+    // the point itself, streamed (nt_ld / nt_st)
+    template <class T>
+    __device__ __forceinline__ T ldn(const G<T> *p) const
+    {
+        return gln(p, o >> (sizeof(T) == 2 ? 1 : 0));
+    }
+    template <class T>
+    __device__ __forceinline__ void stn(G<T> *p, T v) const
+    {
+        gsn(p, o >> (sizeof(T) == 2 ? 1 : 0), v);
     }
     __device__ __forceinline__ void up() { o += sz; }
 };
