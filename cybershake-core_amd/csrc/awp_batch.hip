@@ -271,14 +271,20 @@ extern "C" int awp_batch_create(awp_batch **out, awp_ctx *const *runs, int nruns
     for (int r = 0; r < nruns; ++r) b->o32 = b->o32 && runs[r]->l.o32;
     b->l = runs[0]->l;
     if (b->split) {
-        // two-stage kernels: 64 x {1, 2, 4, 8} tiles, z unroll 1 or 2; 64 x 16
-        // tiles in the byte-offset stress form without z unroll
+        // two-stage kernels: {64, 128} x {1, 2, 4, 8} tiles, z unroll 1 or 2;
+        // blocks of more than 512 threads (64 x 16, 128 x 8) in the
+        // byte-offset stress form without z unroll
         if (b->l.unroll != 2) b->l.unroll = 1;
+        if (b->l.tile_xw != 2) b->l.tile_xw = 1;
         const bool tall_ok = b->o32 && b->l.o32_s && b->l.unroll == 1;
-        if (b->l.tile_y != 1 && b->l.tile_y != 2 && b->l.tile_y != 4 && !(b->l.tile_y == 16 && tall_ok))
+        if (b->l.tile_y != 1 && b->l.tile_y != 2 && b->l.tile_y != 4 && b->l.tile_y != 16) b->l.tile_y = 8;
+        if (b->l.tile_y * b->l.tile_xw > 16 || (b->l.tile_y * b->l.tile_xw > 8 && !tall_ok)) {
             b->l.tile_y = 8;
+            b->l.tile_xw = 1;
+        }
     } else {
         b->l.tile_y = (b->l.tile_y == 4 || b->l.tile_y == 7 || b->l.tile_y == 8) ? b->l.tile_y : 8;
+        b->l.tile_xw = 1;
     }
     b->nchunk.assign(nruns, 1);
     b->kc_split = SPLIT_KC;
@@ -414,6 +420,7 @@ extern "C" int awp_batch_tune(awp_batch *b, int reps)
     struct Cand {
         bool split, persist;
         int ty, pf, nchunk;   // nchunk: z chunks per run (fused), chunk length in levels (two-stage)
+        int xw;      // two-stage: waves per tile row
         int sform;   // two-stage stress kernel: 0 = 64-bit offsets, blocks as registers allow;
                      // 1 = 32-bit offsets, 2 blocks per CU
         bool patch;  // two-stage: the runs' tile patches (else rows of tiles)
@@ -426,19 +433,24 @@ extern "C" int awp_batch_tune(awp_batch *b, int reps)
             for (int un : {1, 2})
                 for (int kc : {SPLIT_KC, 2 * SPLIT_KC})
                     for (int sf : {0, 1})
-                        for (bool pt : {true, false}) cands.push_back({true, ps, ty, un, kc, sf, pt});
-        // 64 x 16 tiles (1024-thread blocks): byte-offset stress form, no z
-        // unroll
-        if (b->o32)
-            for (int kc : {SPLIT_KC, 2 * SPLIT_KC})
-                for (bool pt : {true, false}) cands.push_back({true, ps, 16, 1, kc, 1, pt});
+                        for (bool pt : {true, false}) cands.push_back({true, ps, ty, un, kc, 1, sf, pt});
+        // 128 x 4 tiles; 64 x 16 and 128 x 8 tiles (1024-thread blocks:
+        // byte-offset stress form, no z unroll)
+        for (int kc : {SPLIT_KC, 2 * SPLIT_KC})
+            for (bool pt : {true, false}) {
+                for (int sf : {0, 1}) cands.push_back({true, ps, 4, 1, kc, 2, sf, pt});
+                if (b->o32) {
+                    cands.push_back({true, ps, 16, 1, kc, 1, 1, pt});
+                    cands.push_back({true, ps, 8, 1, kc, 2, 1, pt});
+                }
+            }
         // fused: tile rows x prefetch depth (64x8 tiles only without
         // prefetch: their 13-wave blocks have 128 registers per lane) x z
         // chunks
         const int tp[6][2] = {{8, 0}, {7, 0}, {7, 1}, {7, 2}, {4, 0}, {4, 2}};
         if (b->can_fuse)
             for (const auto &v : tp)
-                for (int nc : {2, 4}) cands.push_back({false, ps, v[0], v[1], nc, 0, true});
+                for (int nc : {2, 4}) cands.push_back({false, ps, v[0], v[1], nc, 1, 0, true});
     }
     const int spw0 = b->l.spw, sph0 = b->l.sph;
     auto apply_form = [b, spw0, sph0](int sf, bool patch) {
@@ -449,13 +461,14 @@ extern "C" int awp_batch_tune(awp_batch *b, int reps)
         set_block_caps(b->l, b->device);
     };
     double best = 1e30;
-    Cand pick = {b->split, b->persist, b->l.tile_y, b->l.unroll, b->split ? b->kc_split : 1, b->l.o32_s ? 1 : 0,
-                 true};
+    Cand pick = {b->split, b->persist, b->l.tile_y, b->l.unroll, b->split ? b->kc_split : 1, b->l.tile_xw,
+                 b->l.o32_s ? 1 : 0, true};
     for (const Cand &cd : cands) {
         b->split = cd.split;
         b->persist = cd.persist;
         apply_form(cd.sform, cd.patch);
         b->l.tile_y = cd.ty;
+        b->l.tile_xw = cd.xw;
         b->l.unroll = cd.pf;
         if (cd.split)
             b->kc_split = cd.nchunk;
@@ -481,6 +494,7 @@ extern "C" int awp_batch_tune(awp_batch *b, int reps)
     b->persist = pick.persist;
     apply_form(pick.sform, pick.patch);
     b->l.tile_y = pick.ty;
+    b->l.tile_xw = pick.xw;
     b->l.unroll = pick.pf;
     if (pick.split)
         b->kc_split = pick.nchunk;

@@ -42,6 +42,7 @@ static Launch choose_launch(const Geo &g, bool fused, bool mixed, int device)
 {
     Launch l;
     l.tile_y = 8;
+    l.tile_xw = 1;
     l.unroll = 1;
     const char *un = getenv("AWP_UNROLL");
     if (un && atoi(un) == 2) l.unroll = 2;
@@ -106,9 +107,15 @@ static Launch choose_launch(const Geo &g, bool fused, bool mixed, int device)
             l.sph = h;
         }
     }
-    // 16-row tiles (1024-thread blocks) exist only in the byte-offset
-    // stress form without z unroll
-    if (!fused && l.tile_y == 16 && (!l.o32 || !l.o32_s || l.unroll != 1)) l.tile_y = 8;
+    // AWP_TILE_XW=2: two-stage tiles of two waves side by side in x (128
+    // points wide).  Blocks of more than 512 threads (64 x 16, 128 x 8
+    // tiles) exist only in the byte-offset stress form without z unroll.
+    const char *xw = getenv("AWP_TILE_XW");
+    if (!fused && xw && atoi(xw) == 2 && l.tile_y <= 8) l.tile_xw = 2;
+    if (!fused && l.tile_y * l.tile_xw > 8 && (!l.o32 || !l.o32_s || l.unroll != 1)) {
+        l.tile_y = 8;
+        l.tile_xw = 1;
+    }
     if (fused) {
         // the fused step keeps its load prefetch depth (0, 1, 2) in `unroll`
         const char *pf = getenv("AWP_PF");

@@ -32,7 +32,7 @@ __device__ inline void velocity_tile(const Geo &g, const State &s0, const Coef &
 {
     const GState s = as_global(s0);
     // the velocity stage reads only FP32 arrays in both storage modes
-    const int i = bx * TX + threadIdx.x + 1;
+    const int i = (bx * blockDim.z + threadIdx.z) * TX + threadIdx.x + 1;
     const int j = by * blockDim.y + threadIdx.y + 1;
     const int kb = bz * kc + 1;
     const int ke = min(kb + kc - 1, g.nz);
@@ -155,7 +155,8 @@ __device__ inline void stress_tile_e(const Geo &g, const State &s0, const Coef &
     const GState s = as_global(s0);
     using Q = typename P::Q;
     using R = typename P::R;
-    const int ir = bx * TX + threadIdx.x + 1;
+    const int seg = bx * blockDim.z + threadIdx.z;   // the wave's 64-point row segment
+    const int ir = seg * TX + threadIdx.x + 1;
     const int jr = by * blockDim.y + threadIdx.y + 1;
     const int kb = bz * kc + 1;
     const int ke = min(kb + kc - 1, g.nz);
@@ -251,7 +252,7 @@ __device__ inline void stress_tile_e(const Geo &g, const State &s0, const Coef &
 
         float r1, r2, r3, r4, r5, r6;
         if constexpr (P::mixed) {
-            const unsigned long long ew = ((const G<unsigned long long> *)s.rexp)[rexp_idx(g, s.nseg, bx, j, k)];
+            const unsigned long long ew = ((const G<unsigned long long> *)s.rexp)[rexp_idx(g, s.nseg, seg, j, k)];
             r1 = r_decode(R1[o], rexp_of(ew, 0)); r2 = r_decode(R2[o], rexp_of(ew, 1));
             r3 = r_decode(R3[o], rexp_of(ew, 2)); r4 = r_decode(R4[o], rexp_of(ew, 3));
             r5 = r_decode(R5[o], rexp_of(ew, 4)); r6 = r_decode(R6[o], rexp_of(ew, 5));
@@ -310,7 +311,7 @@ __device__ inline void stress_tile_e(const Geo &g, const State &s0, const Coef &
                 if (act) RR[q][o] = r_encode(rv[q], e[q]);
             }
             if (act && threadIdx.x == 0)
-                ((G<unsigned long long> *)s.rexp)[rexp_idx(g, s.nseg, bx, j, k)] = rexp_pack(e);
+                ((G<unsigned long long> *)s.rexp)[rexp_idx(g, s.nseg, seg, j, k)] = rexp_pack(e);
         } else {
             R1[o] = r1; R2[o] = r2; R3[o] = r3;
             R4[o] = r4; R5[o] = r5; R6[o] = r6;
@@ -353,7 +354,8 @@ __device__ inline void stress_tile_b(const Geo &g, const State &s0, const Coef &
     const GState s = as_global(s0);
     using Q = typename P::Q;
     using R = typename P::R;
-    const int ir = bx * TX + threadIdx.x + 1;
+    const int seg = bx * blockDim.z + threadIdx.z;   // the wave's 64-point row segment
+    const int ir = seg * TX + threadIdx.x + 1;
     const int jr = by * blockDim.y + threadIdx.y + 1;
     const int kb = bz * kc + 1;
     const int ke = min(kb + kc - 1, g.nz);
@@ -404,7 +406,7 @@ __device__ inline void stress_tile_b(const Geo &g, const State &s0, const Coef &
     float xz_f1 = 0.f, xz_f2 = 0.f, yz_f1 = 0.f, yz_f2 = 0.f, zz_f1 = 0.f;
     // mixed storage: offset of this row segment's exponent at level k in
     // each exponent array, and its step per level
-    O xo = P::mixed ? (O)rexp_idx(g, s.nseg, bx, j, kb) * 8 : (O)0;
+    O xo = P::mixed ? (O)rexp_idx(g, s.nseg, seg, j, kb) * 8 : (O)0;
     const O xstep = (O)g.ny * (O)s.nseg * 8;
 
 #pragma unroll U
@@ -1018,15 +1020,16 @@ void set_block_caps(Launch &l, int device)
     l.lds_s = lds_cap(l.bpc_s, cu, blk);
 }
 
-// tiles of more than 8 rows: the 1024-thread variants (byte-offset stress
-// form, no z unroll; choose_launch and the tuner keep to that)
-static bool tall(const Launch &l) { return l.tile_y > 8; }
+// blocks of more than 512 threads (64 x 16 tiles, 128 x 8 tiles): the
+// 1024-thread variants (byte-offset stress form, no z unroll; choose_launch
+// and the tuner keep to that)
+static bool tall(const Launch &l) { return l.tile_y * l.tile_xw > 8; }
 
 template <class P, int U>
 static int split_grid_t(int device, const Launch &l, int *gs)
 {
     int nv = 0, ns = 0, ncu = 0;
-    const int nt = TX * l.tile_y;
+    const int nt = TX * l.tile_y * l.tile_xw;
     // blocks of the variants in use that fit on a CU at once (a run whose
     // arrays need 64-bit offsets walks more tiles per block)
     hipError_t ev, es;
@@ -1065,7 +1068,7 @@ template <int U, int MAXT>
 static void vel_batch_t(const Launch &l, const SJob *d_jobs, int njobs, int total, int per, int grid_v,
                         hipStream_t st)
 {
-    hipLaunchKernelGGL((k_velocity_batch<U, MAXT>), dim3(persistent_grid(grid_v, per)), dim3(TX, l.tile_y),
+    hipLaunchKernelGGL((k_velocity_batch<U, MAXT>), dim3(persistent_grid(grid_v, per)), dim3(TX, l.tile_y, l.tile_xw),
                        l.lds_v, st, d_jobs, njobs, total, per);
 }
 
@@ -1073,7 +1076,7 @@ template <class P, int U, bool O32, int MAXT>
 static void str_batch_t(const Launch &l, const SJob *d_jobs, int njobs, int total, int per, int grid_s,
                         hipStream_t st)
 {
-    hipLaunchKernelGGL((k_stress_batch<P, U, O32, MAXT>), dim3(persistent_grid(grid_s, per)), dim3(TX, l.tile_y),
+    hipLaunchKernelGGL((k_stress_batch<P, U, O32, MAXT>), dim3(persistent_grid(grid_s, per)), dim3(TX, l.tile_y, l.tile_xw),
                        l.lds_s, st, d_jobs, njobs, total, per);
 }
 

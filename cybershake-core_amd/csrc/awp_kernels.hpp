@@ -327,6 +327,7 @@ struct Coef {
 
 struct Launch {
     int tile_y;      // threads per block in y
+    int tile_xw;     // two-stage kernels: waves per tile row (1 or 2: tiles 64 or 128 points wide)
     int kc;          // z levels per block (chunk)
     int nchunk;      // number of z chunks
     int unroll;      // z-loop unroll factor (1 or 2); prefetch depth of the fused step
@@ -446,7 +447,7 @@ static inline SJob split_job(const Geo &g, const State &s, const Coef &c, const 
     q.s = s;
     q.c = c;
     q.kc = l.kc;
-    q.nbx = (g.nx + TX - 1) / TX;
+    q.nbx = (g.nx + TX * l.tile_xw - 1) / (TX * l.tile_xw);
     q.nby = (g.ny + l.tile_y - 1) / l.tile_y;
     q.nbz = l.nchunk;
     q.tile0 = tile0;
@@ -456,7 +457,7 @@ static inline SJob split_job(const Geo &g, const State &s, const Coef &c, const 
     return q;
 }
 static inline int split_tiles(const SJob &q) { return q.nbx * q.nby * q.nbz; }
-static inline int split_fs_blocks(const SJob &q) { return q.nbx * ((q.g.ny + 3) / 4); }
+static inline int split_fs_blocks(const SJob &q) { return ((q.g.nx + TX - 1) / TX) * ((q.g.ny + 3) / 4); }
 
 // velocity, free-surface and stress stages of every run: three launches,
 // the first and last over the tile list (grid_v / grid_s blocks, from

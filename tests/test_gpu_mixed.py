@@ -66,7 +66,8 @@ def test_mixed_media_bitwise(case_dirs):
             assert np.max(np.abs(a - b)) <= np.max(np.abs(b)) * 2.0 ** -11
 
 
-@pytest.mark.parametrize("scheme", ["split", "fused", "split_o32", "split_o64", "split_ty16"])
+@pytest.mark.parametrize("scheme", ["split", "fused", "split_o32", "split_o64", "split_ty16",
+                                    "split_xw2", "split_xw2_ty4", "split_xw2_o64"])
 @pytest.mark.parametrize("name", list(cases.CASES))
 def test_mixed_fields_bitwise_after_steps(case_dirs, name, scheme, monkeypatch):
     d, _ = case_dirs[name]
@@ -81,6 +82,14 @@ def test_mixed_fields_bitwise_after_steps(case_dirs, name, scheme, monkeypatch):
     if scheme == "split_ty16":
         # 64 x 16 tiles: the 1024-thread variants of the two-stage kernels
         monkeypatch.setenv("AWP_TILE_Y", "16")
+    if scheme.startswith("split_xw2"):
+        # 128-point-wide tiles (two waves per tile row): 128 x 8 tiles in
+        # 1024-thread blocks, 128 x 4 tiles in both stress forms
+        monkeypatch.setenv("AWP_TILE_XW", "2")
+        if scheme != "split_xw2":
+            monkeypatch.setenv("AWP_TILE_Y", "4")
+        if scheme == "split_xw2_o64":
+            monkeypatch.setenv("AWP_O32", "0")
     with _run(p, media, xyz, stf, pts, awpsgt.AWP_STORAGE_MIXED, nsteps=0) as r:
         for target in (1, 9, 40):
             r.step(target - r.progress()[0])
