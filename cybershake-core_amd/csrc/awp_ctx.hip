@@ -83,6 +83,10 @@ static Launch choose_launch(const Geo &g, bool fused, bool mixed, int device)
     l.bpc_s = 0;
     const char *o32e = getenv("AWP_O32");
     if (o32e && (!strcmp(o32e, "0") || !strcmp(o32e, "1"))) l.o32_s = o32e[0] == '1';
+    // velocity kernel form (AWP_O32V=0/1)
+    l.o32_v = false;
+    const char *o32v = getenv("AWP_O32V");
+    if (o32v && (!strcmp(o32v, "0") || !strcmp(o32v, "1"))) l.o32_v = o32v[0] == '1';
     const char *bpc = getenv("AWP_BPC");
     if (bpc) {
         int v = 0, s2 = 0;
@@ -648,7 +652,7 @@ extern "C" int awp_step(awp_ctx *ctx, int nsteps)
             // block per tile
             sn = ctx->view(0);
             const int all = 1 << 30;
-            HIPTRY(ctx, launch_split_velocity(ctx->l, ctx->d_sjob, 1, ctx->split_total, all, st));
+            HIPTRY(ctx, launch_split_velocity(ctx->l, ctx->l.o32, ctx->d_sjob, 1, ctx->split_total, all, st));
             if (e) HIPTRY(ctx, hipEventRecord(e[1], st));
             HIPTRY(ctx, launch_split_fs(ctx->d_sjob, 1, ctx->split_fs, st));
             if (e) HIPTRY(ctx, hipEventRecord(e[2], st));

@@ -61,7 +61,7 @@ __device__ inline void velocity_tile(const Geo &g, const State &s0, const Coef &
     for (int k = kb; k <= ke; ++k, o += sz) {
         const float xz_p1 = xz[o + sz];
         const float yz_p1 = yz[o + sz];
-        const float zz_p2 = zz[o + 2 * sz];
+        const float zz_p2 = nt_ld(&zz[o + 2 * sz]);   // zz: read at (i, j) only
         const float dA = ld(d1[o]), dB = ld(d1[o - sy]), dC = ld(d1[o + 1]), dD = ld(d1[o + 1 - sy]);
 
         const float xx_0 = xx[o], xx_m1 = xx[o - 1], xx_p1 = xx[o + 1], xx_m2 = xx[o - 2];
@@ -94,6 +94,81 @@ __device__ inline void velocity_tile(const Geo &g, const State &s0, const Coef &
         nt_st(&s.u1[o], u * D);
         nt_st(&s.v1[o], v * D);
         nt_st(&s.w1[o], w * D);
+
+        xz_m2 = xz_m1; xz_m1 = xz_0; xz_0 = xz_p1;
+        yz_m2 = yz_m1; yz_m1 = yz_0; yz_0 = yz_p1;
+        zz_m1 = zz_0; zz_0 = zz_p1; zz_p1 = zz_p2;
+        dA_m = dA; dB_m = dB; dC_m = dC;
+    }
+}
+
+// The same statements through a 32-bit byte offset (Cursor32: uniform array
+// bases, one offset register, neighbours as immediates); needs every array
+// below 4 GiB.
+template <int U>
+__device__ inline void velocity_tile_b(const Geo &g, const State &s0, const Coef &c, const int kc, const int bx,
+                                       const int by, const int bz)
+{
+    const GState s = as_global(s0);
+    const int i = (bx * blockDim.z + threadIdx.z) * TX + threadIdx.x + 1;
+    const int j = by * blockDim.y + threadIdx.y + 1;
+    const int kb = bz * kc + 1;
+    const int ke = min(kb + kc - 1, g.nz);
+    if (i > g.nx || j > g.ny) return;
+    Cursor32 p(g, g.base + (long long)kb * g.sz + (long long)j * g.sy + i);
+
+    const G<float> *__restrict__ xx = s.xx;
+    const G<float> *__restrict__ yy = s.yy;
+    const G<float> *__restrict__ zz = s.zz;
+    const G<float> *__restrict__ xy = s.xy;
+    const G<float> *__restrict__ xz = s.xz;
+    const G<float> *__restrict__ yz = s.yz;
+    const G<float> *__restrict__ d1 = s.d1;
+    const float c1 = c.c1, c2 = c.c2, dth = c.dth;
+    const float fxy = s.fx[i] * s.fy[j];
+
+    float xz_m2 = p.ld(xz, 0, 0, -2), xz_m1 = p.ld(xz, 0, 0, -1), xz_0 = p.ld(xz);
+    float yz_m2 = p.ld(yz, 0, 0, -2), yz_m1 = p.ld(yz, 0, 0, -1), yz_0 = p.ld(yz);
+    float zz_m1 = p.ld(zz, 0, 0, -1), zz_0 = p.ld(zz), zz_p1 = p.ld(zz, 0, 0, 1);
+    float dA_m = ld(p.ld(d1, 0, 0, -1)), dB_m = ld(p.ld(d1, 0, -1, -1)), dC_m = ld(p.ld(d1, 1, 0, -1));
+
+#pragma unroll U
+    for (int k = kb; k <= ke; ++k, p.up()) {
+        const float xz_p1 = p.ld(xz, 0, 0, 1);
+        const float yz_p1 = p.ld(yz, 0, 0, 1);
+        const float zz_p2 = gln(zz, p.at(0, 2));   // zz: read at (i, j) only
+        const float dA = ld(p.ld(d1)), dB = ld(p.ld(d1, 0, -1)), dC = ld(p.ld(d1, 1)), dD = ld(p.ld(d1, 1, -1));
+
+        const float xx_0 = p.ld(xx), xx_m1 = p.ld(xx, -1), xx_p1 = p.ld(xx, 1), xx_m2 = p.ld(xx, -2);
+        const float xy_0 = p.ld(xy), xy_jm1 = p.ld(xy, 0, -1), xy_jp1 = p.ld(xy, 0, 1), xy_jm2 = p.ld(xy, 0, -2);
+        const float xy_ip1 = p.ld(xy, 1), xy_ip2 = p.ld(xy, 2), xy_im1 = p.ld(xy, -1);
+        const float yy_0 = p.ld(yy), yy_jp1 = p.ld(yy, 0, 1), yy_jp2 = p.ld(yy, 0, 2), yy_jm1 = p.ld(yy, 0, -1);
+        const float xz_ip1 = p.ld(xz, 1), xz_ip2 = p.ld(xz, 2), xz_im1 = p.ld(xz, -1);
+        const float yz_jm1 = p.ld(yz, 0, -1), yz_jp1 = p.ld(yz, 0, 1), yz_jm2 = p.ld(yz, 0, -2);
+
+        float d, q, u, v, w;
+        d = 0.25f * ((dA + dB) + (dA_m + dB_m));
+        q = dth / d;
+        u = p.ldn(s.u1) + q * (
+            c1 * (xx_0 - xx_m1) + c2 * (xx_p1 - xx_m2) +
+            c1 * (xy_0 - xy_jm1) + c2 * (xy_jp1 - xy_jm2) +
+            c1 * (xz_0 - xz_m1) + c2 * (xz_p1 - xz_m2));
+        d = 0.25f * ((dA + dC) + (dA_m + dC_m));
+        q = dth / d;
+        v = p.ldn(s.v1) + q * (
+            c1 * (xy_ip1 - xy_0) + c2 * (xy_ip2 - xy_im1) +
+            c1 * (yy_jp1 - yy_0) + c2 * (yy_jp2 - yy_jm1) +
+            c1 * (yz_0 - yz_m1) + c2 * (yz_p1 - yz_m2));
+        d = 0.25f * ((dA + dC) + (dB + dD));
+        q = dth / d;
+        w = p.ldn(s.w1) + q * (
+            c1 * (xz_ip1 - xz_0) + c2 * (xz_ip2 - xz_im1) +
+            c1 * (yz_0 - yz_jm1) + c2 * (yz_jp1 - yz_jm2) +
+            c1 * (zz_p1 - zz_0) + c2 * (zz_p2 - zz_m1));
+        const float D = fxy * s.fz[k];
+        p.stn(s.u1, u * D);
+        p.stn(s.v1, v * D);
+        p.stn(s.w1, w * D);
 
         xz_m2 = xz_m1; xz_m1 = xz_0; xz_0 = xz_p1;
         yz_m2 = yz_m1; yz_m1 = yz_0; yz_0 = yz_p1;
@@ -592,7 +667,7 @@ __device__ inline const SJob &split_tile(const SJob *__restrict__ jobs, const in
 // each other's halo lines from that XCD's L2.
 // MAXT: most threads per block -- 512 for tiles of up to 8 rows, 1024 for
 // 16-row tiles (whose 16 waves leave 128 registers a lane)
-template <int U, int MAXT>
+template <int U, int MAXT, bool O32>
 __global__ void __launch_bounds__(MAXT)
 k_velocity_batch(const SJob *__restrict__ jobs, const int njobs, const int total, const int per)
 {
@@ -602,7 +677,10 @@ k_velocity_batch(const SJob *__restrict__ jobs, const int njobs, const int total
         if (tile >= total) break;
         int bx, by, bz;
         const SJob &J = split_tile(jobs, njobs, tile, bx, by, bz);
-        velocity_tile<U>(J.g, J.s, J.c, J.kc, bx, by, bz);
+        if constexpr (O32)
+            velocity_tile_b<U>(J.g, J.s, J.c, J.kc, bx, by, bz);
+        else
+            velocity_tile<U>(J.g, J.s, J.c, J.kc, bx, by, bz);
     }
 }
 
@@ -1034,10 +1112,13 @@ static int split_grid_t(int device, const Launch &l, int *gs)
     // arrays need 64-bit offsets walks more tiles per block)
     hipError_t ev, es;
     if (tall(l)) {
-        ev = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nv, k_velocity_batch<1, 1024>, nt, l.lds_v);
+        ev = l.o32_v ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nv, k_velocity_batch<1, 1024, true>, nt, l.lds_v)
+                     : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nv, k_velocity_batch<1, 1024, false>, nt,
+                                                                    l.lds_v);
         es = hipOccupancyMaxActiveBlocksPerMultiprocessor(&ns, k_stress_batch<P, 1, true, 1024>, nt, l.lds_s);
     } else {
-        ev = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nv, k_velocity_batch<U, 512>, nt, l.lds_v);
+        ev = l.o32_v ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nv, k_velocity_batch<U, 512, true>, nt, l.lds_v)
+                     : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nv, k_velocity_batch<U, 512, false>, nt, l.lds_v);
         es = l.o32_s ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&ns, k_stress_batch<P, U, true, 512>, nt, l.lds_s)
                      : hipOccupancyMaxActiveBlocksPerMultiprocessor(&ns, k_stress_batch<P, U, false, 512>, nt,
                                                                     l.lds_s);
@@ -1065,11 +1146,14 @@ void split_batch_grids(bool mixed, const Launch &l, int device, int *grid_v, int
 static int persistent_grid(int grid, int per) { return std::max(8, std::min(grid, 8 * per)) / 8 * 8; }
 
 template <int U, int MAXT>
-static void vel_batch_t(const Launch &l, const SJob *d_jobs, int njobs, int total, int per, int grid_v,
+static void vel_batch_t(const Launch &l, bool ov, const SJob *d_jobs, int njobs, int total, int per, int grid_v,
                         hipStream_t st)
 {
-    hipLaunchKernelGGL((k_velocity_batch<U, MAXT>), dim3(persistent_grid(grid_v, per)), dim3(TX, l.tile_y, l.tile_xw),
-                       l.lds_v, st, d_jobs, njobs, total, per);
+    const dim3 grid(persistent_grid(grid_v, per)), block(TX, l.tile_y, l.tile_xw);
+    if (ov)
+        hipLaunchKernelGGL((k_velocity_batch<U, MAXT, true>), grid, block, l.lds_v, st, d_jobs, njobs, total, per);
+    else
+        hipLaunchKernelGGL((k_velocity_batch<U, MAXT, false>), grid, block, l.lds_v, st, d_jobs, njobs, total, per);
 }
 
 template <class P, int U, bool O32, int MAXT>
@@ -1099,18 +1183,19 @@ static hipError_t stress_stage(const Launch &l, bool os, const SJob *d_jobs, int
     return hipSuccess;
 }
 
-hipError_t launch_split_velocity(const Launch &l, const SJob *d_jobs, int njobs, int total, int grid_v,
+hipError_t launch_split_velocity(const Launch &l, bool o32, const SJob *d_jobs, int njobs, int total, int grid_v,
                                  hipStream_t st)
 {
     if (njobs <= 0 || total <= 0) return hipSuccess;
     const int per = (total + 7) / 8;
+    const bool ov = o32 && l.o32_v;
     if (tall(l)) {
         if (l.unroll == 2) return hipErrorInvalidValue;   // no such variant
-        vel_batch_t<1, 1024>(l, d_jobs, njobs, total, per, grid_v, st);
+        vel_batch_t<1, 1024>(l, ov, d_jobs, njobs, total, per, grid_v, st);
     } else if (l.unroll == 2) {
-        vel_batch_t<2, 512>(l, d_jobs, njobs, total, per, grid_v, st);
+        vel_batch_t<2, 512>(l, ov, d_jobs, njobs, total, per, grid_v, st);
     } else {
-        vel_batch_t<1, 512>(l, d_jobs, njobs, total, per, grid_v, st);
+        vel_batch_t<1, 512>(l, ov, d_jobs, njobs, total, per, grid_v, st);
     }
     return hipGetLastError();
 }
@@ -1137,7 +1222,7 @@ hipError_t launch_split_stress(bool mixed, const Launch &l, bool o32, const SJob
 hipError_t launch_step_split_batch(bool mixed, const Launch &l, bool o32, const SJob *d_jobs, int njobs,
                                    int total, int fs_total, int grid_v, int grid_s, hipStream_t st)
 {
-    hipError_t e = launch_split_velocity(l, d_jobs, njobs, total, grid_v, st);
+    hipError_t e = launch_split_velocity(l, o32, d_jobs, njobs, total, grid_v, st);
     if (e == hipSuccess) e = launch_split_fs(d_jobs, njobs, fs_total, st);
     if (e == hipSuccess) e = launch_split_stress(mixed, l, o32, d_jobs, njobs, total, grid_s, st);
     return e;

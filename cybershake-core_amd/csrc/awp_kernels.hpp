@@ -335,6 +335,7 @@ struct Launch {
     int spw, sph;    // two-stage kernels: tile patch width / height (0 = rows of tiles)
     bool o32;        // 32-bit byte offsets reach every array (offsets32() of the run's geometry)
     bool o32_s;      // two-stage stress kernel: its 32-bit byte-offset form (only where o32)
+    bool o32_v;      // two-stage velocity kernel: its 32-bit byte-offset form (only where o32)
     int bpc_v, bpc_s;   // two-stage kernels: most blocks per CU (0 = as registers allow)
     unsigned lds_v, lds_s;   // the dynamic LDS bytes per block that enforce them (set_block_caps)
 };
@@ -465,7 +466,7 @@ static inline int split_fs_blocks(const SJob &q) { return ((q.g.nx + TX - 1) / T
 // the runs' free-surface blocks; o32 = offsets32() holds for every run
 hipError_t launch_step_split_batch(bool mixed, const Launch &l, bool o32, const SJob *d_jobs, int njobs,
                                    int total, int fs_total, int grid_v, int grid_s, hipStream_t st);
-hipError_t launch_split_velocity(const Launch &l, const SJob *d_jobs, int njobs, int total, int grid_v,
+hipError_t launch_split_velocity(const Launch &l, bool o32, const SJob *d_jobs, int njobs, int total, int grid_v,
                                  hipStream_t st);
 hipError_t launch_split_fs(const SJob *d_jobs, int njobs, int fs_total, hipStream_t st);
 hipError_t launch_split_stress(bool mixed, const Launch &l, bool o32, const SJob *d_jobs, int njobs, int total,

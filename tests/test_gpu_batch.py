@@ -42,7 +42,7 @@ def _single(d, storage, nsteps=None):
     return r
 
 
-@pytest.mark.parametrize("scheme", ["split", "fused", "split_ty16", "split_xw2"])
+@pytest.mark.parametrize("scheme", ["split", "fused", "split_ty16", "split_xw2", "split_o32v"])
 @pytest.mark.parametrize("storage", [awpsgt.AWP_STORAGE_FP32, awpsgt.AWP_STORAGE_MIXED])
 def test_batch_equals_single_runs(case_dirs, storage, scheme, monkeypatch):
     dirs = [case_dirs[n][0] for n in SAME_NT]
@@ -52,6 +52,8 @@ def test_batch_equals_single_runs(case_dirs, storage, scheme, monkeypatch):
         monkeypatch.setenv("AWP_TILE_Y", "16")
     if scheme == "split_xw2":
         monkeypatch.setenv("AWP_TILE_XW", "2")
+    if scheme == "split_o32v":
+        monkeypatch.setenv("AWP_O32V", "1")
     runs = [_make(d, storage) for d in dirs]
     monkeypatch.delenv("AWP_SCHEME", raising=False)
     nt = runs[0].p.nt

@@ -67,7 +67,8 @@ def test_mixed_media_bitwise(case_dirs):
 
 
 @pytest.mark.parametrize("scheme", ["split", "fused", "split_o32", "split_o64", "split_ty16",
-                                    "split_xw2", "split_xw2_ty4", "split_xw2_o64"])
+                                    "split_xw2", "split_xw2_ty4", "split_xw2_o64", "split_o32v",
+                                    "split_o64v"])
 @pytest.mark.parametrize("name", list(cases.CASES))
 def test_mixed_fields_bitwise_after_steps(case_dirs, name, scheme, monkeypatch):
     d, _ = case_dirs[name]
@@ -82,6 +83,9 @@ def test_mixed_fields_bitwise_after_steps(case_dirs, name, scheme, monkeypatch):
     if scheme == "split_ty16":
         # 64 x 16 tiles: the 1024-thread variants of the two-stage kernels
         monkeypatch.setenv("AWP_TILE_Y", "16")
+    if scheme in ("split_o32v", "split_o64v"):
+        # both forms of the velocity kernel
+        monkeypatch.setenv("AWP_O32V", "1" if scheme == "split_o32v" else "0")
     if scheme.startswith("split_xw2"):
         # 128-point-wide tiles (two waves per tile row): 128 x 8 tiles in
         # 1024-thread blocks, 128 x 4 tiles in both stress forms

@@ -55,13 +55,15 @@ def test_media_setup_bitwise(case_dirs):
 
 
 @pytest.mark.parametrize("scheme", ["split", "fused", "split_o32", "split_o64", "split_ty16",
-                                    "split_xw2", "split_xw2_ty4", "split_xw2_o64"])
+                                    "split_xw2", "split_xw2_ty4", "split_xw2_o64", "split_o32v",
+                                    "split_o64v"])
 @pytest.mark.parametrize("name", list(cases.CASES))
 def test_fields_bitwise_after_steps(case_dirs, name, scheme, monkeypatch):
     """Both time-step schemes (two-stage default, fused with AWP_SCHEME=fused),
-    both byte-offset forms of the two-stage kernels (32-bit, and 64-bit as
-    grids with arrays of 4 GiB or more take) and their tile shapes (64 x 8,
-    64 x 16, 128 x 8, 128 x 4) against the restatement, bit for bit."""
+    both offset forms of the two-stage stress and velocity kernels (32-bit
+    byte offsets, and the 64-bit element indices that grids with arrays of
+    4 GiB or more take) and their tile shapes (64 x 8, 64 x 16, 128 x 8,
+    128 x 4) against the restatement, bit for bit."""
     d, _ = case_dirs[name]
     p, media, xyz, stf, pts = _inputs(d)
     o = _oracle(d, p, media, xyz, stf, pts)
@@ -74,6 +76,9 @@ def test_fields_bitwise_after_steps(case_dirs, name, scheme, monkeypatch):
     if scheme == "split_ty16":
         # 64 x 16 tiles: the 1024-thread variants of the two-stage kernels
         monkeypatch.setenv("AWP_TILE_Y", "16")
+    if scheme in ("split_o32v", "split_o64v"):
+        # both forms of the velocity kernel
+        monkeypatch.setenv("AWP_O32V", "1" if scheme == "split_o32v" else "0")
     if scheme.startswith("split_xw2"):
         # 128-point-wide tiles (two waves per tile row): 128 x 8 tiles in
         # 1024-thread blocks, 128 x 4 tiles in both stress forms
