@@ -7,10 +7,13 @@ depth gradient times a seeded von Karman perturbation (correlation 5 km,
 Hurst 0.1, 5% std, clipped to +-3 sigma), Vp = 1.73 Vs, Gardner density,
 Qs = 50 Vs[km/s], Qp = 2 Qs; X point force with a Gaussian source time
 function; 1.5M SGT points sampled every 10 steps.  Synthetic inputs are built
-on the device; the timed region is K full time steps (velocity kernel,
+on the device; the timed region is K full time steps with every input
+already resident in HBM.  A step is the two-stage one (velocity kernel,
 free-surface images, stress kernel with sponge and memory variables, source,
-SGT gather -- or, with --scheme fused, the one-sweep fused kernel in place of
-the three) with every input already resident in HBM.
+SGT gather) or the one-sweep fused kernel in place of the first three: by
+default (--scheme auto) the run takes the library's default scheme for its
+storage mode and awp_tune keeps the fastest scheme and launch shape before
+the warm-up; --scheme split / fused fixes the scheme at its default launch.
 
 Batched workloads: "c3" -- 8 runs (4 sites x X/Y force) on grids from
 400x300x150 to 900x700x220, 200k SGT points each; "c5" -- 16 runs (8 sites x
@@ -20,7 +23,7 @@ dt = 0.005 s.  Each wave is one awpsgt.Batch (one persistent launch per stage
 for all its runs, replayed from a HIP graph), tuned before the timed steps.
 
     python bench.py [--steps K] [--warmup W] [--config c2|c1|c3|c5] [--storage fp32|mixed]
-                    [--scheme split|fused] [--compare-fp32]
+                    [--scheme auto|split|fused] [--compare-fp32]
 
 Prints one JSON line.  value = point-updates per second of the whole job
 (Gpt-updates/s).  roofline = the dominant kernel: the stress kernel of the
@@ -158,6 +161,12 @@ def bench_single(a, storage):
     st = awpsgt.AWP_STORAGE_MIXED if storage == "mixed" else awpsgt.AWP_STORAGE_FP32
     run, nkept = make_run(p, st, media_fn, cfg["src"], cfg["nsgt"], 3)
     log("set-up %.1f s, %d SGT points, %.1f GB on device" % (time.time() - t0, nkept, run.device_bytes() / 1e9))
+    if a.scheme == "auto":
+        t1 = time.time()
+        run.tune(3)
+        log("tuned in %.1f s" % (time.time() - t1))
+    fused, tile_y, nchunk = run.scheme()
+    log("time step: %s, 64 x %d tiles, %d z chunks" % ("fused one-sweep" if fused else "two-stage", tile_y, nchunk))
 
     run.step(a.warmup)
     run.synchronize()
@@ -174,7 +183,6 @@ def bench_single(a, storage):
     finite = bool(np.isfinite(run.sgt()).all())
     run.close()
 
-    fused = os.environ.get("AWP_SCHEME") == "fused"
     value = npts_grid * a.steps / el / 1e9
     ms_step = el / a.steps * 1e3
     kern_ms = ts / a.steps                   # fused kernel, or the stress kernel of the two-stage step
@@ -199,7 +207,9 @@ def bench_single(a, storage):
     wl = cfg["workload"] if storage == "fp32" else cfg["workload"].replace(
         ", FP32", ", mixed storage (FP16 r1..r6 with per-row exponents, FP16 qp/qs)")
     config = {"workload": wl, "grid": [p.nx, p.ny, p.nz], "points": npts_grid, "sgt_points": int(nkept),
-              "storage": storage, "step": "fused" if fused else "two-stage", "parallelism": "single GPU"}
+              "storage": storage, "step": "fused" if fused else "two-stage",
+              "launch": {"tile_rows": tile_y, "z_chunks": nchunk, "tuned": a.scheme == "auto"},
+              "parallelism": "single GPU"}
     return value, ms_step, roof, config
 
 
@@ -299,8 +309,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
     ap.add_argument("--storage", default="fp32", choices=["fp32", "mixed"])
-    ap.add_argument("--scheme", default="split", choices=["split", "fused"],
-                    help="time-step scheme of the runs (batches created from fused runs tune over both)")
+    ap.add_argument("--scheme", default="auto", choices=["auto", "split", "fused"],
+                    help="time-step scheme of the runs: auto = single runs take the library default and "
+                         "awp_tune, batches take two-stage runs; fused batches tune over both schemes")
     ap.add_argument("--compare-fp32", action="store_true",
                     help="with --storage mixed: also run the FP32 workload and report the ratio")
     ap.add_argument("--budget-gb", type=float, default=0.0,
@@ -325,11 +336,17 @@ def main():
               "and this build has no multi-GPU path (--gpus %d requested)" % a.gpus, file=sys.stderr)
         sys.exit(2)
 
-    if a.scheme == "fused":
-        os.environ["AWP_SCHEME"] = "fused"
+    batched = a.config in ("c3", "c5")
+    if a.scheme == "auto":
+        # batches: two-stage runs (the fused step's second wavefield buffer
+        # would admit fewer runs per wave)
+        if batched:
+            os.environ["AWP_SCHEME"] = "split"
+        else:
+            os.environ.pop("AWP_SCHEME", None)
     else:
-        os.environ.pop("AWP_SCHEME", None)
-    bench_fn = bench_batched if a.config in ("c3", "c5") else bench_single
+        os.environ["AWP_SCHEME"] = a.scheme
+    bench_fn = bench_batched if batched else bench_single
     value, ms_step, roof, config = bench_fn(a, a.storage)
     if a.compare_fp32 and a.storage == "mixed":
         v32, ms32, _, _ = bench_fn(a, "fp32")

@@ -96,7 +96,7 @@ EXPORTS = [
     "awp_write_sgt_file_rwg", "awp_device_memory", "awp_estimate_bytes", "awp_batch_create", "awp_batch_tune", "awp_batch_step",
     "awp_batch_synchronize", "awp_batch_time", "awp_batch_get_launch", "awp_batch_get_scheme",
     "awp_batch_last_error", "awp_batch_destroy", "awp_write_sgt_head", "awp_checkpoint_save",
-    "awp_checkpoint_load",
+    "awp_checkpoint_load", "awp_tune", "awp_set_scheme", "awp_get_scheme",
 ]
 
 
@@ -146,6 +146,10 @@ def lib() -> ctypes.CDLL:
         "awp_write_sgt_head": ([ctypes.POINTER(HeadParams), cp, sz], ip),
         "awp_checkpoint_save": ([vp, cp], ip),
         "awp_checkpoint_load": ([vp, cp], ip),
+        "awp_tune": ([vp, ip], ip),
+        "awp_set_scheme": ([vp, ip], ip),
+        "awp_get_scheme": ([vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                            ctypes.POINTER(ctypes.c_int)], ip),
         "awp_batch_tune": ([vp, ip], ip),
         "awp_batch_step": ([vp, ip], ip),
         "awp_batch_synchronize": ([vp], ip),
@@ -334,6 +338,22 @@ class Run:
     def checkpoint_load(self, path: str) -> None:
         """Restore a checkpoint into this initialised run of the same set-up."""
         self._do(lib().awp_checkpoint_load(self.h, path.encode()), "awp_checkpoint_load")
+
+    def tune(self, reps: int = 3) -> Tuple[bool, int, int]:
+        """Time the candidate schemes and launch shapes, keep the fastest and
+        start the run again (before its first step); returns scheme()."""
+        self._do(lib().awp_tune(self.h, reps), "awp_tune")
+        return self.scheme()
+
+    def set_scheme(self, fused: bool) -> None:
+        """Switch between the fused and the two-stage step (between steps)."""
+        self._do(lib().awp_set_scheme(self.h, 1 if fused else 0), "awp_set_scheme")
+
+    def scheme(self) -> Tuple[bool, int, int]:
+        """(fused step in use, tile rows, z chunks)."""
+        f, t, n = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        self._do(lib().awp_get_scheme(self.h, ctypes.byref(f), ctypes.byref(t), ctypes.byref(n)), "awp_get_scheme")
+        return bool(f.value), t.value, n.value
 
     def field(self, name: str, kext: int = 0) -> np.ndarray:
         out = np.empty((self.p.nz + kext, self.p.ny, self.p.nx), dtype=np.float32)

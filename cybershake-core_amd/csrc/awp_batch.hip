@@ -263,9 +263,14 @@ extern "C" int awp_batch_create(awp_batch **out, awp_ctx *const *runs, int nruns
     b->runs.assign(runs, runs + nruns);
     b->device = runs[0]->device;
     b->mixed = runs[0]->mixed();
+    // the fused scheme when every run holds both buffers and steps fused
     b->can_fuse = true;
-    for (int r = 0; r < nruns; ++r) b->can_fuse = b->can_fuse && runs[r]->fused;
-    b->split = !b->can_fuse;
+    bool all_fused = true;
+    for (int r = 0; r < nruns; ++r) {
+        b->can_fuse = b->can_fuse && runs[r]->two_bufs;
+        all_fused = all_fused && runs[r]->fused;
+    }
+    b->split = !(b->can_fuse && all_fused);
     b->persist = true;
     b->o32 = true;
     for (int r = 0; r < nruns; ++r) b->o32 = b->o32 && runs[r]->l.o32;
@@ -283,7 +288,9 @@ extern "C" int awp_batch_create(awp_batch **out, awp_ctx *const *runs, int nruns
             b->l.tile_xw = 1;
         }
     } else {
-        b->l.tile_y = (b->l.tile_y == 4 || b->l.tile_y == 7 || b->l.tile_y == 8) ? b->l.tile_y : 8;
+        b->l.tile_y = (b->l.tile_y == 4 || b->l.tile_y == 7 || b->l.tile_y == 8 || b->l.tile_y == 11) ? b->l.tile_y
+                                                                                                     : 8;
+        if (b->l.tile_y == 8 || b->l.tile_y == 11) b->l.unroll = 0;
         b->l.tile_xw = 1;
     }
     b->nchunk.assign(nruns, 1);
@@ -444,10 +451,10 @@ extern "C" int awp_batch_tune(awp_batch *b, int reps)
                     cands.push_back({true, ps, 8, 1, kc, 2, 1, pt});
                 }
             }
-        // fused: tile rows x prefetch depth (64x8 tiles only without
-        // prefetch: their 13-wave blocks have 128 registers per lane) x z
-        // chunks
-        const int tp[6][2] = {{8, 0}, {7, 0}, {7, 1}, {7, 2}, {4, 0}, {4, 2}};
+        // fused: tile rows x prefetch depth (64x8 and 64x11 tiles only
+        // without prefetch: their 13- and 16-wave blocks have 128 registers
+        // per lane) x z chunks
+        const int tp[7][2] = {{11, 0}, {8, 0}, {7, 0}, {7, 1}, {7, 2}, {4, 0}, {4, 2}};
         if (b->can_fuse)
             for (const auto &v : tp)
                 for (int nc : {2, 4}) cands.push_back({false, ps, v[0], v[1], nc, 1, 0, true});

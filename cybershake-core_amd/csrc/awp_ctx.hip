@@ -29,13 +29,38 @@ extern "C" const char *awp_last_error(const awp_ctx *ctx) { return ctx ? ctx->er
 // 32-bit byte offsets where every array is below 4 GiB
 static bool use_o32(const Geo &g) { return offsets32(g); }
 
-// The default is the two-stage step (velocity kernel, then stress kernel);
-// AWP_SCHEME=fused selects the fused one-sweep step, which addresses every
-// array with 32-bit byte offsets.
-static bool want_fused(const Geo &g)
+// Time-step scheme of a new run.  AWP_SCHEME=split: the two-stage step
+// (velocity kernel, then stress kernel, in place); AWP_SCHEME=fused: the
+// one-sweep fused step (a second wavefield buffer; every array addressed
+// with 32-bit byte offsets).  Unset: fused for FP32 storage, two-stage for
+// mixed storage (1500x1000x300: fused FP32 16.2-16.3 ms per step on 64 x 11
+// tiles against 17.35 two-stage; fused mixed 23.6 against 15.7).  awp_tune
+// and awp_set_scheme switch a run that holds both buffers between the two.
+static bool want_fused(const Geo &g, int storage)
 {
+    if (!use_o32(g)) return false;
     const char *scheme = getenv("AWP_SCHEME");
-    return scheme && !strcmp(scheme, "fused") && use_o32(g);
+    if (scheme && !strcmp(scheme, "fused")) return true;
+    if (scheme && !strcmp(scheme, "split")) return false;
+    return storage == AWP_STORAGE_FP32;
+}
+
+// z chunks of `nchunk` requested: the top chunk keeps >= 3 levels (the
+// block reaching k = nz writes the free-surface stress images from the two
+// levels below it)
+static void set_chunks(Launch &l, const Geo &g, int nchunk)
+{
+    nchunk = std::max(1, nchunk);
+    for (;;) {
+        const int kc = (g.nz + nchunk - 1) / nchunk;
+        const int last = g.nz - (nchunk - 1) * kc;
+        if (nchunk == 1 || (last >= 3 && (nchunk - 1) * kc < g.nz)) {
+            l.kc = kc;
+            l.nchunk = (g.nz + kc - 1) / kc;
+            return;
+        }
+        nchunk--;
+    }
 }
 
 static Launch choose_launch(const Geo &g, bool fused, bool mixed, int device)
@@ -47,15 +72,19 @@ static Launch choose_launch(const Geo &g, bool fused, bool mixed, int device)
     const char *un = getenv("AWP_UNROLL");
     if (un && atoi(un) == 2) l.unroll = 2;
     // tuning overrides: AWP_TILE_Y (rows per block: 1, 2, 4, 8 or 16 for
-    // the two-stage kernels, 4, 7 or 8 for the fused one) and AWP_KC (z
+    // the two-stage kernels, 4, 7, 8 or 11 for the fused one) and AWP_KC (z
     // levels per block)
     const char *ty = getenv("AWP_TILE_Y");
-    // fused defaults measured at 1500x1000x300: FP32 64x8 tiles, mixed 64x7
-    // tiles, both without load prefetch (awp_batch_tune searches the rest)
-    if (fused) l.tile_y = mixed ? 7 : 8;
-    if (ty && (atoi(ty) == 1 || atoi(ty) == 2 || atoi(ty) == 4 || atoi(ty) == 7 || atoi(ty) == 8 || atoi(ty) == 16))
+    // fused defaults measured at 1500x1000x300: FP32 64x11 tiles (16-wave
+    // blocks fill a CU's wave slots: 16.2-16.3 ms per step against 17.7-18.1
+    // for 64x8), mixed 64x7 tiles, both without load prefetch (awp_tune and
+    // awp_batch_tune search the rest)
+    if (fused) l.tile_y = mixed ? 7 : 11;
+    if (ty && (atoi(ty) == 1 || atoi(ty) == 2 || atoi(ty) == 4 || atoi(ty) == 7 || atoi(ty) == 8 || atoi(ty) == 11 ||
+               atoi(ty) == 16))
         l.tile_y = atoi(ty);
-    if (fused && l.tile_y != 4 && l.tile_y != 7) l.tile_y = 8;
+    if (fused && l.tile_y != 4 && l.tile_y != 7 && l.tile_y != 11) l.tile_y = 8;
+    if (!fused && l.tile_y == 11) l.tile_y = 8;
     if (!fused && l.tile_y == 7) l.tile_y = 8;
     // tile patches of the fused step (AWP_PATCH=<w>x<h> overrides): 4 x 4
     // tiles of 64 x 8 points (1500x1000x300 FP32, 4 z chunks: 17.21 ms per
@@ -124,17 +153,24 @@ static Launch choose_launch(const Geo &g, bool fused, bool mixed, int device)
         // the fused step keeps its load prefetch depth (0, 1, 2) in `unroll`
         const char *pf = getenv("AWP_PF");
         l.unroll = (pf && atoi(pf) >= 0 && atoi(pf) <= 2) ? atoi(pf) : 0;
-        if (l.tile_y == 8) l.unroll = 0;
+        if (l.tile_y == 8 || l.tile_y == 11) l.unroll = 0;
     }
     const long long bxy = (long long)((g.nx + TX - 1) / TX) * ((g.ny + l.tile_y - 1) / l.tile_y);
     int nchunk;
-    if (fused) {
-        // an even number of chunks of about 75 levels: at 1500x1000x300 the
-        // fused step took 17.2-17.7 ms with 2 or 4 chunks and 19.0-19.6 ms
-        // with 3 or 5 (each chunk recomputes 4 velocity levels)
-        nchunk = 2 * std::max(1, (g.nz + 75) / 150);
+    if (fused && l.tile_y == 11) {
+        // chunks of about 100 levels: at 1500x1000x300 with 4 x 4 patches,
+        // 3 chunks ran 16.21-16.23 ms per step on two boxes; 4 chunks
+        // 16.34 on one and 17.71 on the other, 2 chunks 17.92 (each chunk
+        // recomputes 4 velocity levels)
+        nchunk = std::max(1, (g.nz + 50) / 100);
         nchunk = std::min(nchunk, std::max(1, g.nz / 16));
         (void)bxy;
+    } else if (fused) {
+        // 64 x 8, 64 x 7, 64 x 4 tiles: an even number of chunks of about
+        // 75 levels (1500x1000x300, 64 x 8: 17.2-17.7 ms per step with 2 or
+        // 4 chunks, 19.0-19.6 ms with 3 or 5)
+        nchunk = 2 * std::max(1, (g.nz + 75) / 150);
+        nchunk = std::min(nchunk, std::max(1, g.nz / 16));
     } else {
         // z chunks of about SPLIT_KC levels: with the tile patches, the
         // tiles an XCD holds at once then span few levels and share their
@@ -145,18 +181,7 @@ static Launch choose_launch(const Geo &g, bool fused, bool mixed, int device)
     }
     const char *kc_env = getenv("AWP_KC");
     if (kc_env && atoi(kc_env) >= 3) nchunk = std::max(1, (g.nz + atoi(kc_env) - 1) / atoi(kc_env));
-    for (;;) {
-        int kc = (g.nz + nchunk - 1) / nchunk;
-        int last = g.nz - (nchunk - 1) * kc;
-        // the block reaching k = nz writes the free-surface stress images
-        // from the two levels below it, so the top chunk holds >= 3 levels
-        if (nchunk == 1 || (last >= 3 && (nchunk - 1) * kc < g.nz)) {
-            l.kc = kc;
-            l.nchunk = (g.nz + kc - 1) / kc;
-            break;
-        }
-        nchunk--;
-    }
+    set_chunks(l, g, nchunk);
     return l;
 }
 
@@ -184,7 +209,8 @@ extern "C" int awp_create(awp_ctx **out, const awp_in3d *p, int device, int stor
     ctx->device = device;
     ctx->storage = storage;
     ctx->g = make_geo(p->nx, p->ny, p->nz);
-    ctx->fused = want_fused(ctx->g);
+    ctx->fused = want_fused(ctx->g, storage);
+    ctx->two_bufs = ctx->fused;
     ctx->l = choose_launch(ctx->g, ctx->fused, storage == AWP_STORAGE_MIXED, device);
     ctx->vol = nullptr;
     ctx->prof = nullptr;
@@ -219,7 +245,7 @@ extern "C" int awp_create(awp_ctx **out, const awp_in3d *p, int device, int stor
     // qp qs (4 or 2 bytes)
     const size_t nv = (size_t)ctx->g.nvol;
     const size_t esz = (storage == AWP_STORAGE_MIXED) ? 2 : 4;
-    const int nbuf = ctx->fused ? 2 : 1;
+    const int nbuf = ctx->two_bufs ? 2 : 1;
     // a 2-byte array gets 128 bytes of slack and starts 64 bytes in, which
     // puts its interior rows on 128-byte lines (the pitch is a multiple of 64)
     const size_t eslot = nv * esz + (esz == 2 ? 128 : 0), eoff = (esz == 2 ? 64 : 0);
@@ -577,6 +603,35 @@ extern "C" int awp_set_sgt_points(awp_ctx *ctx, int npts, const int *xyz, int *n
 // ---------------------------------------------------------------------------
 // run
 // ---------------------------------------------------------------------------
+// Uploads the two-stage step's one-entry job table for the run's current
+// launch shape and wavefield buffer (after the stream has drained: its
+// kernels read the table).
+static int upload_sjob(awp_ctx *ctx)
+{
+    HIPTRY(ctx, hipStreamSynchronize(ctx->stream));
+    if (!ctx->d_sjob) HIPTRY(ctx, hipMalloc((void **)&ctx->d_sjob, sizeof(SJob)));
+    const SJob q = split_job(ctx->g, ctx->view(ctx->cur), ctx->c, ctx->l, 0, 0);
+    ctx->split_total = split_tiles(q);
+    ctx->split_fs = split_fs_blocks(q);
+    HIPTRY(ctx, hipMemcpy(ctx->d_sjob, &q, sizeof(SJob), hipMemcpyHostToDevice));
+    return AWP_OK;
+}
+
+// After ctx->fused or ctx->l changed: the two-stage step works in buffer 0,
+// so a state the fused step left in buffer 1 moves there first (the whole
+// buffer: every value the next step reads), and its job table follows the
+// launch shape.
+static int scheme_changed(awp_ctx *ctx)
+{
+    if (!ctx->fused && ctx->cur == 1) {
+        HIPTRY(ctx, hipMemcpyAsync(ctx->wf[0][0], ctx->wf[1][0], (size_t)ctx->g.nvol * 9 * sizeof(float),
+                                   hipMemcpyDeviceToDevice, ctx->stream));
+        ctx->cur = 0;
+    }
+    if (!ctx->fused && ctx->initialised) return upload_sjob(ctx);
+    return AWP_OK;
+}
+
 extern "C" int awp_init(awp_ctx *ctx)
 {
     if (!ctx) return AWP_EINVAL;
@@ -593,15 +648,12 @@ extern "C" int awp_init(awp_ctx *ctx)
         ctx->sg_done = true;
     }
     // the two-stage step's one-run job table (state, coefficients, tiles)
+    ctx->cur = 0;
     if (!ctx->fused) {
-        if (!ctx->d_sjob) HIPTRY(ctx, hipMalloc((void **)&ctx->d_sjob, sizeof(SJob)));
-        const SJob q = split_job(ctx->g, ctx->view(0), ctx->c, ctx->l, 0, 0);
-        ctx->split_total = split_tiles(q);
-        ctx->split_fs = split_fs_blocks(q);
-        HIPTRY(ctx, hipMemcpy(ctx->d_sjob, &q, sizeof(SJob), hipMemcpyHostToDevice));
+        int rc = upload_sjob(ctx);
+        if (rc) return rc;
     }
     // first source sample before the loop (pmcl3d.f:988)
-    ctx->cur = 0;
     HIPTRY(ctx, launch_source(ctx->g, ctx->view(0), ctx->p.igreen, ctx->npsrc, ctx->d_psrc, ctx->d_ptop,
                               ctx->d_stf, ctx->p.nst, 1, ctx->vtst, ctx->vtst1, ctx->stream));
     HIPTRY(ctx, hipStreamSynchronize(ctx->stream));
@@ -649,8 +701,8 @@ extern "C" int awp_step(awp_ctx *ctx, int nsteps)
             sn = sb;
         } else {
             // the stage kernels over this run's one-entry job table, one
-            // block per tile
-            sn = ctx->view(0);
+            // block per tile (in buffer 0)
+            sn = ctx->view(ctx->cur);
             const int all = 1 << 30;
             HIPTRY(ctx, launch_split_velocity(ctx->l, ctx->l.o32, ctx->d_sjob, 1, ctx->split_total, all, st));
             if (e) HIPTRY(ctx, hipEventRecord(e[1], st));
@@ -685,6 +737,120 @@ extern "C" int awp_step(awp_ctx *ctx, int nsteps)
         ctx->t_all = ta;
     }
     return AWP_OK;
+}
+
+extern "C" int awp_set_scheme(awp_ctx *ctx, int fused)
+{
+    if (!ctx) return AWP_EINVAL;
+    if (fused && !ctx->two_bufs)
+        return fail(ctx, AWP_EINVAL, "the fused step needs the second wavefield buffer (a run created with the "
+                                     "fused scheme)");
+    HIPTRY(ctx, hipSetDevice(ctx->device));
+    HIPTRY(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->fused = fused != 0;
+    ctx->l = choose_launch(ctx->g, ctx->fused, ctx->mixed(), ctx->device);
+    return scheme_changed(ctx);
+}
+
+extern "C" int awp_get_scheme(const awp_ctx *ctx, int *fused, int *tile_y, int *nchunk)
+{
+    if (!ctx) return AWP_EINVAL;
+    if (fused) *fused = ctx->fused ? 1 : 0;
+    if (tile_y) *tile_y = ctx->l.tile_y;
+    if (nchunk) *nchunk = ctx->l.nchunk;
+    return AWP_OK;
+}
+
+namespace {
+
+// `n` steps of the step kernels alone (no source, no SGT sample), for timing
+int trial_steps(awp_ctx *ctx, int n)
+{
+    const bool mx = ctx->mixed();
+    const int all = 1 << 30;
+    for (int k = 0; k < n; ++k) {
+        if (ctx->fused) {
+            HIPTRY(ctx, launch_step_fused(mx, ctx->g, ctx->view(ctx->cur), ctx->view(ctx->cur ^ 1), ctx->c, ctx->l,
+                                          ctx->stream));
+            ctx->cur ^= 1;
+        } else {
+            HIPTRY(ctx, launch_split_velocity(ctx->l, ctx->l.o32, ctx->d_sjob, 1, ctx->split_total, all, ctx->stream));
+            HIPTRY(ctx, launch_split_fs(ctx->d_sjob, 1, ctx->split_fs, ctx->stream));
+            HIPTRY(ctx, launch_split_stress(mx, ctx->l, ctx->l.o32, ctx->d_sjob, 1, ctx->split_total, all,
+                                            ctx->stream));
+        }
+    }
+    return AWP_OK;
+}
+
+}  // namespace
+
+extern "C" int awp_tune(awp_ctx *ctx, int reps)
+{
+    if (!ctx) return AWP_EINVAL;
+    if (!ctx->initialised || ctx->step != 0)
+        return fail(ctx, AWP_ESTATE, "awp_tune needs an initialised run that has not stepped");
+    if (reps < 1) reps = 3;
+    HIPTRY(ctx, hipSetDevice(ctx->device));
+    int rc = ensure_events(ctx, 2);
+    if (rc) return rc;
+    // candidates: the two-stage step with its defaults; with both buffers,
+    // the fused step on 64 x 11 tiles (FP32; z chunks of about 100, 75, 60
+    // levels x tile patches 4x4, 2x2, 8x2: which of them runs fastest
+    // varies between boxes at 1500x1000x300) and 64 x 8 tiles, or on 64 x 7
+    // tiles (mixed)
+    struct Cand {
+        bool fused;
+        int ty, kc, pw, ph;
+    };
+    std::vector<Cand> cands;
+    cands.push_back({false, 0, 0, 0, 0});
+    if (ctx->two_bufs) {
+        if (!ctx->mixed()) {
+            const int pat[3][2] = {{4, 4}, {2, 2}, {8, 2}};
+            for (int kc : {100, 75, 60})
+                for (const auto &pp : pat) cands.push_back({true, 11, kc, pp[0], pp[1]});
+            cands.push_back({true, 8, 75, 4, 4});
+        } else {
+            cands.push_back({true, 7, 75, 4, 4});
+        }
+    }
+    auto apply = [ctx](const Cand &cd) -> int {
+        HIPTRY(ctx, hipStreamSynchronize(ctx->stream));
+        ctx->fused = cd.fused;
+        Launch l = choose_launch(ctx->g, cd.fused, ctx->mixed(), ctx->device);
+        if (cd.fused) {
+            l.tile_y = cd.ty;
+            l.unroll = 0;
+            l.pw = cd.pw;
+            l.ph = cd.ph;
+            set_chunks(l, ctx->g, std::min((ctx->g.nz + cd.kc / 2) / cd.kc, std::max(1, ctx->g.nz / 16)));
+        }
+        ctx->l = l;
+        return scheme_changed(ctx);
+    };
+    float best = 1e30f;
+    size_t pick = 0;
+    for (size_t n = 0; n < cands.size(); ++n) {
+        rc = apply(cands[n]);
+        if (!rc) rc = trial_steps(ctx, 1);   // warm-up
+        if (rc) return rc;
+        HIPTRY(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
+        rc = trial_steps(ctx, reps);
+        if (rc) return rc;
+        HIPTRY(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
+        HIPTRY(ctx, hipEventSynchronize(ctx->ev[1]));
+        float ms = 0;
+        HIPTRY(ctx, hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
+        if (ms < best) {
+            best = ms;
+            pick = n;
+        }
+    }
+    rc = apply(cands[pick]);
+    if (rc) return rc;
+    // the trial steps changed the state: start the run again
+    return awp_init(ctx);
 }
 
 extern "C" int awp_synchronize(awp_ctx *ctx)
@@ -723,7 +889,7 @@ extern "C" int awp_estimate_bytes(const awp_in3d *p, int storage, int npts, size
 {
     if (!p || (storage != AWP_STORAGE_FP32 && storage != AWP_STORAGE_MIXED) || npts < 0) return AWP_EINVAL;
     const Geo g = make_geo(p->nx, p->ny, p->nz);
-    const bool fused = want_fused(g);
+    const bool fused = want_fused(g, storage);
     const size_t nv = (size_t)g.nvol;
     const size_t esz = storage == AWP_STORAGE_MIXED ? 2 : 4;
     const size_t eslot = nv * esz + (esz == 2 ? 128 : 0);
@@ -989,7 +1155,7 @@ CkpHead ckp_head_of(const awp_ctx *ctx)
     h.ny = ctx->p.ny;
     h.nz = ctx->p.nz;
     h.storage = ctx->storage;
-    h.fused = ctx->fused ? 1 : 0;
+    h.fused = ctx->two_bufs ? 1 : 0;   // the buffer layout (the step in use may differ)
     h.cur = ctx->cur;
     h.step = ctx->step;
     h.nsamp = ctx->nsamp;
@@ -1065,9 +1231,10 @@ extern "C" int awp_checkpoint_load(awp_ctx *ctx, const char *path)
     } else if (h.nx != want.nx || h.ny != want.ny || h.nz != want.nz || h.storage != want.storage ||
                h.fused != want.fused || h.npts != want.npts || h.nsamp_max != want.nsamp_max || h.nt != want.nt ||
                h.state_bytes != want.state_bytes || h.sgt_bytes != want.sgt_bytes) {
-        rc = fail(ctx, AWP_EINVAL, "checkpoint '%s' is of another run (grid, storage, scheme, points or steps differ)",
-                  path);
-    } else if (h.step < 0 || h.step > h.nt || h.nsamp < 0 || h.nsamp > h.nsamp_max || (h.cur != 0 && h.cur != 1)) {
+        rc = fail(ctx, AWP_EINVAL,
+                  "checkpoint '%s' is of another run (grid, storage, wavefield buffers, points or steps differ)", path);
+    } else if (h.step < 0 || h.step > h.nt || h.nsamp < 0 || h.nsamp > h.nsamp_max || (h.cur != 0 && h.cur != 1) ||
+               (h.cur == 1 && !ctx->two_bufs)) {
         rc = fail(ctx, AWP_EINVAL, "checkpoint '%s' has an invalid step count", path);
     }
     if (!rc) {
@@ -1080,5 +1247,5 @@ extern "C" int awp_checkpoint_load(awp_ctx *ctx, const char *path)
     ctx->step = h.step;
     ctx->nsamp = h.nsamp;
     ctx->cur = h.cur;
-    return AWP_OK;
+    return scheme_changed(ctx);
 }

@@ -34,7 +34,8 @@ struct awp_ctx {
     // wf[cur] and writes wf[cur ^ 1]; the two-stage step uses wf[0] only
     float *wf[2][9];
     int cur;
-    bool fused;
+    bool fused;              // the step in use: fused one-sweep (true) or two-stage
+    bool two_bufs;           // the allocation holds wf[1] (the fused step can run)
     // the two-stage step's job table (one entry: this run), uploaded by
     // awp_init, and its tile and free-surface block counts
     awp::SJob *d_sjob;

@@ -567,9 +567,11 @@ static int grid_t(int device)
 }
 
 // kernel variant of (storage, tile rows, prefetch depth): CALL is run with
-// P, TY, PF of the matching template instance.  64 x 8 tiles come without
-// prefetch only: their 13-wave blocks leave 128 registers a lane, and the
-// prefetching variants spill there.
+// P, TY, PF of the matching template instance.  64 x 8 and 64 x 11 tiles
+// come without prefetch only: their 13- and 16-wave blocks leave 128
+// registers a lane, and the prefetching variants spill there.  64 x 11
+// tiles fill a CU's 16 wave slots (4 per SIMD at up to 128 registers) with
+// one block, where a 13-wave block leaves 3 of them empty.
 #define AWP_FUSED_DISPATCH(mixed, l, CALL)                                           \
     do {                                                                             \
         const int pf_ = (l).unroll;                                                  \
@@ -583,6 +585,8 @@ static int grid_t(int device)
                 if (pf_ == 0) { constexpr int TY = 4, PF = 0; CALL; }                \
                 else if (pf_ == 2) { constexpr int TY = 4, PF = 2; CALL; }           \
                 else { constexpr int TY = 4, PF = 1; CALL; }                         \
+            } else if ((l).tile_y == 11) {                                           \
+                constexpr int TY = 11, PF = 0; CALL;                                 \
             } else {                                                                 \
                 constexpr int TY = 8, PF = 0; CALL;                                  \
             }                                                                        \
@@ -596,6 +600,8 @@ static int grid_t(int device)
                 if (pf_ == 0) { constexpr int TY = 4, PF = 0; CALL; }                \
                 else if (pf_ == 2) { constexpr int TY = 4, PF = 2; CALL; }           \
                 else { constexpr int TY = 4, PF = 1; CALL; }                         \
+            } else if ((l).tile_y == 11) {                                           \
+                constexpr int TY = 11, PF = 0; CALL;                                 \
             } else {                                                                 \
                 constexpr int TY = 8, PF = 0; CALL;                                  \
             }                                                                        \

@@ -25,7 +25,11 @@
  * with its -z scaling for a Z-force run, igreen 3), in one pass.  In batch
  * form the two paths get ".<n>" appended (n = position in the list);
  * AWP_HBM_BUDGET_GB caps the device memory a wave may use (default: the free
- * memory less 2 GiB) and AWP_TUNE=1 runs the batch tuner first.
+ * memory less 2 GiB) and AWP_TUNE=1 runs the batch tuner first.  A single
+ * run picks its fastest time-step scheme and launch shape first (awp_tune;
+ * AWP_TUNE=0 skips that); batch runs take the two-stage step unless
+ * AWP_SCHEME says otherwise (the fused step's second wavefield buffer would
+ * admit fewer runs per wave).
  */
 #define _POSIX_C_SOURCE 200809L
 #include <errno.h>
@@ -268,6 +272,7 @@ int main(int argc, char **argv)
     }
     if (argc >= 2 && !strcmp(argv[1], "--batch")) {
         if (argc < 3) return die("--batch", "no IN3D files given");
+        setenv("AWP_SCHEME", "split", 0);
         return run_batch(argc - 2, argv + 2, device, storage);
     }
 
@@ -290,6 +295,13 @@ int main(int argc, char **argv)
         return die("cordfile", err);  /* no station list: SGT mode needs one */
     printf(" Use SGT output mode, sgt_numsta= %d\n", j.npts);
     if (setup_run(&j, device, storage, err, sizeof(err))) return die("set-up", err);
+    const char *tune_env = getenv("AWP_TUNE");
+    if (!(tune_env && !strcmp(tune_env, "0"))) {
+        int fused = 0, ty = 0, nc = 0;
+        if (awp_tune(j.ctx, 3) || awp_get_scheme(j.ctx, &fused, &ty, &nc))
+            return die("tune", awp_last_error(j.ctx));
+        printf(" time step: %s, 64 x %d tiles, %d z chunks\n", fused ? "fused one-sweep" : "two-stage", ty, nc);
+    }
     double t2 = now_s();
 
     /* CHECKPOINT > 0: resume from output_ckp/awp_ckp.bin when it exists and

@@ -118,11 +118,13 @@ int awp_sgt_file_geometry(const awp_in3d *p, int *nt_out, int *nflush);
 
 /* Create a context on HIP device `device` (allocates every device buffer;
  * replaces the host's cudaMalloc block, pmcl3d.c:537-823, and
- * SetDeviceConstValue, kernel...cu:24).  The time step is the two-stage one
- * (velocity kernel, free-surface images, stress kernel, in place); with
- * AWP_SCHEME=fused in the environment it is the one-sweep fused kernel,
- * which holds a second wavefield buffer (36 more bytes per grid point).
- * Both give the same bits. */
+ * SetDeviceConstValue, kernel...cu:24).  Two time-step schemes give the same
+ * bits: the two-stage step (velocity kernel, free-surface images, stress
+ * kernel, in place) and the one-sweep fused step, which reads one wavefield
+ * buffer and writes a second one (36 more bytes per grid point).  FP32 runs
+ * take the fused step, mixed-storage runs the two-stage one; AWP_SCHEME=split
+ * or AWP_SCHEME=fused in the environment overrides (grids whose arrays reach
+ * 4 GiB always take the two-stage step). */
 int awp_create(awp_ctx **out, const awp_in3d *p, int device, int storage,
                char *err, size_t errlen);
 int awp_destroy(awp_ctx *ctx);
@@ -153,6 +155,21 @@ int awp_set_sgt_points(awp_ctx *ctx, int npts, const int *xyz, int *nkept);
 /* Zero the wavefield and memory variables, inject source sample 1
  * (pmcl3d.f:988) and reset the step counter. */
 int awp_init(awp_ctx *ctx);
+
+/* Time-step scheme of a run; every scheme and launch shape gives the same
+ * bits.  awp_tune times `reps` steps (default 3) of each candidate -- the
+ * two-stage step and, in a run that holds the second wavefield buffer, the
+ * fused step on several tile, z-chunk and tile-patch shapes -- keeps the
+ * fastest and starts the run again (awp_init): call it on an initialised run
+ * before its first step.  awp_set_scheme(ctx, 1 / 0) switches to the fused /
+ * two-stage step with its default launch shape at any point between steps
+ * (the fused step needs the second buffer).  awp_get_scheme reports the step
+ * in use (1 fused, 0 two-stage), its tile rows and its z chunks; any pointer
+ * may be NULL.  No counterpart in the reference, which has one kernel
+ * sequence (AWP-GPU-SGT/src/pmcl3d.c:841-917). */
+int awp_tune(awp_ctx *ctx, int reps);
+int awp_set_scheme(awp_ctx *ctx, int fused);
+int awp_get_scheme(const awp_ctx *ctx, int *fused, int *tile_y, int *nchunk);
 
 /* Advance nsteps full time steps (pmcl3d.f:1183-1557: velocity, sponge,
  * free-surface velocity, stress + memory variables, sponge, source sample
@@ -242,7 +259,8 @@ int awp_device_memory(int device, size_t *free_bytes, size_t *total_bytes);
  * memory variables in their storage format, the SGT samples so far, the
  * step count -- to `path` (through path.tmp, renamed when complete);
  * awp_checkpoint_load restores it into an initialised context of the same
- * grid, storage mode, scheme and SGT points (media and source are the
+ * grid, storage mode, wavefield buffers (one or two; the step in use may
+ * differ) and SGT points (media and source are the
  * caller's inputs as at the start), after which awp_step continues bit for
  * bit as the uninterrupted run (a load that fails after it has started
  * copying leaves the run uninitialised).  Replaces the reference's per-rank restart

@@ -42,12 +42,13 @@ def _single(d, storage, nsteps=None):
     return r
 
 
-@pytest.mark.parametrize("scheme", ["split", "fused", "split_ty16", "split_xw2", "split_o32v"])
+@pytest.mark.parametrize("scheme", ["split", "fused", "split_ty16", "split_xw2", "split_o32v", "fused_ty11"])
 @pytest.mark.parametrize("storage", [awpsgt.AWP_STORAGE_FP32, awpsgt.AWP_STORAGE_MIXED])
 def test_batch_equals_single_runs(case_dirs, storage, scheme, monkeypatch):
     dirs = [case_dirs[n][0] for n in SAME_NT]
-    if scheme == "fused":
-        monkeypatch.setenv("AWP_SCHEME", "fused")
+    monkeypatch.setenv("AWP_SCHEME", "fused" if scheme.startswith("fused") else "split")
+    if scheme == "fused_ty11":
+        monkeypatch.setenv("AWP_TILE_Y", "11")
     if scheme == "split_ty16":
         monkeypatch.setenv("AWP_TILE_Y", "16")
     if scheme == "split_xw2":
@@ -59,9 +60,11 @@ def test_batch_equals_single_runs(case_dirs, storage, scheme, monkeypatch):
     nt = runs[0].p.nt
     assert all(r.p.nt == nt for r in runs)
     with awpsgt.Batch(runs) as b:
-        assert b.fused() == (scheme == "fused")
+        assert b.fused() == scheme.startswith("fused")
         if scheme == "split_ty16":
             assert b.launch()[0] == 16
+        if scheme == "fused_ty11":
+            assert b.launch()[0] == 11
         # uneven calls: graph pairs, odd remainders, a call past the end
         for n in (1, 36, 7, 40, 1000):
             b.step(n)
@@ -96,8 +99,7 @@ def test_batch_uneven_grids_and_tuner(case_dirs, tmp_path, scheme, monkeypatch):
         d = tempfile.mkdtemp()
         cases.write_case(d, spec, media, pts)
         dirs.append(d)
-    if scheme == "fused":
-        monkeypatch.setenv("AWP_SCHEME", "fused")
+    monkeypatch.setenv("AWP_SCHEME", scheme)
     runs = [_make(d, awpsgt.AWP_STORAGE_FP32) for d in dirs]
     monkeypatch.delenv("AWP_SCHEME", raising=False)
     nt = min(r.p.nt for r in runs)
@@ -105,7 +107,7 @@ def test_batch_uneven_grids_and_tuner(case_dirs, tmp_path, scheme, monkeypatch):
         ty, pf, nc = b.tune(1)
         if b.fused():
             assert scheme == "fused"
-            assert ty in (4, 7, 8) and pf in (0, 1, 2) and nc >= 1
+            assert ty in (4, 7, 8, 11) and pf in (0, 1, 2) and nc >= 1
         else:
             assert ty in (4, 8, 16) and pf in (1, 2) and nc >= 1
         assert all(r.progress() == (0, 0) for r in runs)

@@ -27,8 +27,7 @@ CASE = "g7_random_edges"   # 133 steps, points on every edge
 @pytest.mark.parametrize("storage", [awpsgt.AWP_STORAGE_FP32, awpsgt.AWP_STORAGE_MIXED])
 def test_restart_continues_bit_for_bit(case_dirs, storage, scheme, tmp_path, monkeypatch):
     d = case_dirs[CASE][0]
-    if scheme == "fused":
-        monkeypatch.setenv("AWP_SCHEME", "fused")
+    monkeypatch.setenv("AWP_SCHEME", scheme)
     ref = _single(d, storage)            # uninterrupted
     a = _make(d, storage)
     a.step(37)                           # odd: the fused scheme's buffer parity is 1
