@@ -231,6 +231,8 @@ extern "C" int awp_create(awp_ctx **out, const awp_in3d *p, int device, int stor
     ctx->t_vel = ctx->t_str = ctx->t_all = 0;
     ctx->bytes = 0;
     ctx->vol_bytes = ctx->state_bytes = 0;
+    ctx->vol_b = nullptr;
+    ctx->vol_b_bytes = 0;
     ctx->sg_done = false;
 
     e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
@@ -239,13 +241,16 @@ extern "C" int awp_create(awp_ctx **out, const awp_in3d *p, int device, int stor
         delete ctx;
         return AWP_EHIP;
     }
-    // one allocation: 9 FP32 arrays (velocities, stresses) per wavefield
-    // buffer (two for the fused step), r1..r6 at 4 (FP32) or 2 (mixed) bytes
-    // per element -- the state awp_init zeroes -- then d1 mu lam (FP32) and
-    // qp qs (4 or 2 bytes)
+    // one allocation: 9 FP32 arrays (velocities, stresses) of wavefield
+    // buffer 0, r1..r6 at 4 (FP32) or 2 (mixed) bytes per element -- the
+    // state awp_init zeroes -- then d1 mu lam (FP32) and qp qs (4 or 2
+    // bytes).  The fused step's buffer 1 is an allocation of its own: where
+    // it lands relative to buffer 0 sets the fused step's speed (at
+    // 1500x1000x300 some placements run 16.1-16.3 ms per step, others
+    // 17.5-17.6), and awp_tune tries several.
     const size_t nv = (size_t)ctx->g.nvol;
     const size_t esz = (storage == AWP_STORAGE_MIXED) ? 2 : 4;
-    const int nbuf = ctx->two_bufs ? 2 : 1;
+    const int nbuf = 1;
     // a 2-byte array gets 128 bytes of slack and starts 64 bytes in, which
     // puts its interior rows on 128-byte lines (the pitch is a multiple of 64)
     const size_t eslot = nv * esz + (esz == 2 ? 128 : 0), eoff = (esz == 2 ? 64 : 0);
@@ -265,8 +270,23 @@ extern "C" int awp_create(awp_ctx **out, const awp_in3d *p, int device, int stor
     ctx->bytes += nbytes;
     ctx->vol_bytes = nbytes;
     ctx->state_bytes = nv * 9 * nbuf * sizeof(float) + 6 * eslot + rexp_bytes;
-    for (int bsel = 0; bsel < 2; ++bsel)
-        for (int a = 0; a < 9; ++a) ctx->wf[bsel][a] = ctx->vol + (size_t)((bsel % nbuf) * 9 + a) * nv;
+    if (ctx->two_bufs) {
+        ctx->vol_b_bytes = nv * 9 * sizeof(float);
+        e = hipMalloc((void **)&ctx->vol_b, ctx->vol_b_bytes);
+        if (e != hipSuccess) {
+            snprintf(err, errlen, "hipMalloc(%zu bytes) for the second wavefield buffer: %s", ctx->vol_b_bytes,
+                     hipGetErrorString(e));
+            hipFree(ctx->vol);
+            hipStreamDestroy(ctx->stream);
+            delete ctx;
+            return AWP_ENOMEM;
+        }
+        ctx->bytes += ctx->vol_b_bytes;
+    }
+    for (int a = 0; a < 9; ++a) {
+        ctx->wf[0][a] = ctx->vol + (size_t)a * nv;
+        ctx->wf[1][a] = ctx->two_bufs ? ctx->vol_b + (size_t)a * nv : ctx->wf[0][a];
+    }
     ctx->cur = 0;
     float **f32[9] = {&ctx->s.u1, &ctx->s.v1, &ctx->s.w1, &ctx->s.xx, &ctx->s.yy, &ctx->s.zz,
                       &ctx->s.xy, &ctx->s.xz, &ctx->s.yz};
@@ -355,6 +375,7 @@ extern "C" int awp_destroy(awp_ctx *ctx)
     if (ctx->stream) hipStreamSynchronize(ctx->stream);
     for (auto &e : ctx->ev) hipEventDestroy(e);
     hipFree(ctx->vol);
+    hipFree(ctx->vol_b);
     hipFree(ctx->prof);
     hipFree(ctx->d_sjob);
     hipFree(ctx->d_psrc);
@@ -640,6 +661,7 @@ extern "C" int awp_init(awp_ctx *ctx)
     HIPTRY(ctx, hipSetDevice(ctx->device));
     // wavefield and memory variables: the front of the volume allocation
     HIPTRY(ctx, hipMemsetAsync(ctx->vol, 0, ctx->state_bytes, ctx->stream));
+    if (ctx->vol_b) HIPTRY(ctx, hipMemsetAsync(ctx->vol_b, 0, ctx->vol_b_bytes, ctx->stream));
     const size_t nout = (size_t)std::max(ctx->npts, 1) * 6 * std::max(ctx->nsamp_max, 1);
     hipLaunchKernelGGL(k_fill, dim3(1024), dim3(256), 0, ctx->stream, ctx->d_sgt, nout, -1.0f);
     HIPTRY(ctx, hipGetLastError());
@@ -810,6 +832,8 @@ extern "C" int awp_tune(awp_ctx *ctx, int reps)
             const int pat[3][2] = {{4, 4}, {2, 2}, {8, 2}};
             for (int kc : {100, 75, 60})
                 for (const auto &pp : pat) cands.push_back({true, 11, kc, pp[0], pp[1]});
+            cands.push_back({true, 11, 75, 3, 3});
+            cands.push_back({true, 11, 75, 4, 8});
             cands.push_back({true, 8, 75, 4, 4});
         } else {
             cands.push_back({true, 7, 75, 4, 4});
@@ -829,19 +853,25 @@ extern "C" int awp_tune(awp_ctx *ctx, int reps)
         ctx->l = l;
         return scheme_changed(ctx);
     };
+    // one warm-up step, then `reps` timed ones (ms in *ms)
+    auto time_steps = [ctx, reps](float *ms) -> int {
+        int r = trial_steps(ctx, 1);
+        if (r) return r;
+        HIPTRY(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
+        r = trial_steps(ctx, reps);
+        if (r) return r;
+        HIPTRY(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
+        HIPTRY(ctx, hipEventSynchronize(ctx->ev[1]));
+        HIPTRY(ctx, hipEventElapsedTime(ms, ctx->ev[0], ctx->ev[1]));
+        return AWP_OK;
+    };
     float best = 1e30f;
     size_t pick = 0;
     for (size_t n = 0; n < cands.size(); ++n) {
-        rc = apply(cands[n]);
-        if (!rc) rc = trial_steps(ctx, 1);   // warm-up
-        if (rc) return rc;
-        HIPTRY(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
-        rc = trial_steps(ctx, reps);
-        if (rc) return rc;
-        HIPTRY(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
-        HIPTRY(ctx, hipEventSynchronize(ctx->ev[1]));
         float ms = 0;
-        HIPTRY(ctx, hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
+        rc = apply(cands[n]);
+        if (!rc) rc = time_steps(&ms);
+        if (rc) return rc;
         if (ms < best) {
             best = ms;
             pick = n;
@@ -849,6 +879,51 @@ extern "C" int awp_tune(awp_ctx *ctx, int reps)
     }
     rc = apply(cands[pick]);
     if (rc) return rc;
+    if (ctx->fused && ctx->vol_b) {
+        // Where the second buffer lands in HBM relative to the first sets
+        // the fused step's speed (1500x1000x300: 16.1-16.3 or 17.5-17.6 ms
+        // per step, stable for the life of the allocation).  Try up to five
+        // more buffers while device memory allows and keep the fastest.
+        const size_t nv = (size_t)ctx->g.nvol;
+        auto use_b = [ctx, nv](float *vb) {
+            ctx->vol_b = vb;
+            for (int a = 0; a < 9; ++a) ctx->wf[1][a] = vb + (size_t)a * nv;
+            ctx->cur = 0;   // trial steps start from buffer 0
+        };
+        std::vector<float *> tried(1, ctx->vol_b);
+        size_t keep = 0;
+        float bestp = 0;
+        use_b(ctx->vol_b);
+        rc = time_steps(&bestp);
+        for (int t = 0; !rc && t < 5; ++t) {
+            size_t fr = 0, tot = 0;
+            if (hipMemGetInfo(&fr, &tot) != hipSuccess || fr < ctx->vol_b_bytes + ((size_t)4 << 30)) break;
+            float *nb = nullptr;
+            if (hipMalloc((void **)&nb, ctx->vol_b_bytes) != hipSuccess) {
+                (void)hipGetLastError();
+                break;
+            }
+            tried.push_back(nb);
+            use_b(nb);
+            float ms = 0;
+            // zero, as awp_init leaves a buffer (the step reads its ghost layers)
+            if (hipMemsetAsync(nb, 0, ctx->vol_b_bytes, ctx->stream) != hipSuccess) {
+                rc = fail(ctx, AWP_EHIP, "hipMemsetAsync of a trial buffer failed");
+                break;
+            }
+            rc = time_steps(&ms);
+            if (!rc && ms < bestp) {
+                bestp = ms;
+                keep = tried.size() - 1;
+            }
+        }
+        hipError_t se = hipStreamSynchronize(ctx->stream);
+        for (size_t k = 0; k < tried.size(); ++k)
+            if (k != keep) hipFree(tried[k]);
+        use_b(tried[keep]);
+        if (rc) return rc;
+        HIPTRY(ctx, se);
+    }
     // the trial steps changed the state: start the run again
     return awp_init(ctx);
 }
@@ -1126,14 +1201,15 @@ extern "C" int awp_get_field(awp_ctx *ctx, int which, int kext, float *host)
 // ---------------------------------------------------------------------------
 // checkpoint / restart (the reference's restart files: pmcl3d.f:1016-1097,
 // 1964-2045).  File: a 72-byte header, then the state part of the volume
-// allocation (wavefield buffers, memory variables, their exponents) and the
-// SGT sample buffer, byte for byte as they sit in HBM, streamed through a
-// pinned staging buffer.
+// allocation (wavefield buffer 0, memory variables, their exponents), the
+// second wavefield buffer where the run holds one, and the SGT sample
+// buffer, byte for byte as they sit in HBM, streamed through a pinned
+// staging buffer.
 // ---------------------------------------------------------------------------
 namespace {
 
 struct CkpHead {
-    char magic[8];                  // "AWPCKP01"
+    char magic[8];                  // "AWPCKP02"
     int32_t nx, ny, nz, storage;
     int32_t fused, cur, step, nsamp;
     int32_t npts, nsamp_max, nt, reserved;
@@ -1150,7 +1226,7 @@ CkpHead ckp_head_of(const awp_ctx *ctx)
 {
     CkpHead h;
     memset(&h, 0, sizeof h);
-    memcpy(h.magic, "AWPCKP01", 8);
+    memcpy(h.magic, "AWPCKP02", 8);
     h.nx = ctx->p.nx;
     h.ny = ctx->p.ny;
     h.nz = ctx->p.nz;
@@ -1162,7 +1238,7 @@ CkpHead ckp_head_of(const awp_ctx *ctx)
     h.npts = ctx->npts;
     h.nsamp_max = ctx->nsamp_max;
     h.nt = ctx->p.nt;
-    h.state_bytes = ctx->state_bytes;
+    h.state_bytes = ctx->state_bytes + ctx->vol_b_bytes;
     h.sgt_bytes = sgt_buffer_bytes(ctx);
     return h;
 }
@@ -1191,6 +1267,7 @@ int ckp_run(awp_ctx *ctx, FILE *f, bool save)
     char *stage = nullptr;
     HIPTRY(ctx, hipHostMalloc((void **)&stage, chunk, hipHostMallocDefault));
     int rc = ckp_stream(ctx, f, (char *)ctx->vol, ctx->state_bytes, stage, chunk, save);
+    if (!rc && ctx->vol_b) rc = ckp_stream(ctx, f, (char *)ctx->vol_b, ctx->vol_b_bytes, stage, chunk, save);
     if (!rc) rc = ckp_stream(ctx, f, (char *)ctx->d_sgt, sgt_buffer_bytes(ctx), stage, chunk, save);
     hipHostFree(stage);
     return rc;

@@ -29,7 +29,9 @@ struct awp_ctx {
     Coef c;
     Launch l;
     hipStream_t stream;
-    float *vol;              // one allocation for the volume arrays
+    float *vol;              // one allocation for the volume arrays (wavefield buffer 0)
+    float *vol_b;            // wavefield buffer 1, an allocation of its own (two_bufs)
+    size_t vol_b_bytes;
     // wavefield buffers (u1 v1 w1 xx yy zz xy xz yz): the fused step reads
     // wf[cur] and writes wf[cur ^ 1]; the two-stage step uses wf[0] only
     float *wf[2][9];
@@ -63,7 +65,7 @@ struct awp_ctx {
     double t_vel, t_str, t_all;
     size_t bytes;
     size_t vol_bytes;        // the volume allocation
-    size_t state_bytes;      // its wavefield + memory-variable part (zeroed by awp_init)
+    size_t state_bytes;      // its wavefield + memory-variable part (zeroed by awp_init, as is vol_b)
     bool sg_done;            // SGT coefficients computed from FP32 media
     char err[512];
 
