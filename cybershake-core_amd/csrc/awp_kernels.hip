@@ -241,6 +241,10 @@ __device__ inline void stress_tile_e(const Geo &g, const State &s0, const Coef &
     const bool act = ir <= g.nx && jr <= g.ny;
     if constexpr (!P::mixed) {
         if (!act) return;
+    } else {
+        // a wave whose whole row segment lies beyond nx (the second wave of
+        // a 128-point tile at the grid's edge) has no point and no exponent
+        if (seg * TX >= g.nx || jr > g.ny) return;
     }
     const int i = min(ir, g.nx), j = min(jr, g.ny);
     const long long sy = g.sy, sz = g.sz;
@@ -440,6 +444,10 @@ __device__ inline void stress_tile_b(const Geo &g, const State &s0, const Coef &
     const bool act = ir <= g.nx && jr <= g.ny;
     if constexpr (!P::mixed) {
         if (!act) return;
+    } else {
+        // a wave whose whole row segment lies beyond nx (the second wave of
+        // a 128-point tile at the grid's edge) has no point and no exponent
+        if (seg * TX >= g.nx || jr > g.ny) return;
     }
     const int i = min(ir, g.nx), j = min(jr, g.ny);
     Cursor32 p(g, g.base + (long long)kb * g.sz + (long long)j * g.sy + i);
