@@ -415,8 +415,8 @@ def estimate_bytes(p: In3d, storage: int = AWP_STORAGE_FP32, npts: int = 0) -> T
 class Batch:
     """Initialised runs advanced together (``awp_batch``): one persistent
     launch per stage of a time step over every run's tiles (two-stage
-    scheme, or the fused one when every run was created with
-    AWP_SCHEME=fused), replayed from a HIP graph.  The runs stay usable
+    scheme, or the fused one when every run holds two wavefield buffers
+    and steps fused), replayed from a HIP graph.  The runs stay usable
     (``sgt()``, ``write_sgt_file``) and must outlive the batch."""
 
     def __init__(self, runs: Sequence["Run"]):

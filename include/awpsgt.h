@@ -302,8 +302,8 @@ int awp_write_sgt_head(const awp_head_params *hp, char *err, size_t errlen);
  * tiles of every run (two-stage scheme: velocity, free-surface images,
  * stress; fused scheme: the one-sweep kernel), then one launch for every
  * run's source and SGT sample, replayed from a HIP graph.  The fused scheme
- * is used when every run was created with it (AWP_SCHEME=fused: two
- * wavefield buffers), else the two-stage one.  Each run keeps its own
+ * is used when every run holds two wavefield buffers and steps fused (the
+ * FP32 default, or AWP_SCHEME=fused), else the two-stage one.  Each run keeps its own
  * outputs (awp_get_sgt, awp_write_sgt_file).  The reference runs one SGT
  * component per MPI job (AWP-ODC-SGT/utils/awp_odc_wrapper.sh:14); these
  * entry points have no counterpart in it. */
