@@ -818,9 +818,10 @@ extern "C" int awp_tune(awp_ctx *ctx, int reps)
     if (rc) return rc;
     // candidates: the two-stage step with its defaults; with both buffers,
     // the fused step on 64 x 11 tiles (FP32; z chunks of about 100, 75, 60
-    // levels x tile patches 4x4, 2x2, 8x2: which of them runs fastest
-    // varies between boxes at 1500x1000x300) and 64 x 8 tiles, or on 64 x 7
-    // tiles (mixed)
+    // levels x tile patches 4x4, 2x2, 8x2, and larger patches) and 64 x 8
+    // tiles, or on 64 x 7 tiles (mixed).  The fastest is searched, then the
+    // fused step's second buffer is placed, then the fused shapes are
+    // searched again on that placement.
     struct Cand {
         bool fused;
         int ty, kc, pw, ph;
@@ -832,8 +833,11 @@ extern "C" int awp_tune(awp_ctx *ctx, int reps)
             const int pat[3][2] = {{4, 4}, {2, 2}, {8, 2}};
             for (int kc : {100, 75, 60})
                 for (const auto &pp : pat) cands.push_back({true, 11, kc, pp[0], pp[1]});
+            // larger patches: more of an XCD's 32 resident tiles are
+            // neighbours that share their halo lines
+            const int big[3][2] = {{4, 8}, {8, 4}, {6, 5}};
+            for (const auto &pp : big) cands.push_back({true, 11, 100, pp[0], pp[1]});
             cands.push_back({true, 11, 75, 3, 3});
-            cands.push_back({true, 11, 75, 4, 8});
             cands.push_back({true, 8, 75, 4, 4});
         } else {
             cands.push_back({true, 7, 75, 4, 4});
@@ -865,19 +869,24 @@ extern "C" int awp_tune(awp_ctx *ctx, int reps)
         HIPTRY(ctx, hipEventElapsedTime(ms, ctx->ev[0], ctx->ev[1]));
         return AWP_OK;
     };
-    float best = 1e30f;
+    // the fastest of the candidates (of the fused ones only: fused_only)
     size_t pick = 0;
-    for (size_t n = 0; n < cands.size(); ++n) {
-        float ms = 0;
-        rc = apply(cands[n]);
-        if (!rc) rc = time_steps(&ms);
-        if (rc) return rc;
-        if (ms < best) {
-            best = ms;
-            pick = n;
+    auto search = [&](bool fused_only) -> int {
+        float best = 1e30f;
+        for (size_t n = 0; n < cands.size(); ++n) {
+            if (fused_only && !cands[n].fused) continue;
+            float ms = 0;
+            int r = apply(cands[n]);
+            if (!r) r = time_steps(&ms);
+            if (r) return r;
+            if (ms < best) {
+                best = ms;
+                pick = n;
+            }
         }
-    }
-    rc = apply(cands[pick]);
+        return apply(cands[pick]);
+    };
+    rc = search(false);
     if (rc) return rc;
     if (ctx->fused && ctx->vol_b) {
         // Where the second buffer lands in HBM relative to the first sets
@@ -923,6 +932,10 @@ extern "C" int awp_tune(awp_ctx *ctx, int reps)
         use_b(tried[keep]);
         if (rc) return rc;
         HIPTRY(ctx, se);
+        // the launch shapes again, on the buffer kept (the first search ran
+        // on whatever placement the run started with)
+        rc = search(true);
+        if (rc) return rc;
     }
     // the trial steps changed the state: start the run again
     return awp_init(ctx);
