@@ -838,6 +838,9 @@ extern "C" int awp_tune(awp_ctx *ctx, int reps)
             const int big[3][2] = {{4, 8}, {8, 4}, {6, 5}};
             for (const auto &pp : big) cands.push_back({true, 11, 100, pp[0], pp[1]});
             cands.push_back({true, 11, 75, 3, 3});
+            // two chunks: less velocity recomputed at chunk edges
+            cands.push_back({true, 11, 150, 4, 4});
+            cands.push_back({true, 11, 150, 2, 2});
             cands.push_back({true, 8, 75, 4, 4});
         } else {
             cands.push_back({true, 7, 75, 4, 4});
