@@ -2,7 +2,7 @@
 the reference solver's own outputs.  FP32 device results are compared
 bit-for-bit with the FP32 restatement (same IEEE operation sequence, no FMA
 contraction) and within rel L2 <= 1e-4 per SGT component with the reference
-binary's files (the tolerance the north star states for FP32)."""
+binary's files (the task's FP32 tolerance)."""
 import hashlib
 import os
 import subprocess
