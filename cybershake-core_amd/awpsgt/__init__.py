@@ -480,9 +480,10 @@ class Batch:
 
 
 def run_dir(dirpath: str, device: int = 0, storage: int = AWP_STORAGE_FP32,
-            nsteps: Optional[int] = None) -> Tuple[In3d, Run]:
+            nsteps: Optional[int] = None, tune: bool = True) -> Tuple[In3d, Run]:
     """Read a run directory's IN3D and inputs (paths relative to it), run it
-    and return (params, run).  The SGT file is not written."""
+    (after awp_tune unless tune=False, as the C host does) and return
+    (params, run).  The SGT file is not written."""
     p = read_in3d(os.path.join(dirpath, "IN3D"))
     validate_in3d(p)
     media = read_media(os.path.join(dirpath, p.s("invel")), p)
@@ -493,6 +494,8 @@ def run_dir(dirpath: str, device: int = 0, storage: int = AWP_STORAGE_FP32,
     r.set_media(media)
     r.set_source(xyz, stf)
     r.init()
+    if tune:
+        r.tune()
     r.step(p.nt if nsteps is None else nsteps)
     r.synchronize()
     return p, r
