@@ -819,9 +819,9 @@ extern "C" int awp_tune(awp_ctx *ctx, int reps)
     // candidates: the two-stage step with its defaults; with both buffers,
     // the fused step on 64 x 11 tiles (FP32; z chunks of about 100, 75, 60
     // levels x tile patches 4x4, 2x2, 8x2, and larger patches) and 64 x 8
-    // tiles, or on 64 x 7 tiles (mixed).  The fastest is searched, then the
-    // fused step's second buffer is placed, then the fused shapes are
-    // searched again on that placement.
+    // tiles, or on 64 x 7 tiles (mixed).  The fastest is searched, then, if
+    // it is fused, the second buffer is placed and all are searched again on
+    // that placement.
     struct Cand {
         bool fused;
         int ty, kc, pw, ph;
@@ -872,12 +872,11 @@ extern "C" int awp_tune(awp_ctx *ctx, int reps)
         HIPTRY(ctx, hipEventElapsedTime(ms, ctx->ev[0], ctx->ev[1]));
         return AWP_OK;
     };
-    // the fastest of the candidates (of the fused ones only: fused_only)
+    // the fastest of the candidates
     size_t pick = 0;
-    auto search = [&](bool fused_only) -> int {
+    auto search = [&]() -> int {
         float best = 1e30f;
         for (size_t n = 0; n < cands.size(); ++n) {
-            if (fused_only && !cands[n].fused) continue;
             float ms = 0;
             int r = apply(cands[n]);
             if (!r) r = time_steps(&ms);
@@ -889,7 +888,7 @@ extern "C" int awp_tune(awp_ctx *ctx, int reps)
         }
         return apply(cands[pick]);
     };
-    rc = search(false);
+    rc = search();
     if (rc) return rc;
     if (ctx->fused && ctx->vol_b) {
         // Where the second buffer lands in HBM relative to the first sets
@@ -935,9 +934,10 @@ extern "C" int awp_tune(awp_ctx *ctx, int reps)
         use_b(tried[keep]);
         if (rc) return rc;
         HIPTRY(ctx, se);
-        // the launch shapes again, on the buffer kept (the first search ran
-        // on whatever placement the run started with)
-        rc = search(true);
+        // every candidate again, on the buffer kept (the first search ran on
+        // whatever placement the run started with; where no placement is
+        // fast, the two-stage step may win now)
+        rc = search();
         if (rc) return rc;
     }
     // the trial steps changed the state: start the run again
