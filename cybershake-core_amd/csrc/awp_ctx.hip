@@ -152,8 +152,15 @@ static Launch choose_launch(const Geo &g, bool fused, bool mixed, int device)
     if (fused) {
         // the fused step keeps its load prefetch depth (0, 1, 2) in `unroll`
         const char *pf = getenv("AWP_PF");
-        l.unroll = (pf && atoi(pf) >= 0 && atoi(pf) <= 2) ? atoi(pf) : 0;
-        if (l.tile_y == 8 || l.tile_y == 11) l.unroll = 0;
+        // (3: no prefetch, two levels per barrier, on 64 x 11 tiles only --
+        // their default: 1500x1000x300 16.17-16.27 ms per step against
+        // 16.48-16.58 with one level per barrier on the same buffers)
+        l.unroll = (pf && atoi(pf) >= 0 && atoi(pf) <= 3) ? atoi(pf) : (l.tile_y == 11 ? 3 : 0);
+        if (l.tile_y == 11) {
+            if (l.unroll != 0 && l.unroll != 3) l.unroll = 3;
+        } else if (l.tile_y == 8 || l.unroll == 3) {
+            l.unroll = 0;
+        }
     }
     const long long bxy = (long long)((g.nx + TX - 1) / TX) * ((g.ny + l.tile_y - 1) / l.tile_y);
     int nchunk;
@@ -852,7 +859,7 @@ extern "C" int awp_tune(awp_ctx *ctx, int reps)
         Launch l = choose_launch(ctx->g, cd.fused, ctx->mixed(), ctx->device);
         if (cd.fused) {
             l.tile_y = cd.ty;
-            l.unroll = 0;
+            l.unroll = cd.ty == 11 ? 3 : 0;   // 64 x 11: two levels per barrier
             l.pw = cd.pw;
             l.ph = cd.ph;
             set_chunks(l, ctx->g, std::min((ctx->g.nz + cd.kc / 2) / cd.kc, std::max(1, ctx->g.nz / 16)));

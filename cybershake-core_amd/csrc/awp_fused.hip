@@ -37,13 +37,14 @@ namespace awp {
 
 namespace {
 
-template <int TY>
+// PFC: the variant code of step_tile (3: two levels per barrier)
+template <int TY, int PFC>
 struct FShape {
     static constexpr int RW = TX + 4;          // region columns i0-2 .. i0+65
     static constexpr int RH = TY + 4;          // region rows j0-2 .. j0+TY+1
     static constexpr int PLANE = RW * RH;
     static constexpr int SLOT = 3 * PLANE;     // u, v, w of one level
-    static constexpr int NSLOT = 6;            // ring of levels
+    static constexpr int NSLOT = PFC == 3 ? 8 : 6;   // ring of levels
     static constexpr int NW = TY + 5;          // waves per block
 };
 
@@ -85,13 +86,17 @@ struct StrIn {
 // what it needs; 1 = the stress stage's loads are issued before the velocity
 // stage's arithmetic; 2 = as 1, and the next level's velocity loads are
 // issued before the stress arithmetic.
+// PFC = 3: no prefetch, and the velocities of two levels per barrier (then
+// the stresses of two levels), which halves the block's barriers.
 // One column tile (bx, by, z chunk bz) of one run: the whole block calls it.
-template <class P, int TY, int PF>
+template <class P, int TY, int PFC>
 __device__ __forceinline__ void step_tile(const Geo &g, const State &a0, const State &b0, const Coef &c,
                                           const int kc, const int bx, const int by, const int bz, float *lds)
 {
     const GState a = as_global(a0), b = as_global(b0);
-    using Sh = FShape<TY>;
+    using Sh = FShape<TY, PFC>;
+    constexpr int PF = PFC == 3 ? 0 : PFC;   // prefetch depth
+    constexpr int LV = PFC == 3 ? 2 : 1;     // levels per barrier
     using Q = typename P::Q;
     using R = typename P::R;
     const int lane = threadIdx.x, wv = threadIdx.y;
@@ -235,256 +240,295 @@ __device__ __forceinline__ void step_tile(const Geo &g, const State &a0, const S
         if (dom && m0 <= g.nz) load_vel(m0, vin);
     }
 
-    for (int m = m0; m <= ke + 2; ++m) {
+    // Old xz / yz / zz of the level a stress stage updates: the bottom of the
+    // velocity stage's z queues when that stage computed level s + 2.
+    struct Old {
+        float xz, yz, zz;
+    };
+    // velocity stage of level m into the LDS ring, then the z queues advance
+    auto vel_level = [&](const int m, Old &old) {
         // ring slot of level l (l >= -1)
         float *slot = lds + ((m + 6) % Sh::NSLOT) * Sh::SLOT;
-        const int s = m - 2;
-        const bool do_str = smain && s >= kb;
-        StrIn sin_ = {};
-        if constexpr (PF >= 1) {
-            if (do_str) load_str(s, sin_);
-        }
-        if constexpr (PF == 1) {
-            if (dom && m <= g.nz) load_vel(m, vin);
-            // keep this level's loads together, ahead of all arithmetic, so
-            // they are in flight at once
-            __builtin_amdgcn_sched_barrier(0);
-        }
-        // ================= velocity of level m (operator.f:92-244) ==========
-        float un = 0.f, vn = 0.f, wn = 0.f;
-        float xz_p1 = 0.f, yz_p1 = 0.f, zz_p2 = 0.f;
-        if (dom && m <= g.nz) {
-            if constexpr (PF == 0) load_vel(m, vin);
-            const VelIn &x = vin;
-            xz_p1 = x.xz_p1; yz_p1 = x.yz_p1; zz_p2 = x.zz_p2;
-            if (m >= 1) {
-                float d, q, u, v, w;
-                d = 0.25f * ((x.dA + x.dB) + (dA_m + dB_m));
-                q = dth / d;
-                u = x.u + q * (
-                    c1 * (x.xx_0 - x.xx_m1) + c2 * (x.xx_p1 - x.xx_m2) +
-                    c1 * (x.xy_0 - x.xy_jm1) + c2 * (x.xy_jp1 - x.xy_jm2) +
-                    c1 * (xz_0 - xz_m1) + c2 * (x.xz_p1 - xz_m2));
-                d = 0.25f * ((x.dA + x.dC) + (dA_m + dC_m));
-                q = dth / d;
-                v = x.v + q * (
-                    c1 * (x.xy_ip1 - x.xy_0) + c2 * (x.xy_ip2 - x.xy_im1) +
-                    c1 * (x.yy_jp1 - x.yy_0) + c2 * (x.yy_jp2 - x.yy_jm1) +
-                    c1 * (yz_0 - yz_m1) + c2 * (x.yz_p1 - yz_m2));
-                d = 0.25f * ((x.dA + x.dC) + (x.dB + x.dD));
-                q = dth / d;
-                w = x.w + q * (
-                    c1 * (x.xz_ip1 - xz_0) + c2 * (x.xz_ip2 - x.xz_im1) +
-                    c1 * (yz_0 - x.yz_jm1) + c2 * (x.yz_jp1 - x.yz_jm2) +
-                    c1 * (zz_p1 - zz_0) + c2 * (x.zz_p2 - zz_m1));
-                const float D = fxy * x.fz;       // addcrjvel, cerjan.f:292-328
-                un = u * D;
-                vn = v * D;
-                wn = w * D;
-                if (is_main && m >= kb && m <= ke) {
-                    const unsigned o = ob + (unsigned)m * sz4;
-                    gsn(b.u1, o, un);
-                    gsn(b.v1, o, vn);
-                    gsn(b.w1, o, wn);
+            // ================= velocity of level m (operator.f:92-244) ==========
+            float un = 0.f, vn = 0.f, wn = 0.f;
+            float xz_p1 = 0.f, yz_p1 = 0.f, zz_p2 = 0.f;
+            if (dom && m <= g.nz) {
+                if constexpr (PF == 0) load_vel(m, vin);
+                const VelIn &x = vin;
+                xz_p1 = x.xz_p1; yz_p1 = x.yz_p1; zz_p2 = x.zz_p2;
+                if (m >= 1) {
+                    float d, q, u, v, w;
+                    d = 0.25f * ((x.dA + x.dB) + (dA_m + dB_m));
+                    q = dth / d;
+                    u = x.u + q * (
+                        c1 * (x.xx_0 - x.xx_m1) + c2 * (x.xx_p1 - x.xx_m2) +
+                        c1 * (x.xy_0 - x.xy_jm1) + c2 * (x.xy_jp1 - x.xy_jm2) +
+                        c1 * (xz_0 - xz_m1) + c2 * (x.xz_p1 - xz_m2));
+                    d = 0.25f * ((x.dA + x.dC) + (dA_m + dC_m));
+                    q = dth / d;
+                    v = x.v + q * (
+                        c1 * (x.xy_ip1 - x.xy_0) + c2 * (x.xy_ip2 - x.xy_im1) +
+                        c1 * (x.yy_jp1 - x.yy_0) + c2 * (x.yy_jp2 - x.yy_jm1) +
+                        c1 * (yz_0 - yz_m1) + c2 * (x.yz_p1 - yz_m2));
+                    d = 0.25f * ((x.dA + x.dC) + (x.dB + x.dD));
+                    q = dth / d;
+                    w = x.w + q * (
+                        c1 * (x.xz_ip1 - xz_0) + c2 * (x.xz_ip2 - x.xz_im1) +
+                        c1 * (yz_0 - x.yz_jm1) + c2 * (x.yz_jp1 - x.yz_jm2) +
+                        c1 * (zz_p1 - zz_0) + c2 * (x.zz_p2 - zz_m1));
+                    const float D = fxy * x.fz;       // addcrjvel, cerjan.f:292-328
+                    un = u * D;
+                    vn = v * D;
+                    wn = w * D;
+                    if (is_main && m >= kb && m <= ke) {
+                        const unsigned o = ob + (unsigned)m * sz4;
+                        gsn(b.u1, o, un);
+                        gsn(b.v1, o, vn);
+                        gsn(b.w1, o, wn);
+                    }
                 }
+                dA_m = x.dA; dB_m = x.dB; dC_m = x.dC;
             }
-            dA_m = x.dA; dB_m = x.dB; dC_m = x.dC;
-        }
-        if (m <= g.nz) {
-            if (active) {
+            if (m <= g.nz) {
+                if (active) {
+                    slot[lpos] = un;
+                    slot[PL + lpos] = vn;
+                    slot[2 * PL + lpos] = wn;
+                }
+            } else if (is_main) {
+                if (m == g.nz + 1 && smain) {
+                    // free-surface velocity images fvelxy / fvelz (operator.f:246-306)
+                    // from the level-nz plane (and w at nz-1)
+                    const float *pl = lds + ((g.nz + 6) % Sh::NSLOT) * Sh::SLOT;
+                    const float *pl1 = lds + ((g.nz - 1 + 6) % Sh::NSLOT) * Sh::SLOT;
+                    const float *pu = pl, *pv = pl + PL, *pw = pl + 2 * PL;
+                    const float w0 = pw[lpos], u0 = pu[lpos], v0 = pv[lpos];
+                    const float ut = u0 - (w0 - pw[lpos - 1]);
+                    const float vt = v0 - (pw[lpos + Sh::RW] - w0);
+                    const float ut_ip1 = (i < g.nx) ? (pu[lpos + 1] - (pw[lpos + 1] - w0)) : 0.0f;
+                    const float vt_jm1 = (j > 1) ? (pv[lpos - Sh::RW] - (w0 - pw[lpos - Sh::RW])) : 0.0f;
+                    const unsigned o = ob + (unsigned)g.nz * sz4;
+                    const float xl = 1.0f / gl(lam, o);
+                    const float xm = 2.0f / gl(mu, o);
+                    const float xl2m = xm + xl;
+                    const float wt = pl1[2 * PL + lpos] - (xl / xl2m) * (
+                        (ut_ip1 - ut) +
+                        (pu[lpos + 1] - u0) +
+                        (vt - vt_jm1) +
+                        (v0 - pv[lpos - Sh::RW]));
+                    un = ut;
+                    vn = vt;
+                    wn = wt;
+                    if (ke == g.nz) {
+                        gsn(b.u1, o + sz4, un);
+                        gsn(b.v1, o + sz4, vn);
+                        gsn(b.w1, o + sz4, wn);
+                    }
+                }
+                // level nz+1: the images; level nz+2: zero (the reference never sets it)
                 slot[lpos] = un;
                 slot[PL + lpos] = vn;
                 slot[2 * PL + lpos] = wn;
             }
-        } else if (is_main) {
-            if (m == g.nz + 1 && smain) {
-                // free-surface velocity images fvelxy / fvelz (operator.f:246-306)
-                // from the level-nz plane (and w at nz-1)
-                const float *pl = lds + ((g.nz + 6) % Sh::NSLOT) * Sh::SLOT;
-                const float *pl1 = lds + ((g.nz - 1 + 6) % Sh::NSLOT) * Sh::SLOT;
-                const float *pu = pl, *pv = pl + PL, *pw = pl + 2 * PL;
-                const float w0 = pw[lpos], u0 = pu[lpos], v0 = pv[lpos];
-                const float ut = u0 - (w0 - pw[lpos - 1]);
-                const float vt = v0 - (pw[lpos + Sh::RW] - w0);
-                const float ut_ip1 = (i < g.nx) ? (pu[lpos + 1] - (pw[lpos + 1] - w0)) : 0.0f;
-                const float vt_jm1 = (j > 1) ? (pv[lpos - Sh::RW] - (w0 - pw[lpos - Sh::RW])) : 0.0f;
-                const unsigned o = ob + (unsigned)g.nz * sz4;
-                const float xl = 1.0f / gl(lam, o);
-                const float xm = 2.0f / gl(mu, o);
-                const float xl2m = xm + xl;
-                const float wt = pl1[2 * PL + lpos] - (xl / xl2m) * (
-                    (ut_ip1 - ut) +
-                    (pu[lpos + 1] - u0) +
-                    (vt - vt_jm1) +
-                    (v0 - pv[lpos - Sh::RW]));
-                un = ut;
-                vn = vt;
-                wn = wt;
-                if (ke == g.nz) {
-                    gsn(b.u1, o + sz4, un);
-                    gsn(b.v1, o + sz4, vn);
-                    gsn(b.w1, o + sz4, wn);
-                }
-            }
-            // level nz+1: the images; level nz+2: zero (the reference never sets it)
-            slot[lpos] = un;
-            slot[PL + lpos] = vn;
-            slot[2 * PL + lpos] = wn;
-        }
 
-        __syncthreads();
-
-        if constexpr (PF >= 2) {
-            if (dom && m + 1 <= g.nz) load_vel(m + 1, vin);
-            __builtin_amdgcn_sched_barrier(0);
-        }
-
-        // ================= stress of level s = m-2 (viscoop.f:49-191) =======
-        float rn[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};   // new memory variables (mixed: stored below)
-        if (do_str) {
-            if constexpr (PF < 1) load_str(s, sin_);
-            const StrIn &x = sin_;
-            const unsigned o = ob + (unsigned)s * sz4;
-            const float *ps = lds + ((s + 6) % Sh::NSLOT) * Sh::SLOT;          // level s
-            const float *pm1 = lds + ((s - 1 + 6) % Sh::NSLOT) * Sh::SLOT;     // s-1
-            const float *pm2 = lds + ((s - 2 + 6) % Sh::NSLOT) * Sh::SLOT;     // s-2
-            const float *pp1 = lds + ((s + 1 + 6) % Sh::NSLOT) * Sh::SLOT;     // s+1
-            const float *pp2 = lds + ((s + 2 + 6) % Sh::NSLOT) * Sh::SLOT;     // s+2
-            const float *pu = ps, *pv = ps + PL, *pw = ps + 2 * PL;
-            const float uc = pu[lpos], vc = pv[lpos], wc = pw[lpos];
-            const float u_ip1 = pu[lpos + 1], u_ip2 = pu[lpos + 2], u_im1 = pu[lpos - 1];
-            const float u_jp1 = pu[lpos + Sh::RW], u_jp2 = pu[lpos + 2 * Sh::RW], u_jm1 = pu[lpos - Sh::RW];
-            const float v_jm1 = pv[lpos - Sh::RW], v_jp1 = pv[lpos + Sh::RW], v_jm2 = pv[lpos - 2 * Sh::RW];
-            const float v_im1 = pv[lpos - 1], v_ip1 = pv[lpos + 1], v_im2 = pv[lpos - 2];
-            const float w_im1 = pw[lpos - 1], w_ip1 = pw[lpos + 1], w_im2 = pw[lpos - 2];
-            const float w_jp1 = pw[lpos + Sh::RW], w_jp2 = pw[lpos + 2 * Sh::RW], w_jm1 = pw[lpos - Sh::RW];
-            // own column along z
-            const float uc_p1 = pp1[lpos], uc_p2 = pp2[lpos], uc_m1 = pm1[lpos];
-            const float vc_p1 = pp1[PL + lpos], vc_p2 = pp2[PL + lpos], vc_m1 = pm1[PL + lpos];
-            const float wc_p1 = pp1[2 * PL + lpos], wc_m1 = pm1[2 * PL + lpos], wc_m2 = pm2[2 * PL + lpos];
-
-            const bool odd = (((g.nz + 1) - s) & 1) != 0;
-            const float vx1 = odd ? t1o : t1e;
-            const float vx2 = odd ? t2o : t2e;
-
-            float vxl = 8.0f / (x.lA + x.lB + x.lC + x.lD + lA_m + lB_m + lC_m + lD_m);
-            const float vxm = 8.0f / (x.mA + x.mB + x.mC + x.mD + mA_m + mB_m + mC_m + mD_m);
-            vxl = vxl + 2.0f * vxm;
-            float vqpa = 0.125f * (x.pA + x.pB + x.pC + x.pD + pA_m + pB_m + pC_m + pD_m);
-            float vqsa = 0.125f * (x.sA + x.sB + x.sC + x.sD + sA_m + sB_m + sC_m + sD_m);
-            if constexpr (P::mixed) {
-                vqpa = vqpa * a.ms.iqp;
-                vqsa = vqsa * a.ms.iqs;
-            }
-
-            const float vt1 = c1 * (u_ip1 - uc) + c2 * (u_ip2 - u_im1);
-            const float vt2 = c1 * (vc - v_jm1) + c2 * (v_jp1 - v_jm2);
-            const float vt3 = c1 * (wc - wc_m1) + c2 * (wc_p1 - wc_m2);
-            const float vt4 = c1 * (u_jp1 - uc) + c2 * (u_jp2 - u_jm1) +
-                              c1 * (vc - v_im1) + c2 * (v_ip1 - v_im2);
-            const float vt5 = c1 * (uc_p1 - uc) + c2 * (uc_p2 - uc_m1) +
-                              c1 * (wc - w_im1) + c2 * (w_ip1 - w_im2);
-            const float vt6 = c1 * (vc_p1 - vc) + c2 * (vc_p2 - vc_m1);
-            const float vt7 = c1 * (w_jp1 - wc) + c2 * (w_jp2 - w_jm1);
-
-            const float vh1 = -vxm * vqsa;
-            const float vtmp = vxl * (vt1 + vt2 + vt3);
-            const float va1 = -vqpa * vxl * dh1 * (vt1 + vt2 + vt3) / 2.0f;
-
-            float r1 = x.r1, r2 = x.r2, r3 = x.r3, r4 = x.r4, r5 = x.r5, r6 = x.r6;
-            float xx = x.xx, yy = x.yy, xy = x.xy;
-            float zz = zz_m2, xz = xz_m2, yz = yz_m2;   // old values of level s
-
-            xx = xx + dth * (vtmp - 2.0f * vxm * (vt2 + vt3)) + dt * r1;
-            yy = yy + dth * (vtmp - 2.0f * vxm * (vt1 + vt3)) + dt * r2;
-            zz = zz + dth * (vtmp - 2.0f * vxm * (vt1 + vt2)) + dt * r3;
-
-            float vxmu = 2.0f / (x.mA + mA_m);
-            float vaas = 0.5f * (x.sA + sA_m);
-            if constexpr (P::mixed) vaas = vaas * a.ms.iqs;
-            const float vh2_r4 = -vaas * vxmu / 2.0f;
-            xy = xy + vxmu * dth * vt4 + dt * r4;
-            vxmu = 2.0f / (x.mA + x.mC);
-            vaas = 0.5f * (x.sA + x.sC);
-            if constexpr (P::mixed) vaas = vaas * a.ms.iqs;
-            const float vh2_r5 = -vaas * vxmu / 2.0f;
-            xz = xz + vxmu * dth * vt5 + dt * r5;
-            vxmu = 2.0f / (x.mA + x.mB);
-            vaas = 0.5f * (x.sA + x.sB);
-            if constexpr (P::mixed) vaas = vaas * a.ms.iqs;
-            const float vh2_r6 = -vaas * vxmu / 2.0f;
-            yz = yz + vxmu * (dth * vt6 + dth * vt7) + dt * r6;
-
-            r1 = (vx2 * r1 - vh1 * (vt2 + vt3) * dh1 + va1) * vx1;
-            r2 = (vx2 * r2 - vh1 * (vt1 + vt3) * dh1 + va1) * vx1;
-            r3 = (vx2 * r3 - vh1 * (vt1 + vt2) * dh1 + va1) * vx1;
-            r4 = (vx2 * r4 + vh2_r4 * dh1 * vt4) * vx1;
-            r5 = (vx2 * r5 + vh2_r5 * dh1 * vt5) * vx1;
-            r6 = (vx2 * r6 + vh2_r6 * dh1 * (vt6 + vt7)) * vx1;
-
-            xx = xx + dt * r1;
-            yy = yy + dt * r2;
-            zz = zz + dt * r3;
-            xy = xy + dt * r4;
-            xz = xz + dt * r5;
-            yz = yz + dt * r6;
-
-            const float D = fxy * x.fz;       // addcrjstr, cerjan.f:249-290
-            xx = xx * D; yy = yy * D; zz = zz * D;
-            xz = xz * D; yz = yz * D; xy = xy * D;
-
-            // memory variables in place (own point only); in mixed storage
-            // they are encoded below, after the row segment's maximum
-            if constexpr (!P::mixed) {
-                gsn(R1, o, r1); gsn(R2, o, r2); gsn(R3, o, r3);
-                gsn(R4, o, r4); gsn(R5, o, r5); gsn(R6, o, r6);
-            } else {
-                rn[0] = r1; rn[1] = r2; rn[2] = r3; rn[3] = r4; rn[4] = r5; rn[5] = r6;
-            }
-            gsn(b.xx, o, xx); gsn(b.yy, o, yy); gsn(b.xy, o, xy);
-            if (s == g.nz) {
-                // fstr (operator.f:575-612): images above the surface, xz = yz = 0 on it
-                const unsigned o1 = o + sz4, o2 = o1 + sz4;
-                gsn(b.zz, o, zz);
-                gsn(b.zz, o1, -zz);
-                gsn(b.zz, o2, -zz_f1);
-                gsn(b.xz, o, 0.0f);
-                gsn(b.yz, o, 0.0f);
-                gsn(b.xz, o1, -xz_f1);
-                gsn(b.xz, o2, -xz_f2);
-                gsn(b.yz, o1, -yz_f1);
-                gsn(b.yz, o2, -yz_f2);
-            } else {
-                gsn(b.zz, o, zz); gsn(b.xz, o, xz); gsn(b.yz, o, yz);
-            }
-            xz_f2 = xz_f1; xz_f1 = xz;
-            yz_f2 = yz_f1; yz_f1 = yz;
-            zz_f1 = zz;
-            lA_m = x.lA; lB_m = x.lB; lC_m = x.lC; lD_m = x.lD;
-            mA_m = x.mA; mB_m = x.mB; mC_m = x.mC; mD_m = x.mD;
-            pA_m = x.pA; pB_m = x.pB; pC_m = x.pC; pD_m = x.pD;
-            sA_m = x.sA; sB_m = x.sB; sC_m = x.sC; sD_m = x.sD;
-        }
-        if constexpr (P::mixed) {
-            // whole waves (one row segment each): lanes outside the domain hold 0
-            if (is_main && s >= kb) {
-                const unsigned orr = (ob + (unsigned)s * sz4) >> RSH;
-                G<R> *RR[6] = {R1, R2, R3, R4, R5, R6};
-                int e[6];
-#pragma unroll
-                for (int q = 0; q < 6; ++q) {
-                    e[q] = row_exponent(wave_max(fabsf(rn[q])));
-                    if (do_str) gsn(RR[q], orr, r_encode(rn[q], e[q]));
-                }
-                if (do_str && lane == 0)
-                    ((G<unsigned long long> *)a.rexp)[rexp_idx(g, a.nseg, bx, j, s)] = rexp_pack(e);
-            }
-        }
-
+        old = {xz_m2, yz_m2, zz_m2};
         // advance the old-stress queues to level m+1
         xz_m2 = xz_m1; xz_m1 = xz_0; xz_0 = xz_p1;
         yz_m2 = yz_m1; yz_m1 = yz_0; yz_0 = yz_p1;
         zz_m2 = zz_m1; zz_m1 = zz_0; zz_0 = zz_p1; zz_p1 = zz_p2;
+    };
+    // stress stage of level s = m - 2 from the ring (sin_: its loads, when
+    // issued ahead)
+    auto str_level = [&](const int m, const Old &old, StrIn &sin_) {
+        const int s = m - 2;
+        const bool do_str = smain && s >= kb;
+            // ================= stress of level s = m-2 (viscoop.f:49-191) =======
+            float rn[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};   // new memory variables (mixed: stored below)
+            if (do_str) {
+                if constexpr (PF < 1) load_str(s, sin_);
+                const StrIn &x = sin_;
+                const unsigned o = ob + (unsigned)s * sz4;
+                const float *ps = lds + ((s + 6) % Sh::NSLOT) * Sh::SLOT;          // level s
+                const float *pm1 = lds + ((s - 1 + 6) % Sh::NSLOT) * Sh::SLOT;     // s-1
+                const float *pm2 = lds + ((s - 2 + 6) % Sh::NSLOT) * Sh::SLOT;     // s-2
+                const float *pp1 = lds + ((s + 1 + 6) % Sh::NSLOT) * Sh::SLOT;     // s+1
+                const float *pp2 = lds + ((s + 2 + 6) % Sh::NSLOT) * Sh::SLOT;     // s+2
+                const float *pu = ps, *pv = ps + PL, *pw = ps + 2 * PL;
+                const float uc = pu[lpos], vc = pv[lpos], wc = pw[lpos];
+                const float u_ip1 = pu[lpos + 1], u_ip2 = pu[lpos + 2], u_im1 = pu[lpos - 1];
+                const float u_jp1 = pu[lpos + Sh::RW], u_jp2 = pu[lpos + 2 * Sh::RW], u_jm1 = pu[lpos - Sh::RW];
+                const float v_jm1 = pv[lpos - Sh::RW], v_jp1 = pv[lpos + Sh::RW], v_jm2 = pv[lpos - 2 * Sh::RW];
+                const float v_im1 = pv[lpos - 1], v_ip1 = pv[lpos + 1], v_im2 = pv[lpos - 2];
+                const float w_im1 = pw[lpos - 1], w_ip1 = pw[lpos + 1], w_im2 = pw[lpos - 2];
+                const float w_jp1 = pw[lpos + Sh::RW], w_jp2 = pw[lpos + 2 * Sh::RW], w_jm1 = pw[lpos - Sh::RW];
+                // own column along z
+                const float uc_p1 = pp1[lpos], uc_p2 = pp2[lpos], uc_m1 = pm1[lpos];
+                const float vc_p1 = pp1[PL + lpos], vc_p2 = pp2[PL + lpos], vc_m1 = pm1[PL + lpos];
+                const float wc_p1 = pp1[2 * PL + lpos], wc_m1 = pm1[2 * PL + lpos], wc_m2 = pm2[2 * PL + lpos];
+
+                const bool odd = (((g.nz + 1) - s) & 1) != 0;
+                const float vx1 = odd ? t1o : t1e;
+                const float vx2 = odd ? t2o : t2e;
+
+                float vxl = 8.0f / (x.lA + x.lB + x.lC + x.lD + lA_m + lB_m + lC_m + lD_m);
+                const float vxm = 8.0f / (x.mA + x.mB + x.mC + x.mD + mA_m + mB_m + mC_m + mD_m);
+                vxl = vxl + 2.0f * vxm;
+                float vqpa = 0.125f * (x.pA + x.pB + x.pC + x.pD + pA_m + pB_m + pC_m + pD_m);
+                float vqsa = 0.125f * (x.sA + x.sB + x.sC + x.sD + sA_m + sB_m + sC_m + sD_m);
+                if constexpr (P::mixed) {
+                    vqpa = vqpa * a.ms.iqp;
+                    vqsa = vqsa * a.ms.iqs;
+                }
+
+                const float vt1 = c1 * (u_ip1 - uc) + c2 * (u_ip2 - u_im1);
+                const float vt2 = c1 * (vc - v_jm1) + c2 * (v_jp1 - v_jm2);
+                const float vt3 = c1 * (wc - wc_m1) + c2 * (wc_p1 - wc_m2);
+                const float vt4 = c1 * (u_jp1 - uc) + c2 * (u_jp2 - u_jm1) +
+                                  c1 * (vc - v_im1) + c2 * (v_ip1 - v_im2);
+                const float vt5 = c1 * (uc_p1 - uc) + c2 * (uc_p2 - uc_m1) +
+                                  c1 * (wc - w_im1) + c2 * (w_ip1 - w_im2);
+                const float vt6 = c1 * (vc_p1 - vc) + c2 * (vc_p2 - vc_m1);
+                const float vt7 = c1 * (w_jp1 - wc) + c2 * (w_jp2 - w_jm1);
+
+                const float vh1 = -vxm * vqsa;
+                const float vtmp = vxl * (vt1 + vt2 + vt3);
+                const float va1 = -vqpa * vxl * dh1 * (vt1 + vt2 + vt3) / 2.0f;
+
+                float r1 = x.r1, r2 = x.r2, r3 = x.r3, r4 = x.r4, r5 = x.r5, r6 = x.r6;
+                float xx = x.xx, yy = x.yy, xy = x.xy;
+                float zz = old.zz, xz = old.xz, yz = old.yz;   // old values of level s
+
+                xx = xx + dth * (vtmp - 2.0f * vxm * (vt2 + vt3)) + dt * r1;
+                yy = yy + dth * (vtmp - 2.0f * vxm * (vt1 + vt3)) + dt * r2;
+                zz = zz + dth * (vtmp - 2.0f * vxm * (vt1 + vt2)) + dt * r3;
+
+                float vxmu = 2.0f / (x.mA + mA_m);
+                float vaas = 0.5f * (x.sA + sA_m);
+                if constexpr (P::mixed) vaas = vaas * a.ms.iqs;
+                const float vh2_r4 = -vaas * vxmu / 2.0f;
+                xy = xy + vxmu * dth * vt4 + dt * r4;
+                vxmu = 2.0f / (x.mA + x.mC);
+                vaas = 0.5f * (x.sA + x.sC);
+                if constexpr (P::mixed) vaas = vaas * a.ms.iqs;
+                const float vh2_r5 = -vaas * vxmu / 2.0f;
+                xz = xz + vxmu * dth * vt5 + dt * r5;
+                vxmu = 2.0f / (x.mA + x.mB);
+                vaas = 0.5f * (x.sA + x.sB);
+                if constexpr (P::mixed) vaas = vaas * a.ms.iqs;
+                const float vh2_r6 = -vaas * vxmu / 2.0f;
+                yz = yz + vxmu * (dth * vt6 + dth * vt7) + dt * r6;
+
+                r1 = (vx2 * r1 - vh1 * (vt2 + vt3) * dh1 + va1) * vx1;
+                r2 = (vx2 * r2 - vh1 * (vt1 + vt3) * dh1 + va1) * vx1;
+                r3 = (vx2 * r3 - vh1 * (vt1 + vt2) * dh1 + va1) * vx1;
+                r4 = (vx2 * r4 + vh2_r4 * dh1 * vt4) * vx1;
+                r5 = (vx2 * r5 + vh2_r5 * dh1 * vt5) * vx1;
+                r6 = (vx2 * r6 + vh2_r6 * dh1 * (vt6 + vt7)) * vx1;
+
+                xx = xx + dt * r1;
+                yy = yy + dt * r2;
+                zz = zz + dt * r3;
+                xy = xy + dt * r4;
+                xz = xz + dt * r5;
+                yz = yz + dt * r6;
+
+                const float D = fxy * x.fz;       // addcrjstr, cerjan.f:249-290
+                xx = xx * D; yy = yy * D; zz = zz * D;
+                xz = xz * D; yz = yz * D; xy = xy * D;
+
+                // memory variables in place (own point only); in mixed storage
+                // they are encoded below, after the row segment's maximum
+                if constexpr (!P::mixed) {
+                    gsn(R1, o, r1); gsn(R2, o, r2); gsn(R3, o, r3);
+                    gsn(R4, o, r4); gsn(R5, o, r5); gsn(R6, o, r6);
+                } else {
+                    rn[0] = r1; rn[1] = r2; rn[2] = r3; rn[3] = r4; rn[4] = r5; rn[5] = r6;
+                }
+                gsn(b.xx, o, xx); gsn(b.yy, o, yy); gsn(b.xy, o, xy);
+                if (s == g.nz) {
+                    // fstr (operator.f:575-612): images above the surface, xz = yz = 0 on it
+                    const unsigned o1 = o + sz4, o2 = o1 + sz4;
+                    gsn(b.zz, o, zz);
+                    gsn(b.zz, o1, -zz);
+                    gsn(b.zz, o2, -zz_f1);
+                    gsn(b.xz, o, 0.0f);
+                    gsn(b.yz, o, 0.0f);
+                    gsn(b.xz, o1, -xz_f1);
+                    gsn(b.xz, o2, -xz_f2);
+                    gsn(b.yz, o1, -yz_f1);
+                    gsn(b.yz, o2, -yz_f2);
+                } else {
+                    gsn(b.zz, o, zz); gsn(b.xz, o, xz); gsn(b.yz, o, yz);
+                }
+                xz_f2 = xz_f1; xz_f1 = xz;
+                yz_f2 = yz_f1; yz_f1 = yz;
+                zz_f1 = zz;
+                lA_m = x.lA; lB_m = x.lB; lC_m = x.lC; lD_m = x.lD;
+                mA_m = x.mA; mB_m = x.mB; mC_m = x.mC; mD_m = x.mD;
+                pA_m = x.pA; pB_m = x.pB; pC_m = x.pC; pD_m = x.pD;
+                sA_m = x.sA; sB_m = x.sB; sC_m = x.sC; sD_m = x.sD;
+            }
+            if constexpr (P::mixed) {
+                // whole waves (one row segment each): lanes outside the domain hold 0
+                if (is_main && s >= kb) {
+                    const unsigned orr = (ob + (unsigned)s * sz4) >> RSH;
+                    G<R> *RR[6] = {R1, R2, R3, R4, R5, R6};
+                    int e[6];
+    #pragma unroll
+                    for (int q = 0; q < 6; ++q) {
+                        e[q] = row_exponent(wave_max(fabsf(rn[q])));
+                        if (do_str) gsn(RR[q], orr, r_encode(rn[q], e[q]));
+                    }
+                    if (do_str && lane == 0)
+                        ((G<unsigned long long> *)a.rexp)[rexp_idx(g, a.nseg, bx, j, s)] = rexp_pack(e);
+                }
+            }
+
+    };
+
+    if constexpr (LV == 1) {
+        for (int m = m0; m <= ke + 2; ++m) {
+            const int s = m - 2;
+            const bool do_str = smain && s >= kb;
+            StrIn sin_ = {};
+            if constexpr (PF >= 1) {
+                if (do_str) load_str(s, sin_);
+            }
+            if constexpr (PF == 1) {
+                if (dom && m <= g.nz) load_vel(m, vin);
+                // keep this level's loads together, ahead of all arithmetic, so
+                // they are in flight at once
+                __builtin_amdgcn_sched_barrier(0);
+            }
+            Old old;
+            vel_level(m, old);
+            __syncthreads();
+
+            if constexpr (PF >= 2) {
+                if (dom && m + 1 <= g.nz) load_vel(m + 1, vin);
+                __builtin_amdgcn_sched_barrier(0);
+            }
+
+            str_level(m, old, sin_);
+        }
+    } else {
+        // two levels per barrier: velocities of m and m+1, one barrier, the
+        // stresses of m-2 and m-1 (reading levels m-4 .. m+1; the 8-level
+        // ring keeps the next pair's writes, m+2 and m+3, off them)
+        for (int m = m0; m <= ke + 2; m += 2) {
+            const bool two = m + 1 <= ke + 2;   // uniform over the block
+            Old old0, old1;
+            vel_level(m, old0);
+            if (two) {
+                // the free-surface images at nz+1 read level nz of every lane
+                if (m + 1 == g.nz + 1) __syncthreads();
+                vel_level(m + 1, old1);
+            }
+            __syncthreads();
+            StrIn sin0 = {}, sin1 = {};
+            str_level(m, old0, sin0);
+            if (two) str_level(m + 1, old1, sin1);
+        }
     }
 }
 
@@ -492,7 +536,7 @@ template <class P, int TY, int PF>
 __global__ void __launch_bounds__(64 * (TY + 5))
 k_step(const Geo g, const State a, const State b, const Coef c, const int kc, const FTiles tl)
 {
-    using Sh = FShape<TY>;
+    using Sh = FShape<TY, PF>;
     __shared__ float lds[Sh::NSLOT * Sh::SLOT];
     int bx, by, bz;
     if (!ftile_of(tl, bx, by, bz)) return;   // uniform over the block
@@ -508,7 +552,7 @@ template <class P, int TY, int PF>
 __global__ void __launch_bounds__(64 * (TY + 5))
 k_step_batch(const FJob *__restrict__ jobs, const int njobs, const int total, const int per, const int par)
 {
-    using Sh = FShape<TY>;
+    using Sh = FShape<TY, PF>;
     __shared__ float lds[Sh::NSLOT * Sh::SLOT];
     const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3, nslot = gridDim.x >> 3;
     for (int k = slot; k < per; k += nslot) {
@@ -586,7 +630,8 @@ static int grid_t(int device)
                 else if (pf_ == 2) { constexpr int TY = 4, PF = 2; CALL; }           \
                 else { constexpr int TY = 4, PF = 1; CALL; }                         \
             } else if ((l).tile_y == 11) {                                           \
-                constexpr int TY = 11, PF = 0; CALL;                                 \
+                if (pf_ == 3) { constexpr int TY = 11, PF = 3; CALL; }               \
+                else { constexpr int TY = 11, PF = 0; CALL; }                        \
             } else {                                                                 \
                 constexpr int TY = 8, PF = 0; CALL;                                  \
             }                                                                        \
@@ -601,7 +646,8 @@ static int grid_t(int device)
                 else if (pf_ == 2) { constexpr int TY = 4, PF = 2; CALL; }           \
                 else { constexpr int TY = 4, PF = 1; CALL; }                         \
             } else if ((l).tile_y == 11) {                                           \
-                constexpr int TY = 11, PF = 0; CALL;                                 \
+                if (pf_ == 3) { constexpr int TY = 11, PF = 3; CALL; }               \
+                else { constexpr int TY = 11, PF = 0; CALL; }                        \
             } else {                                                                 \
                 constexpr int TY = 8, PF = 0; CALL;                                  \
             }                                                                        \

@@ -68,16 +68,19 @@ def test_mixed_media_bitwise(case_dirs):
 
 @pytest.mark.parametrize("scheme", ["split", "fused", "split_o32", "split_o64", "split_ty16",
                                     "split_xw2", "split_xw2_ty4", "split_xw2_o64", "split_o32v",
-                                    "split_o64v", "fused_ty11"])
+                                    "split_o64v", "fused_ty11", "fused_lv2"])
 @pytest.mark.parametrize("name", list(cases.CASES))
 def test_mixed_fields_bitwise_after_steps(case_dirs, name, scheme, monkeypatch):
     d, _ = case_dirs[name]
     p, media, xyz, stf, pts = _inputs(d)
     o = _mixed_oracle(d, media, xyz, stf, pts)
     monkeypatch.setenv("AWP_SCHEME", "fused" if scheme.startswith("fused") else "split")
-    if scheme == "fused_ty11":
+    if scheme in ("fused_ty11", "fused_lv2"):
         # 64 x 11 tiles: 16-wave blocks
         monkeypatch.setenv("AWP_TILE_Y", "11")
+    if scheme in ("fused_ty11", "fused_lv2"):
+        # one level per barrier, or two (the 64 x 11 default)
+        monkeypatch.setenv("AWP_PF", "0" if scheme == "fused_ty11" else "3")
     if scheme in ("split_o32", "split_o64"):
         # both byte-offset widths of the two-stage kernels, whichever the
         # storage mode uses by default
