@@ -900,8 +900,9 @@ extern "C" int awp_tune(awp_ctx *ctx, int reps)
     if (ctx->fused && ctx->vol_b) {
         // Where the second buffer lands in HBM relative to the first sets
         // the fused step's speed (1500x1000x300: 16.1-16.3 or 17.5-17.6 ms
-        // per step, stable for the life of the allocation).  Try up to five
-        // more buffers while device memory allows and keep the fastest.
+        // per step, stable for the life of the allocation; on some boxes one
+        // placement in four is fast).  Try up to nine more buffers while
+        // device memory allows and keep the fastest.
         const size_t nv = (size_t)ctx->g.nvol;
         auto use_b = [ctx, nv](float *vb) {
             ctx->vol_b = vb;
@@ -913,7 +914,7 @@ extern "C" int awp_tune(awp_ctx *ctx, int reps)
         float bestp = 0;
         use_b(ctx->vol_b);
         rc = time_steps(&bestp);
-        for (int t = 0; !rc && t < 5; ++t) {
+        for (int t = 0; !rc && t < 9; ++t) {
             size_t fr = 0, tot = 0;
             if (hipMemGetInfo(&fr, &tot) != hipSuccess || fr < ctx->vol_b_bytes + ((size_t)4 << 30)) break;
             float *nb = nullptr;
