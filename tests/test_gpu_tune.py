@@ -102,3 +102,33 @@ def test_scheme_errors(case_dirs, monkeypatch):
     with pytest.raises(awpsgt.AwpError, match="not stepped"):
         r.tune()
     r.close()
+
+
+@pytest.mark.parametrize("nz", [33, 34])
+def test_fused_pairs_meet_the_free_surface(nz, tmp_path, monkeypatch):
+    """Two levels per barrier on 64 x 11 tiles: with nz = 33 the level pair
+    (nz, nz + 1) falls in one group, so the free-surface images at nz + 1
+    wait for level nz behind an extra barrier; with nz = 34 they do not.
+    Both equal the two-stage step bit for bit, in one z chunk and in three."""
+    import cases
+    spec = cases.tiny_spec("g1_fx_layered")
+    spec.nx, spec.ny, spec.nz = 70, 30, nz
+    spec.npx = spec.npy = spec.npz = 1
+    spec.src = (35, 15, 6)
+    media = awpsgt.inputs.layered_media(70, 30, nz, [700.0, 1500.0, 2500.0])
+    pts = awpsgt.inputs.random_cordfile_points(60, 5, (1, 70), (1, 30), (1, nz))
+    d = str(tmp_path)
+    cases.write_case(d, spec, media, pts)
+    ref = _two_stage(d, awpsgt.AWP_STORAGE_FP32, monkeypatch)
+    monkeypatch.setenv("AWP_SCHEME", "fused")
+    monkeypatch.setenv("AWP_TILE_Y", "11")
+    monkeypatch.setenv("AWP_PF", "3")
+    for kc in ("40", "12"):
+        monkeypatch.setenv("AWP_KC", kc)
+        r = _make(d, awpsgt.AWP_STORAGE_FP32)
+        assert r.scheme() == (True, 11, 1 if kc == "40" else 3)
+        r.step(r.p.nt)
+        r.synchronize()
+        _same(r, ref)
+        r.close()
+    ref.close()
