@@ -290,7 +290,8 @@ extern "C" int awp_batch_create(awp_batch **out, awp_ctx *const *runs, int nruns
     } else {
         b->l.tile_y = (b->l.tile_y == 4 || b->l.tile_y == 7 || b->l.tile_y == 8 || b->l.tile_y == 11) ? b->l.tile_y
                                                                                                      : 8;
-        if (b->l.tile_y == 8 || b->l.tile_y == 11) b->l.unroll = 0;
+        // 64 x 8: no prefetch; 64 x 11: none, with one or two levels per barrier
+        if (b->l.tile_y == 8 || (b->l.tile_y == 11 && b->l.unroll != 3)) b->l.unroll = 0;
         b->l.tile_xw = 1;
     }
     b->nchunk.assign(nruns, 1);
@@ -454,7 +455,7 @@ extern "C" int awp_batch_tune(awp_batch *b, int reps)
         // fused: tile rows x prefetch depth (64x8 and 64x11 tiles only
         // without prefetch: their 13- and 16-wave blocks have 128 registers
         // per lane) x z chunks
-        const int tp[7][2] = {{11, 0}, {8, 0}, {7, 0}, {7, 1}, {7, 2}, {4, 0}, {4, 2}};
+        const int tp[8][2] = {{11, 3}, {11, 0}, {8, 0}, {7, 0}, {7, 1}, {7, 2}, {4, 0}, {4, 2}};
         if (b->can_fuse)
             for (const auto &v : tp)
                 for (int nc : {2, 4}) cands.push_back({false, ps, v[0], v[1], nc, 1, 0, true});

@@ -107,7 +107,7 @@ def test_batch_uneven_grids_and_tuner(case_dirs, tmp_path, scheme, monkeypatch):
         ty, pf, nc = b.tune(1)
         if b.fused():
             assert scheme == "fused"
-            assert ty in (4, 7, 8, 11) and pf in (0, 1, 2) and nc >= 1
+            assert ty in (4, 7, 8, 11) and pf in (0, 1, 2, 3) and nc >= 1
         else:
             assert ty in (4, 8, 16) and pf in (1, 2) and nc >= 1
         assert all(r.progress() == (0, 0) for r in runs)
