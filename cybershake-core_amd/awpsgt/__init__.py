@@ -83,6 +83,32 @@ class AwpError(RuntimeError):
     pass
 
 
+class SgtScan(ctypes.Structure):
+    """awp_sgt_scan_result (include/awpsgt.h)."""
+    _fields_ = [("status", ctypes.c_int), ("offset", ctypes.c_longlong), ("nvalues", ctypes.c_longlong)]
+
+
+SGT_CMP_BINS = 11
+SGT_CMP_CUTOFFS = [0.0001, 0.001, 0.01, 0.1, 0.3, 1, 3, 10, 100, 1000]
+
+
+class SgtCmp(ctypes.Structure):
+    """awp_sgt_cmp (include/awpsgt.h)."""
+    _fields_ = [("nvalues", ctypes.c_longlong), ("avg_diff", ctypes.c_double), ("max_diff", ctypes.c_float),
+                ("max_diff_index", ctypes.c_longlong), ("max_abs_diff", ctypes.c_float),
+                ("pct_count", ctypes.c_longlong), ("avg_pct", ctypes.c_double), ("avg_abs_pct", ctypes.c_double),
+                ("max_pct", ctypes.c_float), ("max_pct_index", ctypes.c_longlong),
+                ("bins", ctypes.c_longlong * SGT_CMP_BINS), ("max_series_norm", ctypes.c_double),
+                ("series_checked", ctypes.c_longlong), ("series_below_floor", ctypes.c_longlong),
+                ("worst_rel_l2", ctypes.c_double), ("worst_series", ctypes.c_longlong),
+                ("worst_abs_below_floor", ctypes.c_double)]
+
+    def as_dict(self) -> dict:
+        d = {k: getattr(self, k) for k, _ in self._fields_ if k != "bins"}
+        d["bins"] = list(self.bins)
+        return d
+
+
 _lib = None
 
 # every symbol include/awpsgt.h declares
@@ -97,6 +123,7 @@ EXPORTS = [
     "awp_batch_synchronize", "awp_batch_time", "awp_batch_get_launch", "awp_batch_get_scheme",
     "awp_batch_last_error", "awp_batch_destroy", "awp_write_sgt_head", "awp_checkpoint_save",
     "awp_checkpoint_load", "awp_tune", "awp_set_scheme", "awp_get_scheme",
+    "awp_get_trace", "awp_sgt_scan", "awp_md5_file", "awp_md5_bytes", "awp_md5_check", "awp_sgt_compare",
 ]
 
 
@@ -159,6 +186,12 @@ def lib() -> ctypes.CDLL:
         "awp_batch_get_scheme": ([vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)], ip),
         "awp_batch_last_error": ([vp], ctypes.c_char_p),
         "awp_batch_destroy": ([vp], ip),
+        "awp_get_trace": ([vp, vp, sz, vp], ip),
+        "awp_sgt_scan": ([cp, ctypes.c_longlong, ctypes.POINTER(SgtScan), cp, sz], ip),
+        "awp_md5_file": ([cp, cp, cp, sz], ip),
+        "awp_md5_bytes": ([vp, sz, cp], ip),
+        "awp_md5_check": ([cp, cp, cp, sz], ip),
+        "awp_sgt_compare": ([cp, cp, ctypes.c_longlong, ip, ctypes.c_double, ctypes.POINTER(SgtCmp), cp, sz], ip),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -244,6 +277,45 @@ def read_sgt16_points(path: str, first: int, count: int, nt_out: int, layout: in
     return out
 
 
+def sgt_scan(path: str, zero_run: int = 24000) -> Tuple[int, int, int]:
+    """NaN / zero-run scan of a raw SGT file (SgtTest check_for_nans):
+    (status 0 clean / 2 NaN / 3 zero run, byte offset or -1, floats read)."""
+    r = SgtScan()
+    err = ctypes.create_string_buffer(512)
+    _check(lib().awp_sgt_scan(path.encode(), int(zero_run), ctypes.byref(r), err, 512), err.value.decode())
+    return r.status, r.offset, r.nvalues
+
+
+def md5_file(path: str) -> str:
+    hexd = ctypes.create_string_buffer(33)
+    err = ctypes.create_string_buffer(512)
+    _check(lib().awp_md5_file(path.encode(), hexd, err, 512), err.value.decode())
+    return hexd.value.decode()
+
+
+def md5_bytes(data: bytes) -> str:
+    hexd = ctypes.create_string_buffer(33)
+    buf = ctypes.create_string_buffer(bytes(data), len(data))
+    _check(lib().awp_md5_bytes(buf, len(data), hexd), "awp_md5_bytes")
+    return hexd.value.decode()
+
+
+def md5_check(path: str, md5path: str) -> bool:
+    """True when the file's MD5 equals the one in md5path (CheckSgt)."""
+    err = ctypes.create_string_buffer(512)
+    return _check(lib().awp_md5_check(path.encode(), md5path.encode(), err, 512), err.value.decode()) == 0
+
+
+def sgt_compare(ref_path: str, test_path: str, npts: int, nt: int, floor: float = 1e-6) -> dict:
+    """compare_sgts statistics and the worst per-series relative L2 error of
+    two raw SGT files of npts x 6 x nt float32 (see awp_sgt_compare)."""
+    c = SgtCmp()
+    err = ctypes.create_string_buffer(512)
+    _check(lib().awp_sgt_compare(ref_path.encode(), test_path.encode(), int(npts), int(nt), float(floor),
+                                 ctypes.byref(c), err, 512), err.value.decode())
+    return c.as_dict()
+
+
 def read_media(path: str, p: In3d) -> np.ndarray:
     n = p.nx * p.ny * p.nz * p.nvar
     return np.fromfile(path, dtype=np.float32, count=n).reshape(p.nz, p.ny, p.nx, p.nvar)
@@ -318,6 +390,14 @@ class Run:
         out = np.empty((self.npts, 6, self.nsamp_max), dtype=np.float32)
         self._do(lib().awp_get_sgt(self.h, out.ctypes.data, out.size), "awp_get_sgt")
         return out
+
+    def trace(self) -> Tuple[np.ndarray, Tuple[int, int, int]]:
+        """The per-step CHKJ trace: ([steps][3] u1 v1 w1, traced node i j k)."""
+        ijk = np.zeros(3, dtype=np.int32)
+        n = self.progress()[0]
+        out = np.empty((n, 3), dtype=np.float32)
+        self._do(lib().awp_get_trace(self.h, out.ctypes.data, out.size, ijk.ctypes.data), "awp_get_trace")
+        return out, (int(ijk[0]), int(ijk[1]), int(ijk[2]))
 
     def write_sgt_file(self, path: str) -> None:
         self._do(lib().awp_write_sgt_file(self.h, path.encode()), "awp_write_sgt_file")

@@ -29,7 +29,7 @@ HIP_FLAGS = [
 C_FLAGS = ["-std=c11", "-O2", "-fPIC", "-Wall", "-Wextra", "-Wno-unused-parameter", "-I" + INCLUDE]
 
 HIP_SOURCES = ["csrc/awp_kernels.hip", "csrc/awp_fused.hip", "csrc/awp_ctx.hip", "csrc/awp_batch.hip"]
-C_SOURCES = ["host/awp_io.c", "host/awp_head.c"]
+C_SOURCES = ["host/awp_io.c", "host/awp_head.c", "host/awp_sgtcheck.c"]
 
 
 def _run(cmd, quiet=True):
@@ -68,6 +68,11 @@ def build(verbose: bool = False) -> str:
                                   "-L" + libdir, "-lawpsgt", "-Wl,-rpath,$ORIGIN/../lib", "-lm"],
              quiet=not verbose)
         shutil.move(exe_tmp, os.path.join(bindir, "awp_pmcl3d"))
+        tool_tmp = os.path.join(tmp, "awp_sgt")
+        _run(["gcc"] + C_FLAGS + [os.path.join(PKG, "host", "awp_sgt.c"), "-o", tool_tmp,
+                                  "-L" + libdir, "-lawpsgt", "-Wl,-rpath,$ORIGIN/../lib", "-lm"],
+             quiet=not verbose)
+        shutil.move(tool_tmp, os.path.join(bindir, "awp_sgt"))
         return so
     finally:
         shutil.rmtree(tmp, ignore_errors=True)

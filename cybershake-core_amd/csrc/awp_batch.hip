@@ -152,6 +152,8 @@ static int upload_tables(awp_batch *b)
         x.sg = c->d_sg;
         x.sgt = c->d_sgt;
         x.counter = b->d_counter + r;
+        x.trace = c->d_trace;
+        x.tpos = c->trace_pos;
         b->max_kept = std::max(b->max_kept, c->nkept);
     }
     b->total = tile0;
@@ -271,6 +273,11 @@ extern "C" int awp_batch_create(awp_batch **out, awp_ctx *const *runs, int nruns
         all_fused = all_fused && runs[r]->fused;
     }
     b->split = !(b->can_fuse && all_fused);
+    // AWP_BATCH_SCHEME=fused / split overrides (fused only where every run
+    // holds both buffers; each run's own step is unchanged)
+    const char *bs = getenv("AWP_BATCH_SCHEME");
+    if (bs && !strcmp(bs, "fused") && b->can_fuse) b->split = false;
+    if (bs && !strcmp(bs, "split")) b->split = true;
     b->persist = true;
     b->o32 = true;
     for (int r = 0; r < nruns; ++r) b->o32 = b->o32 && runs[r]->l.o32;

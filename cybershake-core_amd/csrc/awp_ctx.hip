@@ -241,6 +241,8 @@ extern "C" int awp_create(awp_ctx **out, const awp_in3d *p, int device, int stor
     ctx->vol_b = nullptr;
     ctx->vol_b_bytes = 0;
     ctx->sg_done = false;
+    ctx->d_trace = nullptr;
+    ctx->trace_pos = 0;
 
     e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
@@ -371,6 +373,20 @@ extern "C" int awp_create(awp_ctx **out, const awp_in3d *p, int device, int stor
     }
     ctx->vtst = dt / (dh * dh * dh);
     ctx->vtst1 = 1.0f / (dh * dh);
+    // the reference's master rank traces its local node (ND+2, ND+2, ND+2),
+    // global (ND+2, ND+2, ND+2) counted from the bottom (pmcl3d.f:1476-1479;
+    // rank 0 holds the first block of every axis); clamped into the grid
+    ctx->trace_ijk[0] = std::min(p->nd + 2, p->nx);
+    ctx->trace_ijk[1] = std::min(p->nd + 2, p->ny);
+    ctx->trace_ijk[2] = std::min(p->nd + 2, p->nz);
+    ctx->trace_pos = gidx(ctx->g, ctx->trace_ijk[0], ctx->trace_ijk[1], ctx->trace_ijk[2]);
+    e = hipMalloc((void **)&ctx->d_trace, (size_t)std::max(p->nt, 1) * 3 * sizeof(float));
+    if (e != hipSuccess) {
+        snprintf(err, errlen, "hipMalloc trace: %s", hipGetErrorString(e));
+        awp_destroy(ctx);
+        return AWP_ENOMEM;
+    }
+    ctx->bytes += (size_t)std::max(p->nt, 1) * 3 * sizeof(float);
     *out = ctx;
     return AWP_OK;
 }
@@ -384,6 +400,7 @@ extern "C" int awp_destroy(awp_ctx *ctx)
     hipFree(ctx->vol);
     hipFree(ctx->vol_b);
     hipFree(ctx->prof);
+    hipFree(ctx->d_trace);
     hipFree(ctx->d_sjob);
     hipFree(ctx->d_psrc);
     hipFree(ctx->d_ptop);
@@ -684,7 +701,7 @@ extern "C" int awp_init(awp_ctx *ctx)
     }
     // first source sample before the loop (pmcl3d.f:988)
     HIPTRY(ctx, launch_source(ctx->g, ctx->view(0), ctx->p.igreen, ctx->npsrc, ctx->d_psrc, ctx->d_ptop,
-                              ctx->d_stf, ctx->p.nst, 1, ctx->vtst, ctx->vtst1, ctx->stream));
+                              ctx->d_stf, ctx->p.nst, 1, ctx->vtst, ctx->vtst1, true, nullptr, 0, ctx->stream));
     HIPTRY(ctx, hipStreamSynchronize(ctx->stream));
     ctx->step = 0;
     ctx->nsamp = 0;
@@ -715,6 +732,13 @@ extern "C" int awp_step(awp_ctx *ctx, int nsteps)
         if (rc) return rc;
     }
     hipStream_t st = ctx->stream;
+    if (!ctx->fused && ctx->cur != 0 && todo > 0) {
+        // a fused batch stepped this two-stage run and left its state in
+        // buffer 1: the two-stage step works in buffer 0 (its job table
+        // points there)
+        int rc = scheme_changed(ctx);
+        if (rc) return rc;
+    }
     for (int n = 0; n < todo; ++n) {
         const int i = ++ctx->step;
         hipEvent_t *e = ctx->timing ? &ctx->ev[(size_t)6 * n] : nullptr;
@@ -740,9 +764,9 @@ extern "C" int awp_step(awp_ctx *ctx, int nsteps)
             HIPTRY(ctx, launch_split_stress(mx, ctx->l, ctx->l.o32, ctx->d_sjob, 1, ctx->split_total, all, st));
         }
         if (e) HIPTRY(ctx, hipEventRecord(e[3], st));
-        if (i + 1 <= p.nst)
-            HIPTRY(ctx, launch_source(ctx->g, sn, p.igreen, ctx->npsrc, ctx->d_psrc, ctx->d_ptop, ctx->d_stf,
-                                      p.nst, i + 1, ctx->vtst, ctx->vtst1, st));
+        // source sample i + 1 (while the source lasts), then the trace row
+        HIPTRY(ctx, launch_source(ctx->g, sn, p.igreen, ctx->npsrc, ctx->d_psrc, ctx->d_ptop, ctx->d_stf, p.nst,
+                                  i + 1, ctx->vtst, ctx->vtst1, i + 1 <= p.nst, ctx->d_trace, ctx->trace_pos, st));
         if (i % p.ntiskp_sgt == 0 && ctx->nsamp < ctx->nsamp_max) {
             HIPTRY(ctx, launch_sgt_gather(sn, ctx->nkept, ctx->d_kpos, ctx->d_kidx, ctx->d_sg, ctx->d_sgt,
                                           ctx->nsamp_max, ctx->nsamp, st));
@@ -1043,6 +1067,21 @@ extern "C" int awp_get_sgt(awp_ctx *ctx, float *host, size_t count)
     return AWP_OK;
 }
 
+extern "C" int awp_get_trace(awp_ctx *ctx, float *host, size_t count, int *ijk)
+{
+    if (!ctx) return AWP_EINVAL;
+    if (ijk) {
+        for (int a = 0; a < 3; ++a) ijk[a] = ctx->trace_ijk[a];
+    }
+    if (!host) return AWP_OK;
+    const size_t n = (size_t)ctx->step * 3;
+    if (count < n) return fail(ctx, AWP_EINVAL, "buffer holds %zu floats, need %zu", count, n);
+    HIPTRY(ctx, hipSetDevice(ctx->device));
+    HIPTRY(ctx, hipStreamSynchronize(ctx->stream));
+    if (n) HIPTRY(ctx, hipMemcpy(host, ctx->d_trace, n * sizeof(float), hipMemcpyDeviceToHost));
+    return AWP_OK;
+}
+
 extern "C" int awp_write_sgt_file(awp_ctx *ctx, const char *path)
 {
     if (!ctx || !path) return AWP_EINVAL;
@@ -1226,14 +1265,14 @@ extern "C" int awp_get_field(awp_ctx *ctx, int which, int kext, float *host)
 // checkpoint / restart (the reference's restart files: pmcl3d.f:1016-1097,
 // 1964-2045).  File: a 72-byte header, then the state part of the volume
 // allocation (wavefield buffer 0, memory variables, their exponents), the
-// second wavefield buffer where the run holds one, and the SGT sample
-// buffer, byte for byte as they sit in HBM, streamed through a pinned
-// staging buffer.
+// second wavefield buffer where the run holds one, the SGT sample buffer
+// and the per-step trace, byte for byte as they sit in HBM, streamed
+// through a pinned staging buffer.
 // ---------------------------------------------------------------------------
 namespace {
 
 struct CkpHead {
-    char magic[8];                  // "AWPCKP02"
+    char magic[8];                  // "AWPCKP03"
     int32_t nx, ny, nz, storage;
     int32_t fused, cur, step, nsamp;
     int32_t npts, nsamp_max, nt, reserved;
@@ -1250,7 +1289,7 @@ CkpHead ckp_head_of(const awp_ctx *ctx)
 {
     CkpHead h;
     memset(&h, 0, sizeof h);
-    memcpy(h.magic, "AWPCKP02", 8);
+    memcpy(h.magic, "AWPCKP03", 8);
     h.nx = ctx->p.nx;
     h.ny = ctx->p.ny;
     h.nz = ctx->p.nz;
@@ -1293,6 +1332,9 @@ int ckp_run(awp_ctx *ctx, FILE *f, bool save)
     int rc = ckp_stream(ctx, f, (char *)ctx->vol, ctx->state_bytes, stage, chunk, save);
     if (!rc && ctx->vol_b) rc = ckp_stream(ctx, f, (char *)ctx->vol_b, ctx->vol_b_bytes, stage, chunk, save);
     if (!rc) rc = ckp_stream(ctx, f, (char *)ctx->d_sgt, sgt_buffer_bytes(ctx), stage, chunk, save);
+    if (!rc)
+        rc = ckp_stream(ctx, f, (char *)ctx->d_trace, (size_t)std::max(ctx->p.nt, 1) * 3 * sizeof(float), stage,
+                        chunk, save);
     hipHostFree(stage);
     return rc;
 }

@@ -43,6 +43,11 @@ struct awp_ctx {
     awp::SJob *d_sjob;
     int split_total, split_fs;
     float *prof;             // fx | fy | fz
+    // the per-step job-value trace (CHKJ, pmcl3d.f:1476-1479): u1 v1 w1 at
+    // node trace_ijk after every step, [nt][3]
+    float *d_trace;
+    long long trace_pos;
+    int trace_ijk[3];
     // sources
     int npsrc;
     long long *d_psrc;       // [npsrc] node index

@@ -762,12 +762,26 @@ __device__ void apply_sources(const State &s, const int igreen, const int npsrc,
     }
 }
 
+// The job-value trace the reference's master rank writes to CHKJ after
+// every step (pmcl3d.f:1476-1479): u1, v1, w1 at node tpos, as row `row`
+// of trace[][3].
+__device__ inline void record_trace(const State &s, float *trace, const long long tpos, const int row)
+{
+    trace[3 * row + 0] = s.u1[tpos];
+    trace[3 * row + 1] = s.v1[tpos];
+    trace[3 * row + 2] = s.w1[tpos];
+}
+
+// Source sample `istep` (when `apply`), then the trace row of the step just
+// completed (istep - 2) when `trace` is set.
 __global__ void k_source(const State s, const int igreen, const int npsrc, const long long *psrc_idx,
                          const long long *psrc_top, const long long sz, const float *stf, const int nst,
-                         const int istep, const float vtst, const float vtst1)
+                         const int istep, const float vtst, const float vtst1, const int apply, float *trace,
+                         const long long tpos)
 {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    apply_sources(s, igreen, npsrc, psrc_idx, psrc_top, sz, stf, nst, istep, vtst, vtst1);
+    if (apply) apply_sources(s, igreen, npsrc, psrc_idx, psrc_top, sz, stf, nst, istep, vtst, vtst1);
+    if (trace) record_trace(s, trace, tpos, istep - 2);
 }
 
 // ---------------------------------------------------------------------------
@@ -810,16 +824,29 @@ __global__ void k_sgt_gather(const State s, const int nkept, const long long *kp
 }
 
 // The source and SGT work of one step for every run of a batch (the step's
-// number comes from the run's device counter, so the launch is the same
-// every step and can be replayed from a graph): block (x, r) handles run r,
-// block x = 0 thread 0 its sources, all blocks its SGT points.
+// number comes from the run's device counter, so the launches are the same
+// every step and can be replayed from a graph).  The sources come first, in
+// a launch of their own (thread 0 of block r: run r's sources), because the
+// surface sources (igreen 4..6) write stresses at k = nz that an SGT point
+// on the surface reads: the reference applies the source sample, then takes
+// the SGT sample (pmcl3d.f:1381-1383, 1536-1557).  Then block (x, r) of the
+// gather launch handles run r's SGT points.
+__global__ void __launch_bounds__(64) k_src_batch(const AuxJob *__restrict__ jobs, const int par)
+{
+    const AuxJob &J = jobs[blockIdx.x];
+    const int i = *J.counter + 1;    // this step
+    if (threadIdx.x != 0) return;
+    if (i + 1 <= J.nst)
+        apply_sources(J.s[par ^ 1], J.igreen, J.npsrc, J.psrc, J.ptop, J.g.sz, J.stf, J.nst, i + 1, J.vtst,
+                      J.vtst1);
+    if (J.trace) record_trace(J.s[par ^ 1], J.trace, J.tpos, i - 1);
+}
+
 __global__ void __launch_bounds__(256) k_aux_batch(const AuxJob *__restrict__ jobs, const int par)
 {
     const AuxJob &J = jobs[blockIdx.y];
     const int i = *J.counter + 1;    // this step
     const State &s = J.s[par ^ 1];   // the state after it
-    if (blockIdx.x == 0 && threadIdx.x == 0 && i + 1 <= J.nst)
-        apply_sources(s, J.igreen, J.npsrc, J.psrc, J.ptop, J.g.sz, J.stf, J.nst, i + 1, J.vtst, J.vtst1);
     if (i % J.ntiskp == 0) {
         const int sample = i / J.ntiskp - 1;
         if (sample < J.nsamp_max)
@@ -1074,10 +1101,10 @@ k_pack_field(const Geo g, const T *src, const float inv_scale, const int kext, f
 // ---------------------------------------------------------------------------
 hipError_t launch_source(const Geo &g, const State &s, int igreen, int npsrc, const long long *psrc_idx,
                          const long long *psrc_top, const float *stf, int nst, int istep, float vtst,
-                         float vtst1, hipStream_t st)
+                         float vtst1, bool apply, float *trace, long long tpos, hipStream_t st)
 {
     hipLaunchKernelGGL(k_source, dim3(1), dim3(64), 0, st, s, igreen, npsrc, psrc_idx, psrc_top, g.sz, stf, nst,
-                       istep, vtst, vtst1);
+                       istep, vtst, vtst1, apply ? 1 : 0, trace, tpos);
     return hipGetLastError();
 }
 
@@ -1240,8 +1267,11 @@ hipError_t launch_aux_batch(const AuxJob *d_jobs, int njobs, int max_kept, int p
 {
     if (njobs <= 0) return hipSuccess;
     const int gx = std::max(1, std::min((max_kept + 255) / 256, 1024));
-    hipLaunchKernelGGL(k_aux_batch, dim3(gx, njobs), dim3(256), 0, st, d_jobs, par);
+    hipLaunchKernelGGL(k_src_batch, dim3(njobs), dim3(64), 0, st, d_jobs, par);
     hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_aux_batch, dim3(gx, njobs), dim3(256), 0, st, d_jobs, par);
+    e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_count_batch, dim3((njobs + 63) / 64), dim3(64), 0, st, d_jobs, njobs);
     return hipGetLastError();

@@ -380,9 +380,11 @@ struct MediaConst {
 };
 
 // host launchers (awp_kernels.hip); `mixed` selects the storage mode
+// source sample istep (when apply); with trace set, then row istep - 2 of
+// the per-step trace (u1 v1 w1 at element tpos)
 hipError_t launch_source(const Geo &g, const State &s, int igreen, int npsrc, const long long *psrc_idx,
                          const long long *psrc_top, const float *stf, int nst, int istep, float vtst,
-                         float vtst1, hipStream_t st);
+                         float vtst1, bool apply, float *trace, long long tpos, hipStream_t st);
 hipError_t launch_sgt_coef(const Media32 &m, int nkept, const long long *kpos, float *sg, hipStream_t st);
 hipError_t launch_sgt_gather(const State &s, int nkept, const long long *kpos, const int *kidx,
                              const float *sg, float *out, int nsamp_max, int sample, hipStream_t st);
@@ -486,6 +488,8 @@ struct AuxJob {
     const float *sg;
     float *sgt;
     int *counter;               // steps completed before the current one
+    float *trace;               // [nt][3] u1 v1 w1 at tpos after each step
+    long long tpos;
 };
 // the source and SGT work of one step for every run, then the counters
 hipError_t launch_aux_batch(const AuxJob *d_jobs, int njobs, int max_kept, int par, hipStream_t st);

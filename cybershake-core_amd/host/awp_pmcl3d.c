@@ -33,6 +33,7 @@
  */
 #define _POSIX_C_SOURCE 200809L
 #include <errno.h>
+#include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -168,9 +169,69 @@ done:
     return rc;
 }
 
+/* A value in Fortran's E10.3 edit descriptor: 0.ddd mantissa, two-digit
+ * exponent ("-0.542E-42", " 0.000E+00"); a three-digit exponent drops the E. */
+static void fort_e10_3(float v, char out[16])
+{
+    if (isnan(v)) {
+        snprintf(out, 16, "%10s", "NaN");
+        return;
+    }
+    if (isinf(v)) {
+        snprintf(out, 16, "%10s", v < 0 ? "-Infinity" : "Infinity");
+        return;
+    }
+    if (v == 0.0f) {
+        snprintf(out, 16, "%10s", signbit(v) ? "-0.000E+00" : "0.000E+00");
+        return;
+    }
+    /* three significant digits d.dd x 10^e  ->  0.ddd x 10^(e+1) */
+    char t[32];
+    snprintf(t, sizeof t, "%.2e", fabs((double)v));
+    const int e = atoi(strchr(t, 'e') + 1) + 1;
+    const char sign = v < 0 ? '-' : ' ';
+    char body[24];
+    if (e >= -99 && e <= 99)
+        snprintf(body, sizeof body, "0.%c%c%cE%c%02d", t[0], t[2], t[3], e < 0 ? '-' : '+', e < 0 ? -e : e);
+    else
+        snprintf(body, sizeof body, "0.%c%c%c%c%03d", t[0], t[2], t[3], e < 0 ? '-' : '+', e < 0 ? -e : e);
+    snprintf(out, 16, "%c%s", sign, body);
+}
+
+/* The CHKJ trace (pmcl3d.f:1476-1479: write(24,'(I6,3E10.3)') i, u1, v1, w1
+ * at the master rank's node (ND+2, ND+2, ND+2) after every step). */
+static int write_chkj(job_t *j)
+{
+    if (!j->p.chkj[0]) return 0;
+    int steps = 0;
+    if (awp_progress(j->ctx, &steps, NULL)) return die("CHKJ", awp_last_error(j->ctx));
+    float *tr = (float *)malloc(sizeof(float) * 3 * (size_t)(steps > 0 ? steps : 1));
+    if (!tr) return die("CHKJ", "out of host memory");
+    if (awp_get_trace(j->ctx, tr, (size_t)3 * steps, NULL)) {
+        free(tr);
+        return die("CHKJ", awp_last_error(j->ctx));
+    }
+    FILE *f = fopen(j->p.chkj, "w");
+    if (!f) {
+        free(tr);
+        return die("CHKJ", "cannot create the file");
+    }
+    for (int i = 0; i < steps; ++i) {
+        char a[16], b[16], c[16];
+        fort_e10_3(tr[3 * i], a);
+        fort_e10_3(tr[3 * i + 1], b);
+        fort_e10_3(tr[3 * i + 2], c);
+        fprintf(f, "%6d%s%s%s\n", i + 1, a, b, c);
+    }
+    free(tr);
+    if (fclose(f)) return die("CHKJ", "write failed");
+    return 0;
+}
+
 static int write_outputs(job_t *j, const char *fp16_path, const char *rwg_path)
 {
     if (awp_write_sgt_file(j->ctx, j->p.sgtgro)) return die("SGT output", awp_last_error(j->ctx));
+    if (write_chkj(j)) return 1;
     if (fp16_path && fp16_path[0] && awp_write_sgt_file_fp16(j->ctx, fp16_path))
         return die("FP16 SGT output", awp_last_error(j->ctx));
     if (rwg_path && rwg_path[0] && awp_write_sgt_file_rwg(j->ctx, rwg_path, j->p.igreen == 3))

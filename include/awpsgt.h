@@ -112,6 +112,58 @@ int awp_sgt_select(const awp_in3d *p, int npts, const int *xyz, int *keep, int *
  * (point, component) slot and the number of WRITE_STEP buffers flushed. */
 int awp_sgt_file_geometry(const awp_in3d *p, int *nt_out, int *nflush);
 
+/* ---- acceptance checks of a raw SGT file (host only) --------------------- */
+
+/* NaN / zero-run scan of a float32 file (SgtTest/src/check_for_nans.c:19-56,
+ * which a CyberShake SGT must pass): status 0 clean, 2 a NaN at byte
+ * `offset`, 3 `zero_run` consecutive values with |v| < 1e-20 ending at byte
+ * `offset` (the reference uses 24000); nvalues = floats read. */
+typedef struct {
+    int status;
+    long long offset;
+    long long nvalues;
+} awp_sgt_scan_result;
+int awp_sgt_scan(const char *path, long long zero_run, awp_sgt_scan_result *out, char *err, size_t errlen);
+
+/* MD5 (RFC 1321) of a file as 32 lower-case hex digits, and the check
+ * CheckSgt/CheckSgt.py:44-78 makes: the file's digest against the first
+ * word of the first line of an md5 file (0 equal, 1 different, < 0 error). */
+int awp_md5_file(const char *path, char hex[33], char *err, size_t errlen);
+int awp_md5_bytes(const void *data, size_t n, char hex[33]);
+int awp_md5_check(const char *path, const char *md5path, char *err, size_t errlen);
+
+/* Comparison of two raw SGT files of npts points x 6 components x nt
+ * samples (AWP-HIP-SGT/utils/compare_sgts.c:7-119): its statistics -- mean
+ * difference (test - ref), largest signed difference and its float index,
+ * percent differences over the values whose reference value is > 1e-10
+ * (mean, mean absolute, largest absolute and its index, histogram over
+ * |pct| < 1e-4, 1e-3, 1e-2, 0.1, 0.3, 1, 3, 10, 100, 1000, >= 1000) -- and
+ * the relative L2 error of every (point, component) time series whose
+ * reference norm is at least floor_frac x the largest series norm: the
+ * worst of them and its series index (point * 6 + component), the counts
+ * checked and below the floor, and the largest absolute L2 difference
+ * among the series below the floor. */
+#define AWP_SGT_CMP_BINS 11
+typedef struct {
+    long long nvalues;
+    double avg_diff;
+    float max_diff;            /* largest test - ref (signed, as the reference tool) */
+    long long max_diff_index;
+    float max_abs_diff;
+    long long pct_count;
+    double avg_pct, avg_abs_pct;
+    float max_pct;
+    long long max_pct_index;
+    long long bins[AWP_SGT_CMP_BINS];
+    double max_series_norm;
+    long long series_checked, series_below_floor;
+    double worst_rel_l2;
+    long long worst_series;
+    double worst_abs_below_floor;
+} awp_sgt_cmp;
+int awp_sgt_compare(const char *ref_path, const char *test_path, long long npts, int nt, double floor_frac,
+                    awp_sgt_cmp *out, char *err, size_t errlen);
+
 /* ---------------------------------------------------------------------- */
 /* Device run (HIP)                                                        */
 /* ---------------------------------------------------------------------- */
@@ -185,6 +237,14 @@ int awp_progress(const awp_ctx *ctx, int *step, int *nsamples);
  * nt / NTISKP_SGT; slots of points the rule drops and samples not yet taken
  * are -1.  (Strains exx eyy ezz exy exz eyz, pmcl3d.f:1546-1557.) */
 int awp_get_sgt(awp_ctx *ctx, float *host, size_t count);
+
+/* The job-value trace the reference's master rank writes to the CHKJ file
+ * after every step (pmcl3d.f:1476-1479, format (I6,3E10.3)): u1, v1, w1 at
+ * the node (ND+2, ND+2, ND+2) counted from the bottom (the reference's rank 0
+ * holds the first block of every axis; clamped into the grid), [step][3]
+ * float32 for the steps completed so far.  ijk (may be NULL) receives the
+ * node; host may be NULL to query it only. */
+int awp_get_trace(awp_ctx *ctx, float *host, size_t count, int *ijk);
 
 /* Write the raw SGT file exactly as the reference does (pmcl3d.f:1582-1601):
  * only full WRITE_STEP buffers, each at byte offset

@@ -7,6 +7,7 @@ with MPICH's mpiexec, and stores what it wrote:
   <case>.npz   sgt       raw SGT file of the run as float32 (whole file)
                nbytes    file size in bytes
                inputs_sha256  SHA-256 of IN3D+media+source+cordfile
+               chkj      the CHKJ text file (per-step u1 v1 w1 trace), bytes
   c1_subset.npz  every 10th cordfile point of the 200x200x100 / 1500-step
                run, the SHA-256 of the whole SGT file, per-component L2
                norms over the whole file, and the solver's own timing lines.
@@ -57,9 +58,11 @@ def do_case(name: str) -> None:
         sha = cases.write_case(d, spec, media, pts)
         run_reference(d, spec.npx * spec.npy * spec.npz, 600)
         raw = open(os.path.join(d, "sgt"), "rb").read()
+        chkj = open(os.path.join(d, "CHKJ"), "rb").read()
         np.savez_compressed(os.path.join(HERE, name + ".npz"),
                             sgt=np.frombuffer(raw, dtype=np.float32),
-                            nbytes=np.int64(len(raw)), inputs_sha256=np.array(sha))
+                            nbytes=np.int64(len(raw)), inputs_sha256=np.array(sha),
+                            chkj=np.frombuffer(chkj, dtype=np.uint8))
         print(name, "bytes", len(raw), "sha", sha[:12])
     finally:
         shutil.rmtree(d, ignore_errors=True)

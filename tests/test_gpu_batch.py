@@ -183,3 +183,67 @@ def test_host_batch_mode_writes_each_runs_file(case_dirs, tmp_path):
         ref.close()
         assert open(os.path.join(run, "r%d" % k, "sgt"), "rb").read() == open(want, "rb").read(), name
         assert open(os.path.join(run, "rwg.%d" % k), "rb").read() == open(want + ".rwg", "rb").read(), name
+
+
+def test_two_stage_run_after_fused_batch(case_dirs, monkeypatch):
+    """A run created with both wavefield buffers but switched to the
+    two-stage step, advanced an odd number of steps by a fused batch (which
+    leaves its state in buffer 1), then stepped on its own: the two-stage
+    step must continue from buffer 1's state, bit for bit as a two-stage run
+    stepped alone."""
+    d = case_dirs["g1_fx_layered"][0]
+    monkeypatch.setenv("AWP_SCHEME", "fused")
+    r = _make(d, awpsgt.AWP_STORAGE_FP32)
+    monkeypatch.delenv("AWP_SCHEME", raising=False)
+    r.set_scheme(False)
+    assert r.scheme()[0] is False
+    monkeypatch.setenv("AWP_BATCH_SCHEME", "fused")
+    with awpsgt.Batch([r]) as b:
+        assert b.fused()
+        b.step(7)
+    monkeypatch.delenv("AWP_BATCH_SCHEME", raising=False)
+    assert r.progress()[0] == 7
+    r.step(r.p.nt)
+    r.synchronize()
+    monkeypatch.setenv("AWP_SCHEME", "split")
+    ref = _single(d, awpsgt.AWP_STORAGE_FP32)
+    monkeypatch.delenv("AWP_SCHEME", raising=False)
+    assert np.array_equal(r.sgt(), ref.sgt())
+    for f in ("u1", "v1", "w1", "xx", "zz", "xz", "yz", "r1", "r6"):
+        k = 2 if f in ("u1", "w1", "xz") else 0
+        assert np.array_equal(r.field(f, k), ref.field(f, k)), f
+    ref.close()
+    r.close()
+
+
+@pytest.mark.parametrize("scheme", ["split", "fused"])
+@pytest.mark.parametrize("name", ["g4_surface_xz", "g5_surface_yz", "g6_surface_zz"])
+def test_batch_surface_source_sampled_at_source_cell(case_dirs, name, scheme, monkeypatch, tmp_path):
+    """Surface sources (igreen 4..6) write stresses at k = nz; SGT points on
+    the source's column at the surface (and just below) read them.  The batch
+    applies the source sample before taking the SGT sample, as the single run
+    and the reference do (pmcl3d.f:1381-1383): outputs equal the single run's
+    bit for bit, with several runs in the batch."""
+    spec, media, pts = cases.tiny_inputs(name)
+    sx, sy, _ = spec.src
+    extra = np.array([[sx, sy, 1], [sx, sy, 2], [sx + 1, sy, 1], [sx, sy + 1, 1]], dtype=np.int64)
+    pts = np.concatenate([extra, pts])
+    d = str(tmp_path / name)
+    cases.write_case(d, spec, media, pts)
+    dirs = [d, case_dirs["g1_fx_layered"][0], case_dirs[name][0]]
+    monkeypatch.setenv("AWP_SCHEME", scheme)
+    runs = [_make(x, awpsgt.AWP_STORAGE_FP32) for x in dirs]
+    with awpsgt.Batch(runs) as b:
+        assert b.fused() == (scheme == "fused")
+        b.step(runs[0].p.nt)
+    for x, r in zip(dirs, runs):
+        ref = _single(x, awpsgt.AWP_STORAGE_FP32)
+        assert np.array_equal(r.sgt(), ref.sgt()), x
+        ref.close()
+        r.close()
+    # the surface points really see the source
+    monkeypatch.setenv("AWP_SCHEME", "split")
+    ref = _single(d, awpsgt.AWP_STORAGE_FP32)
+    s = ref.sgt()
+    assert np.any(s[0, :, :] != 0) and np.all(np.isfinite(s[0]))
+    ref.close()
