@@ -54,69 +54,20 @@ BYTES_UPDATE = {"fp32": 176, "mixed": 148}
 BYTES_STR = {"fp32": 124, "mixed": 96}     # the two-stage stress kernel alone
 BYTES_FUSED = {"fp32": 140, "mixed": 116}  # what the fused sweep must move
 
-CONFIGS = {
-    "c2": dict(nx=1500, ny=1000, nz=300, dh=100.0, dt=0.005, nsgt=1_500_000, src=(750, 500, 10),
-               workload="single-GPU site shape 1500x1000x300 (450M points), h=100 m, dt=0.005 s, "
-                        "von Karman Vs 500->4500 m/s, Qs=50 Vs_km, X force, 1.5M SGT points every 10 steps, FP32"),
-    "c1": dict(nx=200, ny=200, nz=100, dh=200.0, dt=0.01, nsgt=2000, src=(100, 100, 10),
-               workload="reference CPU case 200x200x100, h=200 m, dt=0.01 s, layered medium, "
-                        "X force, 2000 SGT points every 10 steps, FP32"),
-    "c3": dict(nsites=4, lo=(400, 300, 150), hi=(900, 700, 220), seed0=10, dh=100.0, dt=0.005, nsgt=200_000,
-               workload="batched: 8 runs (4 sites x X/Y force), grids 400x300x150 .. 900x700x220, h=100 m, "
-                        "dt=0.005 s, von Karman media per site, 200k SGT points per run, FP32"),
-    "c5": dict(nsites=8, lo=(500, 400, 150), hi=(1200, 900, 280), seed0=20, dh=100.0, dt=0.005, nsgt=200_000,
-               workload="batched queue: 16 runs (8 sites x X/Y force), grids 500x400x150 .. 1200x900x280, "
-                        "h=100 m, dt=0.005 s, admitted by device-memory budget, 200k SGT points per run, FP32"),
-}
-
-
-def build_params(cfg, nt, nx=None, ny=None, nz=None, igreen=1):
-    import awpsgt
-    p = awpsgt.In3d()
-    p.igreen = igreen
-    p.dh, p.dt = cfg["dh"], cfg["dt"]
-    p.tmax = (nt - 1) * cfg["dt"]
-    p.npc, p.nd, p.arbc, p.pht = 0, 20, 0.92, 0.1
-    p.nsrc, p.nst = 1, nt
-    p.nx, p.ny, p.nz = nx or cfg["nx"], ny or cfg["ny"], nz or cfg["nz"]
-    p.npx = p.npy = p.npz = 1
-    p.ifault, p.mediarestart, p.nvar, p.socalq, p.nve = 0, 1, 5, 1, 1
-    p.fl, p.fh, p.fp = 0.01, 25.0, 0.5
-    p.read_step, p.ntiskp_sgt = nt, 10
-    p.write_step = max(1, nt // 10)
-    p.nt = nt
-    awpsgt.validate_in3d(p)
-    return p
+from awpsgt.workloads import CONFIGS, build_params, make_run  # noqa: E402
 
 
 def log(msg):
     print("[bench] " + msg, file=sys.stderr, flush=True)
 
-
-def make_run(p, storage, media_fn, src, nsgt, seed_pts):
-    """Run with device media (media_fn() -> torch tensor), a Gaussian force
-    at src and nsgt random SGT points; initialised."""
-    import numpy as np
-    import torch
-    import awpsgt
-    from awpsgt.inputs import gaussian_stf, random_cordfile_points
-    run = awpsgt.Run(p, device=0, storage=storage)
-    pts = random_cordfile_points(nsgt, seed_pts, (1, p.nx), (1, p.ny), (1, p.nz))
-    nkept = run.set_sgt_points(pts.astype(np.int32))
-    media = media_fn()
-    if isinstance(media, np.ndarray):
-        run.set_media(media)
-    else:
-        torch.cuda.synchronize()
-        run.set_media_device(media.data_ptr())
-        torch.cuda.synchronize()
-    del media
-    torch.cuda.empty_cache()
-    stf = np.zeros((1, p.nt, 6), dtype=np.float32)
-    stf[0, :, p.igreen - 1] = gaussian_stf(p.nt, p.dt, 0.3, 1.2, 1e9)
-    run.set_source(np.array([src], dtype=np.int32), stf)
-    run.init()
-    return run, nkept
+# SGT error of the storage modes against the FP64 restatement, per component
+# (rel L2 over the whole series set; 800x600x200, 1500 steps, 2000 points:
+# tests/test_gpu_c4.py measures and bounds them)
+SGT_ERROR_VS_FP64 = {
+    "fp32": {"rel_l2": [1.8e-6, 2.7e-6], "peak": [1.6e-6, 2.7e-6]},
+    "mixed": {"rel_l2": [1.7e-4, 3.1e-4], "peak": [1.1e-4, 3.0e-4]},
+    "case": "800x600x200, h=200 m, 1500 steps, layered medium, 2000 SGT points; range over the six components",
+}
 
 
 def cpu_leg(a):
@@ -124,9 +75,15 @@ def cpu_leg(a):
         return None
     sys.path.insert(0, REPO)
     from oracle import cpu_baseline
-    ncores = min(len(os.sched_getaffinity(0)), 16)
-    log("cpu baseline on %d cores ..." % ncores)
-    return cpu_baseline.measure(ncores)
+    # the GPU lease's share of the host is 16 cores, whatever the affinity
+    # mask shows; both are stated
+    aff = len(os.sched_getaffinity(0))
+    ncores = min(aff, 16)
+    log("cpu baseline on %d cores (%d in the affinity mask) ..." % (ncores, aff))
+    out = cpu_baseline.measure(ncores)
+    out["cores_in_affinity_mask"] = aff
+    out["cores_note"] = "MPI ranks = the 16-core host share of one GPU's lease"
+    return out
 
 
 def traffic_of(name):
@@ -142,8 +99,8 @@ def bench_single(a, storage):
     import numpy as np
     import torch
     import awpsgt
-    from awpsgt.inputs import layered_media
     from awpsgt.media import c2_media
+    from awpsgt.workloads import layered_nvar5
 
     cfg = CONFIGS[a.config]
     nt = a.warmup + a.steps
@@ -152,12 +109,9 @@ def bench_single(a, storage):
     log("%s: grid %dx%dx%d, %d warmup + %d timed steps" % (a.config, p.nx, p.ny, p.nz, a.warmup, a.steps))
     t0 = time.time()
     if a.config == "c2":
-        media_fn = lambda: c2_media(p.nx, p.ny, p.nz, p.dh, seed=2)  # noqa: E731
+        media_fn = lambda: c2_media(p.nx, p.ny, p.nz, p.dh, seed=cfg["seed"])  # noqa: E731
     else:
-        def media_fn():
-            m = layered_media(p.nx, p.ny, p.nz, [600.0, 1400.0, 2200.0, 3000.0, 3800.0])
-            vs = m[..., 1]
-            return np.stack([m[..., 0], vs, m[..., 2], 0.1 * vs, 0.05 * vs], -1).astype(np.float32)
+        media_fn = lambda: layered_nvar5(p)  # noqa: E731
     st = awpsgt.AWP_STORAGE_MIXED if storage == "mixed" else awpsgt.AWP_STORAGE_FP32
     run, nkept = make_run(p, st, media_fn, cfg["src"], cfg["nsgt"], 3)
     log("set-up %.1f s, %d SGT points, %.1f GB on device" % (time.time() - t0, nkept, run.device_bytes() / 1e9))
@@ -213,27 +167,30 @@ def bench_single(a, storage):
     return value, ms_step, roof, config
 
 
-def bench_batched(a, storage):
+def bench_batched(a, storage, config=None, steps=None, warmup=None):
+    """One batched workload (c3 / c5) in one storage mode: runs admitted in
+    waves under the budget, each wave one tuned awpsgt.Batch; returns
+    (Gpt-updates/s, ms per step, roofline, config)."""
     import numpy as np
     import torch
     import awpsgt
-    from awpsgt.batchq import Need, admit, site_grids
-    from awpsgt.media import c2_media
+    from awpsgt.batchq import Need, admit
+    from awpsgt.workloads import batch_jobs, batch_run
 
-    cfg = CONFIGS[a.config]
-    nt = a.warmup + a.steps
+    config = config or a.config
+    steps = steps or a.steps
+    warmup = a.warmup if warmup is None else warmup
+    cfg = CONFIGS[config]
+    nt = warmup + steps
     st = awpsgt.AWP_STORAGE_MIXED if storage == "mixed" else awpsgt.AWP_STORAGE_FP32
-    jobs = []
-    for s_, (nx, ny, nz) in enumerate(site_grids(cfg["nsites"], cfg["lo"], cfg["hi"])):
-        for ig in (1, 2):
-            p = build_params(cfg, nt, nx, ny, nz, igreen=ig)
-            steady, peak = awpsgt.estimate_bytes(p, st, cfg["nsgt"])
-            jobs.append(dict(site=s_, seed=cfg["seed0"] + s_, igreen=ig, p=p, need=Need(steady, peak)))
+    jobs = batch_jobs(config, nt)
+    for j in jobs:
+        j["need"] = Need(*awpsgt.estimate_bytes(j["p"], st, cfg["nsgt"]))
     free, _ = torch.cuda.mem_get_info()
     budget = int(a.budget_gb * 1e9) if a.budget_gb else free - (8 << 30)
     waves = admit([j["need"] for j in jobs], budget)
-    log("%s: %d runs, %.1f GB steady in total, budget %.1f GB -> %d wave(s) of %s runs"
-        % (a.config, len(jobs), sum(j["need"].steady for j in jobs) / 1e9, budget / 1e9, len(waves),
+    log("%s %s: %d runs, %.1f GB steady in total, budget %.1f GB -> %d wave(s) of %s runs"
+        % (config, storage, len(jobs), sum(j["need"].steady for j in jobs) / 1e9, budget / 1e9, len(waves),
            "/".join(str(len(w)) for w in waves)))
     tot_pts = 0
     tot_time = 0.0
@@ -245,61 +202,94 @@ def bench_batched(a, storage):
         runs = []
         t0 = time.time()
         for k in w:
-            j = jobs[k]
-            p = j["p"]
-            fn = (lambda p=p, seed=j["seed"]: c2_media(p.nx, p.ny, p.nz, p.dh, seed=seed))
-            r, _ = make_run(p, st, fn, (p.nx // 2, p.ny // 2, 10), cfg["nsgt"], 100 + k)
+            r, _ = batch_run(config, jobs[k], k, st)
             runs.append(r)
         pts = sum(r.p.nx * r.p.ny * r.p.nz for r in runs)
         log("wave of %d runs, %.0fM points, set-up %.1f s, %.1f GB on device"
             % (len(runs), pts / 1e6, time.time() - t0, sum(r.device_bytes() for r in runs) / 1e9))
         with awpsgt.Batch(runs) as b:
             if not a.no_tune:
+                t1 = time.time()
                 launch = b.tune(2)
-                log("tuned: %s, %s grids, tile rows %d, %s %d, z chunks %d (%.2f ms per step's kernels)"
-                    % ("fused" if b.fused() else "two-stage",
+                log("tuned in %.1f s: %s, %s grids, tile rows %d, %s %d, z chunks %d (%.2f ms per step's kernels)"
+                    % (time.time() - t1, "fused" if b.fused() else "two-stage",
                        "persistent" if b.scheme()[1] else "one-block-per-tile", launch[0],
                        "prefetch" if b.fused() else "z unroll", launch[1], launch[2], b.time_ms()))
             else:
                 launch = b.launch()
             fused, persist = b.scheme()
-            b.step(a.warmup)
+            b.step(warmup)
             torch.cuda.synchronize()
             t = time.perf_counter()
-            b.step(a.steps)
+            b.step(steps)
             el = time.perf_counter() - t
-            kern.append(b.time_ms() / a.steps)
+            kern.append(b.time_ms() / steps)
         finite = all(bool(np.isfinite(r.sgt()).all()) for r in runs)
         for r in runs:
             assert r.progress()[0] == nt
             r.close()
         torch.cuda.empty_cache()
-        log("  %.3f ms/step (device %.3f), %.2f Gpt/s, finite=%s" % (el / a.steps * 1e3, kern[-1],
-                                                                    pts * a.steps / el / 1e9, finite))
-        tot_pts += pts * a.steps
+        log("  %.3f ms/step (device %.3f), %.2f Gpt/s, finite=%s" % (el / steps * 1e3, kern[-1],
+                                                                    pts * steps / el / 1e9, finite))
+        tot_pts += pts * steps
         tot_time += el
     value = tot_pts / tot_time / 1e9
-    pts_step = tot_pts / a.steps
-    ms_step = tot_time / a.steps * 1e3
-    ach = BYTES_UPDATE[storage] * value
+    pts_step = tot_pts / steps
+    ms_step = tot_time / steps * 1e3
+    dev_ms = sum(kern)
+    ach = BYTES_UPDATE[storage] * pts_step / (dev_ms * 1e-3) / 1e9
+    tname = "batch_%s_%s_traffic.txt" % (config, storage)
     roof = {"bound": "hbm",
-            "kernel": ("k_step_batch (fused sweep over every run of a wave)" if fused else
-                       "k_velocity_batch + k_fs_velocity_batch + k_stress_batch (every run of a wave), "
-                       "whole step"),
+            "kernel": ("whole batched step: k_step_batch (fused sweep over every run of a wave) + sources/SGT"
+                       if fused else
+                       "whole batched step: k_velocity_batch + k_fs_velocity_batch + k_stress_batch (every run "
+                       "of a wave) + sources/SGT"),
             "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
-            "traffic": None, "algorithmic_bytes_per_point": BYTES_UPDATE[storage],
-            "avg_step_ms_per_wave": [round(x, 4) for x in kern]}
+            "traffic": traffic_of(tname), "algorithmic_bytes_per_point": BYTES_UPDATE[storage],
+            "algorithmic_bytes_per_step": int(BYTES_UPDATE[storage] * pts_step),
+            "device_ms_per_step": round(dev_ms, 4)}
     if fused:
         roof["fused_kernel_min_bytes_per_point"] = BYTES_FUSED[storage]
     wl = cfg["workload"] if storage == "fp32" else cfg["workload"].replace(
         ", FP32", ", mixed storage (FP16 r1..r6 with per-row exponents, FP16 qp/qs)")
-    config = {"workload": wl, "runs": len(jobs), "waves": [len(w) for w in waves],
-              "points_per_step": int(pts_step), "storage": storage,
-              "step": ("fused" if fused else "two-stage") + ", one launch per stage over all runs, HIP graph",
-              "launch": {"tile_rows": launch[0], ("prefetch" if fused else "z_unroll"): launch[1],
-                         "z_chunks": launch[2], "persistent_grid": persist} if launch else None,
-              "budget_gb": round(budget / 1e9, 1), "parallelism": "single GPU"}
-    return value, ms_step, roof, config
+    conf = {"workload": wl, "runs": len(jobs), "waves": [len(w) for w in waves],
+            "points_per_step": int(pts_step), "storage": storage,
+            "step": ("fused" if fused else "two-stage") + ", one launch per stage over all runs, HIP graph",
+            "launch": {"tile_rows": launch[0], ("prefetch" if fused else "z_unroll"): launch[1],
+                       "z_chunks": launch[2], "persistent_grid": persist} if launch else None,
+            "budget_gb": round(budget / 1e9, 1), "parallelism": "single GPU"}
+    return value, ms_step, roof, conf
+
+
+def batched_record(a):
+    """The metric's second half: the batched workload in FP32 and in mixed
+    storage, each run created with both wavefield buffers where the budget
+    allows so that the batch tuner picks the faster scheme for each storage
+    mode (the comparison is best against best), their ratio and the SGT
+    error each storage mode is stated at."""
+    cfg = a.batched
+    old = os.environ.get("AWP_SCHEME")
+    os.environ["AWP_SCHEME"] = "fused"
+    try:
+        v32, ms32, roof32, c32 = bench_batched(a, "fp32", cfg, a.batched_steps, 2)
+        vmx, msmx, roofmx, cmx = bench_batched(a, "mixed", cfg, a.batched_steps, 2)
+    finally:
+        if old is None:
+            os.environ.pop("AWP_SCHEME", None)
+        else:
+            os.environ["AWP_SCHEME"] = old
+    log("batched %s: mixed %.3f Gpt/s vs FP32 %.3f Gpt/s: x%.3f" % (cfg, vmx, v32, vmx / v32))
+    return {"config": cfg, "workload": CONFIGS[cfg]["workload"].replace(", FP32", ""),
+            "steps": a.batched_steps, "warmup": 2,
+            "fp32": {"value": round(v32, 4), "unit": "Gpt-updates/s", "ms_per_step": round(ms32, 4),
+                     "step": c32["step"], "launch": c32["launch"], "waves": c32["waves"], "roofline": roof32},
+            "mixed": {"value": round(vmx, 4), "unit": "Gpt-updates/s", "ms_per_step": round(msmx, 4),
+                      "step": cmx["step"], "launch": cmx["launch"], "waves": cmx["waves"], "roofline": roofmx},
+            "mixed_vs_fp32": round(vmx / v32, 4),
+            "target_mixed_vs_fp32": 1.6,
+            "bytes_ceiling_mixed_vs_fp32": {"two_stage": round(BYTES_UPDATE["fp32"] / BYTES_UPDATE["mixed"], 3),
+                                             "fused": round(BYTES_FUSED["fp32"] / BYTES_FUSED["mixed"], 3)},
+            "sgt_error_vs_fp64": SGT_ERROR_VS_FP64}
 
 
 def main():
@@ -318,6 +308,10 @@ def main():
                     help="device-memory budget for the batched configs (default: free memory - 8 GiB)")
     ap.add_argument("--no-tune", action="store_true", help="batched configs: skip the tuner")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--batched", default="c3", choices=["c3", "c5", "none"],
+                    help="with the single-grid line: also run this batched workload in FP32 and in mixed "
+                         "storage and add their ratio as the 'batched' sub-record")
+    ap.add_argument("--batched-steps", type=int, default=10)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain device copy of 4 GiB five times (a known-size kernel "
                          "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE counters)")
@@ -353,6 +347,9 @@ def main():
         config["fp32_value"] = round(v32, 4)
         config["mixed_vs_fp32"] = round(value / v32, 4)
         log("mixed %.3f Gpt/s vs FP32 %.3f Gpt/s: x%.3f" % (value, v32, value / v32))
+    sub = None
+    if not batched and a.batched != "none":
+        sub = batched_record(a)
     cpu = cpu_leg(a)
     out = {
         "metric": "Gpt-updates/s + % HBM roofline; batched mixed-storage vs FP32 at stated SGT error",
@@ -371,6 +368,8 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
     }
+    if sub is not None:
+        out["batched"] = sub
     print(json.dumps(out))
 
 

@@ -134,6 +134,15 @@ def lib() -> ctypes.CDLL:
         return _lib
     if not os.path.exists(LIB_PATH):
         raise AwpError("libawpsgt.so not built (%s); run awpsgt.build.build()" % LIB_PATH)
+    # torch ships its own HIP runtime under the same soname as the system's
+    # (libamdhip64.so.7); the first one loaded serves the whole process.  A
+    # process that uses both torch (device memory for synthetic inputs) and
+    # this library must load torch's first: torch fails to find the device
+    # on a runtime of another release, while the library runs on either.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = ctypes.CDLL(LIB_PATH)
     vp, ip, cp, sz = ctypes.c_void_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_size_t
     P = ctypes.POINTER(In3d)

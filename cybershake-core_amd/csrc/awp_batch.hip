@@ -88,16 +88,26 @@ static Launch run_launch(const Geo &g, const Launch &bl, int nchunk, bool split)
 {
     Launch l = bl;
     nchunk = std::max(1, std::min(nchunk, std::max(1, g.nz / (split ? 8 : 16))));
-    for (;;) {
-        const int kc = (g.nz + nchunk - 1) / nchunk;
-        const int last = g.nz - (nchunk - 1) * kc;
-        if (nchunk == 1 || (last >= 3 && (nchunk - 1) * kc < g.nz)) {
-            l.kc = kc;
-            l.nchunk = (g.nz + kc - 1) / kc;
-            return l;
-        }
-        nchunk--;
+    int lo = 1;
+    if (!split) {
+        // the fused step's 32-bit offsets span one chunk
+        const int kmax = std::max(1, fused_kc_max(g));
+        lo = (g.nz + kmax - 1) / kmax;
+        nchunk = std::max(nchunk, lo);
     }
+    auto ok = [&g](int n) {
+        const int kc = (g.nz + n - 1) / n;
+        return n == 1 || (g.nz - (n - 1) * kc >= 3 && (n - 1) * kc < g.nz);
+    };
+    int n = nchunk;
+    while (n > lo && !ok(n)) --n;
+    if (!ok(n)) {
+        n = nchunk;
+        while (n < g.nz && !ok(n)) ++n;
+    }
+    l.kc = (g.nz + n - 1) / n;
+    l.nchunk = (g.nz + l.kc - 1) / l.kc;
+    return l;
 }
 
 // Builds the device job tables from the runs' current buffer parity: in the

@@ -29,16 +29,80 @@ extern "C" const char *awp_last_error(const awp_ctx *ctx) { return ctx ? ctx->er
 // 32-bit byte offsets where every array is below 4 GiB
 static bool use_o32(const Geo &g) { return offsets32(g); }
 
+// Placement of a run's volume arrays.  One allocation holds, in order:
+// the 9 FP32 arrays (velocities, stresses) of wavefield buffer 0, r1..r6 (4
+// bytes FP32, 2 bytes mixed; a 2-byte array starts 64 bytes into its slot
+// so that its interior rows start on 128-byte lines), the mixed mode's
+// row-segment exponents -- together the state awp_init zeroes -- then d1 mu
+// lam (FP32) and qp qs (4 or 2 bytes), and, for a run that holds the fused
+// step's second wavefield buffer, that buffer's 9 arrays.  Consecutive
+// arrays are `pad` bytes apart beyond their own size (AWP_ARRAY_PAD, a
+// multiple of 256; default below), and buffer 1 starts `bshift` further
+// bytes on (AWP_BUF_B_SHIFT).  AWP_BUF_B=sep instead gives buffer 1 an
+// allocation of its own, whose place in HBM the allocator chooses (awp_tune
+// then tries several).
+struct VolLayout {
+    size_t stride4;          // bytes from one 4-byte array to the next
+    size_t wf0[9], r[6], rexp, m[3], q[2];
+    size_t rexp_n;
+    int nseg;
+    size_t state_bytes;      // [0, state_bytes): buffer 0 + r1..r6 + exponents
+    bool b_separate;
+    size_t b_begin, b_bytes; // buffer 1 (offset in the allocation when not separate)
+    size_t total;            // bytes of the allocation
+};
+
+static size_t env_bytes(const char *name, size_t dflt)
+{
+    const char *v = getenv(name);
+    if (!v || !*v) return dflt;
+    const long long x = atoll(v);
+    return x < 0 ? dflt : ((size_t)x + 255) / 256 * 256;
+}
+
+static VolLayout vol_layout(const Geo &g, int storage, bool two_bufs)
+{
+    VolLayout L;
+    const size_t nv = (size_t)g.nvol;
+    const bool mixed = storage == AWP_STORAGE_MIXED;
+    const size_t pad = env_bytes("AWP_ARRAY_PAD", 0);
+    const size_t bshift = env_bytes("AWP_BUF_B_SHIFT", 0);
+    const char *bsep = getenv("AWP_BUF_B");
+    L.b_separate = bsep && !strcmp(bsep, "sep");
+    const size_t a4 = nv * 4 + pad;
+    const size_t eslot = (mixed ? nv * 2 + 128 : nv * 4) + pad, eoff = mixed ? 64 : 0;
+    L.stride4 = a4;
+    size_t o = 0;
+    for (int a = 0; a < 9; ++a, o += a4) L.wf0[a] = o;
+    for (int a = 0; a < 6; ++a, o += eslot) L.r[a] = o + eoff;
+    L.nseg = (g.nx + TX - 1) / TX;
+    L.rexp_n = mixed ? (size_t)g.nz * g.ny * L.nseg : 0;
+    L.rexp = o;
+    o += (8 * L.rexp_n + 255) / 256 * 256;
+    L.state_bytes = o;
+    for (int a = 0; a < 3; ++a, o += a4) L.m[a] = o;
+    for (int a = 0; a < 2; ++a, o += eslot) L.q[a] = o + eoff;
+    L.b_bytes = two_bufs ? 9 * a4 : 0;
+    L.b_begin = 0;
+    if (two_bufs && !L.b_separate) {
+        L.b_begin = o + bshift;
+        o = L.b_begin + L.b_bytes;
+    }
+    L.total = o;
+    return L;
+}
+
 // Time-step scheme of a new run.  AWP_SCHEME=split: the two-stage step
 // (velocity kernel, then stress kernel, in place); AWP_SCHEME=fused: the
 // one-sweep fused step (a second wavefield buffer; every array addressed
-// with 32-bit byte offsets).  Unset: fused for FP32 storage, two-stage for
+// with 32-bit byte offsets from a base rebased per z chunk, so chunks of at
+// most fused_kc_max levels).  Unset: fused for FP32 storage, two-stage for
 // mixed storage (1500x1000x300: fused FP32 16.2-16.3 ms per step on 64 x 11
 // tiles against 17.35 two-stage; fused mixed 23.6 against 15.7).  awp_tune
 // and awp_set_scheme switch a run that holds both buffers between the two.
 static bool want_fused(const Geo &g, int storage)
 {
-    if (!use_o32(g)) return false;
+    if (fused_kc_max(g) < 16) return false;   // planes of more than ~40M points
     const char *scheme = getenv("AWP_SCHEME");
     if (scheme && !strcmp(scheme, "fused")) return true;
     if (scheme && !strcmp(scheme, "split")) return false;
@@ -48,19 +112,28 @@ static bool want_fused(const Geo &g, int storage)
 // z chunks of `nchunk` requested: the top chunk keeps >= 3 levels (the
 // block reaching k = nz writes the free-surface stress images from the two
 // levels below it)
-static void set_chunks(Launch &l, const Geo &g, int nchunk)
+static void set_chunks(Launch &l, const Geo &g, int nchunk, bool fused)
 {
     nchunk = std::max(1, nchunk);
-    for (;;) {
-        const int kc = (g.nz + nchunk - 1) / nchunk;
-        const int last = g.nz - (nchunk - 1) * kc;
-        if (nchunk == 1 || (last >= 3 && (nchunk - 1) * kc < g.nz)) {
-            l.kc = kc;
-            l.nchunk = (g.nz + kc - 1) / kc;
-            return;
-        }
-        nchunk--;
+    int lo = 1;
+    if (fused) {
+        // the fused step's 32-bit offsets span one chunk (fused_kc_max)
+        const int kmax = std::max(1, fused_kc_max(g));
+        lo = (g.nz + kmax - 1) / kmax;
+        nchunk = std::max(nchunk, lo);
     }
+    auto ok = [&g](int n) {
+        const int kc = (g.nz + n - 1) / n;
+        return n == 1 || (g.nz - (n - 1) * kc >= 3 && (n - 1) * kc < g.nz);
+    };
+    int n = nchunk;
+    while (n > lo && !ok(n)) --n;
+    if (!ok(n)) {
+        n = nchunk;
+        while (n < g.nz && !ok(n)) ++n;
+    }
+    l.kc = (g.nz + n - 1) / n;
+    l.nchunk = (g.nz + l.kc - 1) / l.kc;
 }
 
 static Launch choose_launch(const Geo &g, bool fused, bool mixed, int device)
@@ -91,6 +164,17 @@ static Launch choose_launch(const Geo &g, bool fused, bool mixed, int device)
     // step against 17.62 for 2 x 16 and 17.75 for 3 x 8)
     l.pw = 4;
     l.ph = 4;
+    // region-synchronized fused launches (AWP_REGION=<w>x<h>: an XCD's w*h
+    // blocks take w x h neighbouring tiles together); off by default
+    l.rw = l.rh = 0;
+    const char *re = getenv("AWP_REGION");
+    if (re) {
+        int w = 0, h = 0;
+        if (sscanf(re, "%dx%d", &w, &h) == 2 && w >= 1 && h >= 1 && w * h <= 256) {
+            l.rw = w;
+            l.rh = h;
+        }
+    }
     const char *pe = getenv("AWP_PATCH");
     if (pe) {
         int w = 0, h = 0;
@@ -188,7 +272,7 @@ static Launch choose_launch(const Geo &g, bool fused, bool mixed, int device)
     }
     const char *kc_env = getenv("AWP_KC");
     if (kc_env && atoi(kc_env) >= 3) nchunk = std::max(1, (g.nz + atoi(kc_env) - 1) / atoi(kc_env));
-    set_chunks(l, g, nchunk);
+    set_chunks(l, g, nchunk, fused);
     return l;
 }
 
@@ -243,6 +327,7 @@ extern "C" int awp_create(awp_ctx **out, const awp_in3d *p, int device, int stor
     ctx->sg_done = false;
     ctx->d_trace = nullptr;
     ctx->trace_pos = 0;
+    ctx->fsync.counter = nullptr;
 
     e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
@@ -250,25 +335,9 @@ extern "C" int awp_create(awp_ctx **out, const awp_in3d *p, int device, int stor
         delete ctx;
         return AWP_EHIP;
     }
-    // one allocation: 9 FP32 arrays (velocities, stresses) of wavefield
-    // buffer 0, r1..r6 at 4 (FP32) or 2 (mixed) bytes per element -- the
-    // state awp_init zeroes -- then d1 mu lam (FP32) and qp qs (4 or 2
-    // bytes).  The fused step's buffer 1 is an allocation of its own: where
-    // it lands relative to buffer 0 sets the fused step's speed (at
-    // 1500x1000x300 some placements run 16.1-16.3 ms per step, others
-    // 17.5-17.6), and awp_tune tries several.
-    const size_t nv = (size_t)ctx->g.nvol;
-    const size_t esz = (storage == AWP_STORAGE_MIXED) ? 2 : 4;
-    const int nbuf = 1;
-    // a 2-byte array gets 128 bytes of slack and starts 64 bytes in, which
-    // puts its interior rows on 128-byte lines (the pitch is a multiple of 64)
-    const size_t eslot = nv * esz + (esz == 2 ? 128 : 0), eoff = (esz == 2 ? 64 : 0);
-    // mixed storage: one exponent byte per 64-point row segment, level and
-    // memory variable, after r1..r6 (zeroed with them)
-    const int nseg = (p->nx + TX - 1) / TX;
-    const size_t rexp_n = (storage == AWP_STORAGE_MIXED) ? (size_t)p->nz * p->ny * nseg : 0;
-    const size_t rexp_bytes = (8 * rexp_n + 255) / 256 * 256;   // 8-byte slot per row segment
-    const size_t nbytes = nv * (9 * nbuf + 3) * sizeof(float) + 8 * eslot + rexp_bytes;
+    // the volume arrays and their placement (vol_layout)
+    const VolLayout L = vol_layout(ctx->g, storage, ctx->two_bufs);
+    const size_t nbytes = L.total;
     e = hipMalloc((void **)&ctx->vol, nbytes);
     if (e != hipSuccess) {
         snprintf(err, errlen, "hipMalloc(%zu bytes) for the wavefield and media: %s", nbytes, hipGetErrorString(e));
@@ -278,41 +347,45 @@ extern "C" int awp_create(awp_ctx **out, const awp_in3d *p, int device, int stor
     }
     ctx->bytes += nbytes;
     ctx->vol_bytes = nbytes;
-    ctx->state_bytes = nv * 9 * nbuf * sizeof(float) + 6 * eslot + rexp_bytes;
+    ctx->state_bytes = L.state_bytes;
+    ctx->vol_b_own = false;
     if (ctx->two_bufs) {
-        ctx->vol_b_bytes = nv * 9 * sizeof(float);
-        e = hipMalloc((void **)&ctx->vol_b, ctx->vol_b_bytes);
-        if (e != hipSuccess) {
-            snprintf(err, errlen, "hipMalloc(%zu bytes) for the second wavefield buffer: %s", ctx->vol_b_bytes,
-                     hipGetErrorString(e));
-            hipFree(ctx->vol);
-            hipStreamDestroy(ctx->stream);
-            delete ctx;
-            return AWP_ENOMEM;
+        ctx->vol_b_bytes = L.b_bytes;
+        if (L.b_separate) {
+            e = hipMalloc((void **)&ctx->vol_b, ctx->vol_b_bytes);
+            if (e != hipSuccess) {
+                snprintf(err, errlen, "hipMalloc(%zu bytes) for the second wavefield buffer: %s", ctx->vol_b_bytes,
+                         hipGetErrorString(e));
+                hipFree(ctx->vol);
+                hipStreamDestroy(ctx->stream);
+                delete ctx;
+                return AWP_ENOMEM;
+            }
+            ctx->vol_b_own = true;
+            ctx->bytes += ctx->vol_b_bytes;
+        } else {
+            ctx->vol_b = (float *)((char *)ctx->vol + L.b_begin);
         }
-        ctx->bytes += ctx->vol_b_bytes;
     }
+    char *const base = (char *)ctx->vol;
+    ctx->arr_stride = L.stride4;
     for (int a = 0; a < 9; ++a) {
-        ctx->wf[0][a] = ctx->vol + (size_t)a * nv;
-        ctx->wf[1][a] = ctx->two_bufs ? ctx->vol_b + (size_t)a * nv : ctx->wf[0][a];
+        ctx->wf[0][a] = (float *)(base + L.wf0[a]);
+        ctx->wf[1][a] = ctx->two_bufs ? (float *)((char *)ctx->vol_b + (size_t)a * L.stride4) : ctx->wf[0][a];
     }
     ctx->cur = 0;
     float **f32[9] = {&ctx->s.u1, &ctx->s.v1, &ctx->s.w1, &ctx->s.xx, &ctx->s.yy, &ctx->s.zz,
                       &ctx->s.xy, &ctx->s.xz, &ctx->s.yz};
     for (int a = 0; a < 9; ++a) *f32[a] = ctx->wf[0][a];
-    char *rest = (char *)(ctx->vol + 9 * nbuf * nv);
     void **rr[6] = {&ctx->s.r1, &ctx->s.r2, &ctx->s.r3, &ctx->s.r4, &ctx->s.r5, &ctx->s.r6};
-    for (int a = 0; a < 6; ++a) *rr[a] = rest + (size_t)a * eslot + eoff;
-    rest += 6 * eslot;
-    ctx->s.rexp = rexp_n ? (signed char *)rest : nullptr;
-    ctx->s.rexp_n = (long long)rexp_n;
-    ctx->s.nseg = nseg;
-    rest += rexp_bytes;
+    for (int a = 0; a < 6; ++a) *rr[a] = base + L.r[a];
+    ctx->s.rexp = L.rexp_n ? (signed char *)(base + L.rexp) : nullptr;
+    ctx->s.rexp_n = (long long)L.rexp_n;
+    ctx->s.nseg = L.nseg;
     float **m32[3] = {&ctx->s.d1, &ctx->s.mu, &ctx->s.lam};
-    for (int a = 0; a < 3; ++a) *m32[a] = (float *)(rest + (size_t)a * nv * sizeof(float));
-    rest += 3 * nv * sizeof(float);
-    ctx->s.qp = rest + eoff;
-    ctx->s.qs = rest + eslot + eoff;
+    for (int a = 0; a < 3; ++a) *m32[a] = (float *)(base + L.m[a]);
+    ctx->s.qp = base + L.q[0];
+    ctx->s.qs = base + L.q[1];
     MScale one = {1, 1, 1, 1};
     ctx->s.ms = one;
     const size_t np = (size_t)(p->nx + 2) + (p->ny + 2) + (p->nz + 2);
@@ -387,6 +460,16 @@ extern "C" int awp_create(awp_ctx **out, const awp_in3d *p, int device, int stor
         return AWP_ENOMEM;
     }
     ctx->bytes += (size_t)std::max(p->nt, 1) * 3 * sizeof(float);
+    // arrival counters of the region-synchronized fused launches
+    e = hipMalloc((void **)&ctx->fsync.counter, 8 * sizeof(unsigned));
+    if (e == hipSuccess) e = hipMemset(ctx->fsync.counter, 0, 8 * sizeof(unsigned));
+    if (e != hipSuccess) {
+        snprintf(err, errlen, "hipMalloc sync counters: %s", hipGetErrorString(e));
+        awp_destroy(ctx);
+        return AWP_ENOMEM;
+    }
+    for (int x = 0; x < 8; ++x) ctx->fsync.base[x] = 0;
+    ctx->bytes += 8 * sizeof(unsigned);
     *out = ctx;
     return AWP_OK;
 }
@@ -398,9 +481,10 @@ extern "C" int awp_destroy(awp_ctx *ctx)
     if (ctx->stream) hipStreamSynchronize(ctx->stream);
     for (auto &e : ctx->ev) hipEventDestroy(e);
     hipFree(ctx->vol);
-    hipFree(ctx->vol_b);
+    if (ctx->vol_b_own) hipFree(ctx->vol_b);
     hipFree(ctx->prof);
     hipFree(ctx->d_trace);
+    hipFree(ctx->fsync.counter);
     hipFree(ctx->d_sjob);
     hipFree(ctx->d_psrc);
     hipFree(ctx->d_ptop);
@@ -669,7 +753,7 @@ static int upload_sjob(awp_ctx *ctx)
 static int scheme_changed(awp_ctx *ctx)
 {
     if (!ctx->fused && ctx->cur == 1) {
-        HIPTRY(ctx, hipMemcpyAsync(ctx->wf[0][0], ctx->wf[1][0], (size_t)ctx->g.nvol * 9 * sizeof(float),
+        HIPTRY(ctx, hipMemcpyAsync(ctx->wf[0][0], ctx->wf[1][0], 9 * ctx->arr_stride,
                                    hipMemcpyDeviceToDevice, ctx->stream));
         ctx->cur = 0;
     }
@@ -749,7 +833,7 @@ extern "C" int awp_step(awp_ctx *ctx, int nsteps)
             const State sa = ctx->view(ctx->cur), sb = ctx->view(ctx->cur ^ 1);
             if (e) HIPTRY(ctx, hipEventRecord(e[1], st));
             if (e) HIPTRY(ctx, hipEventRecord(e[2], st));
-            HIPTRY(ctx, launch_step_fused(mx, ctx->g, sa, sb, ctx->c, ctx->l, st));
+            HIPTRY(ctx, launch_step_fused(mx, ctx->g, sa, sb, ctx->c, ctx->l, st, &ctx->fsync, ctx->device));
             ctx->cur ^= 1;
             sn = sb;
         } else {
@@ -824,7 +908,7 @@ int trial_steps(awp_ctx *ctx, int n)
     for (int k = 0; k < n; ++k) {
         if (ctx->fused) {
             HIPTRY(ctx, launch_step_fused(mx, ctx->g, ctx->view(ctx->cur), ctx->view(ctx->cur ^ 1), ctx->c, ctx->l,
-                                          ctx->stream));
+                                          ctx->stream, &ctx->fsync, ctx->device));
             ctx->cur ^= 1;
         } else {
             HIPTRY(ctx, launch_split_velocity(ctx->l, ctx->l.o32, ctx->d_sjob, 1, ctx->split_total, all, ctx->stream));
@@ -886,7 +970,7 @@ extern "C" int awp_tune(awp_ctx *ctx, int reps)
             l.unroll = cd.ty == 11 ? 3 : 0;   // 64 x 11: two levels per barrier
             l.pw = cd.pw;
             l.ph = cd.ph;
-            set_chunks(l, ctx->g, std::min((ctx->g.nz + cd.kc / 2) / cd.kc, std::max(1, ctx->g.nz / 16)));
+            set_chunks(l, ctx->g, std::min((ctx->g.nz + cd.kc / 2) / cd.kc, std::max(1, ctx->g.nz / 16)), true);
         }
         ctx->l = l;
         return scheme_changed(ctx);
@@ -921,16 +1005,15 @@ extern "C" int awp_tune(awp_ctx *ctx, int reps)
     };
     rc = search();
     if (rc) return rc;
-    if (ctx->fused && ctx->vol_b) {
+    if (ctx->fused && ctx->vol_b && ctx->vol_b_own) {
         // Where the second buffer lands in HBM relative to the first sets
         // the fused step's speed (1500x1000x300: 16.1-16.3 or 17.5-17.6 ms
         // per step, stable for the life of the allocation; on some boxes one
         // placement in four is fast).  Try up to nine more buffers while
         // device memory allows and keep the fastest.
-        const size_t nv = (size_t)ctx->g.nvol;
-        auto use_b = [ctx, nv](float *vb) {
+        auto use_b = [ctx](float *vb) {
             ctx->vol_b = vb;
-            for (int a = 0; a < 9; ++a) ctx->wf[1][a] = vb + (size_t)a * nv;
+            for (int a = 0; a < 9; ++a) ctx->wf[1][a] = (float *)((char *)vb + (size_t)a * ctx->arr_stride);
             ctx->cur = 0;   // trial steps start from buffer 0
         };
         std::vector<float *> tried(1, ctx->vol_b);
@@ -1014,16 +1097,15 @@ extern "C" int awp_estimate_bytes(const awp_in3d *p, int storage, int npts, size
     const Geo g = make_geo(p->nx, p->ny, p->nz);
     const bool fused = want_fused(g, storage);
     const size_t nv = (size_t)g.nvol;
-    const size_t esz = storage == AWP_STORAGE_MIXED ? 2 : 4;
-    const size_t eslot = nv * esz + (esz == 2 ? 128 : 0);
-    const size_t rexp_n = storage == AWP_STORAGE_MIXED ? (size_t)p->nz * p->ny * ((p->nx + TX - 1) / TX) : 0;
-    const size_t vol = nv * (9 * (fused ? 2 : 1) + 3) * sizeof(float) + 8 * eslot + (8 * rexp_n + 255) / 256 * 256;
+    const VolLayout L = vol_layout(g, storage, fused);
+    const size_t vol = L.total + (L.b_separate ? L.b_bytes : 0);
     const size_t prof = ((size_t)p->nx + p->ny + p->nz + 6) * sizeof(float);
     const size_t nsamp_max = (size_t)std::max(p->nt / std::max(p->ntiskp_sgt, 1), 1);
     const size_t sgt = (size_t)std::max(npts, 1) * 6 * nsamp_max * sizeof(float);
     const size_t pts = (size_t)std::max(npts, 1) * (sizeof(long long) + sizeof(int) + 3 * sizeof(float));
     const size_t src = (size_t)p->nsrc * (2 * sizeof(long long) + (size_t)p->nst * 6 * sizeof(float));
-    const size_t st = vol + prof + sgt + pts + src;
+    const size_t trace = (size_t)std::max(p->nt, 1) * 3 * sizeof(float);
+    const size_t st = vol + prof + sgt + pts + src + trace;
     // awp_set_media: the media file, plus FP32 qp/qs temporaries in mixed mode
     size_t tr = (size_t)p->nx * p->ny * p->nz * p->nvar * sizeof(float);
     if (storage == AWP_STORAGE_MIXED) tr += 2 * nv * sizeof(float) + 2 * 1024 * sizeof(float);

@@ -80,6 +80,7 @@ struct StrIn {
     float r1, r2, r3, r4, r5, r6;
     float xx, yy, xy;
     float fz;                                  // sponge profile at s
+    unsigned long long ew;                     // mixed: the row segment's exponents of r1..r6
 };
 
 // PF selects how far loads run ahead of their use: 0 = each stage loads
@@ -89,11 +90,33 @@ struct StrIn {
 // PFC = 3: no prefetch, and the velocities of two levels per barrier (then
 // the stresses of two levels), which halves the block's barriers.
 // One column tile (bx, by, z chunk bz) of one run: the whole block calls it.
+// The volume arrays of s advanced by `e` elements (the sponge profiles and
+// the exponent slots keep their own indexing).
+template <class P>
+__device__ __forceinline__ GState rebased(const State &s0, const long long e)
+{
+    GState s = as_global(s0);
+    G<float> **f[12] = {&s.u1, &s.v1, &s.w1, &s.xx, &s.yy, &s.zz, &s.xy, &s.xz, &s.yz, &s.d1, &s.mu, &s.lam};
+#pragma unroll
+    for (int q = 0; q < 12; ++q) *f[q] += e;
+    using R = typename P::R;
+    using Q = typename P::Q;
+    G<void> **r[6] = {&s.r1, &s.r2, &s.r3, &s.r4, &s.r5, &s.r6};
+#pragma unroll
+    for (int q = 0; q < 6; ++q) *r[q] = (G<R> *)*r[q] + e;
+    s.qp = (G<Q> *)s.qp + e;
+    s.qs = (G<Q> *)s.qs + e;
+    return s;
+}
+
 template <class P, int TY, int PFC>
 __device__ __forceinline__ void step_tile(const Geo &g, const State &a0, const State &b0, const Coef &c,
                                           const int kc, const int bx, const int by, const int bz, float *lds)
 {
-    const GState a = as_global(a0), b = as_global(b0);
+    // 32-bit byte offsets count from level lz (the chunk's lowest level read)
+    const int lz = max(bz * kc + 1 - 4, -1);
+    const long long rb = (long long)lz * g.sz;
+    const GState a = rebased<P>(a0, rb), b = rebased<P>(b0, rb);
     using Sh = FShape<TY, PFC>;
     constexpr int PF = PFC == 3 ? 0 : PFC;   // prefetch depth
     constexpr int LV = PFC == 3 ? 2 : 1;     // levels per barrier
@@ -125,9 +148,10 @@ __device__ __forceinline__ void step_tile(const Geo &g, const State &a0, const S
     const bool dom = active && i >= 1 && i <= g.nx && j >= 1 && j <= g.ny;
     const bool smain = is_main && dom;
     const long long sy = g.sy, sz = g.sz;
-    // byte offset of (i, j, level 0) in a 4-byte array; level l adds l*sz4
-    // (levels >= -1: the unsigned arithmetic is exact modulo 2^32)
-    const unsigned ob = (unsigned)((g.base + (long long)j * sy + i) * 4);
+    // byte offset of (i, j, level 0) in a 4-byte array from the rebased
+    // pointers, modulo 2^32; level l adds l*sz4 (the offsets of the levels
+    // the tile reads, lz .. lz + kc + 8, are exact: below 2^32)
+    const unsigned ob = (unsigned)((g.base + (long long)j * sy + i - rb) * 4);
     const unsigned sy4 = (unsigned)(sy * 4), sz4 = (unsigned)(sz * 4);
     constexpr int QSH = sizeof(Q) == 2 ? 1 : 0;   // byte offset shift of qp/qs
     constexpr int RSH = sizeof(R) == 2 ? 1 : 0;   // and of r1..r6
@@ -182,6 +206,7 @@ __device__ __forceinline__ void step_tile(const Geo &g, const State &a0, const S
         if constexpr (P::mixed) {
             // FP16 values under the row segment's exponent (one per wave)
             const unsigned long long ew = ((const G<unsigned long long> *)a.rexp)[rexp_idx(g, a.nseg, bx, j, s)];
+            x.ew = ew;
             x.r1 = r_decode(gln(R1, orr), rexp_of(ew, 0)); x.r2 = r_decode(gln(R2, orr), rexp_of(ew, 1));
             x.r3 = r_decode(gln(R3, orr), rexp_of(ew, 2)); x.r4 = r_decode(gln(R4, orr), rexp_of(ew, 3));
             x.r5 = r_decode(gln(R5, orr), rexp_of(ew, 4)); x.r6 = r_decode(gln(R6, orr), rexp_of(ew, 5));
@@ -344,6 +369,7 @@ __device__ __forceinline__ void step_tile(const Geo &g, const State &a0, const S
         const bool do_str = smain && s >= kb;
             // ================= stress of level s = m-2 (viscoop.f:49-191) =======
             float rn[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};   // new memory variables (mixed: stored below)
+            unsigned long long ew_old = 0;                     // their row exponents before the update
             if (do_str) {
                 if constexpr (PF < 1) load_str(s, sin_);
                 const StrIn &x = sin_;
@@ -443,6 +469,7 @@ __device__ __forceinline__ void step_tile(const Geo &g, const State &a0, const S
                     gsn(R4, o, r4); gsn(R5, o, r5); gsn(R6, o, r6);
                 } else {
                     rn[0] = r1; rn[1] = r2; rn[2] = r3; rn[3] = r4; rn[4] = r5; rn[5] = r6;
+                    ew_old = x.ew;
                 }
                 gsn(b.xx, o, xx); gsn(b.yy, o, yy); gsn(b.xy, o, xy);
                 if (s == g.nz) {
@@ -474,9 +501,15 @@ __device__ __forceinline__ void step_tile(const Geo &g, const State &a0, const S
                     const unsigned orr = (ob + (unsigned)s * sz4) >> RSH;
                     G<R> *RR[6] = {R1, R2, R3, R4, R5, R6};
                     int e[6];
+                    // the row's exponents before the update: lane 0 (i = i0,
+                    // inside the domain whenever the row is) loaded the slot
+                    const unsigned long long ew_row =
+                        (unsigned long long)(unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)ew_old) |
+                        ((unsigned long long)(unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(ew_old >> 32))
+                         << 32);
     #pragma unroll
                     for (int q = 0; q < 6; ++q) {
-                        e[q] = row_exponent(wave_max(fabsf(rn[q])));
+                        e[q] = row_exponent_next(rn[q], do_str, rexp_of(ew_row, q));
                         if (do_str) gsn(RR[q], orr, r_encode(rn[q], e[q]));
                     }
                     if (do_str && lane == 0)
@@ -543,6 +576,52 @@ k_step(const Geo g, const State a, const State b, const Coef c, const int kc, co
     step_tile<P, TY, PF>(g, a, b, c, kc, bx, by, bz, lds);
 }
 
+// Region-synchronized form of the one-run sweep: gridDim.x = 8 x (rw * rh)
+// blocks, all resident at once (one per CU); block slot s of XCD x
+// (blockIdx.x = 8 s + x) takes tile (s % rw, s / rw) of each of the XCD's
+// regions in turn (XCD x owns regions [x * per, (x + 1) * per) of the list:
+// z chunk, then region rows, x fastest), and the XCD's blocks meet at a
+// barrier between regions.
+struct FRegions {
+    int nbx, nby, nbz;     // tiles
+    int rw, rh;            // region shape in tiles
+    int nrx, nry;          // regions per chunk in x and y
+    int nreg, per;         // regions in all; per XCD (the last XCDs may have fewer)
+    unsigned *counter;     // [8]
+    unsigned base[8];
+    int spin;              // most polls of one barrier
+};
+
+template <class P, int TY, int PF>
+__global__ void __launch_bounds__(64 * (TY + 5))
+k_step_regions(const Geo g, const State a, const State b, const Coef c, const int kc, const FRegions R)
+{
+    using Sh = FShape<TY, PF>;
+    __shared__ float lds[Sh::NSLOT * Sh::SLOT];
+    const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
+    const unsigned nslot = gridDim.x >> 3;
+    const int sx = slot % R.rw, sy = slot / R.rw;
+    for (int q = 0; q < R.per; ++q) {
+        const int reg = xcd * R.per + q;
+        if (reg >= R.nreg) break;   // uniform over the XCD
+        const int bz = reg / (R.nrx * R.nry);
+        const int rr = reg - bz * (R.nrx * R.nry);
+        const int bx = (rr % R.nrx) * R.rw + sx, by = (rr / R.nrx) * R.rh + sy;
+        if (bx < R.nbx && by < R.nby) step_tile<P, TY, PF>(g, a, b, c, kc, bx, by, bz, lds);
+        __syncthreads();   // the block's waves are done with the LDS ring
+        if (threadIdx.x == 0 && threadIdx.y == 0) {
+            __hip_atomic_fetch_add(R.counter + xcd, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const unsigned target = R.base[xcd] + (unsigned)(q + 1) * nslot;
+            for (int n = 0; n < R.spin; ++n) {
+                const unsigned v = __hip_atomic_load(R.counter + xcd, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if ((int)(v - target) >= 0) break;
+                __builtin_amdgcn_s_sleep(2);
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // Several runs in one launch.  Runs' tiles are numbered consecutively (run r
 // owns [tile0, tile0 + ntiles)); the grid is persistent: block n works on
 // XCD n % 8 and walks that XCD's contiguous range of the global tile list
@@ -583,9 +662,42 @@ FTiles fused_tiles(const Geo &g, const Launch &l)
 
 template <class P, int TY, int PF>
 static hipError_t step_t(const Geo &g, const State &a, const State &b, const Coef &c, const Launch &l,
-                         hipStream_t st)
+                         hipStream_t st, FSync *sync, int device)
 {
     const FTiles t = fused_tiles(g, l);
+    if (sync && l.rw > 0 && l.rh > 0) {
+        // every block of the regions resident at once: one XCD's share of
+        // the CUs times the blocks one CU holds
+        int nb = 0, ncu = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_step_regions<P, TY, PF>, TX * (TY + 5), 0) !=
+                hipSuccess)
+            nb = 0;
+        if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) ncu = 0;
+        const int slots = l.rw * l.rh;
+        if (nb > 0 && slots * 8 <= nb * ncu) {
+            FRegions R;
+            R.nbx = t.nbx;
+            R.nby = t.nby;
+            R.nbz = t.nbz;
+            R.rw = l.rw;
+            R.rh = l.rh;
+            R.nrx = (t.nbx + l.rw - 1) / l.rw;
+            R.nry = (t.nby + l.rh - 1) / l.rh;
+            R.nreg = R.nrx * R.nry * t.nbz;
+            R.per = (R.nreg + 7) / 8;
+            R.counter = sync->counter;
+            for (int x = 0; x < 8; ++x) R.base[x] = sync->base[x];
+            R.spin = 4000;
+            hipLaunchKernelGGL((k_step_regions<P, TY, PF>), dim3(8 * slots), dim3(TX, TY + 5), 0, st, g, a, b, c,
+                               l.kc, R);
+            // each XCD's counter rises by its blocks x the regions it takes
+            for (int x = 0; x < 8; ++x) {
+                const int nr = std::max(0, std::min(R.per, R.nreg - x * R.per));
+                sync->base[x] += (unsigned)(slots * nr);
+            }
+            return hipGetLastError();
+        }
+    }
     hipLaunchKernelGGL((k_step<P, TY, PF>), dim3(8 * t.per), dim3(TX, TY + 5), 0, st, g, a, b, c, l.kc, t);
     return hipGetLastError();
 }
@@ -655,9 +767,9 @@ static int grid_t(int device)
     } while (0)
 
 hipError_t launch_step_fused(bool mixed, const Geo &g, const State &a, const State &b, const Coef &c,
-                             const Launch &l, hipStream_t st)
+                             const Launch &l, hipStream_t st, FSync *sync, int device)
 {
-    AWP_FUSED_DISPATCH(mixed, l, return (step_t<P, TY, PF>(g, a, b, c, l, st)));
+    AWP_FUSED_DISPATCH(mixed, l, return (step_t<P, TY, PF>(g, a, b, c, l, st, sync, device)));
     return hipErrorInvalidValue;
 }
 

@@ -330,8 +330,9 @@ __device__ inline void stress_tile_e(const Geo &g, const State &s0, const Coef &
         const float va1 = -vqpa * vxl * dh1 * (vt1 + vt2 + vt3) / 2.0f;
 
         float r1, r2, r3, r4, r5, r6;
+        unsigned long long ew = 0;   // mixed: the row segment's exponents before this update
         if constexpr (P::mixed) {
-            const unsigned long long ew = ((const G<unsigned long long> *)s.rexp)[rexp_idx(g, s.nseg, seg, j, k)];
+            ew = ((const G<unsigned long long> *)s.rexp)[rexp_idx(g, s.nseg, seg, j, k)];
             r1 = r_decode(R1[o], rexp_of(ew, 0)); r2 = r_decode(R2[o], rexp_of(ew, 1));
             r3 = r_decode(R3[o], rexp_of(ew, 2)); r4 = r_decode(R4[o], rexp_of(ew, 3));
             r5 = r_decode(R5[o], rexp_of(ew, 4)); r6 = r_decode(R6[o], rexp_of(ew, 5));
@@ -386,7 +387,7 @@ __device__ inline void stress_tile_e(const Geo &g, const State &s0, const Coef &
             int e[6];
 #pragma unroll
             for (int q = 0; q < 6; ++q) {
-                e[q] = row_exponent(wave_max(act ? fabsf(rv[q]) : 0.0f));
+                e[q] = row_exponent_next(rv[q], act, rexp_of(ew, q));
                 if (act) RR[q][o] = r_encode(rv[q], e[q]);
             }
             if (act && threadIdx.x == 0)
@@ -538,9 +539,10 @@ __device__ inline void stress_tile_b(const Geo &g, const State &s0, const Coef &
         const float va1 = -vqpa * vxl * dh1 * (vt1 + vt2 + vt3) / 2.0f;
 
         float r1, r2, r3, r4, r5, r6;
+        unsigned long long ew = 0;   // mixed: the row segment's exponents before this update
         if constexpr (P::mixed) {
             // the exponents of this row segment: one 8-byte slot at byte offset xo
-            const unsigned long long ew = gln((const G<unsigned long long> *)s.rexp, xo);
+            ew = gln((const G<unsigned long long> *)s.rexp, xo);
             r1 = r_decode(p.ldn(R1), rexp_of(ew, 0)); r2 = r_decode(p.ldn(R2), rexp_of(ew, 1));
             r3 = r_decode(p.ldn(R3), rexp_of(ew, 2)); r4 = r_decode(p.ldn(R4), rexp_of(ew, 3));
             r5 = r_decode(p.ldn(R5), rexp_of(ew, 4)); r6 = r_decode(p.ldn(R6), rexp_of(ew, 5));
@@ -597,7 +599,7 @@ __device__ inline void stress_tile_b(const Geo &g, const State &s0, const Coef &
             int e[6];
 #pragma unroll
             for (int q = 0; q < 6; ++q) {
-                e[q] = row_exponent(wave_max(act ? fabsf(rv[q]) : 0.0f));
+                e[q] = row_exponent_next(rv[q], act, rexp_of(ew, q));
                 if (act) p.stn(RR[q], r_encode(rv[q], e[q]));
             }
             if (act && threadIdx.x == 0) gsn((G<unsigned long long> *)s.rexp, xo, rexp_pack(e));
@@ -1073,27 +1075,28 @@ __global__ void __launch_bounds__(256) k_to_half(const float *src, _Float16 *dst
 
 __global__ void __launch_bounds__(256)
 k_pack_r(const Geo g, const _Float16 *src, const signed char *ex, const int q, const int nseg, const int kext,
-         float *dst)
+         const int k0, float *dst)
 {
     const int i = blockIdx.x * TX + threadIdx.x + 1;
     const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
-    const int k = blockIdx.z + 1;
+    const int k = blockIdx.z + 1 + k0;
     if (i > g.nx || j > g.ny || k > g.nz + kext) return;
     // levels above the surface are never written: zero
     const float v = k <= g.nz ? r_decode(src[gidx(g, i, j, k)], ex[rexp_idx(g, nseg, (i - 1) / TX, j, k) * 8 + q])
                               : 0.0f;
-    dst[((size_t)(k - 1) * g.ny + (j - 1)) * g.nx + (i - 1)] = v;
+    dst[((size_t)(k - 1 - k0) * g.ny + (j - 1)) * g.nx + (i - 1)] = v;
 }
 
+// levels k0+1 .. k0+gridDim.z of one field, as [k - 1 - k0][j - 1][i - 1]
 template <class T>
 __global__ void __launch_bounds__(256)
-k_pack_field(const Geo g, const T *src, const float inv_scale, const int kext, float *dst)
+k_pack_field(const Geo g, const T *src, const float inv_scale, const int kext, const int k0, float *dst)
 {
     const int i = blockIdx.x * TX + threadIdx.x + 1;
     const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
-    const int k = blockIdx.z + 1;
+    const int k = blockIdx.z + 1 + k0;
     if (i > g.nx || j > g.ny || k > g.nz + kext) return;
-    dst[((size_t)(k - 1) * g.ny + (j - 1)) * g.nx + (i - 1)] = ld(src[gidx(g, i, j, k)]) * inv_scale;
+    dst[((size_t)(k - 1 - k0) * g.ny + (j - 1)) * g.nx + (i - 1)] = ld(src[gidx(g, i, j, k)]) * inv_scale;
 }
 
 // ---------------------------------------------------------------------------
@@ -1311,30 +1314,38 @@ hipError_t launch_to_half(const float *src, _Float16 *dst, float scale, long lon
 hipError_t launch_pack_field(bool mixed, const Geo &g, const State &s, int which, int kext, float *dst,
                              hipStream_t st)
 {
+    return launch_pack_levels(mixed, g, s, which, kext, 0, g.nz + kext, dst, st);
+}
+
+hipError_t launch_pack_levels(bool mixed, const Geo &g, const State &s, int which, int kext, int k0, int nk,
+                              float *dst, hipStream_t st)
+{
+    if (nk <= 0) return hipSuccess;
     dim3 blk(TX, 4);
-    dim3 grd((g.nx + TX - 1) / TX, (g.ny + 3) / 4, g.nz + kext);
+    dim3 grd((g.nx + TX - 1) / TX, (g.ny + 3) / 4, nk);
     const float *f32[9] = {s.u1, s.v1, s.w1, s.xx, s.yy, s.zz, s.xy, s.xz, s.yz};
     void *rr[6] = {s.r1, s.r2, s.r3, s.r4, s.r5, s.r6};
     const float *m32[3] = {s.d1, s.mu, s.lam};
     void *qq[2] = {s.qp, s.qs};
     const float iqs[2] = {s.ms.iqp, s.ms.iqs};
     if (which < 9) {
-        hipLaunchKernelGGL(k_pack_field<float>, grd, blk, 0, st, g, f32[which], 1.0f, kext, dst);
+        hipLaunchKernelGGL(k_pack_field<float>, grd, blk, 0, st, g, f32[which], 1.0f, kext, k0, dst);
     } else if (which >= 15 && which < 18) {
-        hipLaunchKernelGGL(k_pack_field<float>, grd, blk, 0, st, g, m32[which - 15], 1.0f, kext, dst);
+        hipLaunchKernelGGL(k_pack_field<float>, grd, blk, 0, st, g, m32[which - 15], 1.0f, kext, k0, dst);
     } else if (which < 15) {
         if (mixed)
             hipLaunchKernelGGL(k_pack_r, grd, blk, 0, st, g, (const _Float16 *)rr[which - 9],
-                               s.rexp, which - 9, s.nseg, kext, dst);
+                               s.rexp, which - 9, s.nseg, kext, k0, dst);
         else
-            hipLaunchKernelGGL(k_pack_field<float>, grd, blk, 0, st, g, (const float *)rr[which - 9], 1.0f, kext, dst);
+            hipLaunchKernelGGL(k_pack_field<float>, grd, blk, 0, st, g, (const float *)rr[which - 9], 1.0f, kext, k0,
+                               dst);
     } else {
         if (mixed)
             hipLaunchKernelGGL(k_pack_field<_Float16>, grd, blk, 0, st, g, (const _Float16 *)qq[which - 18],
-                               iqs[which - 18], kext, dst);
+                               iqs[which - 18], kext, k0, dst);
         else
             hipLaunchKernelGGL(k_pack_field<float>, grd, blk, 0, st, g, (const float *)qq[which - 18], 1.0f, kext,
-                               dst);
+                               k0, dst);
     }
     return hipGetLastError();
 }

@@ -50,15 +50,15 @@ def _decomp(cores: int):
     return best
 
 
-def _spec(nsteps: int, dec) -> RunSpec:
-    return RunSpec(nx=NX, ny=NY, nz=NZ, dh=100.0, dt=0.005, nt=nsteps, igreen=1,
-                   src=(NX // 2, NY // 2, 10), sigma=0.3, t0=1.2, amp=1e9, nd=20,
+def _spec(nsteps: int, dec, nx=NX, ny=NY, nz=NZ, dh=100.0, dt=0.005) -> RunSpec:
+    return RunSpec(nx=nx, ny=ny, nz=nz, dh=dh, dt=dt, nt=nsteps, igreen=1,
+                   src=(nx // 2, ny // 2, 10), sigma=0.3, t0=1.2, amp=1e9, nd=20,
                    npx=dec[0], npy=dec[1], npz=dec[2], ntiskp_sgt=10,
                    write_step=nsteps // 10)
 
 
-def _ref_wall(d: str, dec, nsteps: int, media, pts) -> float:
-    write_run(d, _spec(nsteps, dec), media, pts)
+def _ref_wall(d: str, dec, nsteps: int, media, pts, **grid) -> float:
+    write_run(d, _spec(nsteps, dec, **grid), media, pts)
     n = dec[0] * dec[1] * dec[2]
     t = time.time()
     r = subprocess.run([MPIEXEC, "-n", str(n), REF, "IN3D"], cwd=d, stdout=subprocess.PIPE,
@@ -68,7 +68,7 @@ def _ref_wall(d: str, dec, nsteps: int, media, pts) -> float:
     return time.time() - t
 
 
-def measure(cores: int, short: int = 10, long: int = 40) -> Dict[str, object]:
+def measure(cores: int, short: int = 10, long: int = 40, c1: bool = True) -> Dict[str, object]:
     cores = max(1, cores)
     d = tempfile.mkdtemp(prefix="awp_cpu_")
     try:
@@ -80,12 +80,25 @@ def measure(cores: int, short: int = 10, long: int = 40) -> Dict[str, object]:
             w1 = _ref_wall(d, dec, short, media, pts)
             w2 = _ref_wall(d, dec, long, media, pts)
             tps = (w2 - w1) / (long - short)
-            return {"value": NX * NY * NZ / tps / 1e9, "unit": "Gpt-updates/s", "cores": n,
-                    "kind": "reference",
-                    "sample": "reference AWP-ODC-SGT CPU solver (oracle/_ref/pmcl3d, amdflang -O2 "
-                              "-fno-automatic), %dx%dx%d layered medium, h=100 m, %d MPI ranks "
-                              "(%dx%dx%d); wall(%d steps) - wall(%d steps) = %.2f s -> %.4f s/step"
-                              % (NX, NY, NZ, n, dec[0], dec[1], dec[2], long, short, w2 - w1, tps)}
+            out = {"value": NX * NY * NZ / tps / 1e9, "unit": "Gpt-updates/s", "cores": n,
+                   "kind": "reference",
+                   "sample": "reference AWP-ODC-SGT CPU solver (oracle/_ref/pmcl3d, amdflang -O2 "
+                             "-fno-automatic), %dx%dx%d layered medium, h=100 m, %d MPI ranks "
+                             "(%dx%dx%d); wall(%d steps) - wall(%d steps) = %.2f s -> %.4f s/step"
+                             % (NX, NY, NZ, n, dec[0], dec[1], dec[2], long, short, w2 - w1, tps)}
+            if c1:
+                # the reference's own CPU case: 200x200x100, h = 200 m, dt = 0.01 s
+                g1 = dict(nx=200, ny=200, nz=100, dh=200.0, dt=0.01)
+                m1 = layered_media(200, 200, 100, LAYERS)
+                p1 = random_cordfile_points(2000, 1, (21, 180), (21, 180), (1, 80))
+                a = _ref_wall(d, dec, 4 * short, m1, p1, **g1)
+                b = _ref_wall(d, dec, 4 * long, m1, p1, **g1)
+                t1 = (b - a) / (4 * long - 4 * short)
+                out["c1"] = {"value": 200 * 200 * 100 / t1 / 1e9, "unit": "Gpt-updates/s", "cores": n,
+                             "sample": "200x200x100 layered (the reference's CPU case), %d MPI ranks; "
+                                       "wall(%d steps) - wall(%d steps) = %.2f s -> %.5f s/step"
+                                       % (n, 4 * long, 4 * short, b - a, t1)}
+            return out
         # fallback: the C restatement with OpenMP
         sys.path.insert(0, REPO)
         from oracle import oracle as orc

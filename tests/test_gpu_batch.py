@@ -247,3 +247,37 @@ def test_batch_surface_source_sampled_at_source_cell(case_dirs, name, scheme, mo
     s = ref.sgt()
     assert np.any(s[0, :, :] != 0) and np.all(np.isfinite(s[0]))
     ref.close()
+
+
+@pytest.mark.parametrize("storage", [awpsgt.AWP_STORAGE_FP32])
+def test_batched_runs_vs_reference_files(case_dirs, storage, tmp_path):
+    """All seven seeded cases in one batch (g7 has more steps: it finishes
+    on its own after the batch): each run's raw SGT file against the
+    reference binary's file, per component (rel L2 <= 1e-4) and per
+    (point, component) series above the floor (rel L2 <= 1e-4)."""
+    from conftest import load_golden, rel_l2
+    from test_sgtcheck import SERIES_FLOOR, per_series_check
+    names = list(cases.CASES)
+    runs = [_make(case_dirs[n][0], storage) for n in names]
+    with awpsgt.Batch(runs) as b:
+        b.step(10_000)
+    worst = 0.0
+    for n, r in zip(names, runs):
+        if r.progress()[0] < r.p.nt:
+            r.step(r.p.nt)
+        out = os.path.join(str(tmp_path), n)
+        r.write_sgt_file(out)
+        mine = np.fromfile(out, dtype=np.float32)
+        ref = load_golden(n)["sgt"]
+        assert mine.size == ref.size, n
+        nt_out, _ = awpsgt.sgt_file_geometry(r.p)
+        npad = (-ref.size) % (6 * nt_out)
+        m = np.concatenate([mine, np.zeros(npad, np.float32)]).reshape(-1, 6, nt_out)
+        g = np.concatenate([ref, np.zeros(npad, np.float32)]).reshape(-1, 6, nt_out)
+        for c in range(6):
+            assert rel_l2(m[:, c], g[:, c]) <= 1e-4, (n, c)
+        cmp_ = per_series_check(ref, mine, nt_out, tmp_path, n)
+        assert cmp_["series_checked"] > 0 and cmp_["worst_rel_l2"] <= 1e-4, (n, cmp_)
+        worst = max(worst, cmp_["worst_rel_l2"])
+        r.close()
+    print("batched runs vs reference files: worst per-series rel L2 %.3g (floor %.0e)" % (worst, SERIES_FLOOR))
