@@ -34,7 +34,7 @@ static bool use_o32(const Geo &g) { return offsets32(g); }
 // bytes FP32, 2 bytes mixed; a 2-byte array starts 64 bytes into its slot
 // so that its interior rows start on 128-byte lines), the mixed mode's
 // row-segment exponents -- together the state awp_init zeroes -- then d1 mu
-// lam (FP32) and qp qs (4 or 2 bytes), and, for a run that holds the fused
+// lam and qp qs (4 or 2 bytes), and, for a run that holds the fused
 // step's second wavefield buffer, that buffer's 9 arrays.  Consecutive
 // arrays are `pad` bytes apart beyond their own size (AWP_ARRAY_PAD, a
 // multiple of 256; default below), and buffer 1 starts `bshift` further
@@ -80,7 +80,7 @@ static VolLayout vol_layout(const Geo &g, int storage, bool two_bufs)
     L.rexp = o;
     o += (8 * L.rexp_n + 255) / 256 * 256;
     L.state_bytes = o;
-    for (int a = 0; a < 3; ++a, o += a4) L.m[a] = o;
+    for (int a = 0; a < 3; ++a, o += eslot) L.m[a] = o + eoff;
     for (int a = 0; a < 2; ++a, o += eslot) L.q[a] = o + eoff;
     L.b_bytes = two_bufs ? 9 * a4 : 0;
     L.b_begin = 0;
@@ -382,20 +382,24 @@ extern "C" int awp_create(awp_ctx **out, const awp_in3d *p, int device, int stor
     ctx->s.rexp = L.rexp_n ? (signed char *)(base + L.rexp) : nullptr;
     ctx->s.rexp_n = (long long)L.rexp_n;
     ctx->s.nseg = L.nseg;
-    float **m32[3] = {&ctx->s.d1, &ctx->s.mu, &ctx->s.lam};
-    for (int a = 0; a < 3; ++a) *m32[a] = (float *)(base + L.m[a]);
+    void **mm[3] = {&ctx->s.d1, &ctx->s.mu, &ctx->s.lam};
+    for (int a = 0; a < 3; ++a) *mm[a] = base + L.m[a];
     ctx->s.qp = base + L.q[0];
     ctx->s.qs = base + L.q[1];
     MScale one = {1, 1, 1, 1};
     ctx->s.ms = one;
+    // sponge profiles fx fy fz, then (mixed) the [3][pz] level profiles of
+    // d1, mu, lam
     const size_t np = (size_t)(p->nx + 2) + (p->ny + 2) + (p->nz + 2);
-    e = hipMalloc((void **)&ctx->prof, np * sizeof(float));
+    const size_t nmb = storage == AWP_STORAGE_MIXED ? 3 * (size_t)ctx->g.pz : 0;
+    e = hipMalloc((void **)&ctx->prof, (np + nmb) * sizeof(float));
     if (e != hipSuccess) {
         snprintf(err, errlen, "hipMalloc profiles: %s", hipGetErrorString(e));
         awp_destroy(ctx);
         return AWP_ENOMEM;
     }
-    ctx->bytes += np * sizeof(float);
+    ctx->bytes += (np + nmb) * sizeof(float);
+    ctx->s.mbar = nmb ? ctx->prof + np : nullptr;
     ctx->s.fx = ctx->prof;
     ctx->s.fy = ctx->prof + (p->nx + 2);
     ctx->s.fz = ctx->prof + (p->nx + 2) + (p->ny + 2);
@@ -537,21 +541,54 @@ static int media_from_device(awp_ctx *ctx, const float *dmedia)
         HIPTRY(ctx, launch_media_ghosts(ctx->g, m, ctx->stream));
         HIPTRY(ctx, hipStreamSynchronize(ctx->stream));
     } else {
-        // d1, mu, lam straight into their FP32 storage, qp and qs into FP32
-        // temporaries that are then stored as FP16 of value * 2^e
+        // The FP32 media are set up in five arrays of wavefield buffer 0
+        // (state that awp_init zeroes before any step), then stored:
+        // d1, mu, lam as FP16 deviations from a profile value per level,
+        // qp and qs as FP16 of value * 2^e.
         const size_t nv = (size_t)ctx->g.nvol;
-        float *tmp = nullptr;
-        hipError_t e = hipMalloc((void **)&tmp, nv * 2 * sizeof(float));
-        if (e != hipSuccess)
-            return fail(ctx, AWP_ENOMEM, "hipMalloc(%zu) for FP32 media set-up: %s", nv * 8, hipGetErrorString(e));
-        Media32 m = {ctx->s.d1, ctx->s.mu, ctx->s.lam, tmp, tmp + nv};
-        e = hipMemsetAsync(tmp, 0, nv * 2 * sizeof(float), ctx->stream);
+        const int nz = ctx->g.nz, pz = ctx->g.pz;
+        float *tmp[5];
+        for (int a = 0; a < 5; ++a) tmp[a] = ctx->wf[0][a];
+        Media32 m = {tmp[0], tmp[1], tmp[2], tmp[3], tmp[4]};
+        hipError_t e = hipSuccess;
+        for (int a = 0; a < 5 && e == hipSuccess; ++a) e = hipMemsetAsync(tmp[a], 0, nv * sizeof(float), ctx->stream);
         if (e == hipSuccess) e = launch_media_setup(ctx->g, m, dmedia, mc, ctx->stream);
         if (e == hipSuccess) e = launch_media_ghosts(ctx->g, m, ctx->stream);
-        // per-array power of two putting the largest magnitude in [2^13, 2^14)
+        // level profiles: (smallest + largest nonzero interior value) / 2 of
+        // each level 1..nz; levels outside take level 1's and level nz's
         const int nblk = 1024;
+        const size_t nrange = 3 * 2 * (size_t)nz;
         float *part = nullptr;
-        if (e == hipSuccess) e = hipMalloc((void **)&part, 2 * nblk * sizeof(float));
+        if (e == hipSuccess) e = hipMalloc((void **)&part, 2 * nblk * sizeof(float) + nrange * sizeof(unsigned));
+        unsigned *range = part ? (unsigned *)(part + 2 * nblk) : nullptr;
+        std::vector<unsigned> hr(nrange);
+        for (size_t q = 0; q < nrange; ++q) hr[q] = (q & 1) ? 0u : 0xFFFFFFFFu;
+        if (e == hipSuccess) e = hipMemcpyAsync(range, hr.data(), nrange * sizeof(unsigned), hipMemcpyHostToDevice, ctx->stream);
+        for (int a = 0; a < 3 && e == hipSuccess; ++a) e = launch_level_range(ctx->g, tmp[a], range + 2 * (size_t)nz * a, ctx->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(hr.data(), range, nrange * sizeof(unsigned), hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        auto key_value = [](unsigned key) {
+            const unsigned b = (key & 0x80000000u) ? (key & 0x7FFFFFFFu) : ~key;
+            float f;
+            memcpy(&f, &b, 4);
+            return f;
+        };
+        std::vector<float> bar(3 * (size_t)pz, 0.0f);
+        for (int a = 0; a < 3; ++a) {
+            std::vector<float> lv(nz, 0.0f);
+            for (int k = 1; k <= nz; ++k) {
+                const unsigned lo = hr[2 * ((size_t)nz * a + k - 1)], hi = hr[2 * ((size_t)nz * a + k - 1) + 1];
+                if (lo != 0xFFFFFFFFu)
+                    lv[k - 1] = (float)(((double)key_value(lo) + (double)key_value(hi)) * 0.5);
+            }
+            for (int k = -1; k <= nz + 2; ++k) bar[mbar_idx(ctx->g, a, k)] = lv[std::min(std::max(k, 1), nz) - 1];
+        }
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(ctx->s.mbar, bar.data(), bar.size() * sizeof(float), hipMemcpyHostToDevice, ctx->stream);
+        void *mdst[3] = {ctx->s.d1, ctx->s.mu, ctx->s.lam};
+        for (int a = 0; a < 3 && e == hipSuccess; ++a)
+            e = launch_media_encode(ctx->g, tmp[a], (_Float16 *)mdst[a], ctx->s.mbar + (size_t)a * pz, ctx->stream);
+        // qp, qs: per-array power of two putting the largest magnitude in [2^13, 2^14)
         const float *arrs[2] = {m.qp, m.qs};
         for (int a = 0; a < 2 && e == hipSuccess; ++a) e = launch_absmax(arrs[a], (long long)nv, part + a * nblk, nblk, ctx->stream);
         std::vector<float> hp(2 * nblk);
@@ -571,12 +608,11 @@ static int media_from_device(awp_ctx *ctx, const float *dmedia)
             e = launch_to_half(arrs[a], (_Float16 *)dst[a], sc[a], (long long)nv, ctx->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
         hipFree(part);
-        hipFree(tmp);
         if (e != hipSuccess) return fail(ctx, AWP_EHIP, "mixed media set-up: %s", hipGetErrorString(e));
         MScale ms = {sc[0], sc[1], isc[0], isc[1]};
         ctx->s.ms = ms;
     }
-    ctx->sg_done = false;  // computed at awp_init from the FP32 mu, lam
+    ctx->sg_done = false;  // computed at awp_init from the stored mu, lam
     ctx->media_set = true;
     ctx->initialised = false;
     return AWP_OK;
@@ -774,7 +810,7 @@ extern "C" int awp_init(awp_ctx *ctx)
     hipLaunchKernelGGL(k_fill, dim3(1024), dim3(256), 0, ctx->stream, ctx->d_sgt, nout, -1.0f);
     HIPTRY(ctx, hipGetLastError());
     if (!ctx->sg_done) {
-        HIPTRY(ctx, launch_sgt_coef(ctx->media32(), ctx->nkept, ctx->d_kpos, ctx->d_sg, ctx->stream));
+        HIPTRY(ctx, launch_sgt_coef(ctx->g, ctx->s, ctx->nkept, ctx->d_kpos, ctx->d_sg, ctx->stream));
         ctx->sg_done = true;
     }
     // the two-stage step's one-run job table (state, coefficients, tiles)
@@ -841,9 +877,9 @@ extern "C" int awp_step(awp_ctx *ctx, int nsteps)
             // block per tile (in buffer 0)
             sn = ctx->view(ctx->cur);
             const int all = 1 << 30;
-            HIPTRY(ctx, launch_split_velocity(ctx->l, ctx->l.o32, ctx->d_sjob, 1, ctx->split_total, all, st));
+            HIPTRY(ctx, launch_split_velocity(ctx->mixed(), ctx->l, ctx->l.o32, ctx->d_sjob, 1, ctx->split_total, all, st));
             if (e) HIPTRY(ctx, hipEventRecord(e[1], st));
-            HIPTRY(ctx, launch_split_fs(ctx->d_sjob, 1, ctx->split_fs, st));
+            HIPTRY(ctx, launch_split_fs(ctx->mixed(), ctx->d_sjob, 1, ctx->split_fs, st));
             if (e) HIPTRY(ctx, hipEventRecord(e[2], st));
             HIPTRY(ctx, launch_split_stress(mx, ctx->l, ctx->l.o32, ctx->d_sjob, 1, ctx->split_total, all, st));
         }
@@ -911,8 +947,8 @@ int trial_steps(awp_ctx *ctx, int n)
                                           ctx->stream, &ctx->fsync, ctx->device));
             ctx->cur ^= 1;
         } else {
-            HIPTRY(ctx, launch_split_velocity(ctx->l, ctx->l.o32, ctx->d_sjob, 1, ctx->split_total, all, ctx->stream));
-            HIPTRY(ctx, launch_split_fs(ctx->d_sjob, 1, ctx->split_fs, ctx->stream));
+            HIPTRY(ctx, launch_split_velocity(ctx->mixed(), ctx->l, ctx->l.o32, ctx->d_sjob, 1, ctx->split_total, all, ctx->stream));
+            HIPTRY(ctx, launch_split_fs(ctx->mixed(), ctx->d_sjob, 1, ctx->split_fs, ctx->stream));
             HIPTRY(ctx, launch_split_stress(mx, ctx->l, ctx->l.o32, ctx->d_sjob, 1, ctx->split_total, all,
                                             ctx->stream));
         }
@@ -1099,16 +1135,19 @@ extern "C" int awp_estimate_bytes(const awp_in3d *p, int storage, int npts, size
     const size_t nv = (size_t)g.nvol;
     const VolLayout L = vol_layout(g, storage, fused);
     const size_t vol = L.total + (L.b_separate ? L.b_bytes : 0);
-    const size_t prof = ((size_t)p->nx + p->ny + p->nz + 6) * sizeof(float);
+    const size_t prof = ((size_t)p->nx + p->ny + p->nz + 6 + (storage == AWP_STORAGE_MIXED ? 3 * (size_t)g.pz : 0)) *
+                        sizeof(float);
     const size_t nsamp_max = (size_t)std::max(p->nt / std::max(p->ntiskp_sgt, 1), 1);
     const size_t sgt = (size_t)std::max(npts, 1) * 6 * nsamp_max * sizeof(float);
     const size_t pts = (size_t)std::max(npts, 1) * (sizeof(long long) + sizeof(int) + 3 * sizeof(float));
     const size_t src = (size_t)p->nsrc * (2 * sizeof(long long) + (size_t)p->nst * 6 * sizeof(float));
     const size_t trace = (size_t)std::max(p->nt, 1) * 3 * sizeof(float);
     const size_t st = vol + prof + sgt + pts + src + trace;
-    // awp_set_media: the media file, plus FP32 qp/qs temporaries in mixed mode
+    // awp_set_media: the media file, plus (mixed mode) the reductions' small
+    // buffers; the FP32 media of the mixed set-up live in wavefield buffer 0
     size_t tr = (size_t)p->nx * p->ny * p->nz * p->nvar * sizeof(float);
-    if (storage == AWP_STORAGE_MIXED) tr += 2 * nv * sizeof(float) + 2 * 1024 * sizeof(float);
+    if (storage == AWP_STORAGE_MIXED) tr += 2 * 1024 * sizeof(float) + 6 * (size_t)p->nz * sizeof(unsigned);
+    (void)nv;
     if (steady) *steady = st;
     if (peak) *peak = st + tr;
     return AWP_OK;

@@ -86,10 +86,10 @@ struct awp_ctx {
         for (int a = 0; a < 9; ++a) *f[a] = wf[which][a];
         return x;
     }
+    // the media arrays of an FP32-storage run
     Media32 media32() const
     {
-        // qp/qs are FP32 only in FP32 mode; the SGT coefficients use mu, lam
-        Media32 m = {s.d1, s.mu, s.lam, mixed() ? nullptr : (float *)s.qp, mixed() ? nullptr : (float *)s.qs};
+        Media32 m = {(float *)s.d1, (float *)s.mu, (float *)s.lam, (float *)s.qp, (float *)s.qs};
         return m;
     }
 };

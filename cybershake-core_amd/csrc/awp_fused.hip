@@ -96,14 +96,18 @@ template <class P>
 __device__ __forceinline__ GState rebased(const State &s0, const long long e)
 {
     GState s = as_global(s0);
-    G<float> **f[12] = {&s.u1, &s.v1, &s.w1, &s.xx, &s.yy, &s.zz, &s.xy, &s.xz, &s.yz, &s.d1, &s.mu, &s.lam};
+    G<float> **f[9] = {&s.u1, &s.v1, &s.w1, &s.xx, &s.yy, &s.zz, &s.xy, &s.xz, &s.yz};
 #pragma unroll
-    for (int q = 0; q < 12; ++q) *f[q] += e;
+    for (int q = 0; q < 9; ++q) *f[q] += e;
     using R = typename P::R;
     using Q = typename P::Q;
+    using M = typename P::M;
     G<void> **r[6] = {&s.r1, &s.r2, &s.r3, &s.r4, &s.r5, &s.r6};
 #pragma unroll
     for (int q = 0; q < 6; ++q) *r[q] = (G<R> *)*r[q] + e;
+    G<void> **m[3] = {&s.d1, &s.mu, &s.lam};
+#pragma unroll
+    for (int q = 0; q < 3; ++q) *m[q] = (G<M> *)*m[q] + e;
     s.qp = (G<Q> *)s.qp + e;
     s.qs = (G<Q> *)s.qs + e;
     return s;
@@ -153,9 +157,12 @@ __device__ __forceinline__ void step_tile(const Geo &g, const State &a0, const S
     // the tile reads, lz .. lz + kc + 8, are exact: below 2^32)
     const unsigned ob = (unsigned)((g.base + (long long)j * sy + i - rb) * 4);
     const unsigned sy4 = (unsigned)(sy * 4), sz4 = (unsigned)(sz * 4);
+    using M = typename P::M;
     constexpr int QSH = sizeof(Q) == 2 ? 1 : 0;   // byte offset shift of qp/qs
     constexpr int RSH = sizeof(R) == 2 ? 1 : 0;   // and of r1..r6
+    constexpr int MSH = sizeof(M) == 2 ? 1 : 0;   // and of d1, mu, lam
     constexpr int QB = (int)sizeof(Q);
+    constexpr int MB = (int)sizeof(M);
     const float c1 = c.c1, c2 = c.c2, dth = c.dth, dt = c.dt, dh1 = c.dh1;
     const float fxy = dom ? a.fx[i] * a.fy[j] : 0.0f;
     // relaxation weights of this column for even / odd depth parity
@@ -165,8 +172,11 @@ __device__ __forceinline__ void step_tile(const Geo &g, const State &a0, const S
     const float t2e = c.tau2[itxy], t2o = c.tau2[itxy + 1];
     const int lpos = rr * Sh::RW + rc;
     constexpr int PL = Sh::PLANE;
-    const G<float> *__restrict__ lam = a.lam;
-    const G<float> *__restrict__ mu = a.mu;
+    const G<M> *__restrict__ d1 = (const G<M> *)a.d1;
+    const G<M> *__restrict__ lam = (const G<M> *)a.lam;
+    const G<M> *__restrict__ mu = (const G<M> *)a.mu;
+    // profile values of level l (mixed storage; 0 unused in FP32)
+    auto mbar = [&](int arr, int l) -> float { return P::mixed ? a.mbar[mbar_idx(g, arr, l)] : 0.0f; };
     const G<Q> *__restrict__ qp = (const G<Q> *)a.qp;
     const G<Q> *__restrict__ qs = (const G<Q> *)a.qs;
     G<R> *__restrict__ R1 = (G<R> *)a.r1;
@@ -184,7 +194,10 @@ __device__ __forceinline__ void step_tile(const Geo &g, const State &a0, const S
         x.xz_p1 = gl(a.xz, o + sz4);
         x.yz_p1 = gl(a.yz, o + sz4);
         x.zz_p2 = gl(a.zz, o + 2 * sz4);
-        x.dA = gl(a.d1, o); x.dB = gl(a.d1, ojm1); x.dC = gl(a.d1, o, 4); x.dD = gl(a.d1, ojm1, 4);
+        const float db = mbar(0, m);
+        const unsigned om = o >> MSH, omj = ojm1 >> MSH;
+        x.dA = mdec(gl(d1, om), db); x.dB = mdec(gl(d1, omj), db);
+        x.dC = mdec(gl(d1, om, MB), db); x.dD = mdec(gl(d1, omj, MB), db);
         x.xx_0 = gl(a.xx, o); x.xx_m1 = gl(a.xx, o, -4); x.xx_p1 = gl(a.xx, o, 4); x.xx_m2 = gl(a.xx, o, -8);
         x.xy_0 = gl(a.xy, o); x.xy_jm1 = gl(a.xy, ojm1); x.xy_jp1 = gl(a.xy, ojp1); x.xy_jm2 = gl(a.xy, ojm2);
         x.xy_ip1 = gl(a.xy, o, 4); x.xy_ip2 = gl(a.xy, o, 8); x.xy_im1 = gl(a.xy, o, -4);
@@ -197,8 +210,14 @@ __device__ __forceinline__ void step_tile(const Geo &g, const State &a0, const S
     // loads of the stress stage at level s (main dom threads)
     auto load_str = [&](int s, StrIn &x) {
         const unsigned o = ob + (unsigned)s * sz4, oj = o - sy4;
-        x.lA = gl(lam, o); x.lB = gl(lam, o, 4); x.lC = gl(lam, oj); x.lD = gl(lam, oj, 4);
-        x.mA = gl(mu, o); x.mB = gl(mu, o, 4); x.mC = gl(mu, oj); x.mD = gl(mu, oj, 4);
+        {
+            const float lb = mbar(2, s), mb = mbar(1, s);
+            const unsigned mo = o >> MSH, mj = oj >> MSH;
+            x.lA = mdec(gl(lam, mo), lb); x.lB = mdec(gl(lam, mo, MB), lb);
+            x.lC = mdec(gl(lam, mj), lb); x.lD = mdec(gl(lam, mj, MB), lb);
+            x.mA = mdec(gl(mu, mo), mb); x.mB = mdec(gl(mu, mo, MB), mb);
+            x.mC = mdec(gl(mu, mj), mb); x.mD = mdec(gl(mu, mj, MB), mb);
+        }
         const unsigned qo = o >> QSH, qj = oj >> QSH;
         x.pA = ld(gl(qp, qo)); x.pB = ld(gl(qp, qo, QB)); x.pC = ld(gl(qp, qj)); x.pD = ld(gl(qp, qj, QB));
         x.sA = ld(gl(qs, qo)); x.sB = ld(gl(qs, qo, QB)); x.sC = ld(gl(qs, qj)); x.sD = ld(gl(qs, qj, QB));
@@ -240,7 +259,9 @@ __device__ __forceinline__ void step_tile(const Geo &g, const State &a0, const S
         }
         if (m0 - 1 >= -1) {
             const unsigned o = ob + (unsigned)(m0 - 1) * sz4;
-            dA_m = gl(a.d1, o); dB_m = gl(a.d1, o - sy4); dC_m = gl(a.d1, o, 4);
+            const float db = mbar(0, m0 - 1);
+            dA_m = mdec(gl(d1, o >> MSH), db); dB_m = mdec(gl(d1, (o - sy4) >> MSH), db);
+            dC_m = mdec(gl(d1, o >> MSH, MB), db);
         }
     }
 
@@ -251,8 +272,14 @@ __device__ __forceinline__ void step_tile(const Geo &g, const State &a0, const S
     float sA_m = 0.f, sB_m = 0.f, sC_m = 0.f, sD_m = 0.f;
     if (smain) {
         const unsigned om = ob + (unsigned)(kb - 1) * sz4, omj = om - sy4;
-        lA_m = gl(lam, om); lB_m = gl(lam, om, 4); lC_m = gl(lam, omj); lD_m = gl(lam, omj, 4);
-        mA_m = gl(mu, om); mB_m = gl(mu, om, 4); mC_m = gl(mu, omj); mD_m = gl(mu, omj, 4);
+        {
+            const float lb = mbar(2, kb - 1), mb = mbar(1, kb - 1);
+            const unsigned mo = om >> MSH, mj = omj >> MSH;
+            lA_m = mdec(gl(lam, mo), lb); lB_m = mdec(gl(lam, mo, MB), lb);
+            lC_m = mdec(gl(lam, mj), lb); lD_m = mdec(gl(lam, mj, MB), lb);
+            mA_m = mdec(gl(mu, mo), mb); mB_m = mdec(gl(mu, mo, MB), mb);
+            mC_m = mdec(gl(mu, mj), mb); mD_m = mdec(gl(mu, mj, MB), mb);
+        }
         const unsigned qo = om >> QSH, qj = omj >> QSH;
         pA_m = ld(gl(qp, qo)); pB_m = ld(gl(qp, qo, QB)); pC_m = ld(gl(qp, qj)); pD_m = ld(gl(qp, qj, QB));
         sA_m = ld(gl(qs, qo)); sB_m = ld(gl(qs, qo, QB)); sC_m = ld(gl(qs, qj)); sD_m = ld(gl(qs, qj, QB));
@@ -333,8 +360,8 @@ __device__ __forceinline__ void step_tile(const Geo &g, const State &a0, const S
                     const float ut_ip1 = (i < g.nx) ? (pu[lpos + 1] - (pw[lpos + 1] - w0)) : 0.0f;
                     const float vt_jm1 = (j > 1) ? (pv[lpos - Sh::RW] - (w0 - pw[lpos - Sh::RW])) : 0.0f;
                     const unsigned o = ob + (unsigned)g.nz * sz4;
-                    const float xl = 1.0f / gl(lam, o);
-                    const float xm = 2.0f / gl(mu, o);
+                    const float xl = 1.0f / mdec(gl(lam, o >> MSH), mbar(2, g.nz));
+                    const float xm = 2.0f / mdec(gl(mu, o >> MSH), mbar(1, g.nz));
                     const float xl2m = xm + xl;
                     const float wt = pl1[2 * PL + lpos] - (xl / xl2m) * (
                         (ut_ip1 - ut) +

@@ -26,12 +26,12 @@ namespace awp {
 // velocity: uxx1/vyy1/wzz1 (operator.f:92-244) fused with addcrjvel
 // (cerjan.f:292-328)
 // ---------------------------------------------------------------------------
-template <int U>
+template <class P, int U>
 __device__ inline void velocity_tile(const Geo &g, const State &s0, const Coef &c, const int kc, const int bx,
                                      const int by, const int bz)
 {
     const GState s = as_global(s0);
-    // the velocity stage reads only FP32 arrays in both storage modes
+    using M = typename P::M;
     const int i = (bx * blockDim.z + threadIdx.z) * TX + threadIdx.x + 1;
     const int j = by * blockDim.y + threadIdx.y + 1;
     const int kb = bz * kc + 1;
@@ -46,7 +46,7 @@ __device__ inline void velocity_tile(const Geo &g, const State &s0, const Coef &
     const G<float> *__restrict__ xy = s.xy;
     const G<float> *__restrict__ xz = s.xz;
     const G<float> *__restrict__ yz = s.yz;
-    const G<float> *__restrict__ d1 = s.d1;
+    const G<M> *__restrict__ d1 = (const G<M> *)s.d1;
     const float c1 = c.c1, c2 = c.c2, dth = c.dth;
     const float fxy = s.fx[i] * s.fy[j];
 
@@ -55,14 +55,17 @@ __device__ inline void velocity_tile(const Geo &g, const State &s0, const Coef &
     float yz_m2 = yz[o - 2 * sz], yz_m1 = yz[o - sz], yz_0 = yz[o];
     float zz_m1 = zz[o - sz], zz_0 = zz[o], zz_p1 = zz[o + sz];
     // density at level k-1: (i,j), (i,j-1), (i+1,j)
-    float dA_m = ld(d1[o - sz]), dB_m = ld(d1[o - sz - sy]), dC_m = ld(d1[o - sz + 1]);
+    const float db_m = P::mixed ? s.mbar[mbar_idx(g, 0, kb - 1)] : 0.0f;
+    float dA_m = mdec(d1[o - sz], db_m), dB_m = mdec(d1[o - sz - sy], db_m), dC_m = mdec(d1[o - sz + 1], db_m);
 
 #pragma unroll U
     for (int k = kb; k <= ke; ++k, o += sz) {
         const float xz_p1 = xz[o + sz];
         const float yz_p1 = yz[o + sz];
         const float zz_p2 = nt_ld(&zz[o + 2 * sz]);   // zz: read at (i, j) only
-        const float dA = ld(d1[o]), dB = ld(d1[o - sy]), dC = ld(d1[o + 1]), dD = ld(d1[o + 1 - sy]);
+        const float db = P::mixed ? s.mbar[mbar_idx(g, 0, k)] : 0.0f;
+        const float dA = mdec(d1[o], db), dB = mdec(d1[o - sy], db), dC = mdec(d1[o + 1], db),
+                    dD = mdec(d1[o + 1 - sy], db);
 
         const float xx_0 = xx[o], xx_m1 = xx[o - 1], xx_p1 = xx[o + 1], xx_m2 = xx[o - 2];
         const float xy_0 = xy[o], xy_jm1 = xy[o - sy], xy_jp1 = xy[o + sy], xy_jm2 = xy[o - 2 * sy];
@@ -105,7 +108,7 @@ __device__ inline void velocity_tile(const Geo &g, const State &s0, const Coef &
 // The same statements through a 32-bit byte offset (Cursor32: uniform array
 // bases, one offset register, neighbours as immediates); needs every array
 // below 4 GiB.
-template <int U>
+template <class P, int U>
 __device__ inline void velocity_tile_b(const Geo &g, const State &s0, const Coef &c, const int kc, const int bx,
                                        const int by, const int bz)
 {
@@ -123,21 +126,26 @@ __device__ inline void velocity_tile_b(const Geo &g, const State &s0, const Coef
     const G<float> *__restrict__ xy = s.xy;
     const G<float> *__restrict__ xz = s.xz;
     const G<float> *__restrict__ yz = s.yz;
-    const G<float> *__restrict__ d1 = s.d1;
+    using M = typename P::M;
+    const G<M> *__restrict__ d1 = (const G<M> *)s.d1;
     const float c1 = c.c1, c2 = c.c2, dth = c.dth;
     const float fxy = s.fx[i] * s.fy[j];
 
     float xz_m2 = p.ld(xz, 0, 0, -2), xz_m1 = p.ld(xz, 0, 0, -1), xz_0 = p.ld(xz);
     float yz_m2 = p.ld(yz, 0, 0, -2), yz_m1 = p.ld(yz, 0, 0, -1), yz_0 = p.ld(yz);
     float zz_m1 = p.ld(zz, 0, 0, -1), zz_0 = p.ld(zz), zz_p1 = p.ld(zz, 0, 0, 1);
-    float dA_m = ld(p.ld(d1, 0, 0, -1)), dB_m = ld(p.ld(d1, 0, -1, -1)), dC_m = ld(p.ld(d1, 1, 0, -1));
+    const float db_m = P::mixed ? s.mbar[mbar_idx(g, 0, kb - 1)] : 0.0f;
+    float dA_m = mdec(p.ld(d1, 0, 0, -1), db_m), dB_m = mdec(p.ld(d1, 0, -1, -1), db_m),
+          dC_m = mdec(p.ld(d1, 1, 0, -1), db_m);
 
 #pragma unroll U
     for (int k = kb; k <= ke; ++k, p.up()) {
         const float xz_p1 = p.ld(xz, 0, 0, 1);
         const float yz_p1 = p.ld(yz, 0, 0, 1);
         const float zz_p2 = gln(zz, p.at(0, 2));   // zz: read at (i, j) only
-        const float dA = ld(p.ld(d1)), dB = ld(p.ld(d1, 0, -1)), dC = ld(p.ld(d1, 1)), dD = ld(p.ld(d1, 1, -1));
+        const float db = P::mixed ? s.mbar[mbar_idx(g, 0, k)] : 0.0f;
+        const float dA = mdec(p.ld(d1), db), dB = mdec(p.ld(d1, 0, -1), db), dC = mdec(p.ld(d1, 1), db),
+                    dD = mdec(p.ld(d1, 1, -1), db);
 
         const float xx_0 = p.ld(xx), xx_m1 = p.ld(xx, -1), xx_p1 = p.ld(xx, 1), xx_m2 = p.ld(xx, -2);
         const float xy_0 = p.ld(xy), xy_jm1 = p.ld(xy, 0, -1), xy_jp1 = p.ld(xy, 0, 1), xy_jm2 = p.ld(xy, 0, -2);
@@ -182,6 +190,7 @@ __device__ inline void velocity_tile_b(const Geo &g, const State &s0, const Coef
 // u(i+1), v(j-1) at nz+1 are recomputed from the same inputs fvelxy uses,
 // so no thread reads another thread's output.
 // ---------------------------------------------------------------------------
+template <class P>
 __device__ inline void fs_velocity_block(const Geo &g, const State &s0, const int bx, const int by)
 {
     const GState s = as_global(s0);
@@ -197,8 +206,11 @@ __device__ inline void fs_velocity_block(const Geo &g, const State &s0, const in
     const float vt = v1[p] - (w1[p + sy] - w0);
     const float ut_ip1 = (i < g.nx) ? (u1[p + 1] - (w1[p + 1] - w0)) : 0.0f;
     const float vt_jm1 = (j > 1) ? (v1[p - sy] - (w0 - w1[p - sy])) : 0.0f;
-    const float xl = 1.0f / s.lam[p];
-    const float xm = 2.0f / s.mu[p];
+    using M = typename P::M;
+    const float mb = P::mixed ? s.mbar[mbar_idx(g, 1, g.nz)] : 0.0f;
+    const float lb = P::mixed ? s.mbar[mbar_idx(g, 2, g.nz)] : 0.0f;
+    const float xl = 1.0f / mdec(((const G<M> *)s.lam)[p], lb);
+    const float xm = 2.0f / mdec(((const G<M> *)s.mu)[p], mb);
     const float xl2m = xm + xl;
     const float wt = w1[p - sz] - (xl / xl2m) * (
         (ut_ip1 - ut) +
@@ -253,8 +265,9 @@ __device__ inline void stress_tile_e(const Geo &g, const State &s0, const Coef &
     const G<float> *__restrict__ u1 = s.u1;
     const G<float> *__restrict__ v1 = s.v1;
     const G<float> *__restrict__ w1 = s.w1;
-    const G<float> *__restrict__ lam = s.lam;
-    const G<float> *__restrict__ mu = s.mu;
+    using M = typename P::M;
+    const G<M> *__restrict__ lam = (const G<M> *)s.lam;
+    const G<M> *__restrict__ mu = (const G<M> *)s.mu;
     const G<Q> *__restrict__ qp = (const G<Q> *)s.qp;
     const G<Q> *__restrict__ qs = (const G<Q> *)s.qs;
     G<R> *__restrict__ R1 = (G<R> *)s.r1;
@@ -277,8 +290,12 @@ __device__ inline void stress_tile_e(const Geo &g, const State &s0, const Coef &
     float w_m2 = w1[o - 2 * sz], w_m1 = w1[o - sz], w_0 = w1[o];
     // media at level k-1 on the 2x2 (x,y) cell: A=(i,j) B=(i+1,j) C=(i,j-1) D=(i+1,j-1)
     long long om = o - sz;
-    float lA_m = ld(lam[om]), lB_m = ld(lam[om + 1]), lC_m = ld(lam[om - sy]), lD_m = ld(lam[om + 1 - sy]);
-    float mA_m = ld(mu[om]), mB_m = ld(mu[om + 1]), mC_m = ld(mu[om - sy]), mD_m = ld(mu[om + 1 - sy]);
+    const float mb_m = P::mixed ? s.mbar[mbar_idx(g, 1, kb - 1)] : 0.0f;
+    const float lb_m = P::mixed ? s.mbar[mbar_idx(g, 2, kb - 1)] : 0.0f;
+    float lA_m = mdec(lam[om], lb_m), lB_m = mdec(lam[om + 1], lb_m), lC_m = mdec(lam[om - sy], lb_m),
+          lD_m = mdec(lam[om + 1 - sy], lb_m);
+    float mA_m = mdec(mu[om], mb_m), mB_m = mdec(mu[om + 1], mb_m), mC_m = mdec(mu[om - sy], mb_m),
+          mD_m = mdec(mu[om + 1 - sy], mb_m);
     float pA_m = ld(qp[om]), pB_m = ld(qp[om + 1]), pC_m = ld(qp[om - sy]), pD_m = ld(qp[om + 1 - sy]);
     float sA_m = ld(qs[om]), sB_m = ld(qs[om + 1]), sC_m = ld(qs[om - sy]), sD_m = ld(qs[om + 1 - sy]);
     // final xz/yz/zz of the two levels below (for the free-surface images)
@@ -289,8 +306,12 @@ __device__ inline void stress_tile_e(const Geo &g, const State &s0, const Coef &
         const float u_p2 = u1[o + 2 * sz];
         const float v_p2 = v1[o + 2 * sz];
         const float w_p1 = w1[o + sz];
-        const float lA = ld(lam[o]), lB = ld(lam[o + 1]), lC = ld(lam[o - sy]), lD = ld(lam[o + 1 - sy]);
-        const float mA = ld(mu[o]), mB = ld(mu[o + 1]), mC = ld(mu[o - sy]), mD = ld(mu[o + 1 - sy]);
+        const float mb = P::mixed ? s.mbar[mbar_idx(g, 1, k)] : 0.0f;
+        const float lb = P::mixed ? s.mbar[mbar_idx(g, 2, k)] : 0.0f;
+        const float lA = mdec(lam[o], lb), lB = mdec(lam[o + 1], lb), lC = mdec(lam[o - sy], lb),
+                    lD = mdec(lam[o + 1 - sy], lb);
+        const float mA = mdec(mu[o], mb), mB = mdec(mu[o + 1], mb), mC = mdec(mu[o - sy], mb),
+                    mD = mdec(mu[o + 1 - sy], mb);
         const float pA = ld(qp[o]), pB = ld(qp[o + 1]), pC = ld(qp[o - sy]), pD = ld(qp[o + 1 - sy]);
         const float sA = ld(qs[o]), sB = ld(qs[o + 1]), sC = ld(qs[o - sy]), sD = ld(qs[o + 1 - sy]);
 
@@ -457,8 +478,9 @@ __device__ inline void stress_tile_b(const Geo &g, const State &s0, const Coef &
     const G<float> *__restrict__ u1 = s.u1;
     const G<float> *__restrict__ v1 = s.v1;
     const G<float> *__restrict__ w1 = s.w1;
-    const G<float> *__restrict__ lam = s.lam;
-    const G<float> *__restrict__ mu = s.mu;
+    using M = typename P::M;
+    const G<M> *__restrict__ lam = (const G<M> *)s.lam;
+    const G<M> *__restrict__ mu = (const G<M> *)s.mu;
     const G<Q> *__restrict__ qp = (const G<Q> *)s.qp;
     const G<Q> *__restrict__ qs = (const G<Q> *)s.qs;
     G<R> *__restrict__ R1 = (G<R> *)s.r1;
@@ -480,8 +502,12 @@ __device__ inline void stress_tile_b(const Geo &g, const State &s0, const Coef &
     float v_m1 = p.ld(v1, 0, 0, -1), v_0 = p.ld(v1), v_p1 = p.ld(v1, 0, 0, 1);
     float w_m2 = p.ld(w1, 0, 0, -2), w_m1 = p.ld(w1, 0, 0, -1), w_0 = p.ld(w1);
     // media at level k-1 on the 2x2 (x,y) cell: A=(i,j) B=(i+1,j) C=(i,j-1) D=(i+1,j-1)
-    float lA_m = p.ld(lam, 0, 0, -1), lB_m = p.ld(lam, 1, 0, -1), lC_m = p.ld(lam, 0, -1, -1), lD_m = p.ld(lam, 1, -1, -1);
-    float mA_m = p.ld(mu, 0, 0, -1), mB_m = p.ld(mu, 1, 0, -1), mC_m = p.ld(mu, 0, -1, -1), mD_m = p.ld(mu, 1, -1, -1);
+    const float mb_m = P::mixed ? s.mbar[mbar_idx(g, 1, kb - 1)] : 0.0f;
+    const float lb_m = P::mixed ? s.mbar[mbar_idx(g, 2, kb - 1)] : 0.0f;
+    float lA_m = mdec(p.ld(lam, 0, 0, -1), lb_m), lB_m = mdec(p.ld(lam, 1, 0, -1), lb_m),
+          lC_m = mdec(p.ld(lam, 0, -1, -1), lb_m), lD_m = mdec(p.ld(lam, 1, -1, -1), lb_m);
+    float mA_m = mdec(p.ld(mu, 0, 0, -1), mb_m), mB_m = mdec(p.ld(mu, 1, 0, -1), mb_m),
+          mC_m = mdec(p.ld(mu, 0, -1, -1), mb_m), mD_m = mdec(p.ld(mu, 1, -1, -1), mb_m);
     float pA_m = ld(p.ld(qp, 0, 0, -1)), pB_m = ld(p.ld(qp, 1, 0, -1));
     float pC_m = ld(p.ld(qp, 0, -1, -1)), pD_m = ld(p.ld(qp, 1, -1, -1));
     float sA_m = ld(p.ld(qs, 0, 0, -1)), sB_m = ld(p.ld(qs, 1, 0, -1));
@@ -498,8 +524,12 @@ __device__ inline void stress_tile_b(const Geo &g, const State &s0, const Coef &
         const float u_p2 = p.ld(u1, 0, 0, 2);
         const float v_p2 = p.ld(v1, 0, 0, 2);
         const float w_p1 = p.ld(w1, 0, 0, 1);
-        const float lA = p.ld(lam), lB = p.ld(lam, 1), lC = p.ld(lam, 0, -1), lD = p.ld(lam, 1, -1);
-        const float mA = p.ld(mu), mB = p.ld(mu, 1), mC = p.ld(mu, 0, -1), mD = p.ld(mu, 1, -1);
+        const float mb = P::mixed ? s.mbar[mbar_idx(g, 1, k)] : 0.0f;
+        const float lb = P::mixed ? s.mbar[mbar_idx(g, 2, k)] : 0.0f;
+        const float lA = mdec(p.ld(lam), lb), lB = mdec(p.ld(lam, 1), lb), lC = mdec(p.ld(lam, 0, -1), lb),
+                    lD = mdec(p.ld(lam, 1, -1), lb);
+        const float mA = mdec(p.ld(mu), mb), mB = mdec(p.ld(mu, 1), mb), mC = mdec(p.ld(mu, 0, -1), mb),
+                    mD = mdec(p.ld(mu, 1, -1), mb);
         const float pA = ld(p.ld(qp)), pB = ld(p.ld(qp, 1)), pC = ld(p.ld(qp, 0, -1)), pD = ld(p.ld(qp, 1, -1));
         const float sA = ld(p.ld(qs)), sB = ld(p.ld(qs, 1)), sC = ld(p.ld(qs, 0, -1)), sD = ld(p.ld(qs, 1, -1));
 
@@ -677,7 +707,7 @@ __device__ inline const SJob &split_tile(const SJob *__restrict__ jobs, const in
 // each other's halo lines from that XCD's L2.
 // MAXT: most threads per block -- 512 for tiles of up to 8 rows, 1024 for
 // 16-row tiles (whose 16 waves leave 128 registers a lane)
-template <int U, int MAXT, bool O32>
+template <class P, int U, int MAXT, bool O32>
 __global__ void __launch_bounds__(MAXT)
 k_velocity_batch(const SJob *__restrict__ jobs, const int njobs, const int total, const int per)
 {
@@ -688,9 +718,9 @@ k_velocity_batch(const SJob *__restrict__ jobs, const int njobs, const int total
         int bx, by, bz;
         const SJob &J = split_tile(jobs, njobs, tile, bx, by, bz);
         if constexpr (O32)
-            velocity_tile_b<U>(J.g, J.s, J.c, J.kc, bx, by, bz);
+            velocity_tile_b<P, U>(J.g, J.s, J.c, J.kc, bx, by, bz);
         else
-            velocity_tile<U>(J.g, J.s, J.c, J.kc, bx, by, bz);
+            velocity_tile<P, U>(J.g, J.s, J.c, J.kc, bx, by, bz);
     }
 }
 
@@ -710,6 +740,7 @@ k_stress_batch(const SJob *__restrict__ jobs, const int njobs, const int total, 
 
 // free-surface images of every run: block b of the launch is block
 // b - fs0 of its run's (nx/64) x (ny/4) surface grid
+template <class P>
 __global__ void __launch_bounds__(256)
 k_fs_velocity_batch(const SJob *__restrict__ jobs, const int njobs)
 {
@@ -719,7 +750,7 @@ k_fs_velocity_batch(const SJob *__restrict__ jobs, const int njobs)
     const SJob &J = jobs[jb];
     const int lb = b - J.fs0;
     const int fnx = (J.g.nx + TX - 1) / TX;
-    fs_velocity_block(J.g, J.s, lb % fnx, lb / fnx);
+    fs_velocity_block<P>(J.g, J.s, lb % fnx, lb / fnx);
 }
 
 // ---------------------------------------------------------------------------
@@ -730,7 +761,6 @@ __device__ void apply_sources(const State &s, const int igreen, const int npsrc,
                               const long long *psrc_top, const long long sz, const float *stf, const int nst,
                               const int istep, const float vtst, const float vtst1)
 {
-    const float *d1 = s.d1;
     for (int n = 0; n < npsrc; ++n) {
         const float *a = stf + ((size_t)n * nst + (istep - 1)) * 6;
         const float axx = a[0], ayy = a[1], azz = a[2];
@@ -739,7 +769,10 @@ __device__ void apply_sources(const State &s, const int igreen, const int npsrc,
         float add = 0.0f;
         if (igreen >= 1 && igreen <= 3) {
             const float amp = igreen == 1 ? axx : (igreen == 2 ? ayy : azz);
-            add = amp * vtst / d1[p];
+            // density at the source: FP32, or (mixed storage, s.mbar set)
+            // FP16 under the profile of its level p / sz - 1
+            const float dd = s.mbar ? mdec(((const _Float16 *)s.d1)[p], s.mbar[p / sz]) : ((const float *)s.d1)[p];
+            add = amp * vtst / dd;
         }
         switch (igreen) {
         case 1: s.u1[p] = s.u1[p] + add; break;
@@ -788,15 +821,25 @@ __global__ void k_source(const State s, const int igreen, const int npsrc, const
 
 // ---------------------------------------------------------------------------
 // SGT: moduli coefficients at the output nodes (pmcl3d.f:1158-1170), from the
-// FP32 media, and the strain gather (pmcl3d.f:1546-1557).  out is
-// [npts][6][nsamp_max].
+// stored media (mixed storage, s.mbar set: decoded under the profile of the
+// node's level kpos / sz - 1), and the strain gather (pmcl3d.f:1546-1557).
+// out is [npts][6][nsamp_max].
 // ---------------------------------------------------------------------------
-__global__ void k_sgt_coef(const Media32 m, const int nkept, const long long *kpos, float *sg)
+__global__ void k_sgt_coef(const State s, const long long sz, const int pz, const int nkept, const long long *kpos,
+                           float *sg)
 {
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= nkept) return;
     const long long p = kpos[n];
-    const float lam = m.lam[p], mu = m.mu[p];
+    float lam, mu;
+    if (s.mbar) {
+        const long long q = p / sz;   // profile index of the level
+        mu = mdec(((const _Float16 *)s.mu)[p], s.mbar[pz + q]);
+        lam = mdec(((const _Float16 *)s.lam)[p], s.mbar[2 * pz + q]);
+    } else {
+        mu = ((const float *)s.mu)[p];
+        lam = ((const float *)s.lam)[p];
+    }
     sg[3 * n + 0] = (1.0f / lam + 1.0f / mu) / (1.0f / mu * (3.0f / lam + 2.0f / mu));
     sg[3 * n + 1] = -((1.0f / lam) / (2.0f / mu * (3.0f / lam + 2.0f / mu)));
     sg[3 * n + 2] = mu;
@@ -946,6 +989,63 @@ __global__ void k_media_corner(const Geo g, const Media32 m)
     m.d1[p] = v;
 }
 
+// Mixed storage of d1, mu, lam.  First, per level k = 1..nz, the smallest
+// and the largest nonzero value of the level's interior (i = 1..nx,
+// j = 1..ny): order-preserving integer keys of the floats meet in atomics,
+// so the result does not depend on the order of the blocks.  Block (y, k)
+// takes rows j = 16y+1 .. 16y+16 of level k.
+__device__ inline unsigned float_key(float f)
+{
+    const unsigned b = __float_as_uint(f);
+    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+
+__global__ void __launch_bounds__(256) k_level_range(const Geo g, const float *a, unsigned *range)
+{
+    __shared__ unsigned red[2][256];
+    const int k = blockIdx.y + 1;
+    const int j0 = blockIdx.x * 16 + 1, j1 = min(j0 + 15, g.ny);
+    unsigned lo = 0xFFFFFFFFu, hi = 0u;
+    for (int j = j0; j <= j1; ++j)
+        for (int i = threadIdx.x + 1; i <= g.nx; i += blockDim.x) {
+            const float v = a[gidx(g, i, j, k)];
+            if (v != 0.0f) {
+                const unsigned key = float_key(v);
+                lo = min(lo, key);
+                hi = max(hi, key);
+            }
+        }
+    red[0][threadIdx.x] = lo;
+    red[1][threadIdx.x] = hi;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            red[0][threadIdx.x] = min(red[0][threadIdx.x], red[0][threadIdx.x + w]);
+            red[1][threadIdx.x] = max(red[1][threadIdx.x], red[1][threadIdx.x + w]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        atomicMin(range + 2 * (k - 1), red[0][0]);
+        atomicMax(range + 2 * (k - 1) + 1, red[1][0]);
+    }
+}
+
+// Then every cell of the padded volume (ghosts included) as the FP16 (round
+// to nearest even) of (float)(x / b - 1) in double, b = bar[z] the profile
+// value of its level plane z = blockIdx.y; a level without a profile value
+// (b = 0) holds zeros.
+__global__ void __launch_bounds__(256)
+k_media_encode(const float *src, _Float16 *dst, const float *bar, const long long sz)
+{
+    const float b = bar[blockIdx.y];
+    const long long z0 = (long long)blockIdx.y * sz;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < sz; t += (long long)gridDim.x * blockDim.x) {
+        const float x = src[z0 + t];
+        dst[z0 + t] = b != 0.0f ? (_Float16)(float)((double)x / (double)b - 1.0) : (_Float16)0.0f;
+    }
+}
+
 __global__ void __launch_bounds__(256) k_absmax(const float *a, const long long n, float *part)
 {
     __shared__ float red[256];
@@ -1087,6 +1187,18 @@ k_pack_r(const Geo g, const _Float16 *src, const signed char *ex, const int q, c
     dst[((size_t)(k - 1 - k0) * g.ny + (j - 1)) * g.nx + (i - 1)] = v;
 }
 
+// levels k0+1 .. k0+gridDim.z of d1, mu or lam in mixed storage (FP16
+// deviations under the level profile bar[k + 1]), as [k - 1 - k0][j - 1][i - 1]
+__global__ void __launch_bounds__(256)
+k_pack_media(const Geo g, const _Float16 *src, const float *bar, const int kext, const int k0, float *dst)
+{
+    const int i = blockIdx.x * TX + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int k = blockIdx.z + 1 + k0;
+    if (i > g.nx || j > g.ny || k > g.nz + kext) return;
+    dst[((size_t)(k - 1 - k0) * g.ny + (j - 1)) * g.nx + (i - 1)] = mdec(src[gidx(g, i, j, k)], bar[k + 1]);
+}
+
 // levels k0+1 .. k0+gridDim.z of one field, as [k - 1 - k0][j - 1][i - 1]
 template <class T>
 __global__ void __launch_bounds__(256)
@@ -1111,10 +1223,10 @@ hipError_t launch_source(const Geo &g, const State &s, int igreen, int npsrc, co
     return hipGetLastError();
 }
 
-hipError_t launch_sgt_coef(const Media32 &m, int nkept, const long long *kpos, float *sg, hipStream_t st)
+hipError_t launch_sgt_coef(const Geo &g, const State &s, int nkept, const long long *kpos, float *sg, hipStream_t st)
 {
     if (nkept <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_sgt_coef, dim3((nkept + 255) / 256), dim3(256), 0, st, m, nkept, kpos, sg);
+    hipLaunchKernelGGL(k_sgt_coef, dim3((nkept + 255) / 256), dim3(256), 0, st, s, g.sz, g.pz, nkept, kpos, sg);
     return hipGetLastError();
 }
 
@@ -1150,13 +1262,13 @@ static int split_grid_t(int device, const Launch &l, int *gs)
     // arrays need 64-bit offsets walks more tiles per block)
     hipError_t ev, es;
     if (tall(l)) {
-        ev = l.o32_v ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nv, k_velocity_batch<1, 1024, true>, nt, l.lds_v)
-                     : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nv, k_velocity_batch<1, 1024, false>, nt,
+        ev = l.o32_v ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nv, k_velocity_batch<P, 1, 1024, true>, nt, l.lds_v)
+                     : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nv, k_velocity_batch<P, 1, 1024, false>, nt,
                                                                     l.lds_v);
         es = hipOccupancyMaxActiveBlocksPerMultiprocessor(&ns, k_stress_batch<P, 1, true, 1024>, nt, l.lds_s);
     } else {
-        ev = l.o32_v ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nv, k_velocity_batch<U, 512, true>, nt, l.lds_v)
-                     : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nv, k_velocity_batch<U, 512, false>, nt, l.lds_v);
+        ev = l.o32_v ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nv, k_velocity_batch<P, U, 512, true>, nt, l.lds_v)
+                     : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nv, k_velocity_batch<P, U, 512, false>, nt, l.lds_v);
         es = l.o32_s ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&ns, k_stress_batch<P, U, true, 512>, nt, l.lds_s)
                      : hipOccupancyMaxActiveBlocksPerMultiprocessor(&ns, k_stress_batch<P, U, false, 512>, nt,
                                                                     l.lds_s);
@@ -1183,15 +1295,32 @@ void split_batch_grids(bool mixed, const Launch &l, int device, int *grid_v, int
 // whole XCD rounds of persistent blocks, no more than the tiles need
 static int persistent_grid(int grid, int per) { return std::max(8, std::min(grid, 8 * per)) / 8 * 8; }
 
-template <int U, int MAXT>
+template <class P, int U, int MAXT>
 static void vel_batch_t(const Launch &l, bool ov, const SJob *d_jobs, int njobs, int total, int per, int grid_v,
                         hipStream_t st)
 {
     const dim3 grid(persistent_grid(grid_v, per)), block(TX, l.tile_y, l.tile_xw);
     if (ov)
-        hipLaunchKernelGGL((k_velocity_batch<U, MAXT, true>), grid, block, l.lds_v, st, d_jobs, njobs, total, per);
+        hipLaunchKernelGGL((k_velocity_batch<P, U, MAXT, true>), grid, block, l.lds_v, st, d_jobs, njobs, total,
+                           per);
     else
-        hipLaunchKernelGGL((k_velocity_batch<U, MAXT, false>), grid, block, l.lds_v, st, d_jobs, njobs, total, per);
+        hipLaunchKernelGGL((k_velocity_batch<P, U, MAXT, false>), grid, block, l.lds_v, st, d_jobs, njobs, total,
+                           per);
+}
+
+template <class P>
+static hipError_t velocity_stage(const Launch &l, bool ov, const SJob *d_jobs, int njobs, int total, int per,
+                                 int grid_v, hipStream_t st)
+{
+    if (tall(l)) {
+        if (l.unroll == 2) return hipErrorInvalidValue;   // no such variant
+        vel_batch_t<P, 1, 1024>(l, ov, d_jobs, njobs, total, per, grid_v, st);
+    } else if (l.unroll == 2) {
+        vel_batch_t<P, 2, 512>(l, ov, d_jobs, njobs, total, per, grid_v, st);
+    } else {
+        vel_batch_t<P, 1, 512>(l, ov, d_jobs, njobs, total, per, grid_v, st);
+    }
+    return hipSuccess;
 }
 
 template <class P, int U, bool O32, int MAXT>
@@ -1221,27 +1350,25 @@ static hipError_t stress_stage(const Launch &l, bool os, const SJob *d_jobs, int
     return hipSuccess;
 }
 
-hipError_t launch_split_velocity(const Launch &l, bool o32, const SJob *d_jobs, int njobs, int total, int grid_v,
-                                 hipStream_t st)
+hipError_t launch_split_velocity(bool mixed, const Launch &l, bool o32, const SJob *d_jobs, int njobs, int total,
+                                 int grid_v, hipStream_t st)
 {
     if (njobs <= 0 || total <= 0) return hipSuccess;
     const int per = (total + 7) / 8;
     const bool ov = o32 && l.o32_v;
-    if (tall(l)) {
-        if (l.unroll == 2) return hipErrorInvalidValue;   // no such variant
-        vel_batch_t<1, 1024>(l, ov, d_jobs, njobs, total, per, grid_v, st);
-    } else if (l.unroll == 2) {
-        vel_batch_t<2, 512>(l, ov, d_jobs, njobs, total, per, grid_v, st);
-    } else {
-        vel_batch_t<1, 512>(l, ov, d_jobs, njobs, total, per, grid_v, st);
-    }
+    const hipError_t e = mixed ? velocity_stage<StoreMixed>(l, ov, d_jobs, njobs, total, per, grid_v, st)
+                               : velocity_stage<StoreF32>(l, ov, d_jobs, njobs, total, per, grid_v, st);
+    if (e != hipSuccess) return e;
     return hipGetLastError();
 }
 
-hipError_t launch_split_fs(const SJob *d_jobs, int njobs, int fs_total, hipStream_t st)
+hipError_t launch_split_fs(bool mixed, const SJob *d_jobs, int njobs, int fs_total, hipStream_t st)
 {
     if (njobs <= 0 || fs_total <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_fs_velocity_batch, dim3(fs_total), dim3(TX, 4), 0, st, d_jobs, njobs);
+    if (mixed)
+        hipLaunchKernelGGL(k_fs_velocity_batch<StoreMixed>, dim3(fs_total), dim3(TX, 4), 0, st, d_jobs, njobs);
+    else
+        hipLaunchKernelGGL(k_fs_velocity_batch<StoreF32>, dim3(fs_total), dim3(TX, 4), 0, st, d_jobs, njobs);
     return hipGetLastError();
 }
 
@@ -1260,8 +1387,8 @@ hipError_t launch_split_stress(bool mixed, const Launch &l, bool o32, const SJob
 hipError_t launch_step_split_batch(bool mixed, const Launch &l, bool o32, const SJob *d_jobs, int njobs,
                                    int total, int fs_total, int grid_v, int grid_s, hipStream_t st)
 {
-    hipError_t e = launch_split_velocity(l, o32, d_jobs, njobs, total, grid_v, st);
-    if (e == hipSuccess) e = launch_split_fs(d_jobs, njobs, fs_total, st);
+    hipError_t e = launch_split_velocity(mixed, l, o32, d_jobs, njobs, total, grid_v, st);
+    if (e == hipSuccess) e = launch_split_fs(mixed, d_jobs, njobs, fs_total, st);
     if (e == hipSuccess) e = launch_split_stress(mixed, l, o32, d_jobs, njobs, total, grid_s, st);
     return e;
 }
@@ -1299,6 +1426,19 @@ hipError_t launch_media_ghosts(const Geo &g, const Media32 &m, hipStream_t st)
     return hipGetLastError();
 }
 
+hipError_t launch_level_range(const Geo &g, const float *a, unsigned *range, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_level_range, dim3((g.ny + 15) / 16, g.nz), dim3(256), 0, st, g, a, range);
+    return hipGetLastError();
+}
+
+hipError_t launch_media_encode(const Geo &g, const float *src, _Float16 *dst, const float *bar, hipStream_t st)
+{
+    const int bx = (int)std::min<long long>((g.sz + 255) / 256, 1024);
+    hipLaunchKernelGGL(k_media_encode, dim3(bx, g.pz), dim3(256), 0, st, src, dst, bar, g.sz);
+    return hipGetLastError();
+}
+
 hipError_t launch_absmax(const float *a, long long n, float *part, int nblk, hipStream_t st)
 {
     hipLaunchKernelGGL(k_absmax, dim3(nblk), dim3(256), 0, st, a, n, part);
@@ -1325,13 +1465,18 @@ hipError_t launch_pack_levels(bool mixed, const Geo &g, const State &s, int whic
     dim3 grd((g.nx + TX - 1) / TX, (g.ny + 3) / 4, nk);
     const float *f32[9] = {s.u1, s.v1, s.w1, s.xx, s.yy, s.zz, s.xy, s.xz, s.yz};
     void *rr[6] = {s.r1, s.r2, s.r3, s.r4, s.r5, s.r6};
-    const float *m32[3] = {s.d1, s.mu, s.lam};
+    const void *mm[3] = {s.d1, s.mu, s.lam};
     void *qq[2] = {s.qp, s.qs};
     const float iqs[2] = {s.ms.iqp, s.ms.iqs};
     if (which < 9) {
         hipLaunchKernelGGL(k_pack_field<float>, grd, blk, 0, st, g, f32[which], 1.0f, kext, k0, dst);
     } else if (which >= 15 && which < 18) {
-        hipLaunchKernelGGL(k_pack_field<float>, grd, blk, 0, st, g, m32[which - 15], 1.0f, kext, k0, dst);
+        if (mixed)
+            hipLaunchKernelGGL(k_pack_media, grd, blk, 0, st, g, (const _Float16 *)mm[which - 15],
+                               s.mbar + (size_t)(which - 15) * g.pz, kext, k0, dst);
+        else
+            hipLaunchKernelGGL(k_pack_field<float>, grd, blk, 0, st, g, (const float *)mm[which - 15], 1.0f, kext,
+                               k0, dst);
     } else if (which < 15) {
         if (mixed)
             hipLaunchKernelGGL(k_pack_r, grd, blk, 0, st, g, (const _Float16 *)rr[which - 9],

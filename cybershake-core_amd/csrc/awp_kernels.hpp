@@ -13,16 +13,18 @@
 // on every side hold the reference's halo values (zero outside the domain,
 // free-surface images at k = nz+1, nz+2).
 //
-// Storage: velocities, stresses, the density d1 and the compliances mu/lam
-// are FP32 always.  The memory variables r1..r6 and the attenuation
-// coefficients qp/qs are FP32 (StoreF32) or, in the mixed mode (StoreMixed),
-// FP16 (see StoreMixed).  Each FP16 qp/qs array holds value * 2^e, a per-array power of two
-// that puts its largest magnitude in [2^13, 2^14); the kernels undo it with
-// an exact multiply, so FP16's only effect is the 11-bit rounding of the
-// stored value.  (Density and the compliances stay FP32: their FP16 rounding
-// is constant across a layer of a layered medium, shifts the wave speeds
-// coherently and took the SGT error past 1e-3.)  Arithmetic is FP32 in both
-// modes.
+// Storage: velocities and stresses are FP32 always.  The memory variables
+// r1..r6, the attenuation coefficients qp/qs, the density d1 and the
+// compliances mu/lam are FP32 (StoreF32) or, in the mixed mode (StoreMixed),
+// FP16 (see StoreMixed).  Each FP16 qp/qs array holds value * 2^e, a
+// per-array power of two that puts its largest magnitude in [2^13, 2^14);
+// the kernels undo it with an exact multiply, so FP16's only effect is the
+// 11-bit rounding of the stored value.  d1/mu/lam are stored as FP16
+// deviations h from an FP32 profile value b(k) per level, x = fma(b, h, b)
+// (mdec): the error is FP16's rounding of x/b - 1, zero on a layered medium
+// (a plain FP16 rounding of the values themselves is constant across a layer,
+// shifts the wave speeds coherently and took the SGT error past 1e-3).
+// Arithmetic is FP32 in both modes.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -104,13 +106,27 @@ template <> __device__ inline _Float16 st<_Float16>(float f) { return (_Float16)
 struct StoreF32 {
     using R = float;   // r1..r6
     using Q = float;   // qp, qs
+    using M = float;   // d1, mu, lam
     static constexpr bool mixed = false;
 };
 struct StoreMixed {
     using R = _Float16;   // r * 2^(15-e), e per row segment (see below)
     using Q = _Float16;
+    using M = _Float16;   // x / b(k) - 1, b(k) the level's profile value
     static constexpr bool mixed = true;
 };
+
+// The density and the compliances from their storage: FP32 as stored, or
+// (mixed) the FP16 deviation h from the level's profile value b,
+// x = fma(b, h, b) -- one rounding, the same on the host's fmaf.
+__host__ __device__ __forceinline__ float mdec(float x, float) { return x; }
+__device__ __forceinline__ float mdec(_Float16 h, float b) { return fmaf(b, (float)h, b); }
+// Profile entry of array a (0 d1, 1 mu, 2 lam) at level k (-1 <= k <= nz+2)
+// in the [3][pz] profile table of a mixed-storage run.
+__host__ __device__ __forceinline__ long long mbar_idx(const Geo &g, int a, int k)
+{
+    return (long long)a * g.pz + k + 1;
+}
 
 // Memory variables in mixed storage: each stored value is the FP16 (round to
 // nearest even) of r * 2^(15-e), where e is shared by the 64 points of one x
@@ -179,7 +195,8 @@ struct MScale {
 struct State {
     float *u1, *v1, *w1;
     float *xx, *yy, *zz, *xy, *xz, *yz;
-    float *d1, *mu, *lam;
+    void *d1, *mu, *lam;       // M arrays of the storage mode
+    float *mbar;               // mixed: [3][pz] level profiles of d1, mu, lam (mbar_idx)
     void *r1, *r2, *r3, *r4, *r5, *r6;
     signed char *rexp;         // mixed: rexp_n 8-byte slots of row-segment exponents
     long long rexp_n;          // nz * ny * nseg
@@ -201,7 +218,8 @@ using G = __attribute__((address_space(1))) T;
 struct GState {
     G<float> *u1, *v1, *w1;
     G<float> *xx, *yy, *zz, *xy, *xz, *yz;
-    G<float> *d1, *mu, *lam;
+    G<void> *d1, *mu, *lam;
+    G<float> *mbar;
     G<void> *r1, *r2, *r3, *r4, *r5, *r6;
     G<signed char> *rexp;
     long long rexp_n;
@@ -217,7 +235,8 @@ __device__ __forceinline__ GState as_global(const State &a)
     s.u1 = (G<float> *)a.u1; s.v1 = (G<float> *)a.v1; s.w1 = (G<float> *)a.w1;
     s.xx = (G<float> *)a.xx; s.yy = (G<float> *)a.yy; s.zz = (G<float> *)a.zz;
     s.xy = (G<float> *)a.xy; s.xz = (G<float> *)a.xz; s.yz = (G<float> *)a.yz;
-    s.d1 = (G<float> *)a.d1; s.mu = (G<float> *)a.mu; s.lam = (G<float> *)a.lam;
+    s.d1 = (G<void> *)a.d1; s.mu = (G<void> *)a.mu; s.lam = (G<void> *)a.lam;
+    s.mbar = (G<float> *)a.mbar;
     s.r1 = (G<void> *)a.r1; s.r2 = (G<void> *)a.r2; s.r3 = (G<void> *)a.r3;
     s.r4 = (G<void> *)a.r4; s.r5 = (G<void> *)a.r5; s.r6 = (G<void> *)a.r6;
     s.rexp = (G<signed char> *)a.rexp;
@@ -427,12 +446,19 @@ struct MediaConst {
 hipError_t launch_source(const Geo &g, const State &s, int igreen, int npsrc, const long long *psrc_idx,
                          const long long *psrc_top, const float *stf, int nst, int istep, float vtst,
                          float vtst1, bool apply, float *trace, long long tpos, hipStream_t st);
-hipError_t launch_sgt_coef(const Media32 &m, int nkept, const long long *kpos, float *sg, hipStream_t st);
+// sg1..sg3 at the SGT nodes from the stored mu, lam (decoded in mixed storage)
+hipError_t launch_sgt_coef(const Geo &g, const State &s, int nkept, const long long *kpos, float *sg, hipStream_t st);
 hipError_t launch_sgt_gather(const State &s, int nkept, const long long *kpos, const int *kidx,
                              const float *sg, float *out, int nsamp_max, int sample, hipStream_t st);
 hipError_t launch_media_setup(const Geo &g, const Media32 &m, const float *media, const MediaConst &mc, hipStream_t st);
 hipError_t launch_media_ghosts(const Geo &g, const Media32 &m, hipStream_t st);
 // per-block maxima of |a| over n elements into part[nblk] (nblk <= 4096)
+// per level k = 1..nz of a (FP32 media array): order-preserving keys of the
+// smallest and largest nonzero interior value, atomically into range[2(k-1)],
+// range[2(k-1)+1] (which start at 0xFFFFFFFF and 0)
+hipError_t launch_level_range(const Geo &g, const float *a, unsigned *range, hipStream_t st);
+// dst = FP16 deviations of src from the level profile bar[0..pz-1] (mixed storage)
+hipError_t launch_media_encode(const Geo &g, const float *src, _Float16 *dst, const float *bar, hipStream_t st);
 hipError_t launch_absmax(const float *a, long long n, float *part, int nblk, hipStream_t st);
 hipError_t launch_to_half(const float *src, _Float16 *dst, float scale, long long n, hipStream_t st);
 hipError_t launch_sgt_fp16(const float *sgt, const int *kidx, int nkept, int nsamp_max, int nsamp, float *scales,
@@ -512,9 +538,9 @@ static inline int split_fs_blocks(const SJob &q) { return ((q.g.nx + TX - 1) / T
 // the runs' free-surface blocks; o32 = offsets32() holds for every run
 hipError_t launch_step_split_batch(bool mixed, const Launch &l, bool o32, const SJob *d_jobs, int njobs,
                                    int total, int fs_total, int grid_v, int grid_s, hipStream_t st);
-hipError_t launch_split_velocity(const Launch &l, bool o32, const SJob *d_jobs, int njobs, int total, int grid_v,
+hipError_t launch_split_velocity(bool mixed, const Launch &l, bool o32, const SJob *d_jobs, int njobs, int total, int grid_v,
                                  hipStream_t st);
-hipError_t launch_split_fs(const SJob *d_jobs, int njobs, int fs_total, hipStream_t st);
+hipError_t launch_split_fs(bool mixed, const SJob *d_jobs, int njobs, int fs_total, hipStream_t st);
 hipError_t launch_split_stress(bool mixed, const Launch &l, bool o32, const SJob *d_jobs, int njobs, int total,
                                int grid_s, hipStream_t st);
 void split_batch_grids(bool mixed, const Launch &l, int device, int *grid_v, int *grid_s);

@@ -66,10 +66,11 @@ typedef REAL real;
  * (i = 64s+1 .. 64s+64) at one (j, k): the exponent of the segment's
  * largest |r| (f * 2^e, f in [0.5, 1)) clamped to [-80, 120]; the new values
  * enter the same step's stresses unrounded, as on the device, (b) the attenuation weights
- * qp, qs stored as FP16 of value * 2^e (d1, mu, lam stay FP32), e
- * per array putting its largest magnitude in
- * [2^13, 2^14), and the kernels' exact multiplications by 2^e / 2^-e where
- * the stored values are used.  Arithmetic stays FP32 in the same order.
+ * qp, qs stored as FP16 of value * 2^e, e per array putting its largest
+ * magnitude in [2^13, 2^14), and the kernels' exact multiplications by
+ * 2^e / 2^-e where the stored values are used, (c) the density and the
+ * compliances d1, mu, lam stored as FP16 deviations from an FP32 profile per
+ * level (quantize_media).  Arithmetic stays FP32 in the same order.
  */
 #if defined(MIXED)
 static inline float as_float(uint32_t u) { float f; memcpy(&f, &u, 4); return f; }
@@ -276,6 +277,45 @@ static void setup_media(oracle_t *o, const float *media, int nvar)
                 o->tau[a][b][c] = EXP((real)0.5 * (LOG(taumax * taumin) + LOG(taumax / taumin) * t));
             }
 }
+
+#if defined(MIXED)
+/* d1, mu, lam (structure.f:488-490 after padmedia) to their mixed storage:
+ * per level k a profile value bar(k), the midrange (min + max) / 2 in double
+ * rounded to float of the level's nonzero interior values (i = 1..nx,
+ * j = 1..ny; levels below 1 and above nz take level 1's and level nz's, 0 for
+ * a level without nonzero values), and per cell the FP16 (round to nearest
+ * even) of h = (float)(x / bar - 1) in double, used as fmaf(bar, h, bar).
+ * On a layered medium h = 0 and the values are exact; zeros stay zero. */
+static void quantize_media(oracle_t *o)
+{
+    real *M[3] = {o->d1, o->mu, o->lam};
+    const int nx = o->nx, ny = o->ny, nz = o->nz;
+    float *bar = (float *)malloc(sizeof(float) * (size_t)nz);
+    for (int a = 0; a < 3; ++a) {
+        for (int k = 1; k <= nz; ++k) {
+            float lo = 0.0f, hi = 0.0f;
+            for (int j = 1; j <= ny; ++j)
+                for (int i = 1; i <= nx; ++i) {
+                    const float v = M[a][IX(o, i, j, k)];
+                    if (v == 0.0f) continue;
+                    if (lo == 0.0f || v < lo) lo = v;
+                    if (v > hi) hi = v;
+                }
+            bar[k - 1] = (float)(((double)lo + (double)hi) * 0.5);
+        }
+        for (int k = -2; k <= nz + 2; ++k) {
+            const float b = bar[(k < 1 ? 1 : (k > nz ? nz : k)) - 1];
+            for (int j = -2; j <= ny + 2; ++j)
+                for (int i = -2; i <= nx + 2; ++i) {
+                    real *v = &M[a][IX(o, i, j, k)];
+                    const float h = b != 0.0f ? round_fp16((float)((double)*v / (double)b - 1.0)) : 0.0f;
+                    *v = fmaf(b, h, b);
+                }
+        }
+    }
+    free(bar);
+}
+#endif
 
 /* --- Cerjan profiles, cerjan.f:4-247 on >= 2 ranks per axis ------------- */
 static void setup_cerjan(oracle_t *o)
@@ -632,6 +672,9 @@ void *SFX(awp_oracle_create)(const awp_oracle_params *prm, const float *media,
     o->fz = (real *)calloc(o->nz + 2, sizeof(real));
 
     setup_media(o, media, prm->nvar);
+#if defined(MIXED)
+    quantize_media(o);
+#endif
     setup_cerjan(o);
 
     /* sources: rdsrcpos (source.f:170-198) + getsrc (source.f:378-402) */
@@ -682,11 +725,12 @@ void *SFX(awp_oracle_create)(const awp_oracle_params *prm, const float *media,
     }
     for (int a = 0; a < 5; ++a) { o->msc[a] = 1; o->misc[a] = 1; }
 #if defined(MIXED)
-    /* media to FP16 storage of value * 2^e (the SGT coefficients above were
-     * taken from the FP32 values, as the device does) */
+    /* qp, qs to FP16 storage of value * 2^e (d1, mu, lam were stored as
+     * deviations from their level profiles by quantize_media, before the
+     * SGT coefficients above were taken from them, as the device does) */
     {
         real *M[5] = {o->d1, o->mu, o->lam, o->qp, o->qs};
-        for (int a = 3; a < 5; ++a) {     /* d1, mu, lam stay FP32 */
+        for (int a = 3; a < 5; ++a) {
             float mx = 0.0f;
             for (size_t q = 0; q < o->nvol; ++q) mx = fmaxf(mx, fabsf(M[a][q]));
             int ex = 0;

@@ -158,7 +158,8 @@ class Oracle:
     """One SGT run of the CPU restatement.  ``dtype`` float32 (reference
     operation order) or float64 (FP64 companion); ``mixed=True`` selects the
     restatement of the device's mixed-storage mode (FP32 arithmetic, FP16
-    memory variables under per-row exponents, FP16 qp/qs)."""
+    memory variables under per-row exponents, FP16 qp/qs, FP16 d1/mu/lam
+    deviations from per-level profiles)."""
 
     def __init__(self, in3d: Dict[str, object], media: np.ndarray, src_xyz: np.ndarray,
                  stf: np.ndarray, pts: np.ndarray, dtype=np.float32, mixed: bool = False):

@@ -3,8 +3,11 @@ the 800x600x200 / 1500-step case (layered medium of the 200x200x100
 reference case, 2000 SGT points; tests/golden/make_fp64.py), per component:
 relative L2 over every 4th point and peak error relative to the component's
 peak.  Bounds: FP32 <= 1e-4 (the FP32 parity bound), mixed storage (FP16
-r1..r6, FP16 qp/qs) <= 1e-3, and the mixed run's FP16 SGT file (per point and
-component FP32 scale) <= 1e-3 as well."""
+r1..r6, FP16 qp/qs, FP16 d1/mu/lam deviations from level profiles) <= 1e-3,
+and the mixed run's FP16 SGT file (per point and component FP32 scale) <= 1e-3
+as well.  On the layered medium the media deviations are zero; a second case
+on a von Karman random medium (the site recipe, where they are not) states
+the same errors against an FP64 run made in the test."""
 import os
 
 import numpy as np
@@ -57,3 +60,45 @@ def test_c4_sgt_error_vs_fp64(c4_dir, tmp_path):
     for name, rel, peak in rows:
         print("%-16s rel L2 %s | peak %s" % (name, " ".join("%s %.2e" % (c, e) for c, e in zip(COMP, rel)),
                                                " ".join("%.2e" % e for e in peak)))
+
+
+@pytest.mark.timeout(600)
+def test_random_medium_sgt_error_vs_fp64():
+    """200x160x100, h = 100 m, dt = 0.005 s, 800 steps on the site media
+    recipe (Vs 500 -> 4500 m/s times a 5 % von Karman perturbation, seed 7),
+    X force at the centre 10 cells deep, 2000 SGT points: FP32 and mixed
+    storage against the FP64 restatement run here (16 threads)."""
+    from awpsgt.media import c2_media
+    from awpsgt.workloads import CONFIGS, build_params, make_run, sgt_points, source_of
+    from oracle import oracle as orc
+    from test_gpu_large import _in3d_dict
+    p = build_params(CONFIGS["c2"], 800, 200, 160, 100)
+    m = c2_media(p.nx, p.ny, p.nz, p.dh, seed=7)
+    media = m.cpu().numpy()
+    del m
+    src = (100, 80, 10)
+    pts = sgt_points(p, 2000, 11)
+    xyz, stf = source_of(p, src)
+    print("[c4] FP64 restatement, %d steps" % p.nt, flush=True)
+    o = orc.Oracle(_in3d_dict(p), media, xyz, stf, pts, dtype=np.float64)
+    o.run(p.nt, 16)
+    ref, _ = o.sgt()
+    o.close()
+    ref = ref.astype(np.float64)
+    rows = []
+    for storage, tol in ((awpsgt.AWP_STORAGE_FP32, 1e-4), (awpsgt.AWP_STORAGE_MIXED, 1e-3)):
+        r, nkept = make_run(p, storage, lambda: media, src, 2000, 11)
+        assert nkept == 2000
+        r.step(p.nt)
+        r.synchronize()
+        s = r.sgt()
+        r.close()
+        live = np.abs(ref).max(axis=(0, 2)) > 0
+        assert live.all()
+        rel, peak = _errors(s, ref)
+        name = "fp32" if storage == awpsgt.AWP_STORAGE_FP32 else "mixed"
+        rows.append((name, rel, peak))
+        assert max(rel) <= tol, (name, rel)
+    for name, rel, peak in rows:
+        print("random medium %-6s rel L2 %s | peak %s" % (
+            name, " ".join("%s %.2e" % (c, e) for c, e in zip(COMP, rel)), " ".join("%.2e" % e for e in peak)))
