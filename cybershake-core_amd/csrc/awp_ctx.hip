@@ -35,20 +35,22 @@ static bool use_o32(const Geo &g) { return offsets32(g); }
 // so that its interior rows start on 128-byte lines), the mixed mode's
 // row-segment exponents -- together the state awp_init zeroes -- then d1 mu
 // lam and qp qs (4 or 2 bytes), and, for a run that holds the fused
-// step's second wavefield buffer, that buffer's 9 arrays.  Consecutive
-// arrays are `pad` bytes apart beyond their own size (AWP_ARRAY_PAD, a
-// multiple of 256; default below), and buffer 1 starts `bshift` further
-// bytes on (AWP_BUF_B_SHIFT).  AWP_BUF_B=sep instead gives buffer 1 an
-// allocation of its own, whose place in HBM the allocator chooses (awp_tune
-// then tries several).
+// step's second wavefield buffer, B_SLACK bytes of room and that buffer's 9
+// arrays.  Consecutive arrays are `pad` bytes apart beyond their own size
+// (AWP_ARRAY_PAD, a multiple of 256; 0 by default).  Buffer 1 starts
+// b_shift bytes into the room (AWP_BUF_B_SHIFT, default 0): awp_tune moves it
+// there, because where buffer 1 sits relative to the other arrays sets the
+// fused step's speed (the same steps, bit for bit, run 15.8 or 17.2 ms at
+// 1500x1000x300 depending on it).
+constexpr size_t B_SLACK = (size_t)8 << 20;
 struct VolLayout {
     size_t stride4;          // bytes from one 4-byte array to the next
     size_t wf0[9], r[6], rexp, m[3], q[2];
     size_t rexp_n;
     int nseg;
     size_t state_bytes;      // [0, state_bytes): buffer 0 + r1..r6 + exponents
-    bool b_separate;
-    size_t b_begin, b_bytes; // buffer 1 (offset in the allocation when not separate)
+    size_t b_room;           // start of buffer 1's room (two buffers)
+    size_t b_begin, b_bytes; // buffer 1: b_room + b_shift, 9 arrays
     size_t total;            // bytes of the allocation
 };
 
@@ -66,9 +68,7 @@ static VolLayout vol_layout(const Geo &g, int storage, bool two_bufs)
     const size_t nv = (size_t)g.nvol;
     const bool mixed = storage == AWP_STORAGE_MIXED;
     const size_t pad = env_bytes("AWP_ARRAY_PAD", 0);
-    const size_t bshift = env_bytes("AWP_BUF_B_SHIFT", 0);
-    const char *bsep = getenv("AWP_BUF_B");
-    L.b_separate = bsep && !strcmp(bsep, "sep");
+    const size_t bshift = std::min(env_bytes("AWP_BUF_B_SHIFT", 0), B_SLACK);
     const size_t a4 = nv * 4 + pad;
     const size_t eslot = (mixed ? nv * 2 + 128 : nv * 4) + pad, eoff = mixed ? 64 : 0;
     L.stride4 = a4;
@@ -83,10 +83,11 @@ static VolLayout vol_layout(const Geo &g, int storage, bool two_bufs)
     for (int a = 0; a < 3; ++a, o += eslot) L.m[a] = o + eoff;
     for (int a = 0; a < 2; ++a, o += eslot) L.q[a] = o + eoff;
     L.b_bytes = two_bufs ? 9 * a4 : 0;
-    L.b_begin = 0;
-    if (two_bufs && !L.b_separate) {
+    L.b_room = L.b_begin = 0;
+    if (two_bufs) {
+        L.b_room = o;
         L.b_begin = o + bshift;
-        o = L.b_begin + L.b_bytes;
+        o += B_SLACK + L.b_bytes;
     }
     L.total = o;
     return L;
@@ -164,17 +165,6 @@ static Launch choose_launch(const Geo &g, bool fused, bool mixed, int device)
     // step against 17.62 for 2 x 16 and 17.75 for 3 x 8)
     l.pw = 4;
     l.ph = 4;
-    // region-synchronized fused launches (AWP_REGION=<w>x<h>: an XCD's w*h
-    // blocks take w x h neighbouring tiles together); off by default
-    l.rw = l.rh = 0;
-    const char *re = getenv("AWP_REGION");
-    if (re) {
-        int w = 0, h = 0;
-        if (sscanf(re, "%dx%d", &w, &h) == 2 && w >= 1 && h >= 1 && w * h <= 256) {
-            l.rw = w;
-            l.rh = h;
-        }
-    }
     const char *pe = getenv("AWP_PATCH");
     if (pe) {
         int w = 0, h = 0;
@@ -327,7 +317,6 @@ extern "C" int awp_create(awp_ctx **out, const awp_in3d *p, int device, int stor
     ctx->sg_done = false;
     ctx->d_trace = nullptr;
     ctx->trace_pos = 0;
-    ctx->fsync.counter = nullptr;
 
     e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
@@ -348,24 +337,10 @@ extern "C" int awp_create(awp_ctx **out, const awp_in3d *p, int device, int stor
     ctx->bytes += nbytes;
     ctx->vol_bytes = nbytes;
     ctx->state_bytes = L.state_bytes;
-    ctx->vol_b_own = false;
     if (ctx->two_bufs) {
         ctx->vol_b_bytes = L.b_bytes;
-        if (L.b_separate) {
-            e = hipMalloc((void **)&ctx->vol_b, ctx->vol_b_bytes);
-            if (e != hipSuccess) {
-                snprintf(err, errlen, "hipMalloc(%zu bytes) for the second wavefield buffer: %s", ctx->vol_b_bytes,
-                         hipGetErrorString(e));
-                hipFree(ctx->vol);
-                hipStreamDestroy(ctx->stream);
-                delete ctx;
-                return AWP_ENOMEM;
-            }
-            ctx->vol_b_own = true;
-            ctx->bytes += ctx->vol_b_bytes;
-        } else {
-            ctx->vol_b = (float *)((char *)ctx->vol + L.b_begin);
-        }
+        ctx->b_room = L.b_room;
+        ctx->vol_b = (float *)((char *)ctx->vol + L.b_begin);
     }
     char *const base = (char *)ctx->vol;
     ctx->arr_stride = L.stride4;
@@ -464,16 +439,6 @@ extern "C" int awp_create(awp_ctx **out, const awp_in3d *p, int device, int stor
         return AWP_ENOMEM;
     }
     ctx->bytes += (size_t)std::max(p->nt, 1) * 3 * sizeof(float);
-    // arrival counters of the region-synchronized fused launches
-    e = hipMalloc((void **)&ctx->fsync.counter, 8 * sizeof(unsigned));
-    if (e == hipSuccess) e = hipMemset(ctx->fsync.counter, 0, 8 * sizeof(unsigned));
-    if (e != hipSuccess) {
-        snprintf(err, errlen, "hipMalloc sync counters: %s", hipGetErrorString(e));
-        awp_destroy(ctx);
-        return AWP_ENOMEM;
-    }
-    for (int x = 0; x < 8; ++x) ctx->fsync.base[x] = 0;
-    ctx->bytes += 8 * sizeof(unsigned);
     *out = ctx;
     return AWP_OK;
 }
@@ -485,10 +450,8 @@ extern "C" int awp_destroy(awp_ctx *ctx)
     if (ctx->stream) hipStreamSynchronize(ctx->stream);
     for (auto &e : ctx->ev) hipEventDestroy(e);
     hipFree(ctx->vol);
-    if (ctx->vol_b_own) hipFree(ctx->vol_b);
     hipFree(ctx->prof);
     hipFree(ctx->d_trace);
-    hipFree(ctx->fsync.counter);
     hipFree(ctx->d_sjob);
     hipFree(ctx->d_psrc);
     hipFree(ctx->d_ptop);
@@ -869,7 +832,7 @@ extern "C" int awp_step(awp_ctx *ctx, int nsteps)
             const State sa = ctx->view(ctx->cur), sb = ctx->view(ctx->cur ^ 1);
             if (e) HIPTRY(ctx, hipEventRecord(e[1], st));
             if (e) HIPTRY(ctx, hipEventRecord(e[2], st));
-            HIPTRY(ctx, launch_step_fused(mx, ctx->g, sa, sb, ctx->c, ctx->l, st, &ctx->fsync, ctx->device));
+            HIPTRY(ctx, launch_step_fused(mx, ctx->g, sa, sb, ctx->c, ctx->l, st));
             ctx->cur ^= 1;
             sn = sb;
         } else {
@@ -944,7 +907,7 @@ int trial_steps(awp_ctx *ctx, int n)
     for (int k = 0; k < n; ++k) {
         if (ctx->fused) {
             HIPTRY(ctx, launch_step_fused(mx, ctx->g, ctx->view(ctx->cur), ctx->view(ctx->cur ^ 1), ctx->c, ctx->l,
-                                          ctx->stream, &ctx->fsync, ctx->device));
+                                          ctx->stream));
             ctx->cur ^= 1;
         } else {
             HIPTRY(ctx, launch_split_velocity(ctx->mixed(), ctx->l, ctx->l.o32, ctx->d_sjob, 1, ctx->split_total, all, ctx->stream));
@@ -1041,53 +1004,40 @@ extern "C" int awp_tune(awp_ctx *ctx, int reps)
     };
     rc = search();
     if (rc) return rc;
-    if (ctx->fused && ctx->vol_b && ctx->vol_b_own) {
-        // Where the second buffer lands in HBM relative to the first sets
-        // the fused step's speed (1500x1000x300: 16.1-16.3 or 17.5-17.6 ms
-        // per step, stable for the life of the allocation; on some boxes one
-        // placement in four is fast).  Try up to nine more buffers while
-        // device memory allows and keep the fastest.
-        auto use_b = [ctx](float *vb) {
-            ctx->vol_b = vb;
-            for (int a = 0; a < 9; ++a) ctx->wf[1][a] = (float *)((char *)vb + (size_t)a * ctx->arr_stride);
+    if (ctx->fused && ctx->vol_b) {
+        // Where buffer 1 sits relative to the other arrays sets the fused
+        // step's speed (1500x1000x300: 15.8 or 17.2 ms per step, stable for
+        // a placement), and which placements are fast changes with where the
+        // allocation lands.  Try buffer 1 at offsets in its room (no
+        // allocation: the room is part of the run's own), keep the fastest,
+        // and search every candidate again on it.
+        static const size_t shifts[] = {0, 4096, 8192, 12288, 65536 + 4096, (size_t)1 << 20, ((size_t)1 << 20) + 8192,
+                                        (size_t)4 << 20, ((size_t)4 << 20) + 4096};
+        auto use_b = [ctx](size_t shift) {
+            ctx->vol_b = (float *)((char *)ctx->vol + ctx->b_room + shift);
+            for (int a = 0; a < 9; ++a) ctx->wf[1][a] = (float *)((char *)ctx->vol_b + (size_t)a * ctx->arr_stride);
             ctx->cur = 0;   // trial steps start from buffer 0
         };
-        std::vector<float *> tried(1, ctx->vol_b);
-        size_t keep = 0;
+        size_t keep = (size_t)((char *)ctx->vol_b - (char *)ctx->vol) - ctx->b_room;
         float bestp = 0;
-        use_b(ctx->vol_b);
         rc = time_steps(&bestp);
-        for (int t = 0; !rc && t < 9; ++t) {
-            size_t fr = 0, tot = 0;
-            if (hipMemGetInfo(&fr, &tot) != hipSuccess || fr < ctx->vol_b_bytes + ((size_t)4 << 30)) break;
-            float *nb = nullptr;
-            if (hipMalloc((void **)&nb, ctx->vol_b_bytes) != hipSuccess) {
-                (void)hipGetLastError();
+        for (size_t t = 0; !rc && t < sizeof(shifts) / sizeof(shifts[0]); ++t) {
+            if (shifts[t] == keep || shifts[t] > B_SLACK) continue;
+            use_b(shifts[t]);
+            // zero, as awp_init leaves the buffer (the step reads its ghost layers)
+            if (hipMemsetAsync(ctx->vol_b, 0, ctx->vol_b_bytes, ctx->stream) != hipSuccess) {
+                rc = fail(ctx, AWP_EHIP, "hipMemsetAsync of the second wavefield buffer failed");
                 break;
             }
-            tried.push_back(nb);
-            use_b(nb);
             float ms = 0;
-            // zero, as awp_init leaves a buffer (the step reads its ghost layers)
-            if (hipMemsetAsync(nb, 0, ctx->vol_b_bytes, ctx->stream) != hipSuccess) {
-                rc = fail(ctx, AWP_EHIP, "hipMemsetAsync of a trial buffer failed");
-                break;
-            }
             rc = time_steps(&ms);
             if (!rc && ms < bestp) {
                 bestp = ms;
-                keep = tried.size() - 1;
+                keep = shifts[t];
             }
         }
-        hipError_t se = hipStreamSynchronize(ctx->stream);
-        for (size_t k = 0; k < tried.size(); ++k)
-            if (k != keep) hipFree(tried[k]);
-        use_b(tried[keep]);
+        use_b(keep);
         if (rc) return rc;
-        HIPTRY(ctx, se);
-        // every candidate again, on the buffer kept (the first search ran on
-        // whatever placement the run started with; where no placement is
-        // fast, the two-stage step may win now)
         rc = search();
         if (rc) return rc;
     }
@@ -1134,7 +1084,7 @@ extern "C" int awp_estimate_bytes(const awp_in3d *p, int storage, int npts, size
     const bool fused = want_fused(g, storage);
     const size_t nv = (size_t)g.nvol;
     const VolLayout L = vol_layout(g, storage, fused);
-    const size_t vol = L.total + (L.b_separate ? L.b_bytes : 0);
+    const size_t vol = L.total;
     const size_t prof = ((size_t)p->nx + p->ny + p->nz + 6 + (storage == AWP_STORAGE_MIXED ? 3 * (size_t)g.pz : 0)) *
                         sizeof(float);
     const size_t nsamp_max = (size_t)std::max(p->nt / std::max(p->ntiskp_sgt, 1), 1);

@@ -30,9 +30,9 @@ struct awp_ctx {
     Launch l;
     hipStream_t stream;
     float *vol;              // one allocation for the volume arrays (wavefield buffer 0)
-    float *vol_b;            // wavefield buffer 1 (two_bufs): inside vol, or an allocation of its own
+    float *vol_b;            // wavefield buffer 1 (two_bufs), inside vol at b_room + its shift
     size_t vol_b_bytes;
-    bool vol_b_own;          // vol_b is an allocation of its own
+    size_t b_room;           // offset in vol of buffer 1's room (B_SLACK bytes more than the buffer)
     size_t arr_stride;       // bytes between consecutive 4-byte volume arrays
     // wavefield buffers (u1 v1 w1 xx yy zz xy xz yz): the fused step reads
     // wf[cur] and writes wf[cur ^ 1]; the two-stage step uses wf[0] only
@@ -45,7 +45,6 @@ struct awp_ctx {
     awp::SJob *d_sjob;
     int split_total, split_fs;
     float *prof;             // fx | fy | fz
-    awp::FSync fsync;        // counters of the region-synchronized fused launches
     // the per-step job-value trace (CHKJ, pmcl3d.f:1476-1479): u1 v1 w1 at
     // node trace_ijk after every step, [nt][3]
     float *d_trace;

@@ -527,20 +527,18 @@ __device__ __forceinline__ void step_tile(const Geo &g, const State &a0, const S
                 if (is_main && s >= kb) {
                     const unsigned orr = (ob + (unsigned)s * sz4) >> RSH;
                     G<R> *RR[6] = {R1, R2, R3, R4, R5, R6};
-                    int e[6];
                     // the row's exponents before the update: lane 0 (i = i0,
                     // inside the domain whenever the row is) loaded the slot
                     const unsigned long long ew_row =
                         (unsigned long long)(unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)ew_old) |
                         ((unsigned long long)(unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(ew_old >> 32))
                          << 32);
+                    const unsigned long long en = row_exponents6(rn, do_str, ew_row);
     #pragma unroll
-                    for (int q = 0; q < 6; ++q) {
-                        e[q] = row_exponent_next(rn[q], do_str, rexp_of(ew_row, q));
-                        if (do_str) gsn(RR[q], orr, r_encode(rn[q], e[q]));
-                    }
+                    for (int q = 0; q < 6; ++q)
+                        if (do_str) gsn(RR[q], orr, r_encode(rn[q], rexp_of(en, q)));
                     if (do_str && lane == 0)
-                        ((G<unsigned long long> *)a.rexp)[rexp_idx(g, a.nseg, bx, j, s)] = rexp_pack(e);
+                        ((G<unsigned long long> *)a.rexp)[rexp_idx(g, a.nseg, bx, j, s)] = en;
                 }
             }
 
@@ -603,52 +601,6 @@ k_step(const Geo g, const State a, const State b, const Coef c, const int kc, co
     step_tile<P, TY, PF>(g, a, b, c, kc, bx, by, bz, lds);
 }
 
-// Region-synchronized form of the one-run sweep: gridDim.x = 8 x (rw * rh)
-// blocks, all resident at once (one per CU); block slot s of XCD x
-// (blockIdx.x = 8 s + x) takes tile (s % rw, s / rw) of each of the XCD's
-// regions in turn (XCD x owns regions [x * per, (x + 1) * per) of the list:
-// z chunk, then region rows, x fastest), and the XCD's blocks meet at a
-// barrier between regions.
-struct FRegions {
-    int nbx, nby, nbz;     // tiles
-    int rw, rh;            // region shape in tiles
-    int nrx, nry;          // regions per chunk in x and y
-    int nreg, per;         // regions in all; per XCD (the last XCDs may have fewer)
-    unsigned *counter;     // [8]
-    unsigned base[8];
-    int spin;              // most polls of one barrier
-};
-
-template <class P, int TY, int PF>
-__global__ void __launch_bounds__(64 * (TY + 5))
-k_step_regions(const Geo g, const State a, const State b, const Coef c, const int kc, const FRegions R)
-{
-    using Sh = FShape<TY, PF>;
-    __shared__ float lds[Sh::NSLOT * Sh::SLOT];
-    const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
-    const unsigned nslot = gridDim.x >> 3;
-    const int sx = slot % R.rw, sy = slot / R.rw;
-    for (int q = 0; q < R.per; ++q) {
-        const int reg = xcd * R.per + q;
-        if (reg >= R.nreg) break;   // uniform over the XCD
-        const int bz = reg / (R.nrx * R.nry);
-        const int rr = reg - bz * (R.nrx * R.nry);
-        const int bx = (rr % R.nrx) * R.rw + sx, by = (rr / R.nrx) * R.rh + sy;
-        if (bx < R.nbx && by < R.nby) step_tile<P, TY, PF>(g, a, b, c, kc, bx, by, bz, lds);
-        __syncthreads();   // the block's waves are done with the LDS ring
-        if (threadIdx.x == 0 && threadIdx.y == 0) {
-            __hip_atomic_fetch_add(R.counter + xcd, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            const unsigned target = R.base[xcd] + (unsigned)(q + 1) * nslot;
-            for (int n = 0; n < R.spin; ++n) {
-                const unsigned v = __hip_atomic_load(R.counter + xcd, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                if ((int)(v - target) >= 0) break;
-                __builtin_amdgcn_s_sleep(2);
-            }
-        }
-        __syncthreads();
-    }
-}
-
 // Several runs in one launch.  Runs' tiles are numbered consecutively (run r
 // owns [tile0, tile0 + ntiles)); the grid is persistent: block n works on
 // XCD n % 8 and walks that XCD's contiguous range of the global tile list
@@ -689,42 +641,9 @@ FTiles fused_tiles(const Geo &g, const Launch &l)
 
 template <class P, int TY, int PF>
 static hipError_t step_t(const Geo &g, const State &a, const State &b, const Coef &c, const Launch &l,
-                         hipStream_t st, FSync *sync, int device)
+                         hipStream_t st)
 {
     const FTiles t = fused_tiles(g, l);
-    if (sync && l.rw > 0 && l.rh > 0) {
-        // every block of the regions resident at once: one XCD's share of
-        // the CUs times the blocks one CU holds
-        int nb = 0, ncu = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_step_regions<P, TY, PF>, TX * (TY + 5), 0) !=
-                hipSuccess)
-            nb = 0;
-        if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) ncu = 0;
-        const int slots = l.rw * l.rh;
-        if (nb > 0 && slots * 8 <= nb * ncu) {
-            FRegions R;
-            R.nbx = t.nbx;
-            R.nby = t.nby;
-            R.nbz = t.nbz;
-            R.rw = l.rw;
-            R.rh = l.rh;
-            R.nrx = (t.nbx + l.rw - 1) / l.rw;
-            R.nry = (t.nby + l.rh - 1) / l.rh;
-            R.nreg = R.nrx * R.nry * t.nbz;
-            R.per = (R.nreg + 7) / 8;
-            R.counter = sync->counter;
-            for (int x = 0; x < 8; ++x) R.base[x] = sync->base[x];
-            R.spin = 4000;
-            hipLaunchKernelGGL((k_step_regions<P, TY, PF>), dim3(8 * slots), dim3(TX, TY + 5), 0, st, g, a, b, c,
-                               l.kc, R);
-            // each XCD's counter rises by its blocks x the regions it takes
-            for (int x = 0; x < 8; ++x) {
-                const int nr = std::max(0, std::min(R.per, R.nreg - x * R.per));
-                sync->base[x] += (unsigned)(slots * nr);
-            }
-            return hipGetLastError();
-        }
-    }
     hipLaunchKernelGGL((k_step<P, TY, PF>), dim3(8 * t.per), dim3(TX, TY + 5), 0, st, g, a, b, c, l.kc, t);
     return hipGetLastError();
 }
@@ -794,9 +713,9 @@ static int grid_t(int device)
     } while (0)
 
 hipError_t launch_step_fused(bool mixed, const Geo &g, const State &a, const State &b, const Coef &c,
-                             const Launch &l, hipStream_t st, FSync *sync, int device)
+                             const Launch &l, hipStream_t st)
 {
-    AWP_FUSED_DISPATCH(mixed, l, return (step_t<P, TY, PF>(g, a, b, c, l, st, sync, device)));
+    AWP_FUSED_DISPATCH(mixed, l, return (step_t<P, TY, PF>(g, a, b, c, l, st)));
     return hipErrorInvalidValue;
 }
 

@@ -27,7 +27,7 @@ namespace awp {
 // (cerjan.f:292-328)
 // ---------------------------------------------------------------------------
 template <class P, int U>
-__device__ inline void velocity_tile(const Geo &g, const State &s0, const Coef &c, const int kc, const int bx,
+__device__ __forceinline__ void velocity_tile(const Geo &g, const State &s0, const Coef &c, const int kc, const int bx,
                                      const int by, const int bz)
 {
     const GState s = as_global(s0);
@@ -109,7 +109,7 @@ __device__ inline void velocity_tile(const Geo &g, const State &s0, const Coef &
 // bases, one offset register, neighbours as immediates); needs every array
 // below 4 GiB.
 template <class P, int U>
-__device__ inline void velocity_tile_b(const Geo &g, const State &s0, const Coef &c, const int kc, const int bx,
+__device__ __forceinline__ void velocity_tile_b(const Geo &g, const State &s0, const Coef &c, const int kc, const int bx,
                                        const int by, const int bz)
 {
     const GState s = as_global(s0);
@@ -191,7 +191,7 @@ __device__ inline void velocity_tile_b(const Geo &g, const State &s0, const Coef
 // so no thread reads another thread's output.
 // ---------------------------------------------------------------------------
 template <class P>
-__device__ inline void fs_velocity_block(const Geo &g, const State &s0, const int bx, const int by)
+__device__ __forceinline__ void fs_velocity_block(const Geo &g, const State &s0, const int bx, const int by)
 {
     const GState s = as_global(s0);
     const int i = bx * TX + threadIdx.x + 1;
@@ -236,7 +236,7 @@ __device__ inline void fs_velocity_block(const Geo &g, const State &s0, const in
 // CU hide the latency of the exponent round trip, and is the faster mixed
 // kernel.
 template <class P, int U>
-__device__ inline void stress_tile_e(const Geo &g, const State &s0, const Coef &c, const int kc, const int bx,
+__device__ __forceinline__ void stress_tile_e(const Geo &g, const State &s0, const Coef &c, const int kc, const int bx,
                                    const int by, const int bz)
 {
     const GState s = as_global(s0);
@@ -405,14 +405,12 @@ __device__ inline void stress_tile_e(const Geo &g, const State &s0, const Coef &
         if constexpr (P::mixed) {
             const float rv[6] = {r1, r2, r3, r4, r5, r6};
             G<R> *RR[6] = {R1, R2, R3, R4, R5, R6};
-            int e[6];
+            const unsigned long long en = row_exponents6(rv, act, ew);
 #pragma unroll
-            for (int q = 0; q < 6; ++q) {
-                e[q] = row_exponent_next(rv[q], act, rexp_of(ew, q));
-                if (act) RR[q][o] = r_encode(rv[q], e[q]);
-            }
+            for (int q = 0; q < 6; ++q)
+                if (act) RR[q][o] = r_encode(rv[q], rexp_of(en, q));
             if (act && threadIdx.x == 0)
-                ((G<unsigned long long> *)s.rexp)[rexp_idx(g, s.nseg, seg, j, k)] = rexp_pack(e);
+                ((G<unsigned long long> *)s.rexp)[rexp_idx(g, s.nseg, seg, j, k)] = en;
         } else {
             R1[o] = r1; R2[o] = r2; R3[o] = r3;
             R4[o] = r4; R5[o] = r5; R6[o] = r6;
@@ -449,7 +447,7 @@ __device__ inline void stress_tile_e(const Geo &g, const State &s0, const Coef &
 }
 
 template <class P, int U>
-__device__ inline void stress_tile_b(const Geo &g, const State &s0, const Coef &c, const int kc, const int bx,
+__device__ __forceinline__ void stress_tile_b(const Geo &g, const State &s0, const Coef &c, const int kc, const int bx,
                                    const int by, const int bz)
 {
     const GState s = as_global(s0);
@@ -626,13 +624,11 @@ __device__ inline void stress_tile_b(const Geo &g, const State &s0, const Coef &
         if constexpr (P::mixed) {
             const float rv[6] = {r1, r2, r3, r4, r5, r6};
             G<R> *RR[6] = {R1, R2, R3, R4, R5, R6};
-            int e[6];
+            const unsigned long long en = row_exponents6(rv, act, ew);
 #pragma unroll
-            for (int q = 0; q < 6; ++q) {
-                e[q] = row_exponent_next(rv[q], act, rexp_of(ew, q));
-                if (act) p.stn(RR[q], r_encode(rv[q], e[q]));
-            }
-            if (act && threadIdx.x == 0) gsn((G<unsigned long long> *)s.rexp, xo, rexp_pack(e));
+            for (int q = 0; q < 6; ++q)
+                if (act) p.stn(RR[q], r_encode(rv[q], rexp_of(en, q)));
+            if (act && threadIdx.x == 0) gsn((G<unsigned long long> *)s.rexp, xo, en);
         } else {
             p.stn(R1, r1); p.stn(R2, r2); p.stn(R3, r3);
             p.stn(R4, r4); p.stn(R5, r5); p.stn(R6, r6);
@@ -669,7 +665,7 @@ __device__ inline void stress_tile_b(const Geo &g, const State &s0, const Coef &
 }
 
 template <class P, int U, bool O32>
-__device__ inline void stress_tile(const Geo &g, const State &s0, const Coef &c, const int kc, const int bx,
+__device__ __forceinline__ void stress_tile(const Geo &g, const State &s0, const Coef &c, const int kc, const int bx,
                                    const int by, const int bz)
 {
     if constexpr (O32)

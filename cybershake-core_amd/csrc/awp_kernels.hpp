@@ -161,24 +161,38 @@ __device__ inline float wave_max(float v)
     const int b = max(__builtin_amdgcn_readlane(x, 32), __builtin_amdgcn_readlane(x, 48));
     return __int_as_float(max(a, b));
 }
-// row_exponent(wave_max(act ? |v| : 0)) for a row whose exponent was e_old
-// (in [REXP_MIN, REXP_MAX]) before this update, all 64 lanes active.  The
-// row's largest value usually stays in its binade from one step to the next:
-// then every |v| < 2^e_old and some |v| >= 2^(e_old-1) (some v != 0 when
-// e_old is the clamp REXP_MIN), which two ballots establish, and the
-// exponent is e_old; a row of zeros gives 0; otherwise the full reduction
-// decides.  The result is the same as the reduction's in every case.
-__device__ inline int row_exponent_next(float v, bool act, int e_old)
+// The exponents of the six memory variables of one row segment after an
+// update, packed as rexp_pack does: row_exponent(wave_max(act ? |v[q]| : 0))
+// for q = 0..5, all 64 lanes active, the same in every lane.  ew_old holds
+// the row's exponents before the update.  A row's largest value usually
+// stays in its binade from one step to the next, which two ballots per array
+// establish (every |v| < 2^e_old, some |v| >= 2^(e_old-1) -- or, at the clamp
+// REXP_MIN, some v != 0): the exponent is then e_old; a third ballot finds an
+// all-zero row (exponent 0).  Only when an array is neither does the wave take
+// the DPP reductions, for all six in one uniform branch (so the six values
+// stay in registers).  The result equals the reductions' in every case.
+__device__ inline unsigned long long row_exponents6(const float (&v)[6], const bool act,
+                                                    const unsigned long long ew_old)
 {
-    const float a = fabsf(v);
-    const float hi = __int_as_float((e_old + 127) << 23);                              // 2^e_old
-    const float lo = e_old == REXP_MIN ? __int_as_float(1) : __int_as_float((e_old + 126) << 23);
-    const unsigned long long below = __builtin_amdgcn_ballot_w64(!act || a < hi);
-    if (below == ~0ull) {
-        if (__builtin_amdgcn_ballot_w64(act && a >= lo) != 0ull) return e_old;
-        if (__builtin_amdgcn_ballot_w64(act && a != 0.0f) == 0ull) return 0;
+    int e[6];
+    bool slow = false;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+        const int eo = rexp_of(ew_old, q);
+        const float a = fabsf(v[q]);
+        const float hi = __int_as_float((eo + 127) << 23);                                     // 2^eo
+        const float lo = eo == REXP_MIN ? __int_as_float(1) : __int_as_float((eo + 126) << 23);
+        const bool below = __builtin_amdgcn_ballot_w64(!act || a < hi) == ~0ull;
+        const bool some = __builtin_amdgcn_ballot_w64(act && a >= lo) != 0ull;
+        const bool zero = __builtin_amdgcn_ballot_w64(act && a != 0.0f) == 0ull;
+        e[q] = zero ? 0 : eo;
+        slow = slow || !(zero || (below && some));
     }
-    return row_exponent(wave_max(act ? a : 0.0f));
+    if (slow) {
+#pragma unroll
+        for (int q = 0; q < 6; ++q) e[q] = row_exponent(wave_max(act ? fabsf(v[q]) : 0.0f));
+    }
+    return rexp_pack(e);
 }
 __device__ inline _Float16 r_encode(float r, int e) { return (_Float16)ldexpf(r, 15 - e); }
 __device__ inline float r_decode(_Float16 h, int e) { return ldexpf((float)h, e - 15); }
@@ -386,20 +400,8 @@ struct Launch {
     bool o32_v;      // two-stage velocity kernel: its 32-bit byte-offset form (only where o32)
     int bpc_v, bpc_s;   // two-stage kernels: most blocks per CU (0 = as registers allow)
     unsigned lds_v, lds_s;   // the dynamic LDS bytes per block that enforce them (set_block_caps)
-    int rw, rh;      // fused step: regions of rw x rh tiles taken by an XCD's blocks together (0: off)
 };
 
-// Region-synchronized launches of the fused step (Launch::rw, rh > 0): the
-// blocks of each XCD, one per CU, take a region of rw x rh neighbouring
-// tiles together and meet at a bounded barrier before the next region, so
-// that the tiles an XCD sweeps at one time stay level with each other in z
-// and read each other's halo lines from its L2.  The barrier orders nothing
-// the results depend on (tiles of one step are independent), and every wait
-// ends after `spin` polls.
-struct FSync {
-    unsigned *counter;     // [8] device arrival counters, one per XCD, never reset
-    unsigned base[8];      // their values when the next launch starts (kept by the host)
-};
 
 // Dynamic LDS a block reserves so that at most `bpc` blocks share a CU:
 // capping the blocks in flight keeps the tiles an XCD works on at once, and
@@ -463,11 +465,10 @@ hipError_t launch_absmax(const float *a, long long n, float *part, int nblk, hip
 hipError_t launch_to_half(const float *src, _Float16 *dst, float scale, long long n, hipStream_t st);
 hipError_t launch_sgt_fp16(const float *sgt, const int *kidx, int nkept, int nsamp_max, int nsamp, float *scales,
                            _Float16 *out, hipStream_t st);
-// the fused one-sweep step (awp_fused.hip): reads state a, writes state b;
-// region-synchronized (Launch::rw, rh) when `sync` is given and the regions'
-// blocks fit on the device at once, else one block per tile
+// the fused one-sweep step (awp_fused.hip): reads state a, writes state b,
+// one block per tile
 hipError_t launch_step_fused(bool mixed, const Geo &g, const State &a, const State &b, const Coef &c,
-                             const Launch &l, hipStream_t st, FSync *sync = nullptr, int device = 0);
+                             const Launch &l, hipStream_t st);
 
 // tiles of one run for the fused step: nbx * nby * nbz column tiles
 // Tile order: within each z chunk, tiles are grouped in patches of pw x ph

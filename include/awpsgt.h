@@ -39,7 +39,8 @@ enum {
 /* Storage modes of the state and media arrays (arithmetic is FP32 in all). */
 enum {
     AWP_STORAGE_FP32 = 0,   /* everything FP32: the reference's storage */
-    AWP_STORAGE_MIXED = 1   /* r1..r6 FP16 with per-row exponents, qp/qs FP16 (scaled), d1/lam/mu FP32 */
+    AWP_STORAGE_MIXED = 1   /* r1..r6 FP16 with per-row exponents, qp/qs FP16 (scaled), d1/lam/mu FP16
+                               deviations from an FP32 profile per level */
 };
 
 /* The IN3D parameters the SGT path uses, in the reference's units.  Filled by
