@@ -29,29 +29,29 @@ extern "C" const char *awp_last_error(const awp_ctx *ctx) { return ctx ? ctx->er
 // 32-bit byte offsets where every array is below 4 GiB
 static bool use_o32(const Geo &g) { return offsets32(g); }
 
-// Placement of a run's volume arrays.  One allocation holds, in order:
-// the 9 FP32 arrays (velocities, stresses) of wavefield buffer 0, r1..r6 (4
-// bytes FP32, 2 bytes mixed; a 2-byte array starts 64 bytes into its slot
-// so that its interior rows start on 128-byte lines), the mixed mode's
-// row-segment exponents -- together the state awp_init zeroes -- then d1 mu
-// lam and qp qs (4 or 2 bytes), and, for a run that holds the fused
-// step's second wavefield buffer, B_SLACK bytes of room and that buffer's 9
-// arrays.  Consecutive arrays are `pad` bytes apart beyond their own size
-// (AWP_ARRAY_PAD, a multiple of 256; 0 by default).  Buffer 1 starts
-// b_shift bytes into the room (AWP_BUF_B_SHIFT, default 0): awp_tune moves it
-// there, because where buffer 1 sits relative to the other arrays sets the
-// fused step's speed (the same steps, bit for bit, run 15.8 or 17.2 ms at
-// 1500x1000x300 depending on it).
-constexpr size_t B_SLACK = (size_t)8 << 20;
+// Placement of a run's volume arrays.  One allocation holds, in order, the
+// media d1 mu lam qp qs (4 bytes FP32, 2 bytes mixed; a 2-byte array starts
+// 64 bytes into its slot so that its interior rows start on 128-byte lines),
+// the mixed mode's row-segment exponents, and the state room: the 9 FP32
+// arrays (velocities, stresses) of wavefield buffer 0, r1..r6 (4 or 2
+// bytes) and, for a run that holds the fused step's second wavefield buffer,
+// its 9 arrays -- consecutive state arrays `spad` bytes apart beyond their
+// own size.  awp_init zeroes the exponents and the whole room.  The room is
+// SPAD_MAX bytes per state array larger than the arrays, so that awp_tune
+// can lay the state out anew at another pad: where the state arrays sit
+// relative to each other and to the media sets the speed of the same steps
+// (1500x1000x300 fused: 15.6-16.0 or 16.6-17.3 ms per step), and which pads
+// are fast depends on where the allocation lands.  AWP_STATE_PAD (a multiple
+// of 256, at most SPAD_MAX) sets the pad a run starts with (default 0).
+constexpr size_t SPAD_MAX = ((size_t)1 << 20) + 4096;
 struct VolLayout {
-    size_t stride4;          // bytes from one 4-byte array to the next
-    size_t wf0[9], r[6], rexp, m[3], q[2];
+    size_t m[3], q[2], rexp;   // media, exponent slots
     size_t rexp_n;
     int nseg;
-    size_t state_bytes;      // [0, state_bytes): buffer 0 + r1..r6 + exponents
-    size_t b_room;           // start of buffer 1's room (two buffers)
-    size_t b_begin, b_bytes; // buffer 1: b_room + b_shift, 9 arrays
-    size_t total;            // bytes of the allocation
+    size_t zero_begin;         // [zero_begin, total): exponents and the state room (zeroed by awp_init)
+    size_t room;               // start of the state room
+    size_t wf[2][9], r[6];     // the state arrays at pad spad
+    size_t total;              // bytes of the allocation
 };
 
 static size_t env_bytes(const char *name, size_t dflt)
@@ -62,35 +62,48 @@ static size_t env_bytes(const char *name, size_t dflt)
     return x < 0 ? dflt : ((size_t)x + 255) / 256 * 256;
 }
 
-static VolLayout vol_layout(const Geo &g, int storage, bool two_bufs)
+static VolLayout vol_layout(const Geo &g, int storage, bool two_bufs, size_t spad)
 {
     VolLayout L;
     const size_t nv = (size_t)g.nvol;
     const bool mixed = storage == AWP_STORAGE_MIXED;
-    const size_t pad = env_bytes("AWP_ARRAY_PAD", 0);
-    const size_t bshift = std::min(env_bytes("AWP_BUF_B_SHIFT", 0), B_SLACK);
-    const size_t a4 = nv * 4 + pad;
-    const size_t eslot = (mixed ? nv * 2 + 128 : nv * 4) + pad, eoff = mixed ? 64 : 0;
-    L.stride4 = a4;
+    spad = std::min(spad, SPAD_MAX);
+    const size_t a4 = nv * 4;
+    const size_t eslot = mixed ? nv * 2 + 128 : nv * 4, eoff = mixed ? 64 : 0;
     size_t o = 0;
-    for (int a = 0; a < 9; ++a, o += a4) L.wf0[a] = o;
-    for (int a = 0; a < 6; ++a, o += eslot) L.r[a] = o + eoff;
-    L.nseg = (g.nx + TX - 1) / TX;
-    L.rexp_n = mixed ? (size_t)g.nz * g.ny * L.nseg : 0;
-    L.rexp = o;
-    o += (8 * L.rexp_n + 255) / 256 * 256;
-    L.state_bytes = o;
     for (int a = 0; a < 3; ++a, o += eslot) L.m[a] = o + eoff;
     for (int a = 0; a < 2; ++a, o += eslot) L.q[a] = o + eoff;
-    L.b_bytes = two_bufs ? 9 * a4 : 0;
-    L.b_room = L.b_begin = 0;
-    if (two_bufs) {
-        L.b_room = o;
-        L.b_begin = o + bshift;
-        o += B_SLACK + L.b_bytes;
+    L.nseg = (g.nx + TX - 1) / TX;
+    L.rexp_n = mixed ? (size_t)g.nz * g.ny * L.nseg : 0;
+    L.zero_begin = L.rexp = o;
+    o += (8 * L.rexp_n + 255) / 256 * 256;
+    L.room = o;
+    const int nstate = two_bufs ? 24 : 15;
+    size_t q = o;
+    for (int a = 0; a < 9; ++a, q += a4 + spad) L.wf[0][a] = q;
+    for (int a = 0; a < 6; ++a, q += eslot + spad) L.r[a] = q + eoff;
+    for (int a = 0; a < 9; ++a) {
+        L.wf[1][a] = two_bufs ? q : L.wf[0][a];
+        if (two_bufs) q += a4 + spad;
     }
-    L.total = o;
+    L.total = o + (two_bufs ? 18 : 9) * a4 + 6 * eslot + (size_t)nstate * SPAD_MAX;
     return L;
+}
+
+// Points the run's state arrays at their places for state pad spad.
+static void place_state(awp_ctx *ctx, size_t spad)
+{
+    const VolLayout L = vol_layout(ctx->g, ctx->storage, ctx->two_bufs, spad);
+    char *const base = (char *)ctx->vol;
+    for (int b = 0; b < 2; ++b)
+        for (int a = 0; a < 9; ++a) ctx->wf[b][a] = (float *)(base + L.wf[b][a]);
+    float **f32[9] = {&ctx->s.u1, &ctx->s.v1, &ctx->s.w1, &ctx->s.xx, &ctx->s.yy, &ctx->s.zz,
+                      &ctx->s.xy, &ctx->s.xz, &ctx->s.yz};
+    for (int a = 0; a < 9; ++a) *f32[a] = ctx->wf[0][a];
+    void **rr[6] = {&ctx->s.r1, &ctx->s.r2, &ctx->s.r3, &ctx->s.r4, &ctx->s.r5, &ctx->s.r6};
+    for (int a = 0; a < 6; ++a) *rr[a] = base + L.r[a];
+    ctx->spad = std::min(spad, SPAD_MAX);
+    ctx->cur = 0;
 }
 
 // Time-step scheme of a new run.  AWP_SCHEME=split: the two-stage step
@@ -311,9 +324,8 @@ extern "C" int awp_create(awp_ctx **out, const awp_in3d *p, int device, int stor
     ctx->timing = false;
     ctx->t_vel = ctx->t_str = ctx->t_all = 0;
     ctx->bytes = 0;
-    ctx->vol_bytes = ctx->state_bytes = 0;
-    ctx->vol_b = nullptr;
-    ctx->vol_b_bytes = 0;
+    ctx->vol_bytes = ctx->zero_begin = 0;
+    ctx->spad = 0;
     ctx->sg_done = false;
     ctx->d_trace = nullptr;
     ctx->trace_pos = 0;
@@ -325,7 +337,7 @@ extern "C" int awp_create(awp_ctx **out, const awp_in3d *p, int device, int stor
         return AWP_EHIP;
     }
     // the volume arrays and their placement (vol_layout)
-    const VolLayout L = vol_layout(ctx->g, storage, ctx->two_bufs);
+    const VolLayout L = vol_layout(ctx->g, storage, ctx->two_bufs, 0);
     const size_t nbytes = L.total;
     e = hipMalloc((void **)&ctx->vol, nbytes);
     if (e != hipSuccess) {
@@ -336,24 +348,9 @@ extern "C" int awp_create(awp_ctx **out, const awp_in3d *p, int device, int stor
     }
     ctx->bytes += nbytes;
     ctx->vol_bytes = nbytes;
-    ctx->state_bytes = L.state_bytes;
-    if (ctx->two_bufs) {
-        ctx->vol_b_bytes = L.b_bytes;
-        ctx->b_room = L.b_room;
-        ctx->vol_b = (float *)((char *)ctx->vol + L.b_begin);
-    }
+    ctx->zero_begin = L.zero_begin;
     char *const base = (char *)ctx->vol;
-    ctx->arr_stride = L.stride4;
-    for (int a = 0; a < 9; ++a) {
-        ctx->wf[0][a] = (float *)(base + L.wf0[a]);
-        ctx->wf[1][a] = ctx->two_bufs ? (float *)((char *)ctx->vol_b + (size_t)a * L.stride4) : ctx->wf[0][a];
-    }
-    ctx->cur = 0;
-    float **f32[9] = {&ctx->s.u1, &ctx->s.v1, &ctx->s.w1, &ctx->s.xx, &ctx->s.yy, &ctx->s.zz,
-                      &ctx->s.xy, &ctx->s.xz, &ctx->s.yz};
-    for (int a = 0; a < 9; ++a) *f32[a] = ctx->wf[0][a];
-    void **rr[6] = {&ctx->s.r1, &ctx->s.r2, &ctx->s.r3, &ctx->s.r4, &ctx->s.r5, &ctx->s.r6};
-    for (int a = 0; a < 6; ++a) *rr[a] = base + L.r[a];
+    place_state(ctx, env_bytes("AWP_STATE_PAD", 0));
     ctx->s.rexp = L.rexp_n ? (signed char *)(base + L.rexp) : nullptr;
     ctx->s.rexp_n = (long long)L.rexp_n;
     ctx->s.nseg = L.nseg;
@@ -752,8 +749,9 @@ static int upload_sjob(awp_ctx *ctx)
 static int scheme_changed(awp_ctx *ctx)
 {
     if (!ctx->fused && ctx->cur == 1) {
-        HIPTRY(ctx, hipMemcpyAsync(ctx->wf[0][0], ctx->wf[1][0], 9 * ctx->arr_stride,
-                                   hipMemcpyDeviceToDevice, ctx->stream));
+        for (int a = 0; a < 9; ++a)
+            HIPTRY(ctx, hipMemcpyAsync(ctx->wf[0][a], ctx->wf[1][a], (size_t)ctx->g.nvol * sizeof(float),
+                                       hipMemcpyDeviceToDevice, ctx->stream));
         ctx->cur = 0;
     }
     if (!ctx->fused && ctx->initialised) return upload_sjob(ctx);
@@ -766,9 +764,8 @@ extern "C" int awp_init(awp_ctx *ctx)
     if (!ctx->media_set || !ctx->source_set || !ctx->points_set)
         return fail(ctx, AWP_ESTATE, "awp_init needs media, source and SGT points first");
     HIPTRY(ctx, hipSetDevice(ctx->device));
-    // wavefield and memory variables: the front of the volume allocation
-    HIPTRY(ctx, hipMemsetAsync(ctx->vol, 0, ctx->state_bytes, ctx->stream));
-    if (ctx->vol_b) HIPTRY(ctx, hipMemsetAsync(ctx->vol_b, 0, ctx->vol_b_bytes, ctx->stream));
+    // the exponent slots and the state room (wavefield buffers, memory variables)
+    HIPTRY(ctx, hipMemsetAsync((char *)ctx->vol + ctx->zero_begin, 0, ctx->vol_bytes - ctx->zero_begin, ctx->stream));
     const size_t nout = (size_t)std::max(ctx->npts, 1) * 6 * std::max(ctx->nsamp_max, 1);
     hipLaunchKernelGGL(k_fill, dim3(1024), dim3(256), 0, ctx->stream, ctx->d_sgt, nout, -1.0f);
     HIPTRY(ctx, hipGetLastError());
@@ -934,8 +931,8 @@ extern "C" int awp_tune(awp_ctx *ctx, int reps)
     // the fused step on 64 x 11 tiles (FP32; z chunks of about 100, 75, 60
     // levels x tile patches 4x4, 2x2, 8x2, and larger patches) and 64 x 8
     // tiles, or on 64 x 7 tiles (mixed).  The fastest is searched, then, if
-    // it is fused, the second buffer is placed and all are searched again on
-    // that placement.
+    // the state is laid out at the fastest pad (place_state) and all are
+    // searched again on that layout.
     struct Cand {
         bool fused;
         int ty, kc, pw, ph;
@@ -988,6 +985,9 @@ extern "C" int awp_tune(awp_ctx *ctx, int reps)
     };
     // the fastest of the candidates
     size_t pick = 0;
+    // AWP_TUNE_LOG=1: every timing on stderr
+    const char *tl = getenv("AWP_TUNE_LOG");
+    const bool log = tl && atoi(tl) == 1;
     auto search = [&]() -> int {
         float best = 1e30f;
         for (size_t n = 0; n < cands.size(); ++n) {
@@ -995,6 +995,10 @@ extern "C" int awp_tune(awp_ctx *ctx, int reps)
             int r = apply(cands[n]);
             if (!r) r = time_steps(&ms);
             if (r) return r;
+            if (log)
+                fprintf(stderr, "awp_tune: %s ty %d kc %d patch %dx%d pad %zu: %.3f ms per step\n",
+                        cands[n].fused ? "fused" : "two-stage", cands[n].ty, cands[n].kc, cands[n].pw, cands[n].ph,
+                        ctx->spad, ms / reps);
             if (ms < best) {
                 best = ms;
                 pick = n;
@@ -1004,39 +1008,36 @@ extern "C" int awp_tune(awp_ctx *ctx, int reps)
     };
     rc = search();
     if (rc) return rc;
-    if (ctx->fused && ctx->vol_b) {
-        // Where buffer 1 sits relative to the other arrays sets the fused
-        // step's speed (1500x1000x300: 15.8 or 17.2 ms per step, stable for
-        // a placement), and which placements are fast changes with where the
-        // allocation lands.  Try buffer 1 at offsets in its room (no
-        // allocation: the room is part of the run's own), keep the fastest,
-        // and search every candidate again on it.
-        static const size_t shifts[] = {0, 4096, 8192, 12288, 65536 + 4096, (size_t)1 << 20, ((size_t)1 << 20) + 8192,
-                                        (size_t)4 << 20, ((size_t)4 << 20) + 4096};
-        auto use_b = [ctx](size_t shift) {
-            ctx->vol_b = (float *)((char *)ctx->vol + ctx->b_room + shift);
-            for (int a = 0; a < 9; ++a) ctx->wf[1][a] = (float *)((char *)ctx->vol_b + (size_t)a * ctx->arr_stride);
-            ctx->cur = 0;   // trial steps start from buffer 0
-        };
-        size_t keep = (size_t)((char *)ctx->vol_b - (char *)ctx->vol) - ctx->b_room;
+    {
+        // Where the state arrays sit relative to each other and to the media
+        // sets the step's speed (1500x1000x300 fused: 15.6-16.0 or 16.6-17.3
+        // ms per step, stable for a placement), and which placements are fast
+        // changes with where the allocation lands.  Lay the state out at
+        // other pads inside its room (no allocation), keep the fastest and
+        // search every candidate again on it.
+        static const size_t pads[] = {0, 256, 512, 768, 1024, 4096 + 256, (size_t)1 << 20, ((size_t)1 << 20) + 256};
+        const size_t start = ctx->spad;
+        size_t keep = start;
         float bestp = 0;
         rc = time_steps(&bestp);
-        for (size_t t = 0; !rc && t < sizeof(shifts) / sizeof(shifts[0]); ++t) {
-            if (shifts[t] == keep || shifts[t] > B_SLACK) continue;
-            use_b(shifts[t]);
-            // zero, as awp_init leaves the buffer (the step reads its ghost layers)
-            if (hipMemsetAsync(ctx->vol_b, 0, ctx->vol_b_bytes, ctx->stream) != hipSuccess) {
-                rc = fail(ctx, AWP_EHIP, "hipMemsetAsync of the second wavefield buffer failed");
-                break;
-            }
+        if (!rc && log) fprintf(stderr, "awp_tune: state pad %zu: %.3f ms per step\n", start, bestp / reps);
+        for (size_t t = 0; !rc && t < sizeof(pads) / sizeof(pads[0]); ++t) {
+            if (pads[t] == start) continue;
+            place_state(ctx, pads[t]);
+            rc = scheme_changed(ctx);
+            // the steps read the ghost layers of both buffers: zero the room
+            if (!rc && hipMemsetAsync((char *)ctx->vol + ctx->zero_begin, 0, ctx->vol_bytes - ctx->zero_begin,
+                                      ctx->stream) != hipSuccess)
+                rc = fail(ctx, AWP_EHIP, "hipMemsetAsync of the state room failed");
             float ms = 0;
-            rc = time_steps(&ms);
+            if (!rc) rc = time_steps(&ms);
+            if (!rc && log) fprintf(stderr, "awp_tune: state pad %zu: %.3f ms per step\n", pads[t], ms / reps);
             if (!rc && ms < bestp) {
                 bestp = ms;
-                keep = shifts[t];
+                keep = pads[t];
             }
         }
-        use_b(keep);
+        place_state(ctx, keep);
         if (rc) return rc;
         rc = search();
         if (rc) return rc;
@@ -1083,7 +1084,7 @@ extern "C" int awp_estimate_bytes(const awp_in3d *p, int storage, int npts, size
     const Geo g = make_geo(p->nx, p->ny, p->nz);
     const bool fused = want_fused(g, storage);
     const size_t nv = (size_t)g.nvol;
-    const VolLayout L = vol_layout(g, storage, fused);
+    const VolLayout L = vol_layout(g, storage, fused, 0);
     const size_t vol = L.total;
     const size_t prof = ((size_t)p->nx + p->ny + p->nz + 6 + (storage == AWP_STORAGE_MIXED ? 3 * (size_t)g.pz : 0)) *
                         sizeof(float);
@@ -1334,16 +1335,18 @@ extern "C" int awp_get_field(awp_ctx *ctx, int which, int kext, float *host)
 
 // ---------------------------------------------------------------------------
 // checkpoint / restart (the reference's restart files: pmcl3d.f:1016-1097,
-// 1964-2045).  File: a 72-byte header, then the state part of the volume
-// allocation (wavefield buffer 0, memory variables, their exponents), the
-// second wavefield buffer where the run holds one, the SGT sample buffer
-// and the per-step trace, byte for byte as they sit in HBM, streamed
-// through a pinned staging buffer.
+// 1964-2045).  File: a 72-byte header, then the memory variables'
+// exponent slots (mixed), the 9 arrays of wavefield buffer 0, r1..r6, the 9
+// arrays of the second wavefield buffer where the run holds one, the SGT
+// sample buffer and the per-step trace -- each array's padded volume as it
+// sits in HBM, without the gaps between arrays, so that runs laid out at
+// different state pads read each other's files -- streamed through a pinned
+// staging buffer.
 // ---------------------------------------------------------------------------
 namespace {
 
 struct CkpHead {
-    char magic[8];                  // "AWPCKP03"
+    char magic[8];                  // "AWPCKP04"
     int32_t nx, ny, nz, storage;
     int32_t fused, cur, step, nsamp;
     int32_t npts, nsamp_max, nt, reserved;
@@ -1356,11 +1359,20 @@ size_t sgt_buffer_bytes(const awp_ctx *ctx)
     return (size_t)std::max(ctx->npts, 1) * 6 * std::max(ctx->nsamp_max, 1) * sizeof(float);
 }
 
+// the state a checkpoint holds: exponent slots, wavefield buffer 0, r1..r6,
+// and buffer 1 where the run holds one (each array's elements, no pads)
+size_t state_payload_bytes(const awp_ctx *ctx)
+{
+    const size_t nv = (size_t)ctx->g.nvol;
+    const size_t rb = ctx->mixed() ? 2 : 4;
+    return 8 * (size_t)ctx->s.rexp_n + 9 * nv * 4 + 6 * nv * rb + (ctx->two_bufs ? 9 * nv * 4 : 0);
+}
+
 CkpHead ckp_head_of(const awp_ctx *ctx)
 {
     CkpHead h;
     memset(&h, 0, sizeof h);
-    memcpy(h.magic, "AWPCKP03", 8);
+    memcpy(h.magic, "AWPCKP04", 8);
     h.nx = ctx->p.nx;
     h.ny = ctx->p.ny;
     h.nz = ctx->p.nz;
@@ -1372,7 +1384,7 @@ CkpHead ckp_head_of(const awp_ctx *ctx)
     h.npts = ctx->npts;
     h.nsamp_max = ctx->nsamp_max;
     h.nt = ctx->p.nt;
-    h.state_bytes = ctx->state_bytes + ctx->vol_b_bytes;
+    h.state_bytes = state_payload_bytes(ctx);
     h.sgt_bytes = sgt_buffer_bytes(ctx);
     return h;
 }
@@ -1400,8 +1412,14 @@ int ckp_run(awp_ctx *ctx, FILE *f, bool save)
     const size_t chunk = (size_t)64 << 20;
     char *stage = nullptr;
     HIPTRY(ctx, hipHostMalloc((void **)&stage, chunk, hipHostMallocDefault));
-    int rc = ckp_stream(ctx, f, (char *)ctx->vol, ctx->state_bytes, stage, chunk, save);
-    if (!rc && ctx->vol_b) rc = ckp_stream(ctx, f, (char *)ctx->vol_b, ctx->vol_b_bytes, stage, chunk, save);
+    const size_t nv = (size_t)ctx->g.nvol;
+    int rc = ckp_stream(ctx, f, (char *)ctx->s.rexp, 8 * (size_t)ctx->s.rexp_n, stage, chunk, save);
+    for (int a = 0; a < 9 && !rc; ++a) rc = ckp_stream(ctx, f, (char *)ctx->wf[0][a], nv * 4, stage, chunk, save);
+    void *rr[6] = {ctx->s.r1, ctx->s.r2, ctx->s.r3, ctx->s.r4, ctx->s.r5, ctx->s.r6};
+    for (int a = 0; a < 6 && !rc; ++a)
+        rc = ckp_stream(ctx, f, (char *)rr[a], nv * (ctx->mixed() ? 2 : 4), stage, chunk, save);
+    for (int a = 0; a < 9 && !rc && ctx->two_bufs; ++a)
+        rc = ckp_stream(ctx, f, (char *)ctx->wf[1][a], nv * 4, stage, chunk, save);
     if (!rc) rc = ckp_stream(ctx, f, (char *)ctx->d_sgt, sgt_buffer_bytes(ctx), stage, chunk, save);
     if (!rc)
         rc = ckp_stream(ctx, f, (char *)ctx->d_trace, (size_t)std::max(ctx->p.nt, 1) * 3 * sizeof(float), stage,

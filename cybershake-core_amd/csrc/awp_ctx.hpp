@@ -29,11 +29,8 @@ struct awp_ctx {
     Coef c;
     Launch l;
     hipStream_t stream;
-    float *vol;              // one allocation for the volume arrays (wavefield buffer 0)
-    float *vol_b;            // wavefield buffer 1 (two_bufs), inside vol at b_room + its shift
-    size_t vol_b_bytes;
-    size_t b_room;           // offset in vol of buffer 1's room (B_SLACK bytes more than the buffer)
-    size_t arr_stride;       // bytes between consecutive 4-byte volume arrays
+    float *vol;              // one allocation for the volume arrays (media, exponents, state room)
+    size_t spad;             // bytes between consecutive state arrays beyond their size (place_state)
     // wavefield buffers (u1 v1 w1 xx yy zz xy xz yz): the fused step reads
     // wf[cur] and writes wf[cur ^ 1]; the two-stage step uses wf[0] only
     float *wf[2][9];
@@ -72,7 +69,7 @@ struct awp_ctx {
     double t_vel, t_str, t_all;
     size_t bytes;
     size_t vol_bytes;        // the volume allocation
-    size_t state_bytes;      // its wavefield + memory-variable part (zeroed by awp_init, as is vol_b)
+    size_t zero_begin;       // [zero_begin, vol_bytes): exponents and state room, zeroed by awp_init
     bool sg_done;            // SGT coefficients computed from FP32 media
     char err[512];
 
