@@ -118,7 +118,7 @@ EXPORTS = [
     "awp_destroy", "awp_last_error", "awp_set_media", "awp_set_media_device",
     "awp_set_media_rwg", "awp_set_source", "awp_set_sgt_points", "awp_init", "awp_step", "awp_synchronize",
     "awp_progress", "awp_get_sgt", "awp_write_sgt_file", "awp_write_sgt_file_fp16",
-    "awp_sgt16_read", "awp_sgt16_read_points", "awp_get_field", "awp_set_timing", "awp_get_timing", "awp_device_bytes",
+    "awp_sgt16_read", "awp_sgt16_read_points", "awp_get_field", "awp_get_field_noalloc", "awp_set_timing", "awp_get_timing", "awp_device_bytes",
     "awp_write_sgt_file_rwg", "awp_device_memory", "awp_estimate_bytes", "awp_batch_create", "awp_batch_tune", "awp_batch_step",
     "awp_batch_synchronize", "awp_batch_time", "awp_batch_get_launch", "awp_batch_get_scheme",
     "awp_batch_last_error", "awp_batch_destroy", "awp_write_sgt_head", "awp_checkpoint_save",
@@ -173,6 +173,7 @@ def lib() -> ctypes.CDLL:
         "awp_sgt16_read": ([cp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), vp, sz, cp, sz], ip),
         "awp_sgt16_read_points": ([cp, ip, ip, ip, vp, sz, cp, sz], ip),
         "awp_get_field": ([vp, ip, ip, vp], ip),
+        "awp_get_field_noalloc": ([vp, ip, ip, vp], ip),
         "awp_set_timing": ([vp, ip], ip),
         "awp_get_timing": ([vp, vp], ip),
         "awp_device_bytes": ([vp, ctypes.POINTER(ctypes.c_size_t)], ip),
@@ -447,6 +448,14 @@ class Run:
     def field(self, name: str, kext: int = 0) -> np.ndarray:
         out = np.empty((self.p.nz + kext, self.p.ny, self.p.nx), dtype=np.float32)
         self._do(lib().awp_get_field(self.h, FIELDS.index(name), kext, out.ctypes.data), "awp_get_field")
+        return out
+
+    def field_noalloc(self, name: str, kext: int = 0) -> np.ndarray:
+        """As field(), through awp_get_field_noalloc (no device allocation:
+        the library's fixed device scratch stages the copy)."""
+        out = np.empty((self.p.nz + kext, self.p.ny, self.p.nx), dtype=np.float32)
+        self._do(lib().awp_get_field_noalloc(self.h, FIELDS.index(name), kext, out.ctypes.data),
+                 "awp_get_field_noalloc")
         return out
 
     def set_timing(self, on: bool) -> None:

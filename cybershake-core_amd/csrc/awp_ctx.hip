@@ -13,6 +13,7 @@
 #include <unistd.h>
 
 #include <algorithm>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -25,6 +26,12 @@ using namespace awp;
 // ---------------------------------------------------------------------------
 
 extern "C" const char *awp_last_error(const awp_ctx *ctx) { return ctx ? ctx->err : "null context"; }
+
+namespace {
+// the device address of the readback scratch, set up once per device
+// (defined with awp_get_field_noalloc below)
+int scratch_setup(int device, char *err, size_t errlen);
+}  // namespace
 
 // 32-bit byte offsets where every array is below 4 GiB
 static bool use_o32(const Geo &g) { return offsets32(g); }
@@ -297,6 +304,8 @@ extern "C" int awp_create(awp_ctx **out, const awp_in3d *p, int device, int stor
         snprintf(err, errlen, "hipSetDevice(%d): %s", device, hipGetErrorString(e));
         return AWP_EHIP;
     }
+    rc = scratch_setup(device, err, errlen);
+    if (rc) return rc;
     awp_ctx *ctx = new awp_ctx();
     memset(ctx->err, 0, sizeof(ctx->err));
     ctx->p = *p;
@@ -1330,6 +1339,71 @@ extern "C" int awp_get_field(awp_ctx *ctx, int which, int kext, float *host)
     if (e == hipSuccess) e = hipMemcpy(host, d, n * sizeof(float), hipMemcpyDeviceToHost);
     hipFree(d);
     if (e != hipSuccess) return fail(ctx, AWP_EHIP, "field copy: %s", hipGetErrorString(e));
+    return AWP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Field readback without allocation (awp_get_field_noalloc).  The staging
+// buffer is a file-scope device array, `scratch`, of SCRATCH_FLOATS floats
+// (32 MiB; every GPU the library runs on holds its own copy until the process
+// ends).  Its device address is taken once per device with that device
+// current (scratch_setup, called by awp_create) and kept in scratch_dev[];
+// host code uses only that address, never the array's name.  Every call on
+// one GPU shares the array: calls that overlap on one GPU race.
+// ---------------------------------------------------------------------------
+namespace {
+constexpr size_t SCRATCH_FLOATS = (size_t)8 << 20;
+constexpr int SCRATCH_MAX_DEVICES = 64;
+__device__ __align__(256) float scratch[SCRATCH_FLOATS];
+float *scratch_dev[SCRATCH_MAX_DEVICES];
+std::mutex scratch_mu;
+
+// the address of `scratch` on `device` (the current device), once per device
+int scratch_setup(int device, char *err, size_t errlen)
+{
+    if (device < 0 || device >= SCRATCH_MAX_DEVICES) {
+        snprintf(err, errlen, "device %d beyond the %d the readback scratch serves", device, SCRATCH_MAX_DEVICES);
+        return AWP_EINVAL;
+    }
+    std::lock_guard<std::mutex> lk(scratch_mu);
+    if (scratch_dev[device]) return AWP_OK;
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+        snprintf(err, errlen, "hipGetSymbolAddress(scratch) on device %d: %s", device, hipGetErrorString(e));
+        return AWP_EHIP;
+    }
+    scratch_dev[device] = (float *)ptr;
+    return AWP_OK;
+}
+}  // namespace
+
+extern "C" int awp_get_field_noalloc(awp_ctx *ctx, int which, int kext, float *host)
+{
+    if (!ctx || !host || which < 0 || which >= 20 || (kext != 0 && kext != 2)) return fail(ctx, AWP_EINVAL, "bad field request");
+    const size_t plane = (size_t)ctx->p.nx * ctx->p.ny;
+    if (plane > SCRATCH_FLOATS)
+        return fail(ctx, AWP_EINVAL, "awp_get_field_noalloc: a level of %zu points exceeds the %zu-point scratch", plane,
+                    SCRATCH_FLOATS);
+    float *d = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(scratch_mu);
+        d = scratch_dev[ctx->device];
+    }
+    if (!d) return fail(ctx, AWP_ESTATE, "awp_get_field_noalloc: no scratch address for device %d", ctx->device);
+    HIPTRY(ctx, hipSetDevice(ctx->device));
+    const int nk = ctx->p.nz + kext;
+    const int per = (int)std::min<size_t>((size_t)nk, SCRATCH_FLOATS / plane);
+    const State st = ctx->view(ctx->cur);
+    // chunks of levels through the scratch; stream order keeps each copy
+    // ahead of the next chunk's pack
+    for (int k0 = 0; k0 < nk; k0 += per) {
+        const int n = std::min(per, nk - k0);
+        HIPTRY(ctx, launch_pack_levels(ctx->mixed(), ctx->g, st, which, kext, k0, n, d, ctx->stream));
+        HIPTRY(ctx, hipMemcpyAsync(host + (size_t)k0 * plane, d, (size_t)n * plane * sizeof(float),
+                                   hipMemcpyDeviceToHost, ctx->stream));
+    }
+    HIPTRY(ctx, hipStreamSynchronize(ctx->stream));
     return AWP_OK;
 }
 

@@ -297,6 +297,16 @@ int awp_sgt16_read_points(const char *path, int first, int count, int layout, fl
  * 17 lam 18 qp 19 qs.  For tests and checkpoints. */
 int awp_get_field(awp_ctx *ctx, int which, int kext, float *host);
 
+/* awp_get_field without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, serving loops, harnesses that forbid hipMalloc).  The
+ * field is packed, a chunk of levels at a time, into a file-scope device
+ * scratch array of 8M floats (32 MiB, held on every GPU the library runs on
+ * until the process ends) and copied to `host`.  Rejects (AWP_EINVAL) grids
+ * whose nx*ny exceeds 8M points.  Calls that overlap on one GPU share the
+ * scratch and race; calls on different GPUs do not. */
+int awp_get_field_noalloc(awp_ctx *ctx, int which, int kext, float *host);
+
 /* Device time of the last awp_step() call per kernel family, from HIP events
  * on the context's stream (milliseconds): out[0] velocity, out[1] stress,
  * out[2] whole step sequence.  Enabled by awp_set_timing(ctx, 1). */
