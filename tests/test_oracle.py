@@ -174,3 +174,33 @@ def test_free_surface_images(case_dirs):
     assert np.array_equal(zz[nz], -zz[nz - 1]) and np.array_equal(zz[nz + 1], -zz[nz - 2])
     assert np.array_equal(xz[nz], -xz[nz - 2]) and np.array_equal(xz[nz + 1], -xz[nz - 3])
     assert not np.any(o.field("u1", 2)[nz + 1])
+
+
+@pytest.mark.parametrize("name", ["g1_fx_layered", "g7_random_edges"])
+def test_mixed_media_storage_closed_form(case_dirs, name):
+    """The mixed restatement stores d1, mu, lam as FP16 deviations h from a
+    per-level profile b: b = the midrange of the level's nonzero interior
+    values (double, rounded to float), h = FP16 of (float)(x / b - 1) (round
+    to nearest even), x' = fma(b, h, b).  Derived here in numpy from the FP32
+    restatement's values: bit for bit; exact on a layered medium; within
+    2^-12 b of x everywhere."""
+    p, media, xyz, stf, pts, o32 = _oracle_for(case_dirs[name][0])
+    om = orc.Oracle(p, media, xyz, stf, pts, mixed=True)
+    for f in ("d1", "mu", "lam"):
+        x = o32.field(f)                       # [k][j][i], k = 1 at the bottom
+        got = om.field(f)
+        nz_ = x != 0
+        lo = np.where(nz_, x, np.inf).min(axis=(1, 2)).astype(np.float64)
+        hi = np.where(nz_, x, -np.inf).max(axis=(1, 2)).astype(np.float64)
+        b = ((lo + hi) * 0.5).astype(np.float32)[:, None, None]
+        h = (x.astype(np.float64) / b.astype(np.float64) - 1.0).astype(np.float32).astype(np.float16)
+        want = (b.astype(np.float64) * h.astype(np.float64) + b.astype(np.float64)).astype(np.float32)
+        assert np.array_equal(got, want), f
+        err = np.abs(got.astype(np.float64) - x)
+        assert np.all(err <= b.astype(np.float64) * 2.0 ** -12 * (1 + 2.0 ** -10) + np.abs(x) * 2.0 ** -24), f
+        if name == "g1_fx_layered":
+            assert np.array_equal(got, x), f
+        else:
+            assert not np.array_equal(got, x), f
+    o32.close()
+    om.close()
