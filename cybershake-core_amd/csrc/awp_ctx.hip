@@ -36,71 +36,52 @@ int scratch_setup(int device, char *err, size_t errlen);
 // 32-bit byte offsets where every array is below 4 GiB
 static bool use_o32(const Geo &g) { return offsets32(g); }
 
-// Placement of a run's volume arrays.  One allocation holds, in order, the
+// Placement of a run's volume arrays.  One allocation holds, in order: the
 // media d1 mu lam qp qs (4 bytes FP32, 2 bytes mixed; a 2-byte array starts
 // 64 bytes into its slot so that its interior rows start on 128-byte lines),
-// the mixed mode's row-segment exponents, and the state room: the 9 FP32
-// arrays (velocities, stresses) of wavefield buffer 0, r1..r6 (4 or 2
-// bytes) and, for a run that holds the fused step's second wavefield buffer,
-// its 9 arrays -- consecutive state arrays `spad` bytes apart beyond their
-// own size.  awp_init zeroes the exponents and the whole room.  The room is
-// SPAD_MAX bytes per state array larger than the arrays, so that awp_tune
-// can lay the state out anew at another pad: where the state arrays sit
-// relative to each other and to the media sets the speed of the same steps
-// (1500x1000x300 fused: 15.6-16.0 or 16.6-17.3 ms per step), and which pads
-// are fast depends on where the allocation lands.  AWP_STATE_PAD (a multiple
-// of 256, at most SPAD_MAX) sets the pad a run starts with (default 0).
-constexpr size_t SPAD_MAX = ((size_t)1 << 20) + 4096;
+// the mixed mode's row-segment exponents, the 9 FP32 arrays (velocities,
+// stresses) of wavefield buffer 0, r1..r6 (4 or 2 bytes) and, for a run that
+// holds the fused step's second wavefield buffer, its 9 arrays.  awp_init
+// zeroes everything after the media.
 struct VolLayout {
     size_t m[3], q[2], rexp;   // media, exponent slots
     size_t rexp_n;
     int nseg;
-    size_t zero_begin;         // [zero_begin, total): exponents and the state room (zeroed by awp_init)
-    size_t room;               // start of the state room
-    size_t wf[2][9], r[6];     // the state arrays at pad spad
-    size_t total;              // bytes of the allocation
+    size_t media_begin, media_end;   // [media_begin, media_end): the media arrays
+    size_t zero_end;                 // [media_end, zero_end): exponents and state (zeroed by awp_init)
+    size_t wf[2][9], r[6];           // the state arrays
+    size_t total;                    // bytes of the allocation
 };
 
-static size_t env_bytes(const char *name, size_t dflt)
-{
-    const char *v = getenv(name);
-    if (!v || !*v) return dflt;
-    const long long x = atoll(v);
-    return x < 0 ? dflt : ((size_t)x + 255) / 256 * 256;
-}
-
-static VolLayout vol_layout(const Geo &g, int storage, bool two_bufs, size_t spad)
+static VolLayout vol_layout(const Geo &g, int storage, bool two_bufs)
 {
     VolLayout L;
     const size_t nv = (size_t)g.nvol;
     const bool mixed = storage == AWP_STORAGE_MIXED;
-    spad = std::min(spad, SPAD_MAX);
     const size_t a4 = nv * 4;
     const size_t eslot = mixed ? nv * 2 + 128 : nv * 4, eoff = mixed ? 64 : 0;
-    size_t o = 0;
+    size_t o = L.media_begin = 0;
     for (int a = 0; a < 3; ++a, o += eslot) L.m[a] = o + eoff;
     for (int a = 0; a < 2; ++a, o += eslot) L.q[a] = o + eoff;
+    L.media_end = o;
     L.nseg = (g.nx + TX - 1) / TX;
     L.rexp_n = mixed ? (size_t)g.nz * g.ny * L.nseg : 0;
-    L.zero_begin = L.rexp = o;
+    L.rexp = o;
     o += (8 * L.rexp_n + 255) / 256 * 256;
-    L.room = o;
-    const int nstate = two_bufs ? 24 : 15;
-    size_t q = o;
-    for (int a = 0; a < 9; ++a, q += a4 + spad) L.wf[0][a] = q;
-    for (int a = 0; a < 6; ++a, q += eslot + spad) L.r[a] = q + eoff;
+    for (int a = 0; a < 9; ++a, o += a4) L.wf[0][a] = o;
+    for (int a = 0; a < 6; ++a, o += eslot) L.r[a] = o + eoff;
     for (int a = 0; a < 9; ++a) {
-        L.wf[1][a] = two_bufs ? q : L.wf[0][a];
-        if (two_bufs) q += a4 + spad;
+        L.wf[1][a] = two_bufs ? o : L.wf[0][a];
+        if (two_bufs) o += a4;
     }
-    L.total = o + (two_bufs ? 18 : 9) * a4 + 6 * eslot + (size_t)nstate * SPAD_MAX;
+    L.zero_end = L.total = o;
     return L;
 }
 
-// Points the run's state arrays at their places for state pad spad.
-static void place_state(awp_ctx *ctx, size_t spad)
+// Points every volume array of the run at its place in the allocation.
+static void place_arrays(awp_ctx *ctx)
 {
-    const VolLayout L = vol_layout(ctx->g, ctx->storage, ctx->two_bufs, spad);
+    const VolLayout L = vol_layout(ctx->g, ctx->storage, ctx->two_bufs);
     char *const base = (char *)ctx->vol;
     for (int b = 0; b < 2; ++b)
         for (int a = 0; a < 9; ++a) ctx->wf[b][a] = (float *)(base + L.wf[b][a]);
@@ -109,7 +90,15 @@ static void place_state(awp_ctx *ctx, size_t spad)
     for (int a = 0; a < 9; ++a) *f32[a] = ctx->wf[0][a];
     void **rr[6] = {&ctx->s.r1, &ctx->s.r2, &ctx->s.r3, &ctx->s.r4, &ctx->s.r5, &ctx->s.r6};
     for (int a = 0; a < 6; ++a) *rr[a] = base + L.r[a];
-    ctx->spad = std::min(spad, SPAD_MAX);
+    void **mm[3] = {&ctx->s.d1, &ctx->s.mu, &ctx->s.lam};
+    for (int a = 0; a < 3; ++a) *mm[a] = base + L.m[a];
+    ctx->s.qp = base + L.q[0];
+    ctx->s.qs = base + L.q[1];
+    ctx->s.rexp = L.rexp_n ? (signed char *)(base + L.rexp) : nullptr;
+    ctx->s.rexp_n = (long long)L.rexp_n;
+    ctx->s.nseg = L.nseg;
+    ctx->zero_begin = L.media_end;
+    ctx->zero_end = L.zero_end;
     ctx->cur = 0;
 }
 
@@ -333,8 +322,7 @@ extern "C" int awp_create(awp_ctx **out, const awp_in3d *p, int device, int stor
     ctx->timing = false;
     ctx->t_vel = ctx->t_str = ctx->t_all = 0;
     ctx->bytes = 0;
-    ctx->vol_bytes = ctx->zero_begin = 0;
-    ctx->spad = 0;
+    ctx->vol_bytes = ctx->zero_begin = ctx->zero_end = 0;
     ctx->sg_done = false;
     ctx->d_trace = nullptr;
     ctx->trace_pos = 0;
@@ -346,7 +334,7 @@ extern "C" int awp_create(awp_ctx **out, const awp_in3d *p, int device, int stor
         return AWP_EHIP;
     }
     // the volume arrays and their placement (vol_layout)
-    const VolLayout L = vol_layout(ctx->g, storage, ctx->two_bufs, 0);
+    const VolLayout L = vol_layout(ctx->g, storage, ctx->two_bufs);
     const size_t nbytes = L.total;
     e = hipMalloc((void **)&ctx->vol, nbytes);
     if (e != hipSuccess) {
@@ -357,16 +345,7 @@ extern "C" int awp_create(awp_ctx **out, const awp_in3d *p, int device, int stor
     }
     ctx->bytes += nbytes;
     ctx->vol_bytes = nbytes;
-    ctx->zero_begin = L.zero_begin;
-    char *const base = (char *)ctx->vol;
-    place_state(ctx, env_bytes("AWP_STATE_PAD", 0));
-    ctx->s.rexp = L.rexp_n ? (signed char *)(base + L.rexp) : nullptr;
-    ctx->s.rexp_n = (long long)L.rexp_n;
-    ctx->s.nseg = L.nseg;
-    void **mm[3] = {&ctx->s.d1, &ctx->s.mu, &ctx->s.lam};
-    for (int a = 0; a < 3; ++a) *mm[a] = base + L.m[a];
-    ctx->s.qp = base + L.q[0];
-    ctx->s.qs = base + L.q[1];
+    place_arrays(ctx);
     MScale one = {1, 1, 1, 1};
     ctx->s.ms = one;
     // sponge profiles fx fy fz, then (mixed) the [3][pz] level profiles of
@@ -774,7 +753,7 @@ extern "C" int awp_init(awp_ctx *ctx)
         return fail(ctx, AWP_ESTATE, "awp_init needs media, source and SGT points first");
     HIPTRY(ctx, hipSetDevice(ctx->device));
     // the exponent slots and the state room (wavefield buffers, memory variables)
-    HIPTRY(ctx, hipMemsetAsync((char *)ctx->vol + ctx->zero_begin, 0, ctx->vol_bytes - ctx->zero_begin, ctx->stream));
+    HIPTRY(ctx, hipMemsetAsync((char *)ctx->vol + ctx->zero_begin, 0, ctx->zero_end - ctx->zero_begin, ctx->stream));
     const size_t nout = (size_t)std::max(ctx->npts, 1) * 6 * std::max(ctx->nsamp_max, 1);
     hipLaunchKernelGGL(k_fill, dim3(1024), dim3(256), 0, ctx->stream, ctx->d_sgt, nout, -1.0f);
     HIPTRY(ctx, hipGetLastError());
@@ -925,6 +904,20 @@ int trial_steps(awp_ctx *ctx, int n)
     return AWP_OK;
 }
 
+// one warm-up step, then `reps` timed ones (milliseconds in *ms)
+int time_trial(awp_ctx *ctx, int reps, float *ms)
+{
+    int r = trial_steps(ctx, 1);
+    if (r) return r;
+    HIPTRY(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
+    r = trial_steps(ctx, reps);
+    if (r) return r;
+    HIPTRY(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
+    HIPTRY(ctx, hipEventSynchronize(ctx->ev[1]));
+    HIPTRY(ctx, hipEventElapsedTime(ms, ctx->ev[0], ctx->ev[1]));
+    return AWP_OK;
+}
+
 }  // namespace
 
 extern "C" int awp_tune(awp_ctx *ctx, int reps)
@@ -939,9 +932,7 @@ extern "C" int awp_tune(awp_ctx *ctx, int reps)
     // candidates: the two-stage step with its defaults; with both buffers,
     // the fused step on 64 x 11 tiles (FP32; z chunks of about 100, 75, 60
     // levels x tile patches 4x4, 2x2, 8x2, and larger patches) and 64 x 8
-    // tiles, or on 64 x 7 tiles (mixed).  The fastest is searched, then, if
-    // the state is laid out at the fastest pad (place_state) and all are
-    // searched again on that layout.
+    // tiles, or on 64 x 7 tiles (mixed).  The fastest is kept.
     struct Cand {
         bool fused;
         int ty, kc, pw, ph;
@@ -981,17 +972,7 @@ extern "C" int awp_tune(awp_ctx *ctx, int reps)
         return scheme_changed(ctx);
     };
     // one warm-up step, then `reps` timed ones (ms in *ms)
-    auto time_steps = [ctx, reps](float *ms) -> int {
-        int r = trial_steps(ctx, 1);
-        if (r) return r;
-        HIPTRY(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
-        r = trial_steps(ctx, reps);
-        if (r) return r;
-        HIPTRY(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
-        HIPTRY(ctx, hipEventSynchronize(ctx->ev[1]));
-        HIPTRY(ctx, hipEventElapsedTime(ms, ctx->ev[0], ctx->ev[1]));
-        return AWP_OK;
-    };
+    auto time_steps = [ctx, reps](float *ms) -> int { return time_trial(ctx, reps, ms); };
     // the fastest of the candidates
     size_t pick = 0;
     // AWP_TUNE_LOG=1: every timing on stderr
@@ -1005,9 +986,9 @@ extern "C" int awp_tune(awp_ctx *ctx, int reps)
             if (!r) r = time_steps(&ms);
             if (r) return r;
             if (log)
-                fprintf(stderr, "awp_tune: %s ty %d kc %d patch %dx%d pad %zu: %.3f ms per step\n",
+                fprintf(stderr, "awp_tune: %s ty %d kc %d patch %dx%d: %.3f ms per step\n",
                         cands[n].fused ? "fused" : "two-stage", cands[n].ty, cands[n].kc, cands[n].pw, cands[n].ph,
-                        ctx->spad, ms / reps);
+                        ms / reps);
             if (ms < best) {
                 best = ms;
                 pick = n;
@@ -1017,40 +998,6 @@ extern "C" int awp_tune(awp_ctx *ctx, int reps)
     };
     rc = search();
     if (rc) return rc;
-    {
-        // Where the state arrays sit relative to each other and to the media
-        // sets the step's speed (1500x1000x300 fused: 15.6-16.0 or 16.6-17.3
-        // ms per step, stable for a placement), and which placements are fast
-        // changes with where the allocation lands.  Lay the state out at
-        // other pads inside its room (no allocation), keep the fastest and
-        // search every candidate again on it.
-        static const size_t pads[] = {0, 256, 512, 768, 1024, 4096 + 256, (size_t)1 << 20, ((size_t)1 << 20) + 256};
-        const size_t start = ctx->spad;
-        size_t keep = start;
-        float bestp = 0;
-        rc = time_steps(&bestp);
-        if (!rc && log) fprintf(stderr, "awp_tune: state pad %zu: %.3f ms per step\n", start, bestp / reps);
-        for (size_t t = 0; !rc && t < sizeof(pads) / sizeof(pads[0]); ++t) {
-            if (pads[t] == start) continue;
-            place_state(ctx, pads[t]);
-            rc = scheme_changed(ctx);
-            // the steps read the ghost layers of both buffers: zero the room
-            if (!rc && hipMemsetAsync((char *)ctx->vol + ctx->zero_begin, 0, ctx->vol_bytes - ctx->zero_begin,
-                                      ctx->stream) != hipSuccess)
-                rc = fail(ctx, AWP_EHIP, "hipMemsetAsync of the state room failed");
-            float ms = 0;
-            if (!rc) rc = time_steps(&ms);
-            if (!rc && log) fprintf(stderr, "awp_tune: state pad %zu: %.3f ms per step\n", pads[t], ms / reps);
-            if (!rc && ms < bestp) {
-                bestp = ms;
-                keep = pads[t];
-            }
-        }
-        place_state(ctx, keep);
-        if (rc) return rc;
-        rc = search();
-        if (rc) return rc;
-    }
     // the trial steps changed the state: start the run again
     return awp_init(ctx);
 }
@@ -1093,7 +1040,7 @@ extern "C" int awp_estimate_bytes(const awp_in3d *p, int storage, int npts, size
     const Geo g = make_geo(p->nx, p->ny, p->nz);
     const bool fused = want_fused(g, storage);
     const size_t nv = (size_t)g.nvol;
-    const VolLayout L = vol_layout(g, storage, fused, 0);
+    const VolLayout L = vol_layout(g, storage, fused);
     const size_t vol = L.total;
     const size_t prof = ((size_t)p->nx + p->ny + p->nz + 6 + (storage == AWP_STORAGE_MIXED ? 3 * (size_t)g.pz : 0)) *
                         sizeof(float);
@@ -1413,9 +1360,7 @@ extern "C" int awp_get_field_noalloc(awp_ctx *ctx, int which, int kext, float *h
 // exponent slots (mixed), the 9 arrays of wavefield buffer 0, r1..r6, the 9
 // arrays of the second wavefield buffer where the run holds one, the SGT
 // sample buffer and the per-step trace -- each array's padded volume as it
-// sits in HBM, without the gaps between arrays, so that runs laid out at
-// different state pads read each other's files -- streamed through a pinned
-// staging buffer.
+// sits in HBM, array by array -- streamed through a pinned staging buffer.
 // ---------------------------------------------------------------------------
 namespace {
 

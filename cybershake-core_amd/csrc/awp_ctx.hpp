@@ -29,8 +29,7 @@ struct awp_ctx {
     Coef c;
     Launch l;
     hipStream_t stream;
-    float *vol;              // one allocation for the volume arrays (media, exponents, state room)
-    size_t spad;             // bytes between consecutive state arrays beyond their size (place_state)
+    float *vol;              // one allocation for the volume arrays (media, exponents, state)
     // wavefield buffers (u1 v1 w1 xx yy zz xy xz yz): the fused step reads
     // wf[cur] and writes wf[cur ^ 1]; the two-stage step uses wf[0] only
     float *wf[2][9];
@@ -69,7 +68,7 @@ struct awp_ctx {
     double t_vel, t_str, t_all;
     size_t bytes;
     size_t vol_bytes;        // the volume allocation
-    size_t zero_begin;       // [zero_begin, vol_bytes): exponents and state room, zeroed by awp_init
+    size_t zero_begin, zero_end;   // in vol: exponents and state, zeroed by awp_init
     bool sg_done;            // SGT coefficients computed from FP32 media
     char err[512];
 

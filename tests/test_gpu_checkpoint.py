@@ -23,16 +23,12 @@ pytestmark = pytest.mark.gpu
 CASE = "g7_random_edges"   # 133 steps, points on every edge
 
 
-@pytest.mark.parametrize("pads", [(0, 0), (768, 1048832)])
 @pytest.mark.parametrize("scheme", ["split", "fused"])
 @pytest.mark.parametrize("storage", [awpsgt.AWP_STORAGE_FP32, awpsgt.AWP_STORAGE_MIXED])
-def test_restart_continues_bit_for_bit(case_dirs, storage, scheme, pads, tmp_path, monkeypatch):
-    """pads: the state pads (gaps between the state arrays) of the saving
-    and of the restoring run -- a checkpoint holds the arrays without them."""
+def test_restart_continues_bit_for_bit(case_dirs, storage, scheme, tmp_path, monkeypatch):
     d = case_dirs[CASE][0]
     monkeypatch.setenv("AWP_SCHEME", scheme)
     ref = _single(d, storage)            # uninterrupted
-    monkeypatch.setenv("AWP_STATE_PAD", str(pads[0]))
     a = _make(d, storage)
     a.step(37)                           # odd: the fused scheme's buffer parity is 1
     ckp = os.path.join(str(tmp_path), "ckp.bin")
@@ -40,7 +36,6 @@ def test_restart_continues_bit_for_bit(case_dirs, storage, scheme, pads, tmp_pat
     assert not os.path.exists(ckp + ".tmp")
     a.step(a.p.nt)
     a.synchronize()
-    monkeypatch.setenv("AWP_STATE_PAD", str(pads[1]))
     b = _make(d, storage)
     b.checkpoint_load(ckp)
     assert b.progress()[0] == 37

@@ -56,21 +56,18 @@ def test_media_setup_bitwise(case_dirs):
 
 @pytest.mark.parametrize("scheme", ["split", "fused", "split_o32", "split_o64", "split_ty16",
                                     "split_xw2", "split_xw2_ty4", "split_xw2_o64", "split_o32v",
-                                    "split_o64v", "fused_ty11", "fused_lv2", "split_pad", "fused_pad"])
+                                    "split_o64v", "fused_ty11", "fused_lv2"])
 @pytest.mark.parametrize("name", list(cases.CASES))
 def test_fields_bitwise_after_steps(case_dirs, name, scheme, monkeypatch):
     """Both time-step schemes (AWP_SCHEME=split two-stage, AWP_SCHEME=fused),
     both offset forms of the two-stage stress and velocity kernels (32-bit
     byte offsets, and the 64-bit element indices that grids with arrays of
     4 GiB or more take), their tile shapes (64 x 8, 64 x 16, 128 x 8,
-    128 x 4) and a state laid out with gaps between its arrays against the
-    restatement, bit for bit."""
+    128 x 4) against the restatement, bit for bit."""
     d, _ = case_dirs[name]
     p, media, xyz, stf, pts = _inputs(d)
     o = _oracle(d, p, media, xyz, stf, pts)
     monkeypatch.setenv("AWP_SCHEME", "fused" if scheme.startswith("fused") else "split")
-    if scheme.endswith("_pad"):
-        monkeypatch.setenv("AWP_STATE_PAD", "1048832")
     if scheme in ("fused_ty11", "fused_lv2"):
         # 64 x 11 tiles: 16-wave blocks
         monkeypatch.setenv("AWP_TILE_Y", "11")
