@@ -174,6 +174,16 @@ static Launch choose_launch(const Geo &g, bool fused, bool mixed, int device)
     // step against 17.62 for 2 x 16 and 17.75 for 3 x 8)
     l.pw = 4;
     l.ph = 4;
+    // tile claim of the fused step (AWP_CLAIM=0/1; AWP_XROT=0..7 rotates
+    // the XCD ranges): persistent blocks that claim tiles per XCD, the
+    // default (1500x1000x300: 15.56-16.92 ms per step against 15.88-17.23
+    // with one block per tile, over the same sequence of runs)
+    l.claim = true;
+    l.xrot = 0;
+    const char *ce = getenv("AWP_CLAIM");
+    if (ce && (!strcmp(ce, "0") || !strcmp(ce, "1"))) l.claim = ce[0] == '1';
+    const char *xr = getenv("AWP_XROT");
+    if (xr) l.xrot = atoi(xr) & 7;
     const char *pe = getenv("AWP_PATCH");
     if (pe) {
         int w = 0, h = 0;
@@ -307,6 +317,7 @@ extern "C" int awp_create(awp_ctx **out, const awp_in3d *p, int device, int stor
     ctx->vol = nullptr;
     ctx->prof = nullptr;
     ctx->d_sjob = nullptr;
+    ctx->d_claim = nullptr;
     ctx->split_total = ctx->split_fs = 0;
     ctx->npsrc = 0;
     ctx->d_psrc = ctx->d_ptop = nullptr;
@@ -359,6 +370,13 @@ extern "C" int awp_create(awp_ctx **out, const awp_in3d *p, int device, int stor
         return AWP_ENOMEM;
     }
     ctx->bytes += (np + nmb) * sizeof(float);
+    e = hipMalloc((void **)&ctx->d_claim, awp::CLAIM_WORDS * sizeof(int));
+    if (e == hipSuccess) e = hipMemset(ctx->d_claim, 0, awp::CLAIM_WORDS * sizeof(int));
+    if (e != hipSuccess) {
+        snprintf(err, errlen, "hipMalloc tile claim counters: %s", hipGetErrorString(e));
+        awp_destroy(ctx);
+        return AWP_ENOMEM;
+    }
     ctx->s.mbar = nmb ? ctx->prof + np : nullptr;
     ctx->s.fx = ctx->prof;
     ctx->s.fy = ctx->prof + (p->nx + 2);
@@ -438,6 +456,7 @@ extern "C" int awp_destroy(awp_ctx *ctx)
     hipFree(ctx->prof);
     hipFree(ctx->d_trace);
     hipFree(ctx->d_sjob);
+    hipFree(ctx->d_claim);
     hipFree(ctx->d_psrc);
     hipFree(ctx->d_ptop);
     hipFree(ctx->d_stf);
@@ -817,7 +836,7 @@ extern "C" int awp_step(awp_ctx *ctx, int nsteps)
             const State sa = ctx->view(ctx->cur), sb = ctx->view(ctx->cur ^ 1);
             if (e) HIPTRY(ctx, hipEventRecord(e[1], st));
             if (e) HIPTRY(ctx, hipEventRecord(e[2], st));
-            HIPTRY(ctx, launch_step_fused(mx, ctx->g, sa, sb, ctx->c, ctx->l, st));
+            HIPTRY(ctx, launch_step_fused(mx, ctx->g, sa, sb, ctx->c, ctx->l, ctx->d_claim, st));
             ctx->cur ^= 1;
             sn = sb;
         } else {
@@ -892,7 +911,7 @@ int trial_steps(awp_ctx *ctx, int n)
     for (int k = 0; k < n; ++k) {
         if (ctx->fused) {
             HIPTRY(ctx, launch_step_fused(mx, ctx->g, ctx->view(ctx->cur), ctx->view(ctx->cur ^ 1), ctx->c, ctx->l,
-                                          ctx->stream));
+                                          ctx->d_claim, ctx->stream));
             ctx->cur ^= 1;
         } else {
             HIPTRY(ctx, launch_split_velocity(ctx->mixed(), ctx->l, ctx->l.o32, ctx->d_sjob, 1, ctx->split_total, all, ctx->stream));

@@ -601,6 +601,30 @@ k_step(const Geo g, const State a, const State b, const Coef c, const int kc, co
     step_tile<P, TY, PF>(g, a, b, c, kc, bx, by, bz, lds);
 }
 
+// The persistent form: each block claims tiles (claim_tile) until none is
+// left.
+template <class P, int TY, int PF>
+__global__ void __launch_bounds__(64 * (TY + 5))
+k_step_claim(const Geo g, const State a, const State b, const Coef c, const int kc, const FTiles tl, int *claim)
+{
+    using Sh = FShape<TY, PF>;
+    __shared__ float lds[Sh::NSLOT * Sh::SLOT];
+    __shared__ int next;
+    const bool lead = threadIdx.x == 0 && threadIdx.y == 0;
+    const int total = tl.nbx * tl.nby * tl.nbz;
+    for (;;) {
+        if (lead) next = claim_tile(claim, tl.per, total, tl.rot);
+        __syncthreads();
+        const int tile = next;
+        if (tile < 0) break;   // uniform over the block
+        int bx, by, bz;
+        patch_tile(tile, tl.nbx, tl.nby, tl.pw, tl.ph, bx, by, bz);
+        step_tile<P, TY, PF>(g, a, b, c, kc, bx, by, bz, lds);
+        __syncthreads();   // the next tile reuses the LDS ring and `next`
+    }
+    if (lead) claim_done(claim, (int)gridDim.x);
+}
+
 // Several runs in one launch.  Runs' tiles are numbered consecutively (run r
 // owns [tile0, tile0 + ntiles)); the grid is persistent: block n works on
 // XCD n % 8 and walks that XCD's contiguous range of the global tile list
@@ -636,15 +660,43 @@ FTiles fused_tiles(const Geo &g, const Launch &l)
     t.ph = l.ph > 0 ? l.ph : 1;
     const long long n = (long long)t.nbx * t.nby * t.nbz;
     t.per = (int)((n + 7) / 8);
+    t.rot = l.xrot & 7;
     return t;
+}
+
+// blocks of k_step_claim resident on the device at once
+template <class P, int TY, int PF>
+static int claim_grid(int device)
+{
+    int nb = 0, ncu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_step_claim<P, TY, PF>, TX * (TY + 5), 0) != hipSuccess)
+        nb = 1;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) ncu = 256;
+    return std::max(1, nb) * std::max(8, ncu);
 }
 
 template <class P, int TY, int PF>
 static hipError_t step_t(const Geo &g, const State &a, const State &b, const Coef &c, const Launch &l,
-                         hipStream_t st)
+                         int *claim, hipStream_t st)
 {
     const FTiles t = fused_tiles(g, l);
-    hipLaunchKernelGGL((k_step<P, TY, PF>), dim3(8 * t.per), dim3(TX, TY + 5), 0, st, g, a, b, c, l.kc, t);
+    if (l.claim && claim) {
+        // the grid size per kernel variant and device, queried once
+        static int cached[16] = {};
+        int dev = 0;
+        hipError_t e = hipGetDevice(&dev);
+        if (e != hipSuccess) return e;
+        int grid = dev >= 0 && dev < 16 ? cached[dev] : 0;
+        if (!grid) {
+            grid = claim_grid<P, TY, PF>(dev);
+            if (dev >= 0 && dev < 16) cached[dev] = grid;
+        }
+        const int ntile = t.nbx * t.nby * t.nbz;
+        hipLaunchKernelGGL((k_step_claim<P, TY, PF>), dim3(std::min(grid, ntile)), dim3(TX, TY + 5), 0, st, g, a,
+                           b, c, l.kc, t, claim);
+    } else {
+        hipLaunchKernelGGL((k_step<P, TY, PF>), dim3(8 * t.per), dim3(TX, TY + 5), 0, st, g, a, b, c, l.kc, t);
+    }
     return hipGetLastError();
 }
 
@@ -713,9 +765,9 @@ static int grid_t(int device)
     } while (0)
 
 hipError_t launch_step_fused(bool mixed, const Geo &g, const State &a, const State &b, const Coef &c,
-                             const Launch &l, hipStream_t st)
+                             const Launch &l, int *claim, hipStream_t st)
 {
-    AWP_FUSED_DISPATCH(mixed, l, return (step_t<P, TY, PF>(g, a, b, c, l, st)));
+    AWP_FUSED_DISPATCH(mixed, l, return (step_t<P, TY, PF>(g, a, b, c, l, claim, st)));
     return hipErrorInvalidValue;
 }
 

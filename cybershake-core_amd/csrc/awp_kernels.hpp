@@ -394,6 +394,8 @@ struct Launch {
     int nchunk;      // number of z chunks
     int unroll;      // z-loop unroll factor (1 or 2); prefetch depth of the fused step
     int pw, ph;      // fused step: tile patch width / height (tiles)
+    bool claim;      // fused step: persistent blocks claim tiles per XCD (claim_tile), else one block per tile
+    int xrot;        // fused step with claim: range of XCD x is (x + xrot) % 8
     int spw, sph;    // two-stage kernels: tile patch width / height (0 = rows of tiles)
     bool o32;        // 32-bit byte offsets reach every array (offsets32() of the run's geometry)
     bool o32_s;      // two-stage stress kernel: its 32-bit byte-offset form (only where o32)
@@ -465,10 +467,12 @@ hipError_t launch_absmax(const float *a, long long n, float *part, int nblk, hip
 hipError_t launch_to_half(const float *src, _Float16 *dst, float scale, long long n, hipStream_t st);
 hipError_t launch_sgt_fp16(const float *sgt, const int *kidx, int nkept, int nsamp_max, int nsamp, float *scales,
                            _Float16 *out, hipStream_t st);
-// the fused one-sweep step (awp_fused.hip): reads state a, writes state b,
-// one block per tile
+// the fused one-sweep step (awp_fused.hip): reads state a, writes state b;
+// one block per tile, or with l.claim persistent blocks claiming tiles
+// through `claim` (CLAIM_WORDS ints of device memory, zero before the first
+// launch; each launch leaves them zero)
 hipError_t launch_step_fused(bool mixed, const Geo &g, const State &a, const State &b, const Coef &c,
-                             const Launch &l, hipStream_t st);
+                             const Launch &l, int *claim, hipStream_t st);
 
 // tiles of one run for the fused step: nbx * nby * nbz column tiles
 // Tile order: within each z chunk, tiles are grouped in patches of pw x ph
@@ -480,7 +484,52 @@ struct FTiles {
     int nbx, nby, nbz;
     int per;   // tiles per XCD (single-run launch)
     int pw, ph;
+    int rot;   // claimed launch: XCD x takes range (x + rot) % 8 first
 };
+
+// Tile claim of a persistent launch.  The tiles [0, total) form 8 ranges of
+// `per` consecutive tiles (the last one shorter).  A block on XCD x takes the
+// next unclaimed tile of range (x + rot) % 8, the XCD read from the
+// hardware register, so that an XCD sweeps one compact part of the domain
+// whatever order the dispatcher hands blocks to XCDs in; when that range is
+// spent it takes tiles of the following ranges, so a faster XCD finishes a
+// slower one's work.  claim[0..7]: next slot of each range; claim[8]: blocks
+// done -- the last block to finish clears all nine for the next launch.
+static constexpr int CLAIM_WORDS = 16;
+
+// the XCD the calling wave runs on
+__device__ inline int xcc_id()
+{
+    unsigned v;
+    asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID)" : "=s"(v));
+    return (int)(v & 7);
+}
+
+// the next tile for a block on this XCD, or -1 when every tile is taken
+// (called by one lane of the block)
+__device__ inline int claim_tile(int *claim, int per, int total, int rot)
+{
+    const int x = (xcc_id() + rot) & 7;
+    for (int d = 0; d < 8; ++d) {
+        const int q = (x + d) & 7;
+        const int n = min(per, total - q * per);   // tiles of range q
+        if (n <= 0) continue;
+        const int k = atomicAdd(&claim[q], 1);
+        if (k < n) return q * per + k;
+    }
+    return -1;
+}
+
+// a block has taken its last (failed) claim; the last of the grid clears the
+// counters (called by one lane of the block)
+__device__ inline void claim_done(int *claim, int nblocks)
+{
+    __threadfence();
+    if (atomicAdd(&claim[8], 1) == nblocks - 1) {
+        __threadfence();
+        for (int q = 0; q < 9; ++q) atomicExch(&claim[q], 0);
+    }
+}
 FTiles fused_tiles(const Geo &g, const Launch &l);
 
 // one run of a batched fused step; s[p] is read and s[p ^ 1] written when
