@@ -370,8 +370,8 @@ extern "C" int awp_create(awp_ctx **out, const awp_in3d *p, int device, int stor
         return AWP_ENOMEM;
     }
     ctx->bytes += (np + nmb) * sizeof(float);
-    e = hipMalloc((void **)&ctx->d_claim, awp::CLAIM_WORDS * sizeof(int));
-    if (e == hipSuccess) e = hipMemset(ctx->d_claim, 0, awp::CLAIM_WORDS * sizeof(int));
+    e = hipMalloc((void **)&ctx->d_claim, awp::CLAIM_WORDS * sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipMemset(ctx->d_claim, 0, awp::CLAIM_WORDS * sizeof(unsigned long long));
     if (e != hipSuccess) {
         snprintf(err, errlen, "hipMalloc tile claim counters: %s", hipGetErrorString(e));
         awp_destroy(ctx);

@@ -605,7 +605,8 @@ k_step(const Geo g, const State a, const State b, const Coef c, const int kc, co
 // left.
 template <class P, int TY, int PF>
 __global__ void __launch_bounds__(64 * (TY + 5))
-k_step_claim(const Geo g, const State a, const State b, const Coef c, const int kc, const FTiles tl, int *claim)
+k_step_claim(const Geo g, const State a, const State b, const Coef c, const int kc, const FTiles tl,
+             unsigned long long *claim)
 {
     using Sh = FShape<TY, PF>;
     __shared__ float lds[Sh::NSLOT * Sh::SLOT];
@@ -677,7 +678,7 @@ static int claim_grid(int device)
 
 template <class P, int TY, int PF>
 static hipError_t step_t(const Geo &g, const State &a, const State &b, const Coef &c, const Launch &l,
-                         int *claim, hipStream_t st)
+                         unsigned long long *claim, hipStream_t st)
 {
     const FTiles t = fused_tiles(g, l);
     if (l.claim && claim) {
@@ -765,7 +766,7 @@ static int grid_t(int device)
     } while (0)
 
 hipError_t launch_step_fused(bool mixed, const Geo &g, const State &a, const State &b, const Coef &c,
-                             const Launch &l, int *claim, hipStream_t st)
+                             const Launch &l, unsigned long long *claim, hipStream_t st)
 {
     AWP_FUSED_DISPATCH(mixed, l, return (step_t<P, TY, PF>(g, a, b, c, l, claim, st)));
     return hipErrorInvalidValue;

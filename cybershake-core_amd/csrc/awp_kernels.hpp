@@ -469,10 +469,10 @@ hipError_t launch_sgt_fp16(const float *sgt, const int *kidx, int nkept, int nsa
                            _Float16 *out, hipStream_t st);
 // the fused one-sweep step (awp_fused.hip): reads state a, writes state b;
 // one block per tile, or with l.claim persistent blocks claiming tiles
-// through `claim` (CLAIM_WORDS ints of device memory, zero before the first
+// through `claim` (CLAIM_WORDS words of device memory, zero before the first
 // launch; each launch leaves them zero)
 hipError_t launch_step_fused(bool mixed, const Geo &g, const State &a, const State &b, const Coef &c,
-                             const Launch &l, int *claim, hipStream_t st);
+                             const Launch &l, unsigned long long *claim, hipStream_t st);
 
 // tiles of one run for the fused step: nbx * nby * nbz column tiles
 // Tile order: within each z chunk, tiles are grouped in patches of pw x ph
@@ -486,16 +486,23 @@ struct FTiles {
     int pw, ph;
     int rot;   // claimed launch: XCD x takes range (x + rot) % 8 first
 };
+FTiles fused_tiles(const Geo &g, const Launch &l);
 
 // Tile claim of a persistent launch.  The tiles [0, total) form 8 ranges of
 // `per` consecutive tiles (the last one shorter).  A block on XCD x takes the
-// next unclaimed tile of range (x + rot) % 8, the XCD read from the
-// hardware register, so that an XCD sweeps one compact part of the domain
-// whatever order the dispatcher hands blocks to XCDs in; when that range is
-// spent it takes tiles of the following ranges, so a faster XCD finishes a
-// slower one's work.  claim[0..7]: next slot of each range; claim[8]: blocks
-// done -- the last block to finish clears all nine for the next launch.
-static constexpr int CLAIM_WORDS = 16;
+// next unclaimed tile from the front of range (x + rot) % 8, the XCD read
+// from the hardware register, so that an XCD sweeps one compact part of the
+// domain whatever order the dispatcher hands blocks to XCDs in.  When that
+// range is spent it takes tiles from the back of the following ranges: a
+// faster XCD finishes a slower one's work on tiles away from the ones the
+// owner is working on.  Counter q
+// (one 64-bit word per 128-byte line, line q): tiles taken from the front
+// of range q (low half) and from its back (high half); a claim is valid
+// while the two sum to less than the range's tiles, so no tile is taken
+// twice.  Line 8: blocks done -- the last block to finish clears all nine
+// for the next launch.
+static constexpr int CLAIM_LINE = 16;                  // 64-bit words per 128-byte line
+static constexpr int CLAIM_WORDS = 9 * CLAIM_LINE;     // 64-bit words of the claim buffer
 
 // the XCD the calling wave runs on
 __device__ inline int xcc_id()
@@ -507,30 +514,30 @@ __device__ inline int xcc_id()
 
 // the next tile for a block on this XCD, or -1 when every tile is taken
 // (called by one lane of the block)
-__device__ inline int claim_tile(int *claim, int per, int total, int rot)
+__device__ inline int claim_tile(unsigned long long *claim, int per, int total, int rot)
 {
     const int x = (xcc_id() + rot) & 7;
     for (int d = 0; d < 8; ++d) {
         const int q = (x + d) & 7;
         const int n = min(per, total - q * per);   // tiles of range q
         if (n <= 0) continue;
-        const int k = atomicAdd(&claim[q], 1);
-        if (k < n) return q * per + k;
+        const unsigned long long o = atomicAdd(&claim[q * CLAIM_LINE], d == 0 ? 1ull : 1ull << 32);
+        const unsigned f = (unsigned)o, b = (unsigned)(o >> 32);
+        if (f + b < (unsigned)n) return q * per + (d == 0 ? (int)f : n - 1 - (int)b);
     }
     return -1;
 }
 
 // a block has taken its last (failed) claim; the last of the grid clears the
 // counters (called by one lane of the block)
-__device__ inline void claim_done(int *claim, int nblocks)
+__device__ inline void claim_done(unsigned long long *claim, int nblocks)
 {
     __threadfence();
-    if (atomicAdd(&claim[8], 1) == nblocks - 1) {
+    if (atomicAdd(&claim[8 * CLAIM_LINE], 1ull) == (unsigned long long)(nblocks - 1)) {
         __threadfence();
-        for (int q = 0; q < 9; ++q) atomicExch(&claim[q], 0);
+        for (int q = 0; q < 9; ++q) atomicExch(&claim[q * CLAIM_LINE], 0ull);
     }
 }
-FTiles fused_tiles(const Geo &g, const Launch &l);
 
 // one run of a batched fused step; s[p] is read and s[p ^ 1] written when
 // the launch's parity is p
