@@ -1,6 +1,7 @@
 // awp_batch.hip -- several SGT runs (components, sites) advanced together on
 // one GPU.  Each stage of a time step is one persistent launch over the
-// tiles of every run (a grid walking the concatenated tile list): in the
+// tiles of every run (a grid walking the concatenated tile list; the fused
+// kernel's blocks claim its tiles per XCD, claim_tile): in the
 // two-stage scheme the velocity kernel, the free-surface images and the
 // stress kernel, in the fused scheme the one-sweep kernel; then one launch
 // for every run's source and SGT sample.  Two steps (both buffer parities of
@@ -50,6 +51,7 @@ struct awp_batch {
     SJob *d_sjobs;
     AuxJob *d_aux;
     int *d_counter;
+    unsigned long long *d_claim;   // the fused kernel's tile claim counters (CLAIM_WORDS)
     int total, fs_total, max_kept;
     int grid, grid_v, grid_s;    // persistent blocks: fused kernel; velocity, stress kernels
     hipGraphExec_t graph;
@@ -195,7 +197,7 @@ static hipError_t launch_kernels(awp_batch *b, int par)
     if (b->split)
         return launch_step_split_batch(b->mixed, b->l, b->o32, b->d_sjobs, n, b->total, b->fs_total,
                                        b->grid_v, b->grid_s, b->stream);
-    return launch_step_fused_batch(b->mixed, b->l, b->d_jobs, n, b->total, b->grid, par, b->stream);
+    return launch_step_fused_batch(b->mixed, b->l, b->d_jobs, n, b->total, b->grid, par, b->d_claim, b->stream);
 }
 
 // grid sizes of the current scheme and launch shape: the blocks that fit on
@@ -317,6 +319,7 @@ extern "C" int awp_batch_create(awp_batch **out, awp_ctx *const *runs, int nruns
     b->d_sjobs = nullptr;
     b->d_aux = nullptr;
     b->d_counter = nullptr;
+    b->d_claim = nullptr;
     b->graph_ok = false;
     const char *g_env = getenv("AWP_GRAPH");
     b->use_graph = !(g_env && !strcmp(g_env, "0"));
@@ -328,6 +331,8 @@ extern "C" int awp_batch_create(awp_batch **out, awp_ctx *const *runs, int nruns
     if (e == hipSuccess) e = hipMalloc((void **)&b->d_sjobs, nruns * sizeof(SJob));
     if (e == hipSuccess) e = hipMalloc((void **)&b->d_aux, nruns * sizeof(AuxJob));
     if (e == hipSuccess) e = hipMalloc((void **)&b->d_counter, nruns * sizeof(int));
+    if (e == hipSuccess) e = hipMalloc((void **)&b->d_claim, awp::CLAIM_WORDS * sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipMemset(b->d_claim, 0, awp::CLAIM_WORDS * sizeof(unsigned long long));
     if (e == hipSuccess) e = hipEventCreate(&b->e0);
     if (e == hipSuccess) e = hipEventCreate(&b->e1);
     if (e != hipSuccess) {
@@ -354,6 +359,7 @@ extern "C" int awp_batch_destroy(awp_batch *b)
     hipFree(b->d_sjobs);
     hipFree(b->d_aux);
     hipFree(b->d_counter);
+    hipFree(b->d_claim);
     hipEventDestroy(b->e0);
     hipEventDestroy(b->e1);
     if (b->stream) hipStreamDestroy(b->stream);

@@ -318,6 +318,8 @@ extern "C" int awp_create(awp_ctx **out, const awp_in3d *p, int device, int stor
     ctx->prof = nullptr;
     ctx->d_sjob = nullptr;
     ctx->d_claim = nullptr;
+    ctx->d_fjob = nullptr;
+    ctx->fgrid = ctx->ftotal = 0;
     ctx->split_total = ctx->split_fs = 0;
     ctx->npsrc = 0;
     ctx->d_psrc = ctx->d_ptop = nullptr;
@@ -457,6 +459,7 @@ extern "C" int awp_destroy(awp_ctx *ctx)
     hipFree(ctx->d_trace);
     hipFree(ctx->d_sjob);
     hipFree(ctx->d_claim);
+    hipFree(ctx->d_fjob);
     hipFree(ctx->d_psrc);
     hipFree(ctx->d_ptop);
     hipFree(ctx->d_stf);
@@ -749,6 +752,38 @@ static int upload_sjob(awp_ctx *ctx)
     return AWP_OK;
 }
 
+// Uploads the claimed fused step's one-entry job table (buffer 0 and 1 of
+// the run; a launch's parity picks which is read) for the run's launch
+// shape, and sizes its persistent grid.
+static int upload_fjob(awp_ctx *ctx)
+{
+    HIPTRY(ctx, hipStreamSynchronize(ctx->stream));
+    if (!ctx->d_fjob) HIPTRY(ctx, hipMalloc((void **)&ctx->d_fjob, sizeof(awp::FJob)));
+    awp::FJob f;
+    f.g = ctx->g;
+    f.s[0] = ctx->view(0);
+    f.s[1] = ctx->view(1);
+    f.c = ctx->c;
+    f.kc = ctx->l.kc;
+    f.t = awp::fused_tiles(ctx->g, ctx->l);
+    f.tile0 = 0;
+    ctx->ftotal = f.t.nbx * f.t.nby * f.t.nbz;
+    ctx->fgrid = awp::fused_batch_grid(ctx->mixed(), ctx->l, ctx->device);
+    HIPTRY(ctx, hipMemcpy(ctx->d_fjob, &f, sizeof(f), hipMemcpyHostToDevice));
+    return AWP_OK;
+}
+
+// the fused step from buffer cur to cur ^ 1: claimed (persistent blocks over
+// the one-entry job table) or static (one block per tile)
+static hipError_t fused_launch(awp_ctx *ctx, hipStream_t st)
+{
+    if (ctx->l.claim)
+        return awp::launch_step_fused_batch(ctx->mixed(), ctx->l, ctx->d_fjob, 1, ctx->ftotal, ctx->fgrid, ctx->cur,
+                                            ctx->d_claim, st);
+    return awp::launch_step_fused(ctx->mixed(), ctx->g, ctx->view(ctx->cur), ctx->view(ctx->cur ^ 1), ctx->c,
+                                  ctx->l, st);
+}
+
 // After ctx->fused or ctx->l changed: the two-stage step works in buffer 0,
 // so a state the fused step left in buffer 1 moves there first (the whole
 // buffer: every value the next step reads), and its job table follows the
@@ -761,8 +796,8 @@ static int scheme_changed(awp_ctx *ctx)
                                        hipMemcpyDeviceToDevice, ctx->stream));
         ctx->cur = 0;
     }
-    if (!ctx->fused && ctx->initialised) return upload_sjob(ctx);
-    return AWP_OK;
+    if (!ctx->initialised) return AWP_OK;
+    return ctx->fused ? upload_fjob(ctx) : upload_sjob(ctx);
 }
 
 extern "C" int awp_init(awp_ctx *ctx)
@@ -782,8 +817,8 @@ extern "C" int awp_init(awp_ctx *ctx)
     }
     // the two-stage step's one-run job table (state, coefficients, tiles)
     ctx->cur = 0;
-    if (!ctx->fused) {
-        int rc = upload_sjob(ctx);
+    {
+        int rc = ctx->fused ? upload_fjob(ctx) : upload_sjob(ctx);
         if (rc) return rc;
     }
     // first source sample before the loop (pmcl3d.f:988)
@@ -833,10 +868,10 @@ extern "C" int awp_step(awp_ctx *ctx, int nsteps)
         const bool mx = ctx->mixed();
         State sn;   // the state after this step
         if (ctx->fused) {
-            const State sa = ctx->view(ctx->cur), sb = ctx->view(ctx->cur ^ 1);
+            const State sb = ctx->view(ctx->cur ^ 1);
             if (e) HIPTRY(ctx, hipEventRecord(e[1], st));
             if (e) HIPTRY(ctx, hipEventRecord(e[2], st));
-            HIPTRY(ctx, launch_step_fused(mx, ctx->g, sa, sb, ctx->c, ctx->l, ctx->d_claim, st));
+            HIPTRY(ctx, fused_launch(ctx, st));
             ctx->cur ^= 1;
             sn = sb;
         } else {
@@ -910,8 +945,7 @@ int trial_steps(awp_ctx *ctx, int n)
     const int all = 1 << 30;
     for (int k = 0; k < n; ++k) {
         if (ctx->fused) {
-            HIPTRY(ctx, launch_step_fused(mx, ctx->g, ctx->view(ctx->cur), ctx->view(ctx->cur ^ 1), ctx->c, ctx->l,
-                                          ctx->d_claim, ctx->stream));
+            HIPTRY(ctx, fused_launch(ctx, ctx->stream));
             ctx->cur ^= 1;
         } else {
             HIPTRY(ctx, launch_split_velocity(ctx->mixed(), ctx->l, ctx->l.o32, ctx->d_sjob, 1, ctx->split_total, all, ctx->stream));

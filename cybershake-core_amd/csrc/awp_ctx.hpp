@@ -39,7 +39,12 @@ struct awp_ctx {
     // the two-stage step's job table (one entry: this run), uploaded by
     // awp_init, and its tile and free-surface block counts
     awp::SJob *d_sjob;
-    unsigned long long *d_claim;   // the fused step's tile claim counters (CLAIM_WORDS, zero between launches)
+    // the claimed fused step's one-entry job table (both buffers), its
+    // persistent grid and tile claim counters (CLAIM_WORDS, zero between
+    // launches)
+    awp::FJob *d_fjob;
+    int fgrid, ftotal;
+    unsigned long long *d_claim;
     int split_total, split_fs;
     float *prof;             // fx | fy | fz
     // the per-step job-value trace (CHKJ, pmcl3d.f:1476-1479): u1 v1 w1 at

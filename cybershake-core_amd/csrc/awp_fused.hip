@@ -601,54 +601,36 @@ k_step(const Geo g, const State a, const State b, const Coef c, const int kc, co
     step_tile<P, TY, PF>(g, a, b, c, kc, bx, by, bz, lds);
 }
 
-// The persistent form: each block claims tiles (claim_tile) until none is
-// left.
+// Runs in one launch (a single run is a one-entry table).  Runs' tiles are
+// numbered consecutively (run r owns [tile0, tile0 + ntiles)); the grid is
+// persistent, each block claiming tiles (claim_tile: a contiguous range of
+// the list per XCD, spent ranges taken over from the back) until none is
+// left.  Each tile's run is looked up in the table, so the kernel keeps no
+// run's arguments live across tiles.  `par` selects each run's (read,
+// write) buffer pair.
 template <class P, int TY, int PF>
 __global__ void __launch_bounds__(64 * (TY + 5))
-k_step_claim(const Geo g, const State a, const State b, const Coef c, const int kc, const FTiles tl,
-             unsigned long long *claim)
+k_step_batch(const FJob *__restrict__ jobs, const int njobs, const int total, const int per, const int par,
+             unsigned long long *claim, const int rot)
 {
     using Sh = FShape<TY, PF>;
     __shared__ float lds[Sh::NSLOT * Sh::SLOT];
     __shared__ int next;
     const bool lead = threadIdx.x == 0 && threadIdx.y == 0;
-    const int total = tl.nbx * tl.nby * tl.nbz;
     for (;;) {
-        if (lead) next = claim_tile(claim, tl.per, total, tl.rot);
+        if (lead) next = claim_tile(claim, per, total, rot);
         __syncthreads();
         const int tile = next;
         if (tile < 0) break;   // uniform over the block
-        int bx, by, bz;
-        patch_tile(tile, tl.nbx, tl.nby, tl.pw, tl.ph, bx, by, bz);
-        step_tile<P, TY, PF>(g, a, b, c, kc, bx, by, bz, lds);
-        __syncthreads();   // the next tile reuses the LDS ring and `next`
-    }
-    if (lead) claim_done(claim, (int)gridDim.x);
-}
-
-// Several runs in one launch.  Runs' tiles are numbered consecutively (run r
-// owns [tile0, tile0 + ntiles)); the grid is persistent: block n works on
-// XCD n % 8 and walks that XCD's contiguous range of the global tile list
-// with stride gridDim.x / 8, so each XCD sweeps neighbouring tiles of one run
-// at a time.  `par` selects each run's (read, write) buffer pair.
-template <class P, int TY, int PF>
-__global__ void __launch_bounds__(64 * (TY + 5))
-k_step_batch(const FJob *__restrict__ jobs, const int njobs, const int total, const int per, const int par)
-{
-    using Sh = FShape<TY, PF>;
-    __shared__ float lds[Sh::NSLOT * Sh::SLOT];
-    const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3, nslot = gridDim.x >> 3;
-    for (int k = slot; k < per; k += nslot) {
-        const int tile = xcd * per + k;
-        if (tile >= total) break;
         int jb = 0;
         while (jb + 1 < njobs && jobs[jb + 1].tile0 <= tile) ++jb;
         const FJob &J = jobs[jb];
         int bx, by, bz;
         patch_tile(tile - J.tile0, J.t.nbx, J.t.nby, J.t.pw, J.t.ph, bx, by, bz);
         step_tile<P, TY, PF>(J.g, J.s[par], J.s[par ^ 1], J.c, J.kc, bx, by, bz, lds);
-        __syncthreads();   // the next tile reuses the LDS ring
+        __syncthreads();   // the next tile reuses the LDS ring and `next`
     }
+    if (lead) claim_done(claim, (int)gridDim.x);
 }
 
 FTiles fused_tiles(const Geo &g, const Launch &l)
@@ -661,53 +643,26 @@ FTiles fused_tiles(const Geo &g, const Launch &l)
     t.ph = l.ph > 0 ? l.ph : 1;
     const long long n = (long long)t.nbx * t.nby * t.nbz;
     t.per = (int)((n + 7) / 8);
-    t.rot = l.xrot & 7;
     return t;
-}
-
-// blocks of k_step_claim resident on the device at once
-template <class P, int TY, int PF>
-static int claim_grid(int device)
-{
-    int nb = 0, ncu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_step_claim<P, TY, PF>, TX * (TY + 5), 0) != hipSuccess)
-        nb = 1;
-    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) ncu = 256;
-    return std::max(1, nb) * std::max(8, ncu);
 }
 
 template <class P, int TY, int PF>
 static hipError_t step_t(const Geo &g, const State &a, const State &b, const Coef &c, const Launch &l,
-                         unsigned long long *claim, hipStream_t st)
+                         hipStream_t st)
 {
     const FTiles t = fused_tiles(g, l);
-    if (l.claim && claim) {
-        // the grid size per kernel variant and device, queried once
-        static int cached[16] = {};
-        int dev = 0;
-        hipError_t e = hipGetDevice(&dev);
-        if (e != hipSuccess) return e;
-        int grid = dev >= 0 && dev < 16 ? cached[dev] : 0;
-        if (!grid) {
-            grid = claim_grid<P, TY, PF>(dev);
-            if (dev >= 0 && dev < 16) cached[dev] = grid;
-        }
-        const int ntile = t.nbx * t.nby * t.nbz;
-        hipLaunchKernelGGL((k_step_claim<P, TY, PF>), dim3(std::min(grid, ntile)), dim3(TX, TY + 5), 0, st, g, a,
-                           b, c, l.kc, t, claim);
-    } else {
-        hipLaunchKernelGGL((k_step<P, TY, PF>), dim3(8 * t.per), dim3(TX, TY + 5), 0, st, g, a, b, c, l.kc, t);
-    }
+    hipLaunchKernelGGL((k_step<P, TY, PF>), dim3(8 * t.per), dim3(TX, TY + 5), 0, st, g, a, b, c, l.kc, t);
     return hipGetLastError();
 }
 
 template <class P, int TY, int PF>
-static hipError_t batch_t(const FJob *d_jobs, int njobs, int total, int grid, int par, hipStream_t st)
+static hipError_t batch_t(const FJob *d_jobs, int njobs, int total, int grid, int par, unsigned long long *claim,
+                          int rot, hipStream_t st)
 {
     const int per = (total + 7) / 8;
-    grid = std::max(8, std::min(grid, 8 * per)) / 8 * 8;   // whole XCD rounds
+    grid = std::max(1, std::min(grid, total));
     hipLaunchKernelGGL((k_step_batch<P, TY, PF>), dim3(grid), dim3(TX, TY + 5), 0, st, d_jobs, njobs, total, per,
-                       par);
+                       par, claim, rot & 7);
     return hipGetLastError();
 }
 
@@ -766,17 +721,17 @@ static int grid_t(int device)
     } while (0)
 
 hipError_t launch_step_fused(bool mixed, const Geo &g, const State &a, const State &b, const Coef &c,
-                             const Launch &l, unsigned long long *claim, hipStream_t st)
+                             const Launch &l, hipStream_t st)
 {
-    AWP_FUSED_DISPATCH(mixed, l, return (step_t<P, TY, PF>(g, a, b, c, l, claim, st)));
+    AWP_FUSED_DISPATCH(mixed, l, return (step_t<P, TY, PF>(g, a, b, c, l, st)));
     return hipErrorInvalidValue;
 }
 
 hipError_t launch_step_fused_batch(bool mixed, const Launch &l, const FJob *d_jobs, int njobs, int total,
-                                   int grid, int par, hipStream_t st)
+                                   int grid, int par, unsigned long long *claim, hipStream_t st)
 {
     if (njobs <= 0 || total <= 0) return hipSuccess;
-    AWP_FUSED_DISPATCH(mixed, l, return (batch_t<P, TY, PF>(d_jobs, njobs, total, grid, par, st)));
+    AWP_FUSED_DISPATCH(mixed, l, return (batch_t<P, TY, PF>(d_jobs, njobs, total, grid, par, claim, l.xrot, st)));
     return hipErrorInvalidValue;
 }
 

@@ -394,8 +394,8 @@ struct Launch {
     int nchunk;      // number of z chunks
     int unroll;      // z-loop unroll factor (1 or 2); prefetch depth of the fused step
     int pw, ph;      // fused step: tile patch width / height (tiles)
-    bool claim;      // fused step: persistent blocks claim tiles per XCD (claim_tile), else one block per tile
-    int xrot;        // fused step with claim: range of XCD x is (x + xrot) % 8
+    bool claim;      // fused step of a single run: persistent blocks claiming tiles per XCD (claim_tile), else one block per tile
+    int xrot;        // claimed fused launches: XCD x starts on range (x + xrot) % 8
     int spw, sph;    // two-stage kernels: tile patch width / height (0 = rows of tiles)
     bool o32;        // 32-bit byte offsets reach every array (offsets32() of the run's geometry)
     bool o32_s;      // two-stage stress kernel: its 32-bit byte-offset form (only where o32)
@@ -467,12 +467,11 @@ hipError_t launch_absmax(const float *a, long long n, float *part, int nblk, hip
 hipError_t launch_to_half(const float *src, _Float16 *dst, float scale, long long n, hipStream_t st);
 hipError_t launch_sgt_fp16(const float *sgt, const int *kidx, int nkept, int nsamp_max, int nsamp, float *scales,
                            _Float16 *out, hipStream_t st);
-// the fused one-sweep step (awp_fused.hip): reads state a, writes state b;
-// one block per tile, or with l.claim persistent blocks claiming tiles
-// through `claim` (CLAIM_WORDS words of device memory, zero before the first
-// launch; each launch leaves them zero)
+// the fused one-sweep step (awp_fused.hip): reads state a, writes state b,
+// one block per tile (the static form; the claimed form of a run is
+// launch_step_fused_batch over a one-entry table)
 hipError_t launch_step_fused(bool mixed, const Geo &g, const State &a, const State &b, const Coef &c,
-                             const Launch &l, unsigned long long *claim, hipStream_t st);
+                             const Launch &l, hipStream_t st);
 
 // tiles of one run for the fused step: nbx * nby * nbz column tiles
 // Tile order: within each z chunk, tiles are grouped in patches of pw x ph
@@ -484,7 +483,6 @@ struct FTiles {
     int nbx, nby, nbz;
     int per;   // tiles per XCD (single-run launch)
     int pw, ph;
-    int rot;   // claimed launch: XCD x takes range (x + rot) % 8 first
 };
 FTiles fused_tiles(const Geo &g, const Launch &l);
 
@@ -550,9 +548,12 @@ struct FJob {
     int tile0;   // first global tile of the run
 };
 // one launch for all runs (d_jobs in device memory, tile0 ascending);
-// tile_y / unroll of `l` select the kernel variant, grid = persistent blocks
+// tile_y / unroll / xrot of `l` select the kernel variant and the XCD range
+// rotation, grid = persistent blocks claiming tiles through `claim`
+// (CLAIM_WORDS words of device memory, zero before the first launch; each
+// launch leaves them zero; launches that share them must not overlap)
 hipError_t launch_step_fused_batch(bool mixed, const Launch &l, const FJob *d_jobs, int njobs, int total,
-                                   int grid, int par, hipStream_t st);
+                                   int grid, int par, unsigned long long *claim, hipStream_t st);
 // blocks of the batched kernel that fit on the device at once
 int fused_batch_grid(bool mixed, const Launch &l, int device);
 
