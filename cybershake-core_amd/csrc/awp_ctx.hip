@@ -321,6 +321,7 @@ extern "C" int awp_create(awp_ctx **out, const awp_in3d *p, int device, int stor
     ctx->d_fjob = nullptr;
     ctx->fgrid = ctx->ftotal = 0;
     ctx->split_total = ctx->split_fs = 0;
+    ctx->sgrid_v = ctx->sgrid_s = 0;
     ctx->npsrc = 0;
     ctx->d_psrc = ctx->d_ptop = nullptr;
     ctx->d_stf = nullptr;
@@ -748,6 +749,7 @@ static int upload_sjob(awp_ctx *ctx)
     const SJob q = split_job(ctx->g, ctx->view(ctx->cur), ctx->c, ctx->l, 0, 0);
     ctx->split_total = split_tiles(q);
     ctx->split_fs = split_fs_blocks(q);
+    awp::split_batch_grids(ctx->mixed(), ctx->l, ctx->device, &ctx->sgrid_v, &ctx->sgrid_s);
     HIPTRY(ctx, hipMemcpy(ctx->d_sjob, &q, sizeof(SJob), hipMemcpyHostToDevice));
     return AWP_OK;
 }
@@ -770,6 +772,23 @@ static int upload_fjob(awp_ctx *ctx)
     ctx->ftotal = f.t.nbx * f.t.nby * f.t.nbz;
     ctx->fgrid = awp::fused_batch_grid(ctx->mixed(), ctx->l, ctx->device);
     HIPTRY(ctx, hipMemcpy(ctx->d_fjob, &f, sizeof(f), hipMemcpyHostToDevice));
+    return AWP_OK;
+}
+
+// the two-stage step's kernels in buffer 0: claimed (persistent blocks) or
+// one block per tile; e: the timing events of the step, or null
+static int split_launch(awp_ctx *ctx, hipStream_t st, hipEvent_t *e)
+{
+    const bool cl = ctx->l.claim;
+    const int gv = cl ? ctx->sgrid_v : 1 << 30, gs = cl ? ctx->sgrid_s : 1 << 30;
+    unsigned long long *claim = cl ? ctx->d_claim : nullptr;
+    HIPTRY(ctx, awp::launch_split_velocity(ctx->mixed(), ctx->l, ctx->l.o32, ctx->d_sjob, 1, ctx->split_total, gv,
+                                           claim, st));
+    if (e) HIPTRY(ctx, hipEventRecord(e[1], st));
+    HIPTRY(ctx, awp::launch_split_fs(ctx->mixed(), ctx->d_sjob, 1, ctx->split_fs, st));
+    if (e) HIPTRY(ctx, hipEventRecord(e[2], st));
+    HIPTRY(ctx, awp::launch_split_stress(ctx->mixed(), ctx->l, ctx->l.o32, ctx->d_sjob, 1, ctx->split_total, gs,
+                                         claim, st));
     return AWP_OK;
 }
 
@@ -875,15 +894,11 @@ extern "C" int awp_step(awp_ctx *ctx, int nsteps)
             ctx->cur ^= 1;
             sn = sb;
         } else {
-            // the stage kernels over this run's one-entry job table, one
-            // block per tile (in buffer 0)
+            // the stage kernels over this run's one-entry job table (in
+            // buffer 0)
             sn = ctx->view(ctx->cur);
-            const int all = 1 << 30;
-            HIPTRY(ctx, launch_split_velocity(ctx->mixed(), ctx->l, ctx->l.o32, ctx->d_sjob, 1, ctx->split_total, all, st));
-            if (e) HIPTRY(ctx, hipEventRecord(e[1], st));
-            HIPTRY(ctx, launch_split_fs(ctx->mixed(), ctx->d_sjob, 1, ctx->split_fs, st));
-            if (e) HIPTRY(ctx, hipEventRecord(e[2], st));
-            HIPTRY(ctx, launch_split_stress(mx, ctx->l, ctx->l.o32, ctx->d_sjob, 1, ctx->split_total, all, st));
+            int rc = split_launch(ctx, st, e);
+            if (rc) return rc;
         }
         if (e) HIPTRY(ctx, hipEventRecord(e[3], st));
         // source sample i + 1 (while the source lasts), then the trace row
@@ -941,17 +956,13 @@ namespace {
 // `n` steps of the step kernels alone (no source, no SGT sample), for timing
 int trial_steps(awp_ctx *ctx, int n)
 {
-    const bool mx = ctx->mixed();
-    const int all = 1 << 30;
     for (int k = 0; k < n; ++k) {
         if (ctx->fused) {
             HIPTRY(ctx, fused_launch(ctx, ctx->stream));
             ctx->cur ^= 1;
         } else {
-            HIPTRY(ctx, launch_split_velocity(ctx->mixed(), ctx->l, ctx->l.o32, ctx->d_sjob, 1, ctx->split_total, all, ctx->stream));
-            HIPTRY(ctx, launch_split_fs(ctx->mixed(), ctx->d_sjob, 1, ctx->split_fs, ctx->stream));
-            HIPTRY(ctx, launch_split_stress(mx, ctx->l, ctx->l.o32, ctx->d_sjob, 1, ctx->split_total, all,
-                                            ctx->stream));
+            int rc = split_launch(ctx, ctx->stream, nullptr);
+            if (rc) return rc;
         }
     }
     return AWP_OK;

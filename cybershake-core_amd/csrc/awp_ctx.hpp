@@ -46,6 +46,7 @@ struct awp_ctx {
     int fgrid, ftotal;
     unsigned long long *d_claim;
     int split_total, split_fs;
+    int sgrid_v, sgrid_s;    // the two-stage kernels' persistent grids (claimed launches)
     float *prof;             // fx | fy | fz
     // the per-step job-value trace (CHKJ, pmcl3d.f:1476-1479): u1 v1 w1 at
     // node trace_ijk after every step, [nt][3]

@@ -697,20 +697,38 @@ __device__ inline const SJob &split_tile(const SJob *__restrict__ jobs, const in
     return J;
 }
 
-// Persistent stage kernels: block n stays on XCD n % 8 and walks that XCD's
-// contiguous range of the global tile list with stride gridDim.x / 8, so the
-// blocks of one XCD work on neighbouring tiles of one run at a time and read
-// each other's halo lines from that XCD's L2.
+// The next tile of a persistent stage launch, the same in every thread of
+// the block, or -1 when none is left: claimed (claim_tile) when `claim` is
+// set, else the static walk -- block n stays on XCD n % 8 and walks that
+// XCD's contiguous range of the global tile list with stride gridDim.x / 8
+// (k: the block's tiles so far), so the blocks of one XCD work on
+// neighbouring tiles of one run at a time and read each other's halo lines
+// from that XCD's L2.  `next`: the block's LDS broadcast slot.
+__device__ inline int stage_tile(unsigned long long *claim, int per, int total, int rot, int k, int *next)
+{
+    if (claim) {
+        __syncthreads();   // every thread has read the previous tile
+        if (threadIdx.x == 0 && threadIdx.y == 0 && threadIdx.z == 0) *next = claim_tile(claim, per, total, rot);
+        __syncthreads();
+        return *next;
+    }
+    const int xcd = blockIdx.x & 7, slot = (int)(blockIdx.x >> 3) + k * (int)(gridDim.x >> 3);
+    const int tile = xcd * per + slot;
+    return (slot < per && tile < total) ? tile : -1;
+}
+
+// Persistent stage kernels over the job table (stage_tile assigns tiles).
 // MAXT: most threads per block -- 512 for tiles of up to 8 rows, 1024 for
 // 16-row tiles (whose 16 waves leave 128 registers a lane)
 template <class P, int U, int MAXT, bool O32>
 __global__ void __launch_bounds__(MAXT)
-k_velocity_batch(const SJob *__restrict__ jobs, const int njobs, const int total, const int per)
+k_velocity_batch(const SJob *__restrict__ jobs, const int njobs, const int total, const int per,
+                 unsigned long long *claim, const int rot)
 {
-    const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3, nslot = gridDim.x >> 3;
-    for (int k = slot; k < per; k += nslot) {
-        const int tile = xcd * per + k;
-        if (tile >= total) break;
+    __shared__ int next;
+    for (int k = 0;; ++k) {
+        const int tile = stage_tile(claim, per, total, rot, k, &next);
+        if (tile < 0) break;
         int bx, by, bz;
         const SJob &J = split_tile(jobs, njobs, tile, bx, by, bz);
         if constexpr (O32)
@@ -718,20 +736,23 @@ k_velocity_batch(const SJob *__restrict__ jobs, const int njobs, const int total
         else
             velocity_tile<P, U>(J.g, J.s, J.c, J.kc, bx, by, bz);
     }
+    if (claim && threadIdx.x == 0 && threadIdx.y == 0 && threadIdx.z == 0) claim_done(claim, (int)gridDim.x);
 }
 
 template <class P, int U, bool O32, int MAXT>
 __global__ void __launch_bounds__(MAXT)
-k_stress_batch(const SJob *__restrict__ jobs, const int njobs, const int total, const int per)
+k_stress_batch(const SJob *__restrict__ jobs, const int njobs, const int total, const int per,
+               unsigned long long *claim, const int rot)
 {
-    const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3, nslot = gridDim.x >> 3;
-    for (int k = slot; k < per; k += nslot) {
-        const int tile = xcd * per + k;
-        if (tile >= total) break;
+    __shared__ int next;
+    for (int k = 0;; ++k) {
+        const int tile = stage_tile(claim, per, total, rot, k, &next);
+        if (tile < 0) break;
         int bx, by, bz;
         const SJob &J = split_tile(jobs, njobs, tile, bx, by, bz);
         stress_tile<P, U, O32>(J.g, J.s, J.c, J.kc, bx, by, bz);
     }
+    if (claim && threadIdx.x == 0 && threadIdx.y == 0 && threadIdx.z == 0) claim_done(claim, (int)gridDim.x);
 }
 
 // free-surface images of every run: block b of the launch is block
@@ -1288,72 +1309,77 @@ void split_batch_grids(bool mixed, const Launch &l, int device, int *grid_v, int
                      : split_grid_t<StoreF32, 1>(device, l, grid_s);
 }
 
-// whole XCD rounds of persistent blocks, no more than the tiles need
-static int persistent_grid(int grid, int per) { return std::max(8, std::min(grid, 8 * per)) / 8 * 8; }
+// whole XCD rounds of persistent blocks, no more than the tiles need (the
+// static walk); claimed launches: no more blocks than tiles
+static int persistent_grid(int grid, int per, int total, bool claimed)
+{
+    if (claimed) return std::max(1, std::min(grid, total));
+    return std::max(8, std::min(grid, 8 * per)) / 8 * 8;
+}
 
 template <class P, int U, int MAXT>
 static void vel_batch_t(const Launch &l, bool ov, const SJob *d_jobs, int njobs, int total, int per, int grid_v,
-                        hipStream_t st)
+                        unsigned long long *claim, hipStream_t st)
 {
-    const dim3 grid(persistent_grid(grid_v, per)), block(TX, l.tile_y, l.tile_xw);
+    const dim3 grid(persistent_grid(grid_v, per, total, claim)), block(TX, l.tile_y, l.tile_xw);
     if (ov)
         hipLaunchKernelGGL((k_velocity_batch<P, U, MAXT, true>), grid, block, l.lds_v, st, d_jobs, njobs, total,
-                           per);
+                           per, claim, l.xrot & 7);
     else
         hipLaunchKernelGGL((k_velocity_batch<P, U, MAXT, false>), grid, block, l.lds_v, st, d_jobs, njobs, total,
-                           per);
+                           per, claim, l.xrot & 7);
 }
 
 template <class P>
 static hipError_t velocity_stage(const Launch &l, bool ov, const SJob *d_jobs, int njobs, int total, int per,
-                                 int grid_v, hipStream_t st)
+                                 int grid_v, unsigned long long *claim, hipStream_t st)
 {
     if (tall(l)) {
         if (l.unroll == 2) return hipErrorInvalidValue;   // no such variant
-        vel_batch_t<P, 1, 1024>(l, ov, d_jobs, njobs, total, per, grid_v, st);
+        vel_batch_t<P, 1, 1024>(l, ov, d_jobs, njobs, total, per, grid_v, claim, st);
     } else if (l.unroll == 2) {
-        vel_batch_t<P, 2, 512>(l, ov, d_jobs, njobs, total, per, grid_v, st);
+        vel_batch_t<P, 2, 512>(l, ov, d_jobs, njobs, total, per, grid_v, claim, st);
     } else {
-        vel_batch_t<P, 1, 512>(l, ov, d_jobs, njobs, total, per, grid_v, st);
+        vel_batch_t<P, 1, 512>(l, ov, d_jobs, njobs, total, per, grid_v, claim, st);
     }
     return hipSuccess;
 }
 
 template <class P, int U, bool O32, int MAXT>
 static void str_batch_t(const Launch &l, const SJob *d_jobs, int njobs, int total, int per, int grid_s,
-                        hipStream_t st)
+                        unsigned long long *claim, hipStream_t st)
 {
-    hipLaunchKernelGGL((k_stress_batch<P, U, O32, MAXT>), dim3(persistent_grid(grid_s, per)), dim3(TX, l.tile_y, l.tile_xw),
-                       l.lds_s, st, d_jobs, njobs, total, per);
+    hipLaunchKernelGGL((k_stress_batch<P, U, O32, MAXT>), dim3(persistent_grid(grid_s, per, total, claim)),
+                       dim3(TX, l.tile_y, l.tile_xw), l.lds_s, st, d_jobs, njobs, total, per, claim, l.xrot & 7);
 }
 
 template <class P>
 static hipError_t stress_stage(const Launch &l, bool os, const SJob *d_jobs, int njobs, int total, int per,
-                               int grid_s, hipStream_t st)
+                               int grid_s, unsigned long long *claim, hipStream_t st)
 {
     if (tall(l)) {
         if (!os || l.unroll == 2) return hipErrorInvalidValue;   // no such variant
-        str_batch_t<P, 1, true, 1024>(l, d_jobs, njobs, total, per, grid_s, st);
+        str_batch_t<P, 1, true, 1024>(l, d_jobs, njobs, total, per, grid_s, claim, st);
         return hipSuccess;
     }
     if (l.unroll == 2) {
-        if (os) str_batch_t<P, 2, true, 512>(l, d_jobs, njobs, total, per, grid_s, st);
-        else str_batch_t<P, 2, false, 512>(l, d_jobs, njobs, total, per, grid_s, st);
+        if (os) str_batch_t<P, 2, true, 512>(l, d_jobs, njobs, total, per, grid_s, claim, st);
+        else str_batch_t<P, 2, false, 512>(l, d_jobs, njobs, total, per, grid_s, claim, st);
     } else {
-        if (os) str_batch_t<P, 1, true, 512>(l, d_jobs, njobs, total, per, grid_s, st);
-        else str_batch_t<P, 1, false, 512>(l, d_jobs, njobs, total, per, grid_s, st);
+        if (os) str_batch_t<P, 1, true, 512>(l, d_jobs, njobs, total, per, grid_s, claim, st);
+        else str_batch_t<P, 1, false, 512>(l, d_jobs, njobs, total, per, grid_s, claim, st);
     }
     return hipSuccess;
 }
 
 hipError_t launch_split_velocity(bool mixed, const Launch &l, bool o32, const SJob *d_jobs, int njobs, int total,
-                                 int grid_v, hipStream_t st)
+                                 int grid_v, unsigned long long *claim, hipStream_t st)
 {
     if (njobs <= 0 || total <= 0) return hipSuccess;
     const int per = (total + 7) / 8;
     const bool ov = o32 && l.o32_v;
-    const hipError_t e = mixed ? velocity_stage<StoreMixed>(l, ov, d_jobs, njobs, total, per, grid_v, st)
-                               : velocity_stage<StoreF32>(l, ov, d_jobs, njobs, total, per, grid_v, st);
+    const hipError_t e = mixed ? velocity_stage<StoreMixed>(l, ov, d_jobs, njobs, total, per, grid_v, claim, st)
+                               : velocity_stage<StoreF32>(l, ov, d_jobs, njobs, total, per, grid_v, claim, st);
     if (e != hipSuccess) return e;
     return hipGetLastError();
 }
@@ -1369,23 +1395,24 @@ hipError_t launch_split_fs(bool mixed, const SJob *d_jobs, int njobs, int fs_tot
 }
 
 hipError_t launch_split_stress(bool mixed, const Launch &l, bool o32, const SJob *d_jobs, int njobs, int total,
-                               int grid_s, hipStream_t st)
+                               int grid_s, unsigned long long *claim, hipStream_t st)
 {
     if (njobs <= 0 || total <= 0) return hipSuccess;
     const int per = (total + 7) / 8;
     const bool os = o32 && l.o32_s;
-    const hipError_t e = mixed ? stress_stage<StoreMixed>(l, os, d_jobs, njobs, total, per, grid_s, st)
-                               : stress_stage<StoreF32>(l, os, d_jobs, njobs, total, per, grid_s, st);
+    const hipError_t e = mixed ? stress_stage<StoreMixed>(l, os, d_jobs, njobs, total, per, grid_s, claim, st)
+                               : stress_stage<StoreF32>(l, os, d_jobs, njobs, total, per, grid_s, claim, st);
     if (e != hipSuccess) return e;
     return hipGetLastError();
 }
 
 hipError_t launch_step_split_batch(bool mixed, const Launch &l, bool o32, const SJob *d_jobs, int njobs,
-                                   int total, int fs_total, int grid_v, int grid_s, hipStream_t st)
+                                   int total, int fs_total, int grid_v, int grid_s, unsigned long long *claim,
+                                   hipStream_t st)
 {
-    hipError_t e = launch_split_velocity(mixed, l, o32, d_jobs, njobs, total, grid_v, st);
+    hipError_t e = launch_split_velocity(mixed, l, o32, d_jobs, njobs, total, grid_v, claim, st);
     if (e == hipSuccess) e = launch_split_fs(mixed, d_jobs, njobs, fs_total, st);
-    if (e == hipSuccess) e = launch_split_stress(mixed, l, o32, d_jobs, njobs, total, grid_s, st);
+    if (e == hipSuccess) e = launch_split_stress(mixed, l, o32, d_jobs, njobs, total, grid_s, claim, st);
     return e;
 }
 

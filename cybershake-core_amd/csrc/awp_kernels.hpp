@@ -593,14 +593,18 @@ static inline int split_fs_blocks(const SJob &q) { return ((q.g.nx + TX - 1) / T
 // velocity, free-surface and stress stages of every run: three launches,
 // the first and last over the tile list (grid_v / grid_s blocks, from
 // split_batch_grids, or one block per tile when larger); fs_total = sum of
-// the runs' free-surface blocks; o32 = offsets32() holds for every run
+// the runs' free-surface blocks; o32 = offsets32() holds for every run.
+// With `claim` (CLAIM_WORDS words, zero before the first launch, left zero
+// by each; launches sharing it must not overlap) the blocks claim tiles per
+// XCD (claim_tile, ranges rotated by l.xrot); without, the static walk.
 hipError_t launch_step_split_batch(bool mixed, const Launch &l, bool o32, const SJob *d_jobs, int njobs,
-                                   int total, int fs_total, int grid_v, int grid_s, hipStream_t st);
-hipError_t launch_split_velocity(bool mixed, const Launch &l, bool o32, const SJob *d_jobs, int njobs, int total, int grid_v,
-                                 hipStream_t st);
+                                   int total, int fs_total, int grid_v, int grid_s, unsigned long long *claim,
+                                   hipStream_t st);
+hipError_t launch_split_velocity(bool mixed, const Launch &l, bool o32, const SJob *d_jobs, int njobs, int total,
+                                 int grid_v, unsigned long long *claim, hipStream_t st);
 hipError_t launch_split_fs(bool mixed, const SJob *d_jobs, int njobs, int fs_total, hipStream_t st);
 hipError_t launch_split_stress(bool mixed, const Launch &l, bool o32, const SJob *d_jobs, int njobs, int total,
-                               int grid_s, hipStream_t st);
+                               int grid_s, unsigned long long *claim, hipStream_t st);
 void split_batch_grids(bool mixed, const Launch &l, int device, int *grid_v, int *grid_s);
 
 // per-run data of the batched source / SGT stage
