@@ -305,6 +305,45 @@ def batched_record(a):
             "sgt_error_vs_fp64": SGT_ERROR_VS_FP64}
 
 
+def claim_ab(a):
+    """One run of the single-grid workload stepped alternately with the
+    claimed launches and with the static ones (AWP_CLAIM=2 / 0, switched
+    between steps on the same buffers), so that both forms see the same
+    memory.  Prints one JSON line; a diagnostic, not the throughput line."""
+    import torch
+    import awpsgt
+    from awpsgt.media import c2_media
+    from awpsgt.workloads import layered_nvar5
+
+    cfg = CONFIGS[a.config]
+    rounds = 3
+    p = build_params(cfg, rounds * 2 * (a.warmup + a.steps))
+    if a.config == "c2":
+        media_fn = lambda: c2_media(p.nx, p.ny, p.nz, p.dh, seed=cfg["seed"])  # noqa: E731
+    else:
+        media_fn = lambda: layered_nvar5(p)  # noqa: E731
+    st = awpsgt.AWP_STORAGE_MIXED if a.storage == "mixed" else awpsgt.AWP_STORAGE_FP32
+    run, _ = make_run(p, st, media_fn, cfg["src"], cfg["nsgt"], 3)
+    fused = run.scheme()[0]
+    out = []
+    for r in range(rounds):
+        for c in ("2", "0"):
+            os.environ["AWP_CLAIM"] = c
+            run.set_scheme(fused)
+            run.step(a.warmup)
+            run.synchronize()
+            t = time.perf_counter()
+            run.step(a.steps)
+            run.synchronize()
+            ms = (time.perf_counter() - t) / a.steps * 1e3
+            log("claim=%s: %.3f ms/step" % (c, ms))
+            out.append({"claim": c != "0", "ms_per_step": round(ms, 4)})
+    run.close()
+    torch.cuda.synchronize()
+    print(json.dumps({"claim_ab": out, "config": a.config, "storage": a.storage,
+                      "step": "fused" if fused else "two-stage"}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -328,7 +367,15 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain device copy of 4 GiB five times (a known-size kernel "
                          "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE counters)")
+    ap.add_argument("--claim-ab", action="store_true",
+                    help="only step one single-grid run alternately with claimed and static tile launches "
+                         "(same buffers) and print their times")
     a = ap.parse_args()
+    if a.claim_ab:
+        if a.scheme != "auto":
+            os.environ["AWP_SCHEME"] = a.scheme
+        claim_ab(a)
+        return
     if a.copy_calibration:
         import torch
         x = torch.ones(1 << 30, device="cuda", dtype=torch.float32)

@@ -196,7 +196,7 @@ static hipError_t launch_kernels(awp_batch *b, int par)
     const int n = (int)b->runs.size();
     if (b->split)
         return launch_step_split_batch(b->mixed, b->l, b->o32, b->d_sjobs, n, b->total, b->fs_total,
-                                       b->grid_v, b->grid_s, b->persist && b->l.claim ? b->d_claim : nullptr,
+                                       b->grid_v, b->grid_s, b->persist && b->l.claim != 0 ? b->d_claim : nullptr,
                                        b->stream);
     return launch_step_fused_batch(b->mixed, b->l, b->d_jobs, n, b->total, b->grid, par, b->d_claim, b->stream);
 }

@@ -174,14 +174,16 @@ static Launch choose_launch(const Geo &g, bool fused, bool mixed, int device)
     // step against 17.62 for 2 x 16 and 17.75 for 3 x 8)
     l.pw = 4;
     l.ph = 4;
-    // tile claim of the fused step (AWP_CLAIM=0/1; AWP_XROT=0..7 rotates
-    // the XCD ranges): persistent blocks that claim tiles per XCD, the
-    // default (1500x1000x300: 15.56-16.92 ms per step against 15.88-17.23
-    // with one block per tile, over the same sequence of runs)
-    l.claim = true;
+    // tile claim (AWP_CLAIM=0 never, 1 when the launch has at least four
+    // tiles per persistent block -- the default --, 2 always; AWP_XROT=0..7
+    // rotates the XCD ranges): persistent blocks that claim tiles per XCD.
+    // 1500x1000x300: fused 16.81-16.92 ms per step against 17.11-17.20 with
+    // one block per tile (same run); 200x200x100 (400 two-stage tiles) one
+    // block per tile is faster
+    l.claim = 1;
     l.xrot = 0;
     const char *ce = getenv("AWP_CLAIM");
-    if (ce && (!strcmp(ce, "0") || !strcmp(ce, "1"))) l.claim = ce[0] == '1';
+    if (ce && (!strcmp(ce, "0") || !strcmp(ce, "1") || !strcmp(ce, "2"))) l.claim = ce[0] - '0';
     const char *xr = getenv("AWP_XROT");
     if (xr) l.xrot = atoi(xr) & 7;
     const char *pe = getenv("AWP_PATCH");
@@ -779,7 +781,8 @@ static int upload_fjob(awp_ctx *ctx)
 // one block per tile; e: the timing events of the step, or null
 static int split_launch(awp_ctx *ctx, hipStream_t st, hipEvent_t *e)
 {
-    const bool cl = ctx->l.claim;
+    const bool cl = ctx->l.claim == 2 ||
+                    (ctx->l.claim == 1 && ctx->split_total >= 4 * std::max(ctx->sgrid_v, ctx->sgrid_s));
     const int gv = cl ? ctx->sgrid_v : 1 << 30, gs = cl ? ctx->sgrid_s : 1 << 30;
     unsigned long long *claim = cl ? ctx->d_claim : nullptr;
     HIPTRY(ctx, awp::launch_split_velocity(ctx->mixed(), ctx->l, ctx->l.o32, ctx->d_sjob, 1, ctx->split_total, gv,
@@ -796,7 +799,7 @@ static int split_launch(awp_ctx *ctx, hipStream_t st, hipEvent_t *e)
 // the one-entry job table) or static (one block per tile)
 static hipError_t fused_launch(awp_ctx *ctx, hipStream_t st)
 {
-    if (ctx->l.claim)
+    if (ctx->l.claim == 2 || (ctx->l.claim == 1 && ctx->ftotal >= 4 * ctx->fgrid))
         return awp::launch_step_fused_batch(ctx->mixed(), ctx->l, ctx->d_fjob, 1, ctx->ftotal, ctx->fgrid, ctx->cur,
                                             ctx->d_claim, st);
     return awp::launch_step_fused(ctx->mixed(), ctx->g, ctx->view(ctx->cur), ctx->view(ctx->cur ^ 1), ctx->c,

@@ -394,7 +394,7 @@ struct Launch {
     int nchunk;      // number of z chunks
     int unroll;      // z-loop unroll factor (1 or 2); prefetch depth of the fused step
     int pw, ph;      // fused step: tile patch width / height (tiles)
-    bool claim;      // fused step of a single run: persistent blocks claiming tiles per XCD (claim_tile), else one block per tile
+    int claim;       // persistent blocks claiming tiles per XCD (claim_tile): 0 never, 1 with >= 4 tiles a block, 2 always
     int xrot;        // claimed fused launches: XCD x starts on range (x + xrot) % 8
     int spw, sph;    // two-stage kernels: tile patch width / height (0 = rows of tiles)
     bool o32;        // 32-bit byte offsets reach every array (offsets32() of the run's geometry)

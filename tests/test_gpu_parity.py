@@ -56,15 +56,16 @@ def test_media_setup_bitwise(case_dirs):
 
 @pytest.mark.parametrize("scheme", ["split", "fused", "split_o32", "split_o64", "split_ty16",
                                     "split_xw2", "split_xw2_ty4", "split_xw2_o64", "split_o32v",
-                                    "split_o64v", "fused_ty11", "fused_lv2", "fused_static", "fused_claim_r5"])
+                                    "split_o64v", "fused_ty11", "fused_lv2", "fused_static", "fused_claim_r5",
+                                    "split_claim"])
 @pytest.mark.parametrize("name", list(cases.CASES))
 def test_fields_bitwise_after_steps(case_dirs, name, scheme, monkeypatch):
     """Both time-step schemes (AWP_SCHEME=split two-stage, AWP_SCHEME=fused),
     both offset forms of the two-stage stress and velocity kernels (32-bit
     byte offsets, and the 64-bit element indices that grids with arrays of
     4 GiB or more take), their tile shapes (64 x 8, 64 x 16, 128 x 8,
-    128 x 4) and the fused step's static and claimed tile assignments (with
-    two XCD range rotations) against the restatement, bit for bit."""
+    128 x 4) and the static and claimed tile assignments (the fused ranges
+    rotated) against the restatement, bit for bit."""
     d, _ = case_dirs[name]
     p, media, xyz, stf, pts = _inputs(d)
     o = _oracle(d, p, media, xyz, stf, pts)
@@ -75,12 +76,13 @@ def test_fields_bitwise_after_steps(case_dirs, name, scheme, monkeypatch):
     if scheme in ("fused_ty11", "fused_lv2"):
         # one level per barrier, or two (the 64 x 11 default)
         monkeypatch.setenv("AWP_PF", "0" if scheme == "fused_ty11" else "3")
-    if scheme in ("fused_static", "fused_claim_r5"):
+    if scheme in ("fused_static", "fused_claim_r5", "split_claim"):
         # one block per tile, or persistent blocks claiming tiles per XCD
-        # (the default) with the XCD ranges rotated; each claimed launch
-        # leaves the counters zero for the next one
-        monkeypatch.setenv("AWP_CLAIM", "0" if scheme == "fused_static" else "1")
-        monkeypatch.setenv("AWP_XROT", "5")
+        # (forced: these grids have too few tiles for the default to claim),
+        # the fused ranges rotated; each claimed launch leaves the counters
+        # zero for the next one
+        monkeypatch.setenv("AWP_CLAIM", "0" if scheme == "fused_static" else "2")
+        monkeypatch.setenv("AWP_XROT", "5" if scheme == "fused_claim_r5" else "0")
     if scheme in ("split_o32", "split_o64"):
         # both byte-offset widths of the two-stage kernels, whichever the
         # storage mode uses by default
