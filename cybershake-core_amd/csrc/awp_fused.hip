@@ -224,7 +224,7 @@ __device__ __forceinline__ void step_tile(const Geo &g, const State &a0, const S
         const unsigned orr = o >> RSH;
         if constexpr (P::mixed) {
             // FP16 values under the row segment's exponent (one per wave)
-            const unsigned long long ew = ((const G<unsigned long long> *)a.rexp)[rexp_idx(g, a.nseg, bx, j, s)];
+            const unsigned long long ew = rexp_load(a.rexp, rexp_idx(g, a.nseg, bx, j, s));
             x.ew = ew;
             x.r1 = r_decode(gln(R1, orr), rexp_of(ew, 0)); x.r2 = r_decode(gln(R2, orr), rexp_of(ew, 1));
             x.r3 = r_decode(gln(R3, orr), rexp_of(ew, 2)); x.r4 = r_decode(gln(R4, orr), rexp_of(ew, 3));

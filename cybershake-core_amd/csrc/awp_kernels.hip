@@ -353,7 +353,7 @@ __device__ __forceinline__ void stress_tile_e(const Geo &g, const State &s0, con
         float r1, r2, r3, r4, r5, r6;
         unsigned long long ew = 0;   // mixed: the row segment's exponents before this update
         if constexpr (P::mixed) {
-            ew = ((const G<unsigned long long> *)s.rexp)[rexp_idx(g, s.nseg, seg, j, k)];
+            ew = rexp_load(s.rexp, rexp_idx(g, s.nseg, seg, j, k));
             r1 = r_decode(R1[o], rexp_of(ew, 0)); r2 = r_decode(R2[o], rexp_of(ew, 1));
             r3 = r_decode(R3[o], rexp_of(ew, 2)); r4 = r_decode(R4[o], rexp_of(ew, 3));
             r5 = r_decode(R5[o], rexp_of(ew, 4)); r6 = r_decode(R6[o], rexp_of(ew, 5));
@@ -570,7 +570,7 @@ __device__ __forceinline__ void stress_tile_b(const Geo &g, const State &s0, con
         unsigned long long ew = 0;   // mixed: the row segment's exponents before this update
         if constexpr (P::mixed) {
             // the exponents of this row segment: one 8-byte slot at byte offset xo
-            ew = gln((const G<unsigned long long> *)s.rexp, xo);
+            ew = rexp_load(s.rexp, (long long)(xo >> 3));
             r1 = r_decode(p.ldn(R1), rexp_of(ew, 0)); r2 = r_decode(p.ldn(R2), rexp_of(ew, 1));
             r3 = r_decode(p.ldn(R3), rexp_of(ew, 2)); r4 = r_decode(p.ldn(R4), rexp_of(ew, 3));
             r5 = r_decode(p.ldn(R5), rexp_of(ew, 4)); r6 = r_decode(p.ldn(R6), rexp_of(ew, 5));

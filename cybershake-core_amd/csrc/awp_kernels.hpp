@@ -167,15 +167,17 @@ __device__ inline float wave_max(float v)
 // the row's exponents before the update.  A row's largest value usually
 // stays in its binade from one step to the next, which two ballots per array
 // establish (every |v| < 2^e_old, some |v| >= 2^(e_old-1) -- or, at the clamp
-// REXP_MIN, some v != 0): the exponent is then e_old; a third ballot finds an
-// all-zero row (exponent 0).  Only when an array is neither does the wave take
-// the DPP reductions, for all six in one uniform branch (so the six values
-// stay in registers).  The result equals the reductions' in every case.
+// REXP_MIN, some v != 0): the exponent is then e_old; for an array that
+// fails them a third ballot finds an all-zero row (exponent 0).  Only when an
+// array is neither does the wave take the DPP reductions, for all six in one
+// uniform branch (so the six values stay in registers).  The result equals
+// the reductions' in every case.
 __device__ inline unsigned long long row_exponents6(const float (&v)[6], const bool act,
                                                     const unsigned long long ew_old)
 {
     int e[6];
-    bool slow = false;
+    bool kept[6];
+    bool all_kept = true;
 #pragma unroll
     for (int q = 0; q < 6; ++q) {
         const int eo = rexp_of(ew_old, q);
@@ -184,18 +186,48 @@ __device__ inline unsigned long long row_exponents6(const float (&v)[6], const b
         const float lo = eo == REXP_MIN ? __int_as_float(1) : __int_as_float((eo + 126) << 23);
         const bool below = __builtin_amdgcn_ballot_w64(!act || a < hi) == ~0ull;
         const bool some = __builtin_amdgcn_ballot_w64(act && a >= lo) != 0ull;
-        const bool zero = __builtin_amdgcn_ballot_w64(act && a != 0.0f) == 0ull;
-        e[q] = zero ? 0 : eo;
-        slow = slow || !(zero || (below && some));
+        kept[q] = below && some;
+        e[q] = eo;
+        all_kept = all_kept && kept[q];
     }
-    if (slow) {
+    if (!all_kept) {
+        // the arrays whose largest value left its binade: an all-zero row
+        // takes exponent 0, any other one the reductions
+        bool slow = false;
 #pragma unroll
-        for (int q = 0; q < 6; ++q) e[q] = row_exponent(wave_max(act ? fabsf(v[q]) : 0.0f));
+        for (int q = 0; q < 6; ++q) {
+            if (!kept[q]) {
+                const bool zero = __builtin_amdgcn_ballot_w64(act && v[q] != 0.0f) == 0ull;
+                e[q] = 0;
+                slow = slow || !zero;
+            }
+        }
+        if (slow) {
+#pragma unroll
+            for (int q = 0; q < 6; ++q) e[q] = row_exponent(wave_max(act ? fabsf(v[q]) : 0.0f));
+        }
     }
     return rexp_pack(e);
 }
+// The exponent slot `slot` of a row segment (the same in every active lane
+// of the wave), read through the scalar data cache: the exponents then live
+// in scalar registers and their unpacking, the bounds of the binade test and
+// the decode offsets are scalar work.  Each slot is read once per launch, by
+// the wave that rewrites it afterwards (with a vector store of its new value,
+// which depends on this read), and a launch starts with the scalar cache
+// invalidated, so the read sees the previous launch's value.
+template <class T>
+__device__ __forceinline__ unsigned long long rexp_load(const T *rexp, long long slot)
+{
+    using CP = const __attribute__((address_space(4))) unsigned long long *;
+    const int u = __builtin_amdgcn_readfirstlane((int)slot);
+    return ((CP)(const void *)rexp)[u];
+}
 __device__ inline _Float16 r_encode(float r, int e) { return (_Float16)ldexpf(r, 15 - e); }
-__device__ inline float r_decode(_Float16 h, int e) { return ldexpf((float)h, e - 15); }
+// h * 2^(e-15) as one multiply by the power of two (exact scaling, the
+// same rounding as ldexpf; e - 15 lies in [REXP_MIN - 15, REXP_MAX - 15],
+// inside the normal exponent range)
+__device__ inline float r_decode(_Float16 h, int e) { return (float)h * __int_as_float((e - 15 + 127) << 23); }
 
 // Power-of-two storage scales of the FP16 qp/qs arrays (1 in FP32 mode) and
 // their inverses.
