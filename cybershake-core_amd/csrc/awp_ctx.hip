@@ -996,7 +996,9 @@ extern "C" int awp_tune(awp_ctx *ctx, int reps)
     HIPTRY(ctx, hipSetDevice(ctx->device));
     int rc = ensure_events(ctx, 2);
     if (rc) return rc;
-    // candidates: the two-stage step with its defaults; with both buffers,
+    // candidates: the two-stage step with its defaults, and on grids of
+    // fewer tiles than the GPU holds blocks (200x200x100: 400 tiles) with
+    // shorter z chunks and 64 x 4 tiles for more of them; with both buffers,
     // the fused step on 64 x 11 tiles (FP32; z chunks of about 100, 75, 60
     // levels x tile patches 4x4, 2x2, 8x2, and larger patches) and 64 x 8
     // tiles, or on 64 x 7 tiles (mixed).  The fastest is kept.
@@ -1006,6 +1008,13 @@ extern "C" int awp_tune(awp_ctx *ctx, int reps)
     };
     std::vector<Cand> cands;
     cands.push_back({false, 0, 0, 0, 0});
+    {
+        const long long tiles = (long long)((ctx->g.nx + TX - 1) / TX) * ((ctx->g.ny + 7) / 8) *
+                                split_nchunk(ctx->g.nz, SPLIT_KC);
+        if (tiles < 4096)
+            for (int ty : {8, 4})
+                for (int kc : {12, 8}) cands.push_back({false, ty, kc, 0, 0});
+    }
     if (ctx->two_bufs) {
         if (!ctx->mixed()) {
             const int pat[3][2] = {{4, 4}, {2, 2}, {8, 2}};
@@ -1020,6 +1029,12 @@ extern "C" int awp_tune(awp_ctx *ctx, int reps)
             cands.push_back({true, 11, 150, 4, 4});
             cands.push_back({true, 11, 150, 2, 2});
             cands.push_back({true, 8, 75, 4, 4});
+            // small grids: more z chunks, or 64 x 4 tiles, for more tiles
+            const long long fxy = (long long)((ctx->g.nx + TX - 1) / TX) * ((ctx->g.ny + 10) / 11);
+            if (fxy * std::max(1, ctx->g.nz / 100) < 4096) {
+                for (int kc : {25, 16}) cands.push_back({true, 11, kc, 2, 2});
+                cands.push_back({true, 4, 16, 4, 4});
+            }
         } else {
             cands.push_back({true, 7, 75, 4, 4});
         }
@@ -1028,6 +1043,11 @@ extern "C" int awp_tune(awp_ctx *ctx, int reps)
         HIPTRY(ctx, hipStreamSynchronize(ctx->stream));
         ctx->fused = cd.fused;
         Launch l = choose_launch(ctx->g, cd.fused, ctx->mixed(), ctx->device);
+        if (!cd.fused && cd.ty) {
+            l.tile_y = cd.ty;
+            l.tile_xw = 1;
+            set_chunks(l, ctx->g, split_nchunk(ctx->g.nz, cd.kc), false);
+        }
         if (cd.fused) {
             l.tile_y = cd.ty;
             l.unroll = cd.ty == 11 ? 3 : 0;   // 64 x 11: two levels per barrier
