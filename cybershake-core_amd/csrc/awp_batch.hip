@@ -492,10 +492,9 @@ extern "C" int awp_batch_tune(awp_batch *b, int reps)
         b->l.sph = patch ? sph0 : 0;
         set_block_caps(b->l, b->device);
     };
-    double best = 1e30;
-    Cand pick = {b->split, b->persist, b->l.tile_y, b->l.unroll, b->split ? b->kc_split : 1, b->l.tile_xw,
-                 b->l.o32_s ? 1 : 0, true};
-    for (const Cand &cd : cands) {
+    // times the step kernels of candidate cd over n launches after one
+    // warm-up (ms in *out)
+    auto time_cand = [&](const Cand &cd, int n, double *out) -> int {
         b->split = cd.split;
         b->persist = cd.persist;
         apply_form(cd.sform, cd.patch);
@@ -509,17 +508,47 @@ extern "C" int awp_batch_tune(awp_batch *b, int reps)
         int rc = upload_tables(b);
         if (rc) return rc;
         set_grids(b);
-        // warm-up, then timed launches of the step kernels alone
         BTRY(b, launch_kernels(b, 0));
         BTRY(b, hipEventRecord(b->e0, b->stream));
-        for (int k = 0; k < reps; ++k) BTRY(b, launch_kernels(b, k & 1));
+        for (int k = 0; k < n; ++k) BTRY(b, launch_kernels(b, k & 1));
         BTRY(b, hipEventRecord(b->e1, b->stream));
         BTRY(b, hipEventSynchronize(b->e1));
         float ms = 0;
         BTRY(b, hipEventElapsedTime(&ms, b->e0, b->e1));
+        *out = ms / n;
+        return AWP_OK;
+    };
+    // every candidate once, then the three fastest again over twice the
+    // launches (one pass over ~100 candidates leaves picks within timing
+    // noise of each other)
+    // AWP_TUNE_LOG=1: every timing on stderr
+    const char *tl = getenv("AWP_TUNE_LOG");
+    const bool log = tl && atoi(tl) == 1;
+    auto show = [&](const char *pass, const Cand &cd, double ms) {
+        if (log)
+            fprintf(stderr, "awp_batch_tune %s: %s %s ty %d xw %d pf/unroll %d chunks %d form %d patch %d: %.3f ms\n",
+                    pass, cd.split ? "two-stage" : "fused", cd.persist ? "persistent" : "per-tile", cd.ty, cd.xw,
+                    cd.pf, cd.nchunk, cd.sform, cd.patch ? 1 : 0, ms);
+    };
+    std::vector<std::pair<double, size_t>> first;
+    for (size_t n = 0; n < cands.size(); ++n) {
+        double ms = 0;
+        int rc = time_cand(cands[n], reps, &ms);
+        if (rc) return rc;
+        show("first", cands[n], ms);
+        first.push_back({ms, n});
+    }
+    std::sort(first.begin(), first.end());
+    double best = 1e30;
+    Cand pick = cands[first[0].second];
+    for (size_t q = 0; q < std::min<size_t>(3, first.size()); ++q) {
+        double ms = 0;
+        int rc = time_cand(cands[first[q].second], 2 * reps, &ms);
+        if (rc) return rc;
+        show("again", cands[first[q].second], ms);
         if (ms < best) {
             best = ms;
-            pick = cd;
+            pick = cands[first[q].second];
         }
     }
     b->split = pick.split;
@@ -539,6 +568,6 @@ extern "C" int awp_batch_tune(awp_batch *b, int reps)
         int rc = awp_init(c);
         if (rc) return bfail(b, rc, "re-initialising a run after tuning: %s", awp_last_error(c));
     }
-    b->t_last = best / reps;
+    b->t_last = best;
     return AWP_OK;
 }
