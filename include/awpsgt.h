@@ -211,8 +211,9 @@ int awp_init(awp_ctx *ctx);
 
 /* Time-step scheme of a run; every scheme and launch shape gives the same
  * bits.  awp_tune times `reps` steps (default 3) of each candidate -- the
- * two-stage step and, in a run that holds the second wavefield buffer, the
- * fused step on several tile, z-chunk and tile-patch shapes -- keeps the
+ * two-stage step (on grids of few tiles also with 64x4 tiles and shorter z
+ * chunks) and, in a run that holds the second wavefield buffer, the fused
+ * step on several tile, z-chunk and tile-patch shapes -- keeps the
  * fastest and starts the run again (awp_init): call it on an initialised run
  * before its first step.  awp_set_scheme(ctx, 1 / 0) switches to the fused /
  * two-stage step with its default launch shape at any point between steps
@@ -382,9 +383,10 @@ typedef struct awp_batch awp_batch;
 int awp_batch_create(awp_batch **out, awp_ctx *const *runs, int nruns, char *err, size_t errlen);
 /* Times the batch's step kernels for each scheme (fused only when the runs
  * hold two buffers) / tile height / prefetch depth or z unroll / z chunking
- * / persistent or one-block-per-tile grid candidate (reps launches each)
- * and keeps the fastest.  Needs runs that
- * have not stepped; re-initialises them (awp_init) afterwards. */
+ * / persistent or one-block-per-tile grid candidate (reps launches each),
+ * times the three fastest again over twice the launches and keeps the
+ * fastest.  Needs runs that have not stepped; re-initialises them
+ * (awp_init) afterwards. */
 int awp_batch_tune(awp_batch *b, int reps);
 /* Advances every run min(nsteps, steps it has left) steps; returns when
  * they are done. */
@@ -398,8 +400,8 @@ int awp_batch_time(const awp_batch *b, double *ms);
 int awp_batch_get_launch(const awp_batch *b, int *tile_y, int *prefetch, int *nchunk);
 /* *fused = 1 for the fused scheme, 0 for the two-stage one; *persistent = 1
  * when the stage kernels run persistent grids (the blocks that fit on the
- * GPU at once walk the tile list), 0 for one block per tile (the tuner
- * picks).  Either pointer may be NULL. */
+ * GPU at once claim the tiles of the list per XCD), 0 for one block per tile
+ * (the tuner picks).  Either pointer may be NULL. */
 int awp_batch_get_scheme(const awp_batch *b, int *fused, int *persistent);
 const char *awp_batch_last_error(const awp_batch *b);
 int awp_batch_destroy(awp_batch *b);
