@@ -731,6 +731,7 @@ hipError_t launch_step_fused_batch(bool mixed, const Launch &l, const FJob *d_jo
                                    int grid, int par, unsigned long long *claim, hipStream_t st)
 {
     if (njobs <= 0 || total <= 0) return hipSuccess;
+    if (!claim) return hipErrorInvalidValue;   // the kernel takes its tiles through the counters
     AWP_FUSED_DISPATCH(mixed, l, return (batch_t<P, TY, PF>(d_jobs, njobs, total, grid, par, claim, l.xrot, st)));
     return hipErrorInvalidValue;
 }
