@@ -57,15 +57,16 @@ def test_media_setup_bitwise(case_dirs):
 @pytest.mark.parametrize("scheme", ["split", "fused", "split_o32", "split_o64", "split_ty16",
                                     "split_xw2", "split_xw2_ty4", "split_xw2_o64", "split_o32v",
                                     "split_o64v", "fused_ty11", "fused_lv2", "fused_static", "fused_claim_r5",
-                                    "split_claim"])
+                                    "split_claim", "split_ty4_kc8"])
 @pytest.mark.parametrize("name", list(cases.CASES))
 def test_fields_bitwise_after_steps(case_dirs, name, scheme, monkeypatch):
     """Both time-step schemes (AWP_SCHEME=split two-stage, AWP_SCHEME=fused),
     both offset forms of the two-stage stress and velocity kernels (32-bit
     byte offsets, and the 64-bit element indices that grids with arrays of
     4 GiB or more take), their tile shapes (64 x 8, 64 x 16, 128 x 8,
-    128 x 4) and the static and claimed tile assignments (the fused ranges
-    rotated) against the restatement, bit for bit."""
+    128 x 4, 64 x 4 in 8-level z chunks) and the static and claimed tile
+    assignments (the fused ranges rotated) against the restatement, bit for
+    bit."""
     d, _ = case_dirs[name]
     p, media, xyz, stf, pts = _inputs(d)
     o = _oracle(d, p, media, xyz, stf, pts)
@@ -87,6 +88,10 @@ def test_fields_bitwise_after_steps(case_dirs, name, scheme, monkeypatch):
         # both byte-offset widths of the two-stage kernels, whichever the
         # storage mode uses by default
         monkeypatch.setenv("AWP_O32", "1" if scheme == "split_o32" else "0")
+    if scheme == "split_ty4_kc8":
+        # 64 x 4 tiles in z chunks of 8 levels (a small-grid tuner candidate)
+        monkeypatch.setenv("AWP_TILE_Y", "4")
+        monkeypatch.setenv("AWP_KC", "8")
     if scheme == "split_ty16":
         # 64 x 16 tiles: the 1024-thread variants of the two-stage kernels
         monkeypatch.setenv("AWP_TILE_Y", "16")
