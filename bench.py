@@ -147,8 +147,8 @@ def bench_single(a, storage):
     ms_step = el / a.steps * 1e3
     kern_ms = ts / a.steps                   # fused kernel, or the stress kernel of the two-stage step
     if fused:
-        kname = ("k_step_claim (fused velocity + stress + sponge + free surface sweep; persistent blocks "
-                 "claiming tiles per XCD; k_step under AWP_CLAIM=0)")
+        kname = ("k_step_batch (fused velocity + stress + sponge + free surface sweep over the run's one-entry "
+                 "job table; persistent blocks claiming tiles per XCD; k_step under AWP_CLAIM=0)")
         bytes_pt = BYTES_UPDATE[storage]
         tname = "step_%s%s_traffic.txt" % (a.config, "" if storage == "fp32" else "_" + storage)
     else:

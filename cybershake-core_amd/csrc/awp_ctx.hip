@@ -107,8 +107,9 @@ static void place_arrays(awp_ctx *ctx)
 // one-sweep fused step (a second wavefield buffer; every array addressed
 // with 32-bit byte offsets from a base rebased per z chunk, so chunks of at
 // most fused_kc_max levels).  Unset: fused for FP32 storage, two-stage for
-// mixed storage (1500x1000x300: fused FP32 16.2-16.3 ms per step on 64 x 11
-// tiles against 17.35 two-stage; fused mixed 23.6 against 15.7).  awp_tune
+// mixed storage (1500x1000x300, one run's buffers: fused FP32 16.8-16.9 ms
+// per step on 64 x 11 tiles against 17.2-17.3 two-stage; fused mixed 23.7
+// against 14.7).  awp_tune
 // and awp_set_scheme switch a run that holds both buffers between the two.
 static bool want_fused(const Geo &g, int storage)
 {
