@@ -360,7 +360,7 @@ def main():
                     help="device-memory budget for the batched configs (default: free memory - 8 GiB)")
     ap.add_argument("--no-tune", action="store_true", help="batched configs: skip the tuner")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--batched", default="c3", choices=["c3", "c5", "none"],
+    ap.add_argument("--batched", default="c5", choices=["c3", "c5", "none"],
                     help="with the single-grid line: also run this batched workload in FP32 and in mixed "
                          "storage and add their ratio as the 'batched' sub-record")
     ap.add_argument("--batched-steps", type=int, default=10)
