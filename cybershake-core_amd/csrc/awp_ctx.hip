@@ -1066,8 +1066,10 @@ extern "C" int awp_tune(awp_ctx *ctx, int reps)
     // AWP_TUNE_LOG=1: every timing on stderr
     const char *tl = getenv("AWP_TUNE_LOG");
     const bool log = tl && atoi(tl) == 1;
+    // every candidate once, then the three fastest again (their first
+    // timings lie within a percent of each other at 1500x1000x300)
     auto search = [&]() -> int {
-        float best = 1e30f;
+        std::vector<std::pair<float, size_t>> first;
         for (size_t n = 0; n < cands.size(); ++n) {
             float ms = 0;
             int r = apply(cands[n]);
@@ -1077,8 +1079,24 @@ extern "C" int awp_tune(awp_ctx *ctx, int reps)
                 fprintf(stderr, "awp_tune: %s ty %d kc %d patch %dx%d: %.3f ms per step\n",
                         cands[n].fused ? "fused" : "two-stage", cands[n].ty, cands[n].kc, cands[n].pw, cands[n].ph,
                         ms / reps);
-            if (ms < best) {
-                best = ms;
+            first.push_back({ms, n});
+        }
+        std::sort(first.begin(), first.end());
+        float best = 1e30f;
+        pick = first[0].second;
+        for (size_t q = 0; q < std::min<size_t>(3, first.size()); ++q) {
+            float ms = 0, ms2 = 0;
+            const size_t n = first[q].second;
+            int r = apply(cands[n]);
+            if (!r) r = time_steps(&ms);
+            if (!r) r = time_steps(&ms2);
+            if (r) return r;
+            if (log)
+                fprintf(stderr, "awp_tune again: %s ty %d kc %d patch %dx%d: %.3f ms per step\n",
+                        cands[n].fused ? "fused" : "two-stage", cands[n].ty, cands[n].kc, cands[n].pw, cands[n].ph,
+                        (ms + ms2) / (2 * reps));
+            if (ms + ms2 < best) {
+                best = ms + ms2;
                 pick = n;
             }
         }

@@ -213,9 +213,9 @@ int awp_init(awp_ctx *ctx);
  * bits.  awp_tune times `reps` steps (default 3) of each candidate -- the
  * two-stage step (on grids of few tiles also with 64x4 tiles and shorter z
  * chunks) and, in a run that holds the second wavefield buffer, the fused
- * step on several tile, z-chunk and tile-patch shapes -- keeps the
- * fastest and starts the run again (awp_init): call it on an initialised run
- * before its first step.  awp_set_scheme(ctx, 1 / 0) switches to the fused /
+ * step on several tile, z-chunk and tile-patch shapes --, times the three
+ * fastest again over twice the steps, keeps the fastest and starts the run
+ * again (awp_init): call it on an initialised run before its first step.  awp_set_scheme(ctx, 1 / 0) switches to the fused /
  * two-stage step with its default launch shape at any point between steps
  * (the fused step needs the second buffer).  awp_get_scheme reports the step
  * in use (1 fused, 0 two-stage), its tile rows and its z chunks; any pointer
