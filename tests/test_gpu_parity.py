@@ -209,15 +209,16 @@ def test_large_shape_properties():
             assert np.array_equal(r4.field(f, k), o.field(f, k)), f
 
 
+@pytest.mark.parametrize("claim", ["2", "0"])
 @pytest.mark.parametrize("storage", [awpsgt.AWP_STORAGE_FP32, awpsgt.AWP_STORAGE_MIXED])
 @pytest.mark.parametrize("scheme", ["split", "fused"])
-def test_ragged_shape_claimed_equals_restatement(scheme, storage, monkeypatch, tmp_path):
+def test_ragged_shape_equals_restatement(scheme, storage, claim, monkeypatch, tmp_path):
     """A ragged shape (200x130x60: rows of three full 64-point segments and
     one of 8 points, partial tiles in y for both tile heights, a random
-    medium) with every launch claiming its tiles per XCD (the default only
-    claims on grids of many tiles), in both schemes and storage modes:
-    after 10 steps every field equals the restatement's (FP32, or the
-    mixed-storage one) bit for bit."""
+    medium) with every launch claiming its tiles per XCD (AWP_CLAIM=2; the
+    default only claims on grids of many tiles) or one block per tile, in
+    both schemes and storage modes: after 10 steps every field equals the
+    restatement's (FP32, or the mixed-storage one) bit for bit."""
     spec = cases.c1_spec()
     spec.nx, spec.ny, spec.nz, spec.nt = 200, 130, 60, 20
     spec.ntiskp_sgt, spec.write_step = 5, 2
@@ -230,7 +231,7 @@ def test_ragged_shape_claimed_equals_restatement(scheme, storage, monkeypatch, t
     cases.write_case(d, spec, media, pts)
     p, media, xyz, stf, pts = _inputs(d)
     monkeypatch.setenv("AWP_SCHEME", scheme)
-    monkeypatch.setenv("AWP_CLAIM", "2")
+    monkeypatch.setenv("AWP_CLAIM", claim)
     q = orc.read_in3d(os.path.join(d, "IN3D"))
     o = orc.Oracle(q, media, xyz, stf, pts, mixed=storage == awpsgt.AWP_STORAGE_MIXED)
     o.run(10, 20)
