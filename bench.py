@@ -344,6 +344,58 @@ def claim_ab(a):
                       "step": "fused" if fused else "two-stage"}))
 
 
+def class_probe(a):
+    """Runs of the single-grid workload created and freed in a fixed order
+    (A; B while A lives; C after A is freed; D after B is freed), each timed
+    over a few steps: a run's speed follows the physical memory its
+    allocation gets, and a run created into memory another run freed runs
+    like that run.  Prints one JSON line; a diagnostic, not the throughput
+    line."""
+    import torch
+    import awpsgt
+    from awpsgt.media import c2_media
+    from awpsgt.workloads import layered_nvar5
+
+    cfg = CONFIGS[a.config]
+    p = build_params(cfg, 4 * (a.warmup + a.steps))
+    if a.config == "c2":
+        media = c2_media(p.nx, p.ny, p.nz, p.dh, seed=cfg["seed"])
+    else:
+        media = layered_nvar5(p)
+    st = awpsgt.AWP_STORAGE_MIXED if a.storage == "mixed" else awpsgt.AWP_STORAGE_FP32
+    out = []
+
+    def make():
+        return make_run(p, st, lambda: media, cfg["src"], cfg["nsgt"], 3)[0]
+
+    def timed(r, name):
+        r.step(a.warmup)
+        r.synchronize()
+        t = time.perf_counter()
+        r.step(a.steps)
+        r.synchronize()
+        ms = (time.perf_counter() - t) / a.steps * 1e3
+        log("%s: %.3f ms/step" % (name, ms))
+        out.append({"run": name, "ms_per_step": round(ms, 4)})
+
+    ra = make()
+    timed(ra, "A")
+    rb = make()
+    timed(rb, "B (A alive)")
+    timed(ra, "A again")
+    ra.close()
+    torch.cuda.synchronize()
+    rc = make()
+    timed(rc, "C (after A freed)")
+    rb.close()
+    torch.cuda.synchronize()
+    rd = make()
+    timed(rd, "D (after B freed)")
+    rc.close()
+    rd.close()
+    print(json.dumps({"class_probe": out, "config": a.config, "storage": a.storage}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -370,7 +422,15 @@ def main():
     ap.add_argument("--claim-ab", action="store_true",
                     help="only step one single-grid run alternately with claimed and static tile launches "
                          "(same buffers) and print their times")
+    ap.add_argument("--class-probe", action="store_true",
+                    help="only create, time and free single-grid runs in a fixed order (the speed a run gets "
+                         "follows the memory its allocation gets) and print their times")
     a = ap.parse_args()
+    if a.class_probe:
+        if a.scheme != "auto":
+            os.environ["AWP_SCHEME"] = a.scheme
+        class_probe(a)
+        return
     if a.claim_ab:
         if a.scheme != "auto":
             os.environ["AWP_SCHEME"] = a.scheme
