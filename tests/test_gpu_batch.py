@@ -42,7 +42,8 @@ def _single(d, storage, nsteps=None):
     return r
 
 
-@pytest.mark.parametrize("scheme", ["split", "fused", "split_ty16", "split_xw2", "split_o32v", "fused_ty11"])
+@pytest.mark.parametrize("scheme", ["split", "fused", "split_ty16", "split_xw2", "split_o32v", "fused_ty11",
+                                    "split_nograph", "fused_nograph"])
 @pytest.mark.parametrize("storage", [awpsgt.AWP_STORAGE_FP32, awpsgt.AWP_STORAGE_MIXED])
 def test_batch_equals_single_runs(case_dirs, storage, scheme, monkeypatch):
     dirs = [case_dirs[n][0] for n in SAME_NT]
@@ -55,6 +56,9 @@ def test_batch_equals_single_runs(case_dirs, storage, scheme, monkeypatch):
         monkeypatch.setenv("AWP_TILE_XW", "2")
     if scheme == "split_o32v":
         monkeypatch.setenv("AWP_O32V", "1")
+    if scheme.endswith("_nograph"):
+        # each step's launches issued directly instead of replaying the graph
+        monkeypatch.setenv("AWP_GRAPH", "0")
     runs = [_make(d, storage) for d in dirs]
     monkeypatch.delenv("AWP_SCHEME", raising=False)
     nt = runs[0].p.nt
