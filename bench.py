@@ -249,7 +249,9 @@ def bench_batched(a, storage, config=None, steps=None, warmup=None):
     ms_step = tot_time / steps * 1e3
     dev_ms = sum(kern)
     ach = BYTES_UPDATE[storage] * pts_step / (dev_ms * 1e-3) / 1e9
-    tname = "batch_%s_%s_traffic.txt" % (config, storage)
+    # measured bytes per step of the two-stage batch (the fused batch has
+    # none measured: traffic null)
+    tname = "batch_%s_%s%s_traffic.txt" % (config, storage, "_fused" if fused else "")
     roof = {"bound": "hbm",
             "kernel": ("whole batched step: k_step_batch (fused sweep over every run of a wave) + sources/SGT"
                        if fused else
