@@ -62,7 +62,10 @@ static VolLayout vol_layout(const Geo &g, int storage, bool two_bufs)
     const size_t eslot = mixed ? nv * 2 + 128 : nv * 4, eoff = mixed ? 64 : 0;
     size_t o = L.media_begin = 0;
     for (int a = 0; a < 3; ++a, o += eslot) L.m[a] = o + eoff;
-    for (int a = 0; a < 2; ++a, o += eslot) L.q[a] = o + eoff;
+    // qp, qs: mixed storage keeps them as one array of (qp, qs) FP16 pairs,
+    // 4 bytes a point, in the two slots (both pointers at its start)
+    const size_t qbase = o;   // a multiple of 128: rows of the pairs start on a line
+    for (int a = 0; a < 2; ++a, o += eslot) L.q[a] = mixed ? qbase : o + eoff;
     L.media_end = o;
     L.nseg = (g.nx + TX - 1) / TX;
     L.rexp_n = mixed ? (size_t)g.nz * g.ny * L.nseg : 0;
@@ -579,9 +582,8 @@ static int media_from_device(awp_ctx *ctx, const float *dmedia)
             sc[a] = ldexpf(1.0f, 14 - ex);    // mx * sc in [2^13, 2^14)
             isc[a] = ldexpf(1.0f, ex - 14);
         }
-        void *dst[2] = {ctx->s.qp, ctx->s.qs};
-        for (int a = 0; a < 2 && e == hipSuccess; ++a)
-            e = launch_to_half(arrs[a], (_Float16 *)dst[a], sc[a], (long long)nv, ctx->stream);
+        if (e == hipSuccess)
+            e = launch_to_qpair(m.qp, m.qs, (unsigned *)ctx->s.qp, sc[0], sc[1], (long long)nv, ctx->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
         hipFree(part);
         if (e != hipSuccess) return fail(ctx, AWP_EHIP, "mixed media set-up: %s", hipGetErrorString(e));

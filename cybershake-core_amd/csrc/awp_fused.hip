@@ -108,8 +108,14 @@ __device__ __forceinline__ GState rebased(const State &s0, const long long e)
     G<void> **m[3] = {&s.d1, &s.mu, &s.lam};
 #pragma unroll
     for (int q = 0; q < 3; ++q) *m[q] = (G<M> *)*m[q] + e;
-    s.qp = (G<Q> *)s.qp + e;
-    s.qs = (G<Q> *)s.qs + e;
+    if constexpr (P::mixed) {
+        // (qp, qs) pairs, one 4-byte word a point
+        s.qp = (G<unsigned> *)s.qp + e;
+        s.qs = s.qp;
+    } else {
+        s.qp = (G<Q> *)s.qp + e;
+        s.qs = (G<Q> *)s.qs + e;
+    }
     return s;
 }
 
@@ -179,6 +185,7 @@ __device__ __forceinline__ void step_tile(const Geo &g, const State &a0, const S
     auto mbar = [&](int arr, int l) -> float { return P::mixed ? a.mbar[mbar_idx(g, arr, l)] : 0.0f; };
     const G<Q> *__restrict__ qp = (const G<Q> *)a.qp;
     const G<Q> *__restrict__ qs = (const G<Q> *)a.qs;
+    const G<unsigned> *__restrict__ qq = (const G<unsigned> *)a.qp;   // mixed: (qp, qs) pairs
     G<R> *__restrict__ R1 = (G<R> *)a.r1;
     G<R> *__restrict__ R2 = (G<R> *)a.r2;
     G<R> *__restrict__ R3 = (G<R> *)a.r3;
@@ -218,9 +225,15 @@ __device__ __forceinline__ void step_tile(const Geo &g, const State &a0, const S
             x.mA = mdec(gl(mu, mo), mb); x.mB = mdec(gl(mu, mo, MB), mb);
             x.mC = mdec(gl(mu, mj), mb); x.mD = mdec(gl(mu, mj, MB), mb);
         }
-        const unsigned qo = o >> QSH, qj = oj >> QSH;
-        x.pA = ld(gl(qp, qo)); x.pB = ld(gl(qp, qo, QB)); x.pC = ld(gl(qp, qj)); x.pD = ld(gl(qp, qj, QB));
-        x.sA = ld(gl(qs, qo)); x.sB = ld(gl(qs, qo, QB)); x.sC = ld(gl(qs, qj)); x.sD = ld(gl(qs, qj, QB));
+        if constexpr (P::mixed) {
+            const unsigned wA = gl(qq, o), wB = gl(qq, o, 4), wC = gl(qq, oj), wD = gl(qq, oj, 4);
+            x.pA = q_lo(wA); x.pB = q_lo(wB); x.pC = q_lo(wC); x.pD = q_lo(wD);
+            x.sA = q_hi(wA); x.sB = q_hi(wB); x.sC = q_hi(wC); x.sD = q_hi(wD);
+        } else {
+            const unsigned qo = o >> QSH, qj = oj >> QSH;
+            x.pA = ld(gl(qp, qo)); x.pB = ld(gl(qp, qo, QB)); x.pC = ld(gl(qp, qj)); x.pD = ld(gl(qp, qj, QB));
+            x.sA = ld(gl(qs, qo)); x.sB = ld(gl(qs, qo, QB)); x.sC = ld(gl(qs, qj)); x.sD = ld(gl(qs, qj, QB));
+        }
         const unsigned orr = o >> RSH;
         if constexpr (P::mixed) {
             // FP16 values under the row segment's exponent (one per wave)
@@ -280,9 +293,15 @@ __device__ __forceinline__ void step_tile(const Geo &g, const State &a0, const S
             mA_m = mdec(gl(mu, mo), mb); mB_m = mdec(gl(mu, mo, MB), mb);
             mC_m = mdec(gl(mu, mj), mb); mD_m = mdec(gl(mu, mj, MB), mb);
         }
-        const unsigned qo = om >> QSH, qj = omj >> QSH;
-        pA_m = ld(gl(qp, qo)); pB_m = ld(gl(qp, qo, QB)); pC_m = ld(gl(qp, qj)); pD_m = ld(gl(qp, qj, QB));
-        sA_m = ld(gl(qs, qo)); sB_m = ld(gl(qs, qo, QB)); sC_m = ld(gl(qs, qj)); sD_m = ld(gl(qs, qj, QB));
+        if constexpr (P::mixed) {
+            const unsigned wA = gl(qq, om), wB = gl(qq, om, 4), wC = gl(qq, omj), wD = gl(qq, omj, 4);
+            pA_m = q_lo(wA); pB_m = q_lo(wB); pC_m = q_lo(wC); pD_m = q_lo(wD);
+            sA_m = q_hi(wA); sB_m = q_hi(wB); sC_m = q_hi(wC); sD_m = q_hi(wD);
+        } else {
+            const unsigned qo = om >> QSH, qj = omj >> QSH;
+            pA_m = ld(gl(qp, qo)); pB_m = ld(gl(qp, qo, QB)); pC_m = ld(gl(qp, qj)); pD_m = ld(gl(qp, qj, QB));
+            sA_m = ld(gl(qs, qo)); sB_m = ld(gl(qs, qo, QB)); sC_m = ld(gl(qs, qj)); sD_m = ld(gl(qs, qj, QB));
+        }
     }
     // final xz/yz/zz of the two levels below (free-surface images)
     float xz_f1 = 0.f, xz_f2 = 0.f, yz_f1 = 0.f, yz_f2 = 0.f, zz_f1 = 0.f;

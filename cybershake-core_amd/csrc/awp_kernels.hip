@@ -270,6 +270,7 @@ __device__ __forceinline__ void stress_tile_e(const Geo &g, const State &s0, con
     const G<M> *__restrict__ mu = (const G<M> *)s.mu;
     const G<Q> *__restrict__ qp = (const G<Q> *)s.qp;
     const G<Q> *__restrict__ qs = (const G<Q> *)s.qs;
+    const G<unsigned> *__restrict__ qq = (const G<unsigned> *)s.qp;   // mixed: (qp, qs) pairs
     G<R> *__restrict__ R1 = (G<R> *)s.r1;
     G<R> *__restrict__ R2 = (G<R> *)s.r2;
     G<R> *__restrict__ R3 = (G<R> *)s.r3;
@@ -296,8 +297,15 @@ __device__ __forceinline__ void stress_tile_e(const Geo &g, const State &s0, con
           lD_m = mdec(lam[om + 1 - sy], lb_m);
     float mA_m = mdec(mu[om], mb_m), mB_m = mdec(mu[om + 1], mb_m), mC_m = mdec(mu[om - sy], mb_m),
           mD_m = mdec(mu[om + 1 - sy], mb_m);
-    float pA_m = ld(qp[om]), pB_m = ld(qp[om + 1]), pC_m = ld(qp[om - sy]), pD_m = ld(qp[om + 1 - sy]);
-    float sA_m = ld(qs[om]), sB_m = ld(qs[om + 1]), sC_m = ld(qs[om - sy]), sD_m = ld(qs[om + 1 - sy]);
+    float pA_m, pB_m, pC_m, pD_m, sA_m, sB_m, sC_m, sD_m;
+    if constexpr (P::mixed) {
+        const unsigned wA = qq[om], wB = qq[om + 1], wC = qq[om - sy], wD = qq[om + 1 - sy];
+        pA_m = q_lo(wA); pB_m = q_lo(wB); pC_m = q_lo(wC); pD_m = q_lo(wD);
+        sA_m = q_hi(wA); sB_m = q_hi(wB); sC_m = q_hi(wC); sD_m = q_hi(wD);
+    } else {
+        pA_m = ld(qp[om]); pB_m = ld(qp[om + 1]); pC_m = ld(qp[om - sy]); pD_m = ld(qp[om + 1 - sy]);
+        sA_m = ld(qs[om]); sB_m = ld(qs[om + 1]); sC_m = ld(qs[om - sy]); sD_m = ld(qs[om + 1 - sy]);
+    }
     // final xz/yz/zz of the two levels below (for the free-surface images)
     float xz_f1 = 0.f, xz_f2 = 0.f, yz_f1 = 0.f, yz_f2 = 0.f, zz_f1 = 0.f;
 
@@ -312,8 +320,15 @@ __device__ __forceinline__ void stress_tile_e(const Geo &g, const State &s0, con
                     lD = mdec(lam[o + 1 - sy], lb);
         const float mA = mdec(mu[o], mb), mB = mdec(mu[o + 1], mb), mC = mdec(mu[o - sy], mb),
                     mD = mdec(mu[o + 1 - sy], mb);
-        const float pA = ld(qp[o]), pB = ld(qp[o + 1]), pC = ld(qp[o - sy]), pD = ld(qp[o + 1 - sy]);
-        const float sA = ld(qs[o]), sB = ld(qs[o + 1]), sC = ld(qs[o - sy]), sD = ld(qs[o + 1 - sy]);
+        float pA, pB, pC, pD, sA, sB, sC, sD;
+        if constexpr (P::mixed) {
+            const unsigned wA = qq[o], wB = qq[o + 1], wC = qq[o - sy], wD = qq[o + 1 - sy];
+            pA = q_lo(wA); pB = q_lo(wB); pC = q_lo(wC); pD = q_lo(wD);
+            sA = q_hi(wA); sB = q_hi(wB); sC = q_hi(wC); sD = q_hi(wD);
+        } else {
+            pA = ld(qp[o]); pB = ld(qp[o + 1]); pC = ld(qp[o - sy]); pD = ld(qp[o + 1 - sy]);
+            sA = ld(qs[o]); sB = ld(qs[o + 1]); sC = ld(qs[o - sy]); sD = ld(qs[o + 1 - sy]);
+        }
 
         const float u_ip1 = u1[o + 1], u_ip2 = u1[o + 2], u_im1 = u1[o - 1];
         const float u_jp1 = u1[o + sy], u_jp2 = u1[o + 2 * sy], u_jm1 = u1[o - sy];
@@ -481,6 +496,7 @@ __device__ __forceinline__ void stress_tile_b(const Geo &g, const State &s0, con
     const G<M> *__restrict__ mu = (const G<M> *)s.mu;
     const G<Q> *__restrict__ qp = (const G<Q> *)s.qp;
     const G<Q> *__restrict__ qs = (const G<Q> *)s.qs;
+    const G<unsigned> *__restrict__ qq = (const G<unsigned> *)s.qp;   // mixed: (qp, qs) pairs
     G<R> *__restrict__ R1 = (G<R> *)s.r1;
     G<R> *__restrict__ R2 = (G<R> *)s.r2;
     G<R> *__restrict__ R3 = (G<R> *)s.r3;
@@ -506,10 +522,18 @@ __device__ __forceinline__ void stress_tile_b(const Geo &g, const State &s0, con
           lC_m = mdec(p.ld(lam, 0, -1, -1), lb_m), lD_m = mdec(p.ld(lam, 1, -1, -1), lb_m);
     float mA_m = mdec(p.ld(mu, 0, 0, -1), mb_m), mB_m = mdec(p.ld(mu, 1, 0, -1), mb_m),
           mC_m = mdec(p.ld(mu, 0, -1, -1), mb_m), mD_m = mdec(p.ld(mu, 1, -1, -1), mb_m);
-    float pA_m = ld(p.ld(qp, 0, 0, -1)), pB_m = ld(p.ld(qp, 1, 0, -1));
-    float pC_m = ld(p.ld(qp, 0, -1, -1)), pD_m = ld(p.ld(qp, 1, -1, -1));
-    float sA_m = ld(p.ld(qs, 0, 0, -1)), sB_m = ld(p.ld(qs, 1, 0, -1));
-    float sC_m = ld(p.ld(qs, 0, -1, -1)), sD_m = ld(p.ld(qs, 1, -1, -1));
+    float pA_m, pB_m, pC_m, pD_m, sA_m, sB_m, sC_m, sD_m;
+    if constexpr (P::mixed) {
+        const unsigned wA = p.ld(qq, 0, 0, -1), wB = p.ld(qq, 1, 0, -1);
+        const unsigned wC = p.ld(qq, 0, -1, -1), wD = p.ld(qq, 1, -1, -1);
+        pA_m = q_lo(wA); pB_m = q_lo(wB); pC_m = q_lo(wC); pD_m = q_lo(wD);
+        sA_m = q_hi(wA); sB_m = q_hi(wB); sC_m = q_hi(wC); sD_m = q_hi(wD);
+    } else {
+        pA_m = ld(p.ld(qp, 0, 0, -1)); pB_m = ld(p.ld(qp, 1, 0, -1));
+        pC_m = ld(p.ld(qp, 0, -1, -1)); pD_m = ld(p.ld(qp, 1, -1, -1));
+        sA_m = ld(p.ld(qs, 0, 0, -1)); sB_m = ld(p.ld(qs, 1, 0, -1));
+        sC_m = ld(p.ld(qs, 0, -1, -1)); sD_m = ld(p.ld(qs, 1, -1, -1));
+    }
     // final xz/yz/zz of the two levels below (for the free-surface images)
     float xz_f1 = 0.f, xz_f2 = 0.f, yz_f1 = 0.f, yz_f2 = 0.f, zz_f1 = 0.f;
     // mixed storage: offset of this row segment's exponent at level k in
@@ -528,8 +552,15 @@ __device__ __forceinline__ void stress_tile_b(const Geo &g, const State &s0, con
                     lD = mdec(p.ld(lam, 1, -1), lb);
         const float mA = mdec(p.ld(mu), mb), mB = mdec(p.ld(mu, 1), mb), mC = mdec(p.ld(mu, 0, -1), mb),
                     mD = mdec(p.ld(mu, 1, -1), mb);
-        const float pA = ld(p.ld(qp)), pB = ld(p.ld(qp, 1)), pC = ld(p.ld(qp, 0, -1)), pD = ld(p.ld(qp, 1, -1));
-        const float sA = ld(p.ld(qs)), sB = ld(p.ld(qs, 1)), sC = ld(p.ld(qs, 0, -1)), sD = ld(p.ld(qs, 1, -1));
+        float pA, pB, pC, pD, sA, sB, sC, sD;
+        if constexpr (P::mixed) {
+            const unsigned wA = p.ld(qq), wB = p.ld(qq, 1), wC = p.ld(qq, 0, -1), wD = p.ld(qq, 1, -1);
+            pA = q_lo(wA); pB = q_lo(wB); pC = q_lo(wC); pD = q_lo(wD);
+            sA = q_hi(wA); sB = q_hi(wB); sC = q_hi(wC); sD = q_hi(wD);
+        } else {
+            pA = ld(p.ld(qp)); pB = ld(p.ld(qp, 1)); pC = ld(p.ld(qp, 0, -1)); pD = ld(p.ld(qp, 1, -1));
+            sA = ld(p.ld(qs)); sB = ld(p.ld(qs, 1)); sC = ld(p.ld(qs, 0, -1)); sD = ld(p.ld(qs, 1, -1));
+        }
 
         const float u_ip1 = p.ld(u1, 1), u_ip2 = p.ld(u1, 2), u_im1 = p.ld(u1, -1);
         const float u_jp1 = p.ld(u1, 0, 1), u_jp2 = p.ld(u1, 0, 2), u_jm1 = p.ld(u1, 0, -1);
@@ -1190,6 +1221,17 @@ __global__ void __launch_bounds__(256) k_to_half(const float *src, _Float16 *dst
         dst[t] = (_Float16)(src[t] * scale);
 }
 
+// qp and qs of every point as one word of two FP16 halves (qp low, qs high),
+// each rounded as k_to_half rounds it
+__global__ void __launch_bounds__(256)
+k_to_qpair(const float *qp, const float *qs, unsigned *dst, const float sp, const float ss, const long long n)
+{
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x) {
+        const _Float16 a = (_Float16)(qp[t] * sp), b = (_Float16)(qs[t] * ss);
+        dst[t] = (unsigned)__builtin_bit_cast(unsigned short, a) | ((unsigned)__builtin_bit_cast(unsigned short, b) << 16);
+    }
+}
+
 __global__ void __launch_bounds__(256)
 k_pack_r(const Geo g, const _Float16 *src, const signed char *ex, const int q, const int nseg, const int kext,
          const int k0, float *dst)
@@ -1226,6 +1268,19 @@ k_pack_field(const Geo g, const T *src, const float inv_scale, const int kext, c
     const int k = blockIdx.z + 1 + k0;
     if (i > g.nx || j > g.ny || k > g.nz + kext) return;
     dst[((size_t)(k - 1 - k0) * g.ny + (j - 1)) * g.nx + (i - 1)] = ld(src[gidx(g, i, j, k)]) * inv_scale;
+}
+
+// one half (0 qp, 1 qs) of the mixed-storage (qp, qs) pairs, unscaled
+__global__ void __launch_bounds__(256)
+k_pack_qpair(const Geo g, const unsigned *src, const int half, const float inv_scale, const int kext, const int k0,
+             float *dst)
+{
+    const int i = blockIdx.x * TX + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int k = blockIdx.z + 1 + k0;
+    if (i > g.nx || j > g.ny || k > g.nz + kext) return;
+    const unsigned w = src[gidx(g, i, j, k)];
+    dst[((size_t)(k - 1 - k0) * g.ny + (j - 1)) * g.nx + (i - 1)] = (half ? q_hi(w) : q_lo(w)) * inv_scale;
 }
 
 // ---------------------------------------------------------------------------
@@ -1474,6 +1529,13 @@ hipError_t launch_to_half(const float *src, _Float16 *dst, float scale, long lon
     return hipGetLastError();
 }
 
+hipError_t launch_to_qpair(const float *qp, const float *qs, unsigned *dst, float sp, float ss, long long n,
+                           hipStream_t st)
+{
+    hipLaunchKernelGGL(k_to_qpair, dim3(4096), dim3(256), 0, st, qp, qs, dst, sp, ss, n);
+    return hipGetLastError();
+}
+
 hipError_t launch_pack_field(bool mixed, const Geo &g, const State &s, int which, int kext, float *dst,
                              hipStream_t st)
 {
@@ -1509,7 +1571,7 @@ hipError_t launch_pack_levels(bool mixed, const Geo &g, const State &s, int whic
                                dst);
     } else {
         if (mixed)
-            hipLaunchKernelGGL(k_pack_field<_Float16>, grd, blk, 0, st, g, (const _Float16 *)qq[which - 18],
+            hipLaunchKernelGGL(k_pack_qpair, grd, blk, 0, st, g, (const unsigned *)s.qp, which - 18,
                                iqs[which - 18], kext, k0, dst);
         else
             hipLaunchKernelGGL(k_pack_field<float>, grd, blk, 0, st, g, (const float *)qq[which - 18], 1.0f, kext,

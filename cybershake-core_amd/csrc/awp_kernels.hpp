@@ -224,6 +224,11 @@ __device__ __forceinline__ unsigned long long rexp_load(const T *rexp, long long
     return ((CP)(const void *)rexp)[u];
 }
 __device__ inline _Float16 r_encode(float r, int e) { return (_Float16)ldexpf(r, 15 - e); }
+// Mixed storage keeps a point's qp and qs side by side in one 4-byte word
+// (FP16 qp in the low half, FP16 qs in the high half), so one load brings
+// both; these unpack the halves.
+__device__ __forceinline__ float q_lo(unsigned w) { return (float)__builtin_bit_cast(_Float16, (unsigned short)(w & 0xFFFFu)); }
+__device__ __forceinline__ float q_hi(unsigned w) { return (float)__builtin_bit_cast(_Float16, (unsigned short)(w >> 16)); }
 // h * 2^(e-15) as one multiply by the power of two (exact scaling, the
 // same rounding as ldexpf; e - 15 lies in [REXP_MIN - 15, REXP_MAX - 15],
 // inside the normal exponent range)
@@ -497,6 +502,9 @@ hipError_t launch_level_range(const Geo &g, const float *a, unsigned *range, hip
 hipError_t launch_media_encode(const Geo &g, const float *src, _Float16 *dst, const float *bar, hipStream_t st);
 hipError_t launch_absmax(const float *a, long long n, float *part, int nblk, hipStream_t st);
 hipError_t launch_to_half(const float *src, _Float16 *dst, float scale, long long n, hipStream_t st);
+// mixed storage: qp, qs (FP32, n points) to (qp, qs) FP16 pairs scaled by sp, ss
+hipError_t launch_to_qpair(const float *qp, const float *qs, unsigned *dst, float sp, float ss, long long n,
+                           hipStream_t st);
 hipError_t launch_sgt_fp16(const float *sgt, const int *kidx, int nkept, int nsamp_max, int nsamp, float *scales,
                            _Float16 *out, hipStream_t st);
 // the fused one-sweep step (awp_fused.hip): reads state a, writes state b,
