@@ -61,7 +61,10 @@ static VolLayout vol_layout(const Geo &g, int storage, bool two_bufs)
     const size_t a4 = nv * 4;
     const size_t eslot = mixed ? nv * 2 + 128 : nv * 4, eoff = mixed ? 64 : 0;
     size_t o = L.media_begin = 0;
-    for (int a = 0; a < 3; ++a, o += eslot) L.m[a] = o + eoff;
+    // d1, mu, lam; mixed storage keeps mu and lam as one array of (mu, lam)
+    // FP16 pairs, 4 bytes a point, in their two slots (both pointers at its
+    // start, a multiple of 128)
+    for (int a = 0; a < 3; ++a, o += eslot) L.m[a] = (mixed && a == 2) ? L.m[1] : (mixed && a == 1) ? o : o + eoff;
     // qp, qs: mixed storage keeps them as one array of (qp, qs) FP16 pairs,
     // 4 bytes a point, in the two slots (both pointers at its start)
     const size_t qbase = o;   // a multiple of 128: rows of the pairs start on a line
@@ -564,9 +567,11 @@ static int media_from_device(awp_ctx *ctx, const float *dmedia)
         }
         if (e == hipSuccess)
             e = hipMemcpyAsync(ctx->s.mbar, bar.data(), bar.size() * sizeof(float), hipMemcpyHostToDevice, ctx->stream);
-        void *mdst[3] = {ctx->s.d1, ctx->s.mu, ctx->s.lam};
-        for (int a = 0; a < 3 && e == hipSuccess; ++a)
-            e = launch_media_encode(ctx->g, tmp[a], (_Float16 *)mdst[a], ctx->s.mbar + (size_t)a * pz, ctx->stream);
+        if (e == hipSuccess)
+            e = launch_media_encode(ctx->g, tmp[0], (_Float16 *)ctx->s.d1, ctx->s.mbar, ctx->stream);
+        if (e == hipSuccess)
+            e = launch_media_encode2(ctx->g, tmp[1], tmp[2], (unsigned *)ctx->s.mu, ctx->s.mbar + (size_t)pz,
+                                     ctx->s.mbar + 2 * (size_t)pz, ctx->stream);
         // qp, qs: per-array power of two putting the largest magnitude in [2^13, 2^14)
         const float *arrs[2] = {m.qp, m.qs};
         for (int a = 0; a < 2 && e == hipSuccess; ++a) e = launch_absmax(arrs[a], (long long)nv, part + a * nblk, nblk, ctx->stream);

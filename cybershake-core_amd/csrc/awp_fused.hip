@@ -105,9 +105,16 @@ __device__ __forceinline__ GState rebased(const State &s0, const long long e)
     G<void> **r[6] = {&s.r1, &s.r2, &s.r3, &s.r4, &s.r5, &s.r6};
 #pragma unroll
     for (int q = 0; q < 6; ++q) *r[q] = (G<R> *)*r[q] + e;
-    G<void> **m[3] = {&s.d1, &s.mu, &s.lam};
+    if constexpr (P::mixed) {
+        // d1 FP16; (mu, lam) pairs, one 4-byte word a point
+        s.d1 = (G<M> *)s.d1 + e;
+        s.mu = (G<unsigned> *)s.mu + e;
+        s.lam = s.mu;
+    } else {
+        G<void> **m[3] = {&s.d1, &s.mu, &s.lam};
 #pragma unroll
-    for (int q = 0; q < 3; ++q) *m[q] = (G<M> *)*m[q] + e;
+        for (int q = 0; q < 3; ++q) *m[q] = (G<M> *)*m[q] + e;
+    }
     if constexpr (P::mixed) {
         // (qp, qs) pairs, one 4-byte word a point
         s.qp = (G<unsigned> *)s.qp + e;
@@ -181,6 +188,7 @@ __device__ __forceinline__ void step_tile(const Geo &g, const State &a0, const S
     const G<M> *__restrict__ d1 = (const G<M> *)a.d1;
     const G<M> *__restrict__ lam = (const G<M> *)a.lam;
     const G<M> *__restrict__ mu = (const G<M> *)a.mu;
+    const G<unsigned> *__restrict__ mlq = (const G<unsigned> *)a.mu;   // mixed: (mu, lam) pairs
     // profile values of level l (mixed storage; 0 unused in FP32)
     auto mbar = [&](int arr, int l) -> float { return P::mixed ? a.mbar[mbar_idx(g, arr, l)] : 0.0f; };
     const G<Q> *__restrict__ qp = (const G<Q> *)a.qp;
@@ -219,11 +227,17 @@ __device__ __forceinline__ void step_tile(const Geo &g, const State &a0, const S
         const unsigned o = ob + (unsigned)s * sz4, oj = o - sy4;
         {
             const float lb = mbar(2, s), mb = mbar(1, s);
-            const unsigned mo = o >> MSH, mj = oj >> MSH;
-            x.lA = mdec(gl(lam, mo), lb); x.lB = mdec(gl(lam, mo, MB), lb);
-            x.lC = mdec(gl(lam, mj), lb); x.lD = mdec(gl(lam, mj, MB), lb);
-            x.mA = mdec(gl(mu, mo), mb); x.mB = mdec(gl(mu, mo, MB), mb);
-            x.mC = mdec(gl(mu, mj), mb); x.mD = mdec(gl(mu, mj, MB), mb);
+            if constexpr (P::mixed) {
+                const unsigned wA = gl(mlq, o), wB = gl(mlq, o, 4), wC = gl(mlq, oj), wD = gl(mlq, oj, 4);
+                x.lA = mdec(h_hi(wA), lb); x.lB = mdec(h_hi(wB), lb); x.lC = mdec(h_hi(wC), lb); x.lD = mdec(h_hi(wD), lb);
+                x.mA = mdec(h_lo(wA), mb); x.mB = mdec(h_lo(wB), mb); x.mC = mdec(h_lo(wC), mb); x.mD = mdec(h_lo(wD), mb);
+            } else {
+                const unsigned mo = o >> MSH, mj = oj >> MSH;
+                x.lA = mdec(gl(lam, mo), lb); x.lB = mdec(gl(lam, mo, MB), lb);
+                x.lC = mdec(gl(lam, mj), lb); x.lD = mdec(gl(lam, mj, MB), lb);
+                x.mA = mdec(gl(mu, mo), mb); x.mB = mdec(gl(mu, mo, MB), mb);
+                x.mC = mdec(gl(mu, mj), mb); x.mD = mdec(gl(mu, mj, MB), mb);
+            }
         }
         if constexpr (P::mixed) {
             const unsigned wA = gl(qq, o), wB = gl(qq, o, 4), wC = gl(qq, oj), wD = gl(qq, oj, 4);
@@ -287,11 +301,17 @@ __device__ __forceinline__ void step_tile(const Geo &g, const State &a0, const S
         const unsigned om = ob + (unsigned)(kb - 1) * sz4, omj = om - sy4;
         {
             const float lb = mbar(2, kb - 1), mb = mbar(1, kb - 1);
-            const unsigned mo = om >> MSH, mj = omj >> MSH;
-            lA_m = mdec(gl(lam, mo), lb); lB_m = mdec(gl(lam, mo, MB), lb);
-            lC_m = mdec(gl(lam, mj), lb); lD_m = mdec(gl(lam, mj, MB), lb);
-            mA_m = mdec(gl(mu, mo), mb); mB_m = mdec(gl(mu, mo, MB), mb);
-            mC_m = mdec(gl(mu, mj), mb); mD_m = mdec(gl(mu, mj, MB), mb);
+            if constexpr (P::mixed) {
+                const unsigned wA = gl(mlq, om), wB = gl(mlq, om, 4), wC = gl(mlq, omj), wD = gl(mlq, omj, 4);
+                lA_m = mdec(h_hi(wA), lb); lB_m = mdec(h_hi(wB), lb); lC_m = mdec(h_hi(wC), lb); lD_m = mdec(h_hi(wD), lb);
+                mA_m = mdec(h_lo(wA), mb); mB_m = mdec(h_lo(wB), mb); mC_m = mdec(h_lo(wC), mb); mD_m = mdec(h_lo(wD), mb);
+            } else {
+                const unsigned mo = om >> MSH, mj = omj >> MSH;
+                lA_m = mdec(gl(lam, mo), lb); lB_m = mdec(gl(lam, mo, MB), lb);
+                lC_m = mdec(gl(lam, mj), lb); lD_m = mdec(gl(lam, mj, MB), lb);
+                mA_m = mdec(gl(mu, mo), mb); mB_m = mdec(gl(mu, mo, MB), mb);
+                mC_m = mdec(gl(mu, mj), mb); mD_m = mdec(gl(mu, mj, MB), mb);
+            }
         }
         if constexpr (P::mixed) {
             const unsigned wA = gl(qq, om), wB = gl(qq, om, 4), wC = gl(qq, omj), wD = gl(qq, omj, 4);
@@ -379,8 +399,15 @@ __device__ __forceinline__ void step_tile(const Geo &g, const State &a0, const S
                     const float ut_ip1 = (i < g.nx) ? (pu[lpos + 1] - (pw[lpos + 1] - w0)) : 0.0f;
                     const float vt_jm1 = (j > 1) ? (pv[lpos - Sh::RW] - (w0 - pw[lpos - Sh::RW])) : 0.0f;
                     const unsigned o = ob + (unsigned)g.nz * sz4;
-                    const float xl = 1.0f / mdec(gl(lam, o >> MSH), mbar(2, g.nz));
-                    const float xm = 2.0f / mdec(gl(mu, o >> MSH), mbar(1, g.nz));
+                    float xl, xm;
+                    if constexpr (P::mixed) {
+                        const unsigned w = gl(mlq, o);   // (mu, lam) pair
+                        xl = 1.0f / mdec(h_hi(w), mbar(2, g.nz));
+                        xm = 2.0f / mdec(h_lo(w), mbar(1, g.nz));
+                    } else {
+                        xl = 1.0f / mdec(gl(lam, o >> MSH), mbar(2, g.nz));
+                        xm = 2.0f / mdec(gl(mu, o >> MSH), mbar(1, g.nz));
+                    }
                     const float xl2m = xm + xl;
                     const float wt = pl1[2 * PL + lpos] - (xl / xl2m) * (
                         (ut_ip1 - ut) +

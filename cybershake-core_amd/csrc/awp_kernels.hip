@@ -209,8 +209,15 @@ __device__ __forceinline__ void fs_velocity_block(const Geo &g, const State &s0,
     using M = typename P::M;
     const float mb = P::mixed ? s.mbar[mbar_idx(g, 1, g.nz)] : 0.0f;
     const float lb = P::mixed ? s.mbar[mbar_idx(g, 2, g.nz)] : 0.0f;
-    const float xl = 1.0f / mdec(((const G<M> *)s.lam)[p], lb);
-    const float xm = 2.0f / mdec(((const G<M> *)s.mu)[p], mb);
+    float xl, xm;
+    if constexpr (P::mixed) {
+        const unsigned w = ((const G<unsigned> *)s.mu)[p];   // (mu, lam) pair
+        xl = 1.0f / mdec(h_hi(w), lb);
+        xm = 2.0f / mdec(h_lo(w), mb);
+    } else {
+        xl = 1.0f / mdec(((const G<M> *)s.lam)[p], lb);
+        xm = 2.0f / mdec(((const G<M> *)s.mu)[p], mb);
+    }
     const float xl2m = xm + xl;
     const float wt = w1[p - sz] - (xl / xl2m) * (
         (ut_ip1 - ut) +
@@ -268,6 +275,7 @@ __device__ __forceinline__ void stress_tile_e(const Geo &g, const State &s0, con
     using M = typename P::M;
     const G<M> *__restrict__ lam = (const G<M> *)s.lam;
     const G<M> *__restrict__ mu = (const G<M> *)s.mu;
+    const G<unsigned> *__restrict__ mlq = (const G<unsigned> *)s.mu;   // mixed: (mu, lam) pairs
     const G<Q> *__restrict__ qp = (const G<Q> *)s.qp;
     const G<Q> *__restrict__ qs = (const G<Q> *)s.qs;
     const G<unsigned> *__restrict__ qq = (const G<unsigned> *)s.qp;   // mixed: (qp, qs) pairs
@@ -293,10 +301,17 @@ __device__ __forceinline__ void stress_tile_e(const Geo &g, const State &s0, con
     long long om = o - sz;
     const float mb_m = P::mixed ? s.mbar[mbar_idx(g, 1, kb - 1)] : 0.0f;
     const float lb_m = P::mixed ? s.mbar[mbar_idx(g, 2, kb - 1)] : 0.0f;
-    float lA_m = mdec(lam[om], lb_m), lB_m = mdec(lam[om + 1], lb_m), lC_m = mdec(lam[om - sy], lb_m),
-          lD_m = mdec(lam[om + 1 - sy], lb_m);
-    float mA_m = mdec(mu[om], mb_m), mB_m = mdec(mu[om + 1], mb_m), mC_m = mdec(mu[om - sy], mb_m),
-          mD_m = mdec(mu[om + 1 - sy], mb_m);
+    float lA_m, lB_m, lC_m, lD_m, mA_m, mB_m, mC_m, mD_m;
+    if constexpr (P::mixed) {
+        const unsigned wA = mlq[om], wB = mlq[om + 1], wC = mlq[om - sy], wD = mlq[om + 1 - sy];
+        lA_m = mdec(h_hi(wA), lb_m); lB_m = mdec(h_hi(wB), lb_m); lC_m = mdec(h_hi(wC), lb_m); lD_m = mdec(h_hi(wD), lb_m);
+        mA_m = mdec(h_lo(wA), mb_m); mB_m = mdec(h_lo(wB), mb_m); mC_m = mdec(h_lo(wC), mb_m); mD_m = mdec(h_lo(wD), mb_m);
+    } else {
+        lA_m = mdec(lam[om], lb_m); lB_m = mdec(lam[om + 1], lb_m); lC_m = mdec(lam[om - sy], lb_m);
+        lD_m = mdec(lam[om + 1 - sy], lb_m);
+        mA_m = mdec(mu[om], mb_m); mB_m = mdec(mu[om + 1], mb_m); mC_m = mdec(mu[om - sy], mb_m);
+        mD_m = mdec(mu[om + 1 - sy], mb_m);
+    }
     float pA_m, pB_m, pC_m, pD_m, sA_m, sB_m, sC_m, sD_m;
     if constexpr (P::mixed) {
         const unsigned wA = qq[om], wB = qq[om + 1], wC = qq[om - sy], wD = qq[om + 1 - sy];
@@ -316,10 +331,15 @@ __device__ __forceinline__ void stress_tile_e(const Geo &g, const State &s0, con
         const float w_p1 = w1[o + sz];
         const float mb = P::mixed ? s.mbar[mbar_idx(g, 1, k)] : 0.0f;
         const float lb = P::mixed ? s.mbar[mbar_idx(g, 2, k)] : 0.0f;
-        const float lA = mdec(lam[o], lb), lB = mdec(lam[o + 1], lb), lC = mdec(lam[o - sy], lb),
-                    lD = mdec(lam[o + 1 - sy], lb);
-        const float mA = mdec(mu[o], mb), mB = mdec(mu[o + 1], mb), mC = mdec(mu[o - sy], mb),
-                    mD = mdec(mu[o + 1 - sy], mb);
+        float lA, lB, lC, lD, mA, mB, mC, mD;
+        if constexpr (P::mixed) {
+            const unsigned wA = mlq[o], wB = mlq[o + 1], wC = mlq[o - sy], wD = mlq[o + 1 - sy];
+            lA = mdec(h_hi(wA), lb); lB = mdec(h_hi(wB), lb); lC = mdec(h_hi(wC), lb); lD = mdec(h_hi(wD), lb);
+            mA = mdec(h_lo(wA), mb); mB = mdec(h_lo(wB), mb); mC = mdec(h_lo(wC), mb); mD = mdec(h_lo(wD), mb);
+        } else {
+            lA = mdec(lam[o], lb); lB = mdec(lam[o + 1], lb); lC = mdec(lam[o - sy], lb); lD = mdec(lam[o + 1 - sy], lb);
+            mA = mdec(mu[o], mb); mB = mdec(mu[o + 1], mb); mC = mdec(mu[o - sy], mb); mD = mdec(mu[o + 1 - sy], mb);
+        }
         float pA, pB, pC, pD, sA, sB, sC, sD;
         if constexpr (P::mixed) {
             const unsigned wA = qq[o], wB = qq[o + 1], wC = qq[o - sy], wD = qq[o + 1 - sy];
@@ -494,6 +514,7 @@ __device__ __forceinline__ void stress_tile_b(const Geo &g, const State &s0, con
     using M = typename P::M;
     const G<M> *__restrict__ lam = (const G<M> *)s.lam;
     const G<M> *__restrict__ mu = (const G<M> *)s.mu;
+    const G<unsigned> *__restrict__ mlq = (const G<unsigned> *)s.mu;   // mixed: (mu, lam) pairs
     const G<Q> *__restrict__ qp = (const G<Q> *)s.qp;
     const G<Q> *__restrict__ qs = (const G<Q> *)s.qs;
     const G<unsigned> *__restrict__ qq = (const G<unsigned> *)s.qp;   // mixed: (qp, qs) pairs
@@ -518,10 +539,17 @@ __device__ __forceinline__ void stress_tile_b(const Geo &g, const State &s0, con
     // media at level k-1 on the 2x2 (x,y) cell: A=(i,j) B=(i+1,j) C=(i,j-1) D=(i+1,j-1)
     const float mb_m = P::mixed ? s.mbar[mbar_idx(g, 1, kb - 1)] : 0.0f;
     const float lb_m = P::mixed ? s.mbar[mbar_idx(g, 2, kb - 1)] : 0.0f;
-    float lA_m = mdec(p.ld(lam, 0, 0, -1), lb_m), lB_m = mdec(p.ld(lam, 1, 0, -1), lb_m),
-          lC_m = mdec(p.ld(lam, 0, -1, -1), lb_m), lD_m = mdec(p.ld(lam, 1, -1, -1), lb_m);
-    float mA_m = mdec(p.ld(mu, 0, 0, -1), mb_m), mB_m = mdec(p.ld(mu, 1, 0, -1), mb_m),
-          mC_m = mdec(p.ld(mu, 0, -1, -1), mb_m), mD_m = mdec(p.ld(mu, 1, -1, -1), mb_m);
+    float lA_m, lB_m, lC_m, lD_m, mA_m, mB_m, mC_m, mD_m;
+    if constexpr (P::mixed) {
+        const unsigned wA = p.ld(mlq, 0, 0, -1), wB = p.ld(mlq, 1, 0, -1), wC = p.ld(mlq, 0, -1, -1), wD = p.ld(mlq, 1, -1, -1);
+        lA_m = mdec(h_hi(wA), lb_m); lB_m = mdec(h_hi(wB), lb_m); lC_m = mdec(h_hi(wC), lb_m); lD_m = mdec(h_hi(wD), lb_m);
+        mA_m = mdec(h_lo(wA), mb_m); mB_m = mdec(h_lo(wB), mb_m); mC_m = mdec(h_lo(wC), mb_m); mD_m = mdec(h_lo(wD), mb_m);
+    } else {
+        lA_m = mdec(p.ld(lam, 0, 0, -1), lb_m); lB_m = mdec(p.ld(lam, 1, 0, -1), lb_m);
+        lC_m = mdec(p.ld(lam, 0, -1, -1), lb_m); lD_m = mdec(p.ld(lam, 1, -1, -1), lb_m);
+        mA_m = mdec(p.ld(mu, 0, 0, -1), mb_m); mB_m = mdec(p.ld(mu, 1, 0, -1), mb_m);
+        mC_m = mdec(p.ld(mu, 0, -1, -1), mb_m); mD_m = mdec(p.ld(mu, 1, -1, -1), mb_m);
+    }
     float pA_m, pB_m, pC_m, pD_m, sA_m, sB_m, sC_m, sD_m;
     if constexpr (P::mixed) {
         const unsigned wA = p.ld(qq, 0, 0, -1), wB = p.ld(qq, 1, 0, -1);
@@ -548,10 +576,17 @@ __device__ __forceinline__ void stress_tile_b(const Geo &g, const State &s0, con
         const float w_p1 = p.ld(w1, 0, 0, 1);
         const float mb = P::mixed ? s.mbar[mbar_idx(g, 1, k)] : 0.0f;
         const float lb = P::mixed ? s.mbar[mbar_idx(g, 2, k)] : 0.0f;
-        const float lA = mdec(p.ld(lam), lb), lB = mdec(p.ld(lam, 1), lb), lC = mdec(p.ld(lam, 0, -1), lb),
-                    lD = mdec(p.ld(lam, 1, -1), lb);
-        const float mA = mdec(p.ld(mu), mb), mB = mdec(p.ld(mu, 1), mb), mC = mdec(p.ld(mu, 0, -1), mb),
-                    mD = mdec(p.ld(mu, 1, -1), mb);
+        float lA, lB, lC, lD, mA, mB, mC, mD;
+        if constexpr (P::mixed) {
+            const unsigned wA = p.ld(mlq), wB = p.ld(mlq, 1), wC = p.ld(mlq, 0, -1), wD = p.ld(mlq, 1, -1);
+            lA = mdec(h_hi(wA), lb); lB = mdec(h_hi(wB), lb); lC = mdec(h_hi(wC), lb); lD = mdec(h_hi(wD), lb);
+            mA = mdec(h_lo(wA), mb); mB = mdec(h_lo(wB), mb); mC = mdec(h_lo(wC), mb); mD = mdec(h_lo(wD), mb);
+        } else {
+            lA = mdec(p.ld(lam), lb); lB = mdec(p.ld(lam, 1), lb); lC = mdec(p.ld(lam, 0, -1), lb);
+            lD = mdec(p.ld(lam, 1, -1), lb);
+            mA = mdec(p.ld(mu), mb); mB = mdec(p.ld(mu, 1), mb); mC = mdec(p.ld(mu, 0, -1), mb);
+            mD = mdec(p.ld(mu, 1, -1), mb);
+        }
         float pA, pB, pC, pD, sA, sB, sC, sD;
         if constexpr (P::mixed) {
             const unsigned wA = p.ld(qq), wB = p.ld(qq, 1), wC = p.ld(qq, 0, -1), wD = p.ld(qq, 1, -1);
@@ -882,8 +917,9 @@ __global__ void k_sgt_coef(const State s, const long long sz, const int pz, cons
     float lam, mu;
     if (s.mbar) {
         const long long q = p / sz;   // profile index of the level
-        mu = mdec(((const _Float16 *)s.mu)[p], s.mbar[pz + q]);
-        lam = mdec(((const _Float16 *)s.lam)[p], s.mbar[2 * pz + q]);
+        const unsigned w = ((const unsigned *)s.mu)[p];   // (mu, lam) pair
+        mu = mdec(h_lo(w), s.mbar[pz + q]);
+        lam = mdec(h_hi(w), s.mbar[2 * pz + q]);
     } else {
         mu = ((const float *)s.mu)[p];
         lam = ((const float *)s.lam)[p];
@@ -1094,6 +1130,22 @@ k_media_encode(const float *src, _Float16 *dst, const float *bar, const long lon
     }
 }
 
+// mu and lam as (mu, lam) pairs, each half rounded as k_media_encode rounds it
+__global__ void __launch_bounds__(256)
+k_media_encode2(const float *mu, const float *lam, unsigned *dst, const float *bar_mu, const float *bar_lam,
+                const long long sz)
+{
+    const float bm = bar_mu[blockIdx.y], bl = bar_lam[blockIdx.y];
+    const long long z0 = (long long)blockIdx.y * sz;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < sz; t += (long long)gridDim.x * blockDim.x) {
+        const float x = mu[z0 + t], y = lam[z0 + t];
+        const _Float16 hm = bm != 0.0f ? (_Float16)(float)((double)x / (double)bm - 1.0) : (_Float16)0.0f;
+        const _Float16 hl = bl != 0.0f ? (_Float16)(float)((double)y / (double)bl - 1.0) : (_Float16)0.0f;
+        dst[z0 + t] = (unsigned)__builtin_bit_cast(unsigned short, hm) |
+                      ((unsigned)__builtin_bit_cast(unsigned short, hl) << 16);
+    }
+}
+
 __global__ void __launch_bounds__(256) k_absmax(const float *a, const long long n, float *part)
 {
     __shared__ float red[256];
@@ -1256,6 +1308,19 @@ k_pack_media(const Geo g, const _Float16 *src, const float *bar, const int kext,
     const int k = blockIdx.z + 1 + k0;
     if (i > g.nx || j > g.ny || k > g.nz + kext) return;
     dst[((size_t)(k - 1 - k0) * g.ny + (j - 1)) * g.nx + (i - 1)] = mdec(src[gidx(g, i, j, k)], bar[k + 1]);
+}
+
+// mu (half 0) or lam (half 1) of the mixed-storage (mu, lam) pairs
+__global__ void __launch_bounds__(256)
+k_pack_media2(const Geo g, const unsigned *src, const int half, const float *bar, const int kext, const int k0,
+              float *dst)
+{
+    const int i = blockIdx.x * TX + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int k = blockIdx.z + 1 + k0;
+    if (i > g.nx || j > g.ny || k > g.nz + kext) return;
+    const unsigned w = src[gidx(g, i, j, k)];
+    dst[((size_t)(k - 1 - k0) * g.ny + (j - 1)) * g.nx + (i - 1)] = mdec(half ? h_hi(w) : h_lo(w), bar[k + 1]);
 }
 
 // levels k0+1 .. k0+gridDim.z of one field, as [k - 1 - k0][j - 1][i - 1]
@@ -1517,6 +1582,14 @@ hipError_t launch_media_encode(const Geo &g, const float *src, _Float16 *dst, co
     return hipGetLastError();
 }
 
+hipError_t launch_media_encode2(const Geo &g, const float *mu, const float *lam, unsigned *dst, const float *bar_mu,
+                                const float *bar_lam, hipStream_t st)
+{
+    const int bx = (int)std::min<long long>((g.sz + 255) / 256, 1024);
+    hipLaunchKernelGGL(k_media_encode2, dim3(bx, g.pz), dim3(256), 0, st, mu, lam, dst, bar_mu, bar_lam, g.sz);
+    return hipGetLastError();
+}
+
 hipError_t launch_absmax(const float *a, long long n, float *part, int nblk, hipStream_t st)
 {
     hipLaunchKernelGGL(k_absmax, dim3(nblk), dim3(256), 0, st, a, n, part);
@@ -1557,8 +1630,13 @@ hipError_t launch_pack_levels(bool mixed, const Geo &g, const State &s, int whic
         hipLaunchKernelGGL(k_pack_field<float>, grd, blk, 0, st, g, f32[which], 1.0f, kext, k0, dst);
     } else if (which >= 15 && which < 18) {
         if (mixed)
-            hipLaunchKernelGGL(k_pack_media, grd, blk, 0, st, g, (const _Float16 *)mm[which - 15],
-                               s.mbar + (size_t)(which - 15) * g.pz, kext, k0, dst);
+        {
+            if (which == 15)
+                hipLaunchKernelGGL(k_pack_media, grd, blk, 0, st, g, (const _Float16 *)mm[0], s.mbar, kext, k0, dst);
+            else
+                hipLaunchKernelGGL(k_pack_media2, grd, blk, 0, st, g, (const unsigned *)s.mu, which - 16,
+                                   s.mbar + (size_t)(which - 15) * g.pz, kext, k0, dst);
+        }
         else
             hipLaunchKernelGGL(k_pack_field<float>, grd, blk, 0, st, g, (const float *)mm[which - 15], 1.0f, kext,
                                k0, dst);

@@ -229,6 +229,10 @@ __device__ inline _Float16 r_encode(float r, int e) { return (_Float16)ldexpf(r,
 // both; these unpack the halves.
 __device__ __forceinline__ float q_lo(unsigned w) { return (float)__builtin_bit_cast(_Float16, (unsigned short)(w & 0xFFFFu)); }
 __device__ __forceinline__ float q_hi(unsigned w) { return (float)__builtin_bit_cast(_Float16, (unsigned short)(w >> 16)); }
+// The same for mu (low half) and lam (high half), whose FP16 deviations
+// mdec decodes.
+__device__ __forceinline__ _Float16 h_lo(unsigned w) { return __builtin_bit_cast(_Float16, (unsigned short)(w & 0xFFFFu)); }
+__device__ __forceinline__ _Float16 h_hi(unsigned w) { return __builtin_bit_cast(_Float16, (unsigned short)(w >> 16)); }
 // h * 2^(e-15) as one multiply by the power of two (exact scaling, the
 // same rounding as ldexpf; e - 15 lies in [REXP_MIN - 15, REXP_MAX - 15],
 // inside the normal exponent range)
@@ -502,6 +506,10 @@ hipError_t launch_level_range(const Geo &g, const float *a, unsigned *range, hip
 hipError_t launch_media_encode(const Geo &g, const float *src, _Float16 *dst, const float *bar, hipStream_t st);
 hipError_t launch_absmax(const float *a, long long n, float *part, int nblk, hipStream_t st);
 hipError_t launch_to_half(const float *src, _Float16 *dst, float scale, long long n, hipStream_t st);
+// mixed storage: mu, lam (FP32) to (mu, lam) pairs of FP16 deviations from
+// their level profiles (k_media_encode's rounding)
+hipError_t launch_media_encode2(const Geo &g, const float *mu, const float *lam, unsigned *dst, const float *bar_mu,
+                                const float *bar_lam, hipStream_t st);
 // mixed storage: qp, qs (FP32, n points) to (qp, qs) FP16 pairs scaled by sp, ss
 hipError_t launch_to_qpair(const float *qp, const float *qs, unsigned *dst, float sp, float ss, long long n,
                            hipStream_t st);
