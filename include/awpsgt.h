@@ -40,7 +40,8 @@ enum {
 enum {
     AWP_STORAGE_FP32 = 0,   /* everything FP32: the reference's storage */
     AWP_STORAGE_MIXED = 1   /* r1..r6 FP16 with per-row exponents, qp/qs FP16 (scaled), d1/lam/mu FP16
-                               deviations from an FP32 profile per level */
+                               deviations from an FP32 profile per level; qp with qs and mu with lam
+                               share one 4-byte word a point */
 };
 
 /* The IN3D parameters the SGT path uses, in the reference's units.  Filled by
